@@ -1,0 +1,165 @@
+/*
+ * gmrf_c.h — C ABI of the MI355X (gfx950) numeric GMRF core.
+ *
+ * Replaces the numeric path behind rust-inla's solver interface:
+ *   trait InlaSolver            crates/inla_math/src/solver.rs:29-44
+ *   FaerCpuSolver               crates/inla_math/src/solver.rs:48-99
+ *   factorize_sparse / refactorize_numeric_arc / symbolic_pattern /
+ *   sparse_solve_in_place / sparse_diagonal_inverse
+ *                               crates/inla_math/src/sparse_ldlt.rs:196-517
+ * and adds what the reference has no interface for: batched numeric
+ * factorisation over one symbolic analysis, batched solves and selected
+ * inversion, and the selected inverse on the pattern of Q.
+ *
+ * Conventions
+ *  - Matrices are square CSC with 64-bit indptr/indices, rows sorted and unique
+ *    within each column, both triangles stored (as sprs::CsMat in the reference,
+ *    crates/inla_math/src/sparse.rs:3). Only entries with row >= col are read
+ *    numerically, as crates/inla_math/src/sparse_ldlt.rs:74-102 does.
+ *  - Every function returns a gmrf_status; on failure gmrf_last_error() returns
+ *    a thread-local message. Status codes map 1:1 onto the reference's
+ *    SolverError (solver.rs:10-17) / MathError (error.rs:7-37).
+ *  - gmrf_solver_* take host pointers (like the reference's slices).
+ *    gmrf_batch_* take DEVICE pointers on the library's device and enqueue
+ *    work on the batch's stream; gmrf_batch_sync() waits for it.
+ *  - A handle is used by one thread at a time; distinct handles may be used
+ *    concurrently (the reference's Send + Sync with &mut self).
+ */
+#ifndef GMRF_C_H
+#define GMRF_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gmrf_status {
+  GMRF_OK = 0,
+  GMRF_SINGULAR = 1,              /* SolverError::Singular            */
+  GMRF_NOT_POSITIVE_DEFINITE = 2, /* SolverError::NotPositiveDefinite */
+  GMRF_SOLVE_FAILURE = 3,         /* SolverError::SolveFailure(msg)   */
+  GMRF_INVALID_ARGUMENT = 4,      /* MathError::{NotSquare, DimensionMismatch, InvalidMatrix, ...} */
+  GMRF_OUT_OF_MEMORY = 5,         /* MathError::OutOfMemory           */
+  GMRF_DEVICE_ERROR = 6           /* HIP runtime failure (no reference counterpart) */
+} gmrf_status;
+
+/* Ordering request for gmrf_analyze. CHOOSER mirrors choose_symmetric_order
+ * (crates/inla_math/src/ordering.rs:149-162): time-major Kronecker relabelling,
+ * else RCM when it halves the envelope, else approximate minimum degree. */
+typedef enum gmrf_order {
+  GMRF_ORDER_CHOOSER = 0,
+  GMRF_ORDER_AMD = 1,
+  GMRF_ORDER_NATURAL = 2,
+  GMRF_ORDER_GIVEN = 3 /* perm[k] = original index at factor position k */
+} gmrf_order;
+
+typedef struct gmrf_symbolic gmrf_symbolic;
+typedef struct gmrf_batch gmrf_batch;
+typedef struct gmrf_solver gmrf_solver;
+
+typedef struct gmrf_symbolic_stats {
+  int64_t n;
+  int64_t nnz;           /* stored entries of the input CSC */
+  int64_t nnz_q_lower;   /* entries with row >= col */
+  int32_t order_kind;    /* 0 AMD, 1 time-major, 2 RCM, 3 natural, 4 given */
+  int32_t nlevels;       /* height of the supernodal tree */
+  int64_t nsuper;
+  int64_t nnz_l;         /* structural nnz(L) incl. the diagonal */
+  int64_t l_size;        /* doubles of the supernodal L storage (incl. relaxation zeros) */
+  int64_t u_size;        /* doubles of the update workspace */
+  int64_t max_front;
+  double flops_ldlt;     /* sum_j r_j (r_j + 3) */
+  double flops_selinv;   /* sum_j 2 r_j (r_j + 1) + n */
+  double flops_solve;    /* 4 sum_j r_j + n */
+  double sum_r2;
+} gmrf_symbolic_stats;
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* gmrf_last_error(void);
+
+/* Select the HIP device used by handles created afterwards (default 0). */
+gmrf_status gmrf_set_device(int32_t device);
+
+/* ---- symbolic analysis (once per sparsity pattern) -------------------------
+ * Replaces factorize_symbolic / symbolic_pattern (sparse_ldlt.rs:159-186,473-477)
+ * and the pattern cache of crates/inla_math/src/scratch.rs:92-118; the cache
+ * here is process-wide, keyed by the exact (indptr, indices) arrays. */
+gmrf_status gmrf_analyze(int64_t n, const int64_t* indptr, const int64_t* indices,
+                         int32_t order, const int64_t* perm, gmrf_symbolic** out);
+void gmrf_symbolic_free(gmrf_symbolic* sym);
+gmrf_status gmrf_symbolic_get_stats(const gmrf_symbolic* sym, gmrf_symbolic_stats* out);
+/* perm[k] = original index at factor position k (orig_of_factor, sparse_ldlt.rs:272-277) */
+gmrf_status gmrf_symbolic_perm(const gmrf_symbolic* sym, int64_t* perm);
+/* column elimination tree in factor order, -1 for roots */
+gmrf_status gmrf_symbolic_etree(const gmrf_symbolic* sym, int64_t* parent);
+/* structural column counts of L (incl. diagonal) in factor order */
+gmrf_status gmrf_symbolic_colcount(const gmrf_symbolic* sym, int64_t* colcount);
+/* supernode partition: sn_start[nsuper+1] and supernodal parents[nsuper] */
+gmrf_status gmrf_symbolic_supernodes(const gmrf_symbolic* sym, int64_t* sn_start,
+                                     int64_t* sn_parent);
+/* Ordering-chooser restatement (ordering.rs:8-162) for inspection:
+ * returns the branch (0 AMD, 1 time-major, 2 RCM) and, when not AMD, fwd[old] = new. */
+int32_t gmrf_choose_order(int64_t n, const int64_t* indptr, const int64_t* indices, int64_t* fwd);
+
+/* ---- batched numeric phase (device pointers) --------------------------------
+ * A batch holds up to `capacity` numeric factorisations of one pattern. */
+gmrf_status gmrf_batch_create(gmrf_symbolic* sym, int32_t capacity, gmrf_batch** out);
+void gmrf_batch_destroy(gmrf_batch* bt);
+/* Numeric LDL^T of `count` matrices; values[b * ld_values + e] is entry e of the
+ * CSC of matrix b (device). Replaces factorize_numeric_ldlt as called from
+ * sparse_ldlt.rs:196-241,480-517 — no pivoting, no regularisation. */
+gmrf_status gmrf_batch_factorize(gmrf_batch* bt, int32_t count, const double* values,
+                                 int64_t ld_values);
+/* log|Q_b| = sum log|D| (ldlt.rs:64-67) and per-matrix status (0 ok,
+ * GMRF_SINGULAR, GMRF_NOT_POSITIVE_DEFINITE). Both outputs are host pointers;
+ * the call synchronises the batch stream. */
+gmrf_status gmrf_batch_logdet(gmrf_batch* bt, int32_t count, double* logdet, int32_t* status);
+/* Device-side variant (no synchronisation): logdet/status are device pointers. */
+gmrf_status gmrf_batch_logdet_device(gmrf_batch* bt, int32_t count, double* logdet,
+                                     int32_t* status);
+/* x_b = Q_b^-1 rhs_b, original order (sparse_ldlt.rs:243-265), device pointers;
+ * x may alias rhs. */
+gmrf_status gmrf_batch_solve(gmrf_batch* bt, int32_t count, const double* rhs, int64_t ld_rhs,
+                             double* x, int64_t ld_x);
+/* Selected inversion on the supernodal pattern of L (Takahashi recurrence). */
+gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count);
+/* diag(Q_b^-1) in original order after gmrf_batch_selinv (sparse_ldlt.rs:421-431). */
+gmrf_status gmrf_batch_selinv_diag(gmrf_batch* bt, int32_t count, double* diag, int64_t ld_diag);
+/* (Q_b^-1)_e for every stored CSC entry e after gmrf_batch_selinv (device). */
+gmrf_status gmrf_batch_selinv_pattern(gmrf_batch* bt, int32_t count, double* sigma,
+                                      int64_t ld_sigma);
+gmrf_status gmrf_batch_sync(gmrf_batch* bt);
+/* The batch's HIP stream (hipStream_t) for callers that enqueue around it. */
+void* gmrf_batch_stream(gmrf_batch* bt);
+/* Enqueue subsequent batch work on the caller's stream (hipStream_t, not owned;
+ * NULL restores the batch's own stream). Keeps ordering with the caller's work
+ * without host synchronisation. */
+gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream);
+
+/* ---- InlaSolver mirror (host pointers) --------------------------------------
+ * gmrf_solver_factorize = InlaSolver::factorize (solver.rs:70-74): symbolic
+ * analysis on the first call or when the pattern changes, numeric otherwise.
+ * On failure the previously stored factor is kept, as in the reference.
+ * solve / diag_inv / log_abs_det before any successful factorize return
+ * GMRF_SOLVE_FAILURE "Factorization not computed yet" (solver.rs:77-79). */
+gmrf_status gmrf_solver_create(gmrf_solver** out);
+void gmrf_solver_destroy(gmrf_solver* s);
+gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indptr,
+                                  const int64_t* indices, const double* data);
+gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x);
+gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out);
+gmrf_status gmrf_solver_log_abs_det(const gmrf_solver* s, double* out);
+int64_t gmrf_solver_n(const gmrf_solver* s);
+
+/* ---- diagnostics -------------------------------------------------------------
+ * T = A B^T for one 64x64 tile through the FP64 MFMA tile engine (A, B are
+ * 64 x K column-major host arrays, K <= 64). No reference counterpart: pins the
+ * MFMA lane layout in the tests. */
+gmrf_status gmrf_probe_mma(const double* A, const double* B, int32_t K, double* T);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMRF_C_H */

@@ -1,0 +1,114 @@
+"""ctypes binding of libgmrf_hip.so (include/gmrf_c.h).
+
+The shared library is built in-tree next to this file (``__graft_entry__.build``
+or ``make -C rust-inla_amd/csrc``). There is no fallback: when the library is
+missing, importing this module raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgmrf_hip.so")
+
+GMRF_OK = 0
+GMRF_SINGULAR = 1
+GMRF_NOT_POSITIVE_DEFINITE = 2
+GMRF_SOLVE_FAILURE = 3
+GMRF_INVALID_ARGUMENT = 4
+GMRF_OUT_OF_MEMORY = 5
+GMRF_DEVICE_ERROR = 6
+
+ORDER_CHOOSER = 0
+ORDER_AMD = 1
+ORDER_NATURAL = 2
+ORDER_GIVEN = 3
+
+_i32 = C.c_int32
+_i64 = C.c_int64
+_dbl = C.c_double
+_vp = C.c_void_p
+_pi64 = C.POINTER(C.c_int64)
+_pdbl = C.POINTER(C.c_double)
+_pi32 = C.POINTER(C.c_int32)
+
+
+class SymbolicStats(C.Structure):
+    _fields_ = [
+        ("n", _i64),
+        ("nnz", _i64),
+        ("nnz_q_lower", _i64),
+        ("order_kind", _i32),
+        ("nlevels", _i32),
+        ("nsuper", _i64),
+        ("nnz_l", _i64),
+        ("l_size", _i64),
+        ("u_size", _i64),
+        ("max_front", _i64),
+        ("flops_ldlt", _dbl),
+        ("flops_selinv", _dbl),
+        ("flops_solve", _dbl),
+        ("sum_r2", _dbl),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/gmrf_c.h
+SIGNATURES = {
+    "gmrf_last_error": (C.c_char_p, []),
+    "gmrf_set_device": (_i32, [_i32]),
+    "gmrf_analyze": (_i32, [_i64, _pi64, _pi64, _i32, _pi64, C.POINTER(_vp)]),
+    "gmrf_symbolic_free": (None, [_vp]),
+    "gmrf_symbolic_get_stats": (_i32, [_vp, C.POINTER(SymbolicStats)]),
+    "gmrf_symbolic_perm": (_i32, [_vp, _pi64]),
+    "gmrf_symbolic_etree": (_i32, [_vp, _pi64]),
+    "gmrf_symbolic_colcount": (_i32, [_vp, _pi64]),
+    "gmrf_symbolic_supernodes": (_i32, [_vp, _pi64, _pi64]),
+    "gmrf_choose_order": (_i32, [_i64, _pi64, _pi64, _pi64]),
+    "gmrf_batch_create": (_i32, [_vp, _i32, C.POINTER(_vp)]),
+    "gmrf_batch_destroy": (None, [_vp]),
+    "gmrf_batch_factorize": (_i32, [_vp, _i32, _vp, _i64]),
+    "gmrf_batch_logdet": (_i32, [_vp, _i32, _pdbl, _pi32]),
+    "gmrf_batch_logdet_device": (_i32, [_vp, _i32, _vp, _vp]),
+    "gmrf_batch_solve": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64]),
+    "gmrf_batch_selinv": (_i32, [_vp, _i32]),
+    "gmrf_batch_selinv_diag": (_i32, [_vp, _i32, _vp, _i64]),
+    "gmrf_batch_selinv_pattern": (_i32, [_vp, _i32, _vp, _i64]),
+    "gmrf_batch_sync": (_i32, [_vp]),
+    "gmrf_batch_stream": (_vp, [_vp]),
+    "gmrf_batch_set_stream": (_i32, [_vp, _vp]),
+    "gmrf_solver_create": (_i32, [C.POINTER(_vp)]),
+    "gmrf_solver_destroy": (None, [_vp]),
+    "gmrf_solver_factorize": (_i32, [_vp, _i64, _pi64, _pi64, _pdbl]),
+    "gmrf_solver_solve": (_i32, [_vp, _pdbl, _pdbl]),
+    "gmrf_solver_diag_inv": (_i32, [_vp, _pdbl]),
+    "gmrf_solver_log_abs_det": (_i32, [_vp, _pdbl]),
+    "gmrf_solver_n": (_i64, [_vp]),
+    "gmrf_probe_mma": (_i32, [_pdbl, _pdbl, _i32, _pdbl]),
+}
+
+
+def _load():
+    # torch bundles its own HIP runtime with the same SONAME (libamdhip64.so.7).
+    # Loading torch first makes this library bind to that runtime instead of
+    # pulling a second copy from /opt/rocm into the process.
+    import torch  # noqa: F401
+
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C rust-inla_amd/csrc`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    msg = lib.gmrf_last_error()
+    return msg.decode() if msg else ""

@@ -1,0 +1,241 @@
+"""Synthetic GMRF workloads, built the way rust-inla builds them.
+
+Everything here produces host-side inputs (CSC patterns, affine precision
+components, projectors, observations); nothing here factorises anything.
+
+* ``lattice_mesh`` restates py-inla/python/inla/spde.py:23-63 (regular lattice
+  over a rectangle, x fastest, two triangles per cell).
+* ``fem_blocks`` restates the mass / stiffness assembly of
+  crates/inla_fmesher/src/fmesher.rs:544-629 with identity diffusion
+  (c0 lumped area/3 on the diagonal, g1 = e_i . e_j / (4 |T|)).
+* ``spde_components`` returns the fixed matrices of
+  Q(tau, kappa) = tau^2 (kappa^4 C0 + 2 kappa^2 G1 + G1 C0^-1 G1)
+  (crates/inla_stats/src/spde.rs:70-142) on one union pattern, so Q(theta) is
+  an affine combination sum_k c_k(theta) Q_k.
+* ``lattice_projector`` restates the barycentric projector of
+  fmesher.rs:175-219 for points inside the lattice mesh.
+* ``besag_components`` restates crates/inla_stats/src/besag.rs:127-152.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+GEOM_EPSILON = 1e-15  # fmesher.rs:5
+
+
+def lattice_mesh(nx: int, ny: int, xlim=(0.0, 1.0), ylim=(0.0, 1.0)):
+    """Vertices (nx*ny, 2) and triangles (2(nx-1)(ny-1), 3); spde.py:23-63."""
+    if nx < 2 or ny < 2:
+        raise ValueError("nx and ny must be >= 2")
+    xs = np.linspace(xlim[0], xlim[1], nx)
+    ys = np.linspace(ylim[0], ylim[1], ny)
+    xx, yy = np.meshgrid(xs, ys, indexing="xy")
+    vertices = np.column_stack([xx.ravel(order="C"), yy.ravel(order="C")])
+    j, i = np.meshgrid(np.arange(ny - 1), np.arange(nx - 1), indexing="ij")
+    i = i.ravel()
+    j = j.ravel()
+    v00 = j * nx + i
+    v10 = j * nx + i + 1
+    v01 = (j + 1) * nx + i
+    v11 = (j + 1) * nx + i + 1
+    t1 = np.stack([v00, v10, v01], axis=1)
+    t2 = np.stack([v10, v11, v01], axis=1)
+    tris = np.empty((2 * t1.shape[0], 3), dtype=np.int64)
+    tris[0::2] = t1
+    tris[1::2] = t2
+    return vertices, tris
+
+
+def fem_blocks(vertices: np.ndarray, tris: np.ndarray):
+    """(c0 diagonal, g1 CSC) with identity diffusion; fmesher.rs:544-629."""
+    n = vertices.shape[0]
+    s0 = vertices[tris[:, 0]]
+    s1 = vertices[tris[:, 1]]
+    s2 = vertices[tris[:, 2]]
+    e = [s2 - s1, s0 - s2, s1 - s0]  # edge_vec(a, b) = a - b
+    area = 0.5 * np.abs((s1[:, 0] - s0[:, 0]) * (s2[:, 1] - s0[:, 1])
+                        - (s1[:, 1] - s0[:, 1]) * (s2[:, 0] - s0[:, 0]))
+    fa = 0.5 * np.abs(e[0][:, 0] * e[1][:, 1] - e[0][:, 1] * e[1][:, 0])
+    fa = np.where(fa <= GEOM_EPSILON, area, fa)
+    c0 = np.zeros(n)
+    np.add.at(c0, tris.ravel(), np.repeat(area / 3.0, 3))
+    rows, cols, vals = [], [], []
+    for a in range(3):
+        for b in range(3):
+            # rotate_perp preserves dot products, so H = I gives e_a . e_b
+            eab = e[a][:, 0] * e[b][:, 0] + e[a][:, 1] * e[b][:, 1]
+            rows.append(tris[:, a])
+            cols.append(tris[:, b])
+            vals.append(eab / (4.0 * fa))
+    g1 = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(n, n))
+    g1.sum_duplicates()
+    g1.sort_indices()
+    return c0, g1
+
+
+def union_pattern(mats):
+    """Common sorted CSC pattern of several square matrices (+ diagonal)."""
+    n = mats[0].shape[0]
+    acc = sp.identity(n, format="csc") * 0.0
+    pat = sp.csc_matrix((np.ones(n), (np.arange(n), np.arange(n))), shape=(n, n))
+    for m in mats:
+        m = sp.csc_matrix(m)
+        pat = pat + sp.csc_matrix((np.ones(m.nnz), m.indices, m.indptr), shape=m.shape)
+    pat = sp.csc_matrix(pat)
+    pat.sum_duplicates()
+    pat.sort_indices()
+    del acc
+    return pat.indptr.astype(np.int64), pat.indices.astype(np.int64)
+
+
+def values_on_pattern(m, indptr, indices):
+    """Values of sparse matrix m at the given CSC pattern (0 where m has none)."""
+    m = sp.csc_matrix(m)
+    m.sum_duplicates()
+    m.sort_indices()
+    n = len(indptr) - 1
+    out = np.zeros(len(indices))
+    for c in range(n):
+        a, b = indptr[c], indptr[c + 1]
+        ma, mb = m.indptr[c], m.indptr[c + 1]
+        if mb == ma:
+            continue
+        pos = np.searchsorted(indices[a:b], m.indices[ma:mb])
+        if np.any(pos >= b - a) or np.any(indices[a:b][np.minimum(pos, b - a - 1)] != m.indices[ma:mb]):
+            raise ValueError("matrix has entries outside the pattern")
+        out[a + pos] = m.data[ma:mb]
+    return out
+
+
+@dataclass
+class AffineGmrf:
+    """Q(theta) = sum_k coef_k(theta) * Q_k on one full symmetric CSC pattern."""
+
+    n: int
+    indptr: np.ndarray
+    indices: np.ndarray
+    components: np.ndarray  # (K, nnz)
+    names: list = field(default_factory=list)
+
+    def values(self, coefs):
+        return np.asarray(coefs) @ self.components
+
+    def csc(self, coefs):
+        return sp.csc_matrix((self.values(coefs), self.indices, self.indptr), shape=(self.n, self.n))
+
+
+def spde_components(nx: int, ny: int) -> tuple[AffineGmrf, np.ndarray, np.ndarray]:
+    """SPDE alpha=2 pieces on the nx x ny lattice: components (C0, G1, G1 C0^-1 G1).
+
+    Q(tau, kappa) = tau^2 kappa^4 C0 + 2 tau^2 kappa^2 G1 + tau^2 G1 C0^-1 G1.
+    """
+    verts, tris = lattice_mesh(nx, ny)
+    c0, g1 = fem_blocks(verts, tris)
+    n = verts.shape[0]
+    C0 = sp.diags(c0, format="csc")
+    GCG = sp.csc_matrix(g1 @ sp.diags(1.0 / c0) @ g1)
+    indptr, indices = union_pattern([C0, g1, GCG])
+    comps = np.stack([values_on_pattern(C0, indptr, indices),
+                      values_on_pattern(g1, indptr, indices),
+                      values_on_pattern(GCG, indptr, indices)])
+    return AffineGmrf(n, indptr, indices, comps, ["C0", "G1", "G1C0invG1"]), verts, tris
+
+
+def spde_coefs(log_tau: float, log_kappa: float):
+    """Coefficients of (C0, G1, GCG) and their derivatives w.r.t. (log tau, log kappa)."""
+    t2 = math.exp(2 * log_tau)
+    k2 = math.exp(2 * log_kappa)
+    c = np.array([t2 * k2 * k2, 2 * t2 * k2, t2])
+    # d/dlog_tau: 2c ; d/dlog_kappa: (4, 2, 0) * c
+    dc = np.array([2 * c, [4 * c[0], 2 * c[1], 0.0]])
+    return c, dc
+
+
+def lattice_projector(nx: int, ny: int, pts: np.ndarray, xlim=(0.0, 1.0), ylim=(0.0, 1.0)):
+    """Barycentric projector A (n_obs x nx*ny) for points inside the lattice mesh."""
+    hx = (xlim[1] - xlim[0]) / (nx - 1)
+    hy = (ylim[1] - ylim[0]) / (ny - 1)
+    fx = (pts[:, 0] - xlim[0]) / hx
+    fy = (pts[:, 1] - ylim[0]) / hy
+    i = np.clip(np.floor(fx).astype(np.int64), 0, nx - 2)
+    j = np.clip(np.floor(fy).astype(np.int64), 0, ny - 2)
+    u = fx - i
+    v = fy - j
+    v00 = j * nx + i
+    v10 = v00 + 1
+    v01 = v00 + nx
+    v11 = v01 + 1
+    lower = (u + v) <= 1.0
+    verts = np.where(lower[:, None], np.stack([v00, v10, v01], 1), np.stack([v10, v11, v01], 1))
+    w = np.where(lower[:, None], np.stack([1 - u - v, u, v], 1),
+                 np.stack([1 - v, u + v - 1, 1 - u], 1))
+    m = pts.shape[0]
+    rows = np.repeat(np.arange(m), 3)
+    keep = np.abs(w.ravel()) > GEOM_EPSILON
+    A = sp.csc_matrix((w.ravel()[keep], (rows[keep], verts.ravel()[keep])), shape=(m, nx * ny))
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def rook_adjacency(nr: int, nc: int):
+    """Rook neighbours on an nr x nc lattice, node = r*nc + c."""
+    adj = [[] for _ in range(nr * nc)]
+    for r in range(nr):
+        for c in range(nc):
+            i = r * nc + c
+            if r > 0:
+                adj[i].append(i - nc)
+            if c > 0:
+                adj[i].append(i - 1)
+            if c + 1 < nc:
+                adj[i].append(i + 1)
+            if r + 1 < nr:
+                adj[i].append(i + nc)
+    return adj
+
+
+def besag_structure(adj):
+    """Unit-tau Besag structure matrix R (besag.rs:127-152 with tau = 1)."""
+    n = len(adj)
+    rows, cols, vals = [], [], []
+    for i, nb in enumerate(adj):
+        rows.append(i)
+        cols.append(i)
+        vals.append(float(len(nb)) if nb else 1.0)
+        for j in nb:
+            rows.append(i)
+            cols.append(j)
+            vals.append(-1.0)
+    R = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+    R.sum_duplicates()
+    R.sort_indices()
+    return R
+
+
+def sample_gmrf(Q: sp.csc_matrix, rng: np.random.Generator) -> np.ndarray:
+    """One draw x ~ N(0, Q^-1) through SuperLU in symmetric mode (data generation)."""
+    from scipy.sparse.linalg import splu
+
+    n = Q.shape[0]
+    lu = splu(sp.csc_matrix(Q), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+              options={"SymmetricMode": True})
+    # Pr Q Pc = L U with U = D L^T when no pivoting happened (Pr = Pc^T).
+    d = lu.U.diagonal()
+    if np.any(d <= 0):
+        raise ValueError("sample_gmrf: Q is not positive definite")
+    z = rng.standard_normal(n) / np.sqrt(d)
+    # Solve U' ... : x_p = L^-T z with L unit lower (U = D L^T  =>  L^T = D^-1 U)
+    from scipy.sparse.linalg import spsolve_triangular
+
+    Lt = sp.diags(1.0 / d) @ lu.U
+    y = spsolve_triangular(sp.csr_matrix(Lt), z, lower=False, unit_diagonal=True)
+    x = np.empty(n)
+    x[lu.perm_c] = y
+    return x
