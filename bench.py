@@ -1,0 +1,192 @@
+#!/usr/bin/env python
+"""Headline benchmark: CCD theta-points/s (LDL^T + selected inversion + gradient), FP64.
+
+Workload: SPDE alpha=2 on a 142 x 142 unit-square lattice (n = 20,164), 50,000
+uniform observations through the barycentric projector, Gaussian likelihood
+(sigma_eps = 0.1), true range 0.3 and sigma 1, field sampled from Q(tau, kappa);
+internal theta = (log prec_eps, log tau, log kappa); CCD with m = 3 gives 15
+theta-points around the mode, each evaluated as one unit of work:
+  prior LDL^T (Q + 1e-12 I), posterior LDL^T (Q + A^T W A), the mode solve,
+  posterior and prior selected inversion, the gradient of the log marginal
+  likelihood (trace contractions), marginal variances.
+One step = all 15 theta-points in one batched call; value = theta-points / s.
+
+The CCD design is scaled with the Hessian from the finite-difference stencil of
+crates/inla_math/src/hyper_opt.rs:197-259, all 19 stencil points evaluated in one
+batch before timing. The mode is the generating theta (the design does not
+depend on how the mode was found).
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rust-inla_amd"))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12   # MI355X dense FP64 (MFMA), FLOP/s
+PEAK_HBM = 8.0e12     # MI355X HBM3E, B/s
+METRIC = "CCD θ-points/sec (LDLᵀ+selinv+grad) FP64, SPDE n≈20k; % FP64-MFMA/HBM peak"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, theta, seconds_hint):
+    """Oracle (CPU restatement, 1 core) on one theta-point of the same workload:
+    prior + posterior LDL^T, mode, Takahashi diag (per CCD node, inference.rs:1306-1385)
+    plus a central finite-difference gradient (2m extra Laplace evaluations), the
+    reference's way of differentiating (hyper_opt.rs:197-259)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    coefs, _, lpar, _ = wl.params(theta)
+    t0 = time.perf_counter()
+    q = wl.gmrf.csc(coefs[0])
+    r = orc.find_mode(q, "gaussian", "identity", wl.y, lpar[0], A=wl.A, keep_post=True)
+    r.post_factor().takahashi_diag()
+    h = 1e-4
+    for l in range(len(theta)):
+        for s in (h, -h):
+            th = np.array(theta, dtype=float)
+            th[l] += s
+            c, _, lp, _ = wl.params(th)
+            orc.find_mode(wl.gmrf.csc(c[0]), "gaussian", "identity", wl.y, lp[0], A=wl.A)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "theta-points/s", "cores": 1, "kind": "port",
+            "sample": f"1 CCD theta-point of the same workload on the oracle (1 thread): node work "
+                      f"(prior+posterior LDLT, mode solve, Takahashi diag) + 2m=6 FD Laplace "
+                      f"evaluations for the gradient; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nx", type=int, default=142)
+    ap.add_argument("--n-obs", type=int, default=50_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(theta-points would shard across GPUs with one all-gather of the node outputs).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from inla_gpu import _lib, ccd
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+    from inla_gpu.solver import Symbolic
+
+    torch.cuda.init()
+    log("== setup: workload")
+    wl = W.spde_gaussian_workload(nx=args.nx, n_obs=args.n_obs)
+    g = wl.gmrf
+    m = 3
+    design_z, design_w = ccd.ccd_design(m, 1.0)
+    npts = len(design_z)
+    stencil = ccd.hessian_points(wl.theta_true, max(1e-4, 0.05 * float(np.max(np.abs(wl.theta_true)))))
+    cap = max(npts, len(stencil))
+    log(f"== setup: model n={g.n} nnz={len(g.indices)} obs={len(wl.y)} capacity={cap}")
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="gaussian", A=wl.A,
+                         capacity=cap)
+    st = Symbolic(g.n, g.indptr, g.indices).stats()
+    # Hessian of the log posterior from one batched evaluation of the stencil
+    hh = max(1e-4, 0.05 * float(np.max(np.abs(wl.theta_true))))
+    c, dc, lp, dl = wl.params(stencil)
+    r = model.eval(c, lpar=lp, want_x=False, want_var=False)
+    lpost = r["mlik"].cpu().numpy() + np.array([wl.log_prior(t)[0] for t in stencil])
+    H = ccd.hessian_from_values(lpost, m, hh)
+    lam, V = ccd.scaling_from_hessian(H)
+    thetas = np.array([ccd.theta_from_z(wl.theta_true, lam, V, z) for z in design_z])
+    coefs, dcoefs, lpar, dlog = wl.params(thetas)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    coefs, dcoefs, lpar, dlog = dev(coefs), dev(dcoefs), dev(lpar), dev(dlog)
+
+    def step():
+        return model.eval(coefs, lpar=lpar, dcoefs=dcoefs, dlog_lpar=dlog, want_x=True, want_var=True)
+
+    log(f"== warmup {args.warmup}")
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    assert (out["status"] == 0).all(), out["status"]
+    log(f"== timed {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = npts * args.steps / (t1 - t0)
+    log(f"   {ms_per_step:.3f} ms/step, {value:.1f} theta-points/s")
+
+    # the same timed loop again with per-launch HIP events for the roofline
+    log("== profiled pass")
+    _lib.profile_enable(True)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    prof = _lib.profile_summary()
+    _lib.profile_enable(False)
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    name, d = dom
+    avg_s = d["ms"] * 1e-3 / d["launches"]
+    fl = d["flops"] / d["launches"]
+    achieved = fl / avg_s
+    bound = "mfma" if name in ("k_factor_large", "k_selinv_large") else "hbm"
+    if bound == "mfma":
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name}
+    else:
+        bw = (d["bytes"] / d["launches"]) / avg_s
+        roof = {"bound": "hbm", "achieved": bw / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": bw / PEAK_HBM, "traffic": None, "kernel": name}
+    kernels = {k: {"launches": v["launches"], "ms_per_step": v["ms"] / args.steps,
+                   "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    log(json.dumps(kernels, indent=1))
+
+    grad = out["grad"].cpu().numpy()
+    mlik = out["mlik"].cpu().numpy()
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "theta-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"spde-gaussian lattice {args.nx}x{args.nx} n={g.n}, "
+                               f"{len(wl.y)} obs, CCD m=3 ({npts} theta-points)",
+                   "n": g.n, "n_obs": len(wl.y), "theta_points": npts, "nnz_q": int(len(g.indices)),
+                   "nnz_l": int(st["nnz_l"]), "nsuper": int(st["nsuper"]), "levels": int(st["nlevels"]),
+                   "flops_ldlt": st["flops_ldlt"], "flops_selinv": st["flops_selinv"],
+                   "mlik_mode": float(mlik[0]), "grad_mode_norm": float(np.linalg.norm(grad[0]))},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("== cpu baseline (oracle, 1 thread, 1 theta-point)")
+        res["cpu_baseline"] = cpu_baseline(wl, wl.theta_true, 25)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -152,8 +152,86 @@ gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out);
 gmrf_status gmrf_solver_log_abs_det(const gmrf_solver* s, double* out);
 int64_t gmrf_solver_n(const gmrf_solver* s);
 
+/* ---- batched Laplace evaluation -------------------------------------------
+ * A latent Gaussian model with an affine precision
+ *   Q(theta) = sum_k coef_k(theta) Q_k   (all Q_k on one full symmetric CSC pattern
+ *                                         that also contains the pattern of A^T A),
+ * observations y_i with linear predictor eta = A x (A: n_obs x n CSC; NULL means
+ * the identity) and one likelihood family. gmrf_model_eval runs, for a batch of
+ * theta-points, what the reference runs per point in
+ * find_latent_mode_a_with_solver (crates/inla_stats/src/inference.rs:776-975):
+ * prior factorisation (Q + 1e-12 I, or Q itself when constraints are given —
+ * the caller then includes kappa A_c^T A_c as a component with a constant
+ * coefficient, constraints.rs:275-313), the Newton loop with step cap and
+ * backtracking, the confirmation factorisation, the Laplace marginal likelihood
+ * (inference.rs:972), diag(Q_post^-1) by selected inversion, and the gradient of
+ * the marginal likelihood w.r.t. theta (no reference counterpart). */
+enum { GMRF_FAMILY_GAUSSIAN = 0, GMRF_FAMILY_POISSON = 1, GMRF_FAMILY_BINOMIAL = 2,
+       GMRF_FAMILY_NBINOMIAL = 3 };
+enum { GMRF_LINK_IDENTITY = 0, GMRF_LINK_LOG = 1, GMRF_LINK_LOGIT = 2 };
+
+typedef struct gmrf_model gmrf_model;
+
+typedef struct gmrf_model_desc {
+  int64_t n;                 /* latent dimension */
+  const int64_t* indptr;     /* full symmetric CSC pattern (host) */
+  const int64_t* indices;
+  int32_t n_comp;            /* K */
+  const double* comps;       /* K x nnz values of Q_1..Q_K on the pattern (host) */
+  int64_t n_obs;
+  const int64_t* a_indptr;   /* A: n_obs x n CSC (host), NULL for eta = x */
+  const int64_t* a_indices;
+  const double* a_values;
+  int32_t family;            /* GMRF_FAMILY_* */
+  int32_t link;              /* GMRF_LINK_* */
+  const double* y;           /* n_obs responses */
+  const double* obs_a;       /* n_obs: Gaussian precision scale / Poisson exposure /
+                                binomial trials / NB exposure (NULL = 1) */
+  const double* obs_b;       /* n_obs: NB size (NULL otherwise) */
+  int32_t n_constr;          /* k hard constraints A_c x = e (projection), 0 for none */
+  const double* constr_a;    /* k x n row-major (host) */
+  const double* constr_e;    /* k */
+  int32_t capacity;          /* max theta-points per gmrf_model_eval */
+} gmrf_model_desc;
+
+typedef struct gmrf_eval_args {
+  int32_t count;             /* theta-points in this call (<= capacity) */
+  int32_t n_theta;           /* m, length of theta (for the gradient) */
+  const double* coefs;       /* device, count x K */
+  const double* lpar;        /* device, count: Gaussian precision multiplier (NULL = 1) */
+  const double* dcoefs;      /* device, count x m x K: d coef_k / d theta_l (NULL: no gradient) */
+  const double* dlog_lpar;   /* device, count x m: d log(lpar) / d theta_l (NULL = 0) */
+  int32_t max_iter;          /* Newton iterations (the reference uses 200) */
+  double tol;                /* Newton tolerance (the reference uses 1e-5) */
+  /* outputs, device pointers (NULL to skip x / var / grad) */
+  double* mlik;              /* count */
+  double* x;                 /* count x n, posterior mode (original order) */
+  double* var;               /* count x n, diag(Q_post^-1) */
+  double* grad;              /* count x m, d mlik / d theta */
+  /* outputs, host pointers */
+  int32_t* status;           /* count: GMRF_OK or the failure of that point */
+  int32_t* n_iter;           /* count: Newton iterations used */
+} gmrf_eval_args;
+
+gmrf_status gmrf_model_create(const gmrf_model_desc* desc, gmrf_model** out);
+void gmrf_model_destroy(gmrf_model* model);
+gmrf_status gmrf_model_eval(gmrf_model* model, gmrf_eval_args* args);
+void* gmrf_model_stream(gmrf_model* model);
+gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream);
+
 /* ---- diagnostics -------------------------------------------------------------
- * T = A B^T for one 64x64 tile through the FP64 MFMA tile engine (A, B are
+ * Per-launch timing of the level kernels with HIP events on the launching
+ * stream (no reference counterpart; used by the benchmark's roofline).
+ * gmrf_profile_enable(1) clears and starts recording; gmrf_profile_summary
+ * waits for the recorded events and returns per-kernel launch counts, total
+ * milliseconds and the algorithmic flops / bytes of the recorded launches;
+ * `names` receives max_entries strings of name_len bytes. */
+gmrf_status gmrf_profile_enable(int32_t on);
+gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_len,
+                                 int64_t* launches, double* total_ms, double* flops, double* bytes,
+                                 int32_t* n_entries);
+
+/* T = A B^T for one 64x64 tile through the FP64 MFMA tile engine (A, B are
  * 64 x K column-major host arrays, K <= 64). No reference counterpart: pins the
  * MFMA lane layout in the tests. */
 gmrf_status gmrf_probe_mma(const double* A, const double* B, int32_t K, double* T);

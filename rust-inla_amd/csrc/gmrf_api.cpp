@@ -10,227 +10,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/gmrf_c.h"
-#include "plan.hpp"
-#include "symbolic.hpp"
+#include "runtime.hpp"
 
-namespace gmrf {
-hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
-                               int64_t, double*, double*, hipStream_t);
-hipError_t launch_factor_large(const DevPlan&, const int32_t*, int, int, const double*, int64_t,
-                               double*, double*, hipStream_t);
-hipError_t launch_logdet(const DevPlan&, int, const double*, double*, int32_t*, hipStream_t);
-hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t);
-hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
-hipError_t launch_diag_scale(const DevPlan&, int, const double*, double*, hipStream_t);
-hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
-                            double*, hipStream_t);
-hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
-                            hipStream_t);
-hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
-                               double*, double*, hipStream_t);
-hipError_t launch_selinv_large(const DevPlan&, const int32_t*, int, int, const double*, double*,
-                               double*, double*, hipStream_t);
-hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
-hipError_t launch_pattern_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
-hipError_t launch_mma_probe(const double*, const double*, int, double*, hipStream_t);
-}  // namespace gmrf
-
-using namespace gmrf;
-
-namespace {
-
+namespace gmrf_rt {
 thread_local std::string g_err;
 int g_device = 0;
-
-gmrf_status fail(gmrf_status st, const std::string& msg) {
-  g_err = msg;
-  return st;
-}
-
-struct HipError : std::runtime_error {
-  hipError_t code;
-  HipError(hipError_t c, const char* what)
-      : std::runtime_error(std::string(what) + ": " + hipGetErrorString(c)), code(c) {}
-};
-
-inline void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw HipError(e, what);
-}
-
-gmrf_status from_exception() {
-  try {
-    throw;
-  } catch (const HipError& e) {
-    if (e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation)
-      return fail(GMRF_OUT_OF_MEMORY, e.what());
-    return fail(GMRF_DEVICE_ERROR, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(GMRF_OUT_OF_MEMORY, "out of memory during sparse factorization");
-  } catch (const std::exception& e) {
-    return fail(GMRF_INVALID_ARGUMENT, e.what());
-  } catch (...) {
-    return fail(GMRF_DEVICE_ERROR, "unknown error");
-  }
-}
-
-// Device allocation owned by a vector of raw pointers freed in the destructor.
-struct DevArena {
-  std::vector<void*> ptrs;
-  ~DevArena() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  T* alloc(size_t count) {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    check(hipMalloc(&p, bytes), "hipMalloc");
-    ptrs.push_back(p);
-    return (T*)p;
-  }
-  template <typename T>
-  T* upload(const std::vector<T>& v) {
-    T* d = alloc<T>(v.size());
-    if (!v.empty()) check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
-    return d;
-  }
-};
-
-struct LevelInfo {
-  int64_t sb, se;     // small fronts [sb, se) in small list
-  int64_t lb, le;     // large fronts [lb, le) in large list
-  int64_t ab, ae;     // all fronts [ab, ae) in all list
-  int32_t maxm_small, maxm_all;
-};
-
-}  // namespace
-
-// Symbolic analysis plus its device plan; shared by batches and solvers.
-struct SymData {
-  Symbolic S;
-  int device = 0;
-  DevArena arena;
-  DevPlan plan{};
-  std::vector<LevelInfo> levels;
-  int32_t* d_small = nullptr;
-  int32_t* d_large = nullptr;
-  int32_t* d_all = nullptr;
-  int order_req = 0;
-  std::vector<int64_t> given;  // given permutation (cache key)
-  std::mutex dev_mu;
-  bool dev_built = false;
-
-  // Upload the plan on first use (symbolic analysis itself needs no device).
-  void ensure_device() {
-    std::lock_guard<std::mutex> lk(dev_mu);
-    if (dev_built) return;
-    build_device();
-    dev_built = true;
-  }
-
-  void build_device() {
-    check(hipSetDevice(device), "hipSetDevice");
-    const int64_t ns = S.nsuper, n = S.n;
-    if (n >= (int64_t)1 << 31 || S.nnz_full >= (int64_t)1 << 31)
-      throw std::runtime_error("pattern too large for 32-bit device indices");
-    std::vector<int32_t> sn_start(ns + 1), sn_r(ns), sn_parent(ns), child_ptr(ns + 1),
-        child_idx(S.child_idx.size()), rows(S.sn_rows.size()), perm(n);
-    for (int64_t s = 0; s <= ns; ++s) sn_start[s] = (int32_t)S.sn_start[s];
-    for (int64_t s = 0; s < ns; ++s) {
-      sn_r[s] = (int32_t)(S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s]);
-      sn_parent[s] = (int32_t)S.sn_parent[s];
-    }
-    for (int64_t s = 0; s <= ns; ++s) child_ptr[s] = (int32_t)S.child_ptr[s];
-    for (size_t q = 0; q < S.child_idx.size(); ++q) child_idx[q] = (int32_t)S.child_idx[q];
-    for (size_t q = 0; q < S.sn_rows.size(); ++q) rows[q] = (int32_t)S.sn_rows[q];
-    for (int64_t k = 0; k < n; ++k) perm[k] = (int32_t)S.perm[k];
-    // input entries per supernode
-    std::vector<int64_t> col2s(n);
-    for (int64_t s = 0; s < ns; ++s)
-      for (int64_t j = S.sn_start[s]; j < S.sn_start[s + 1]; ++j) col2s[j] = s;
-    std::vector<int64_t> fa_ptr(ns + 1, 0);
-    std::vector<int64_t> esn(S.nnz_full, -1);
-    for (int64_t c = 0; c < n; ++c)
-      for (int64_t p = S.indptr[c]; p < S.indptr[c + 1]; ++p) {
-        if (S.amap[p] < 0) continue;
-        int64_t fj = std::min(S.iperm[S.indices[p]], S.iperm[c]);
-        esn[p] = col2s[fj];
-        fa_ptr[esn[p] + 1]++;
-      }
-    for (int64_t s = 0; s < ns; ++s) fa_ptr[s + 1] += fa_ptr[s];
-    std::vector<int32_t> fa_csc(fa_ptr[ns]), fa_slot(fa_ptr[ns]);
-    {
-      std::vector<int64_t> fill(fa_ptr.begin(), fa_ptr.end() - 1);
-      for (int64_t p = 0; p < S.nnz_full; ++p) {
-        if (esn[p] < 0) continue;
-        int64_t s = esn[p], q = fill[s]++;
-        fa_csc[q] = (int32_t)p;
-        int64_t local = S.amap[p] - S.l_off[s];
-        if (local < 0 || local >= ((int64_t)1 << 31)) throw std::runtime_error("front too large");
-        fa_slot[q] = (int32_t)local;
-      }
-    }
-    plan.n = (int32_t)n;
-    plan.nsuper = (int32_t)ns;
-    plan.nnz = S.nnz_full;
-    plan.l_size = S.l_size;
-    plan.u_size = S.u_size;
-    plan.v_size = S.v_size;
-    plan.sn_start = arena.upload(sn_start);
-    plan.sn_r = arena.upload(sn_r);
-    plan.sn_parent = arena.upload(sn_parent);
-    std::vector<int64_t> loff(S.l_off.begin(), S.l_off.begin() + ns);
-    std::vector<int64_t> uoff(S.u_off.begin(), S.u_off.begin() + ns);
-    std::vector<int64_t> voff(S.v_off.begin(), S.v_off.begin() + ns);
-    plan.l_off = arena.upload(loff);
-    plan.u_off = arena.upload(uoff);
-    plan.v_off = arena.upload(voff);
-    plan.rows_ptr = arena.upload(S.sn_rows_ptr);
-    plan.rows = arena.upload(rows);
-    plan.rel = arena.upload(S.rel);
-    plan.child_ptr = arena.upload(child_ptr);
-    plan.child_idx = arena.upload(child_idx);
-    plan.fa_ptr = arena.upload(fa_ptr);
-    plan.fa_csc = arena.upload(fa_csc);
-    plan.fa_slot = arena.upload(fa_slot);
-    plan.perm = arena.upload(perm);
-    plan.diag_slot = arena.upload(S.diag_slot);
-    plan.slot_full = arena.upload(S.slot_full);
-    // level classes
-    std::vector<int32_t> small, large, all;
-    levels.resize(S.nlevels);
-    for (int l = 0; l < S.nlevels; ++l) {
-      LevelInfo& L = levels[l];
-      L.sb = (int64_t)small.size();
-      L.lb = (int64_t)large.size();
-      L.ab = (int64_t)all.size();
-      L.maxm_small = 0;
-      L.maxm_all = 0;
-      for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-        int64_t s = S.level_sn[q];
-        int32_t m = (int32_t)((S.sn_start[s + 1] - S.sn_start[s]) + sn_r[s]);
-        all.push_back((int32_t)s);
-        L.maxm_all = std::max(L.maxm_all, m);
-        if (m <= kSmallFront) {
-          small.push_back((int32_t)s);
-          L.maxm_small = std::max(L.maxm_small, m);
-        } else {
-          large.push_back((int32_t)s);
-        }
-      }
-      L.se = (int64_t)small.size();
-      L.le = (int64_t)large.size();
-      L.ae = (int64_t)all.size();
-    }
-    d_small = arena.upload(small);
-    d_large = arena.upload(large);
-    d_all = arena.upload(all);
-  }
-};
-
-struct gmrf_symbolic {
-  std::shared_ptr<SymData> d;
-};
+Profiler g_prof;
+}  // namespace gmrf_rt
 
 namespace {
 
@@ -238,6 +24,8 @@ namespace {
 std::mutex g_cache_mu;
 std::vector<std::shared_ptr<SymData>> g_cache;  // most recent last
 constexpr size_t kCacheMax = 16;
+
+}  // namespace
 
 std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                         int order, const int64_t* given) {
@@ -274,112 +62,6 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
   return d;
 }
 
-}  // namespace
-
-// Numeric workspace for up to `cap` matrices of one pattern.
-struct gmrf_batch {
-  std::shared_ptr<SymData> sym;
-  int cap = 0;
-  hipStream_t stream = nullptr;  // stream in use (own or caller's)
-  hipStream_t own = nullptr;
-  DevArena arena;
-  double *L = nullptr, *U = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *V = nullptr;
-  double* d_logdet = nullptr;
-  int32_t* d_status = nullptr;
-
-  gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
-    check(hipSetDevice(sym->device), "hipSetDevice");
-    sym->ensure_device();
-    check(hipStreamCreateWithFlags(&own, hipStreamNonBlocking), "hipStreamCreate");
-    stream = own;
-    const Symbolic& Sy = sym->S;
-    L = arena.alloc<double>((size_t)cap * Sy.l_size);
-    U = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.u_size, 1));
-    S = arena.alloc<double>((size_t)cap * Sy.l_size);
-    T = arena.alloc<double>((size_t)cap * Sy.l_size);
-    X = arena.alloc<double>((size_t)cap * Sy.n);
-    V = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.v_size, 1));
-    d_logdet = arena.alloc<double>(cap);
-    d_status = arena.alloc<int32_t>(cap);
-  }
-  ~gmrf_batch() {
-    if (own) {
-      (void)hipStreamSynchronize(own);
-      (void)hipStreamDestroy(own);
-    }
-  }
-
-  void factorize(int count, const double* vals, int64_t ldv) {
-    const DevPlan& P = sym->plan;
-    for (const LevelInfo& lv : sym->levels) {
-      check(launch_factor_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
-                                vals, ldv, L, U, stream),
-            "factor_small");
-      check(launch_factor_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, vals, ldv, L,
-                                U, stream),
-            "factor_large");
-    }
-    check(launch_logdet(P, count, L, d_logdet, d_status, stream), "logdet");
-  }
-
-  void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx) {
-    const DevPlan& P = sym->plan;
-    check(launch_gather_perm(P, count, rhs, ldr, X, stream), "gather");
-    for (const LevelInfo& lv : sym->levels)
-      check(launch_solve_fwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, L, X,
-                             V, stream),
-            "solve_fwd");
-    check(launch_diag_scale(P, count, L, X, stream), "diag_scale");
-    for (size_t l = sym->levels.size(); l-- > 0;) {
-      const LevelInfo& lv = sym->levels[l];
-      check(launch_solve_bwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, L, X,
-                             stream),
-            "solve_bwd");
-    }
-    check(launch_scatter_perm(P, count, X, x, ldx, stream), "scatter");
-  }
-
-  void selinv(int count) {
-    const DevPlan& P = sym->plan;
-    for (size_t l = sym->levels.size(); l-- > 0;) {
-      const LevelInfo& lv = sym->levels[l];
-      check(launch_selinv_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, L, S, U, T,
-                                stream),
-            "selinv_large");
-      check(launch_selinv_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
-                                L, S, U, stream),
-            "selinv_small");
-    }
-  }
-};
-
-// InlaSolver mirror: one matrix, host pointers, double-buffered so that a failed
-// factorize leaves the previous factor usable (solver.rs:70-74 keeps it).
-struct gmrf_solver {
-  struct Slot {
-    std::unique_ptr<gmrf_batch> bt;
-    double logdet = 0.0;
-    bool have_selinv = false;
-  };
-  Slot slot[2];
-  int cur = -1;
-  double* d_vals = nullptr;
-  size_t d_vals_cap = 0;
-  double* d_vec = nullptr;
-  size_t d_vec_cap = 0;
-  ~gmrf_solver() {
-    if (d_vals) (void)hipFree(d_vals);
-    if (d_vec) (void)hipFree(d_vec);
-  }
-  void ensure(double*& p, size_t& cap, size_t need) {
-    if (cap >= need) return;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    check(hipMalloc(&p, need * sizeof(double)), "hipMalloc");
-    cap = need;
-  }
-};
 
 extern "C" {
 
@@ -705,6 +387,60 @@ gmrf_status gmrf_solver_log_abs_det(const gmrf_solver* s, double* out) {
   if (!out) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   *out = s->slot[s->cur].logdet;
   return GMRF_OK;
+}
+
+gmrf_status gmrf_profile_enable(int32_t on) {
+  try {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (on) g_prof.clear();
+    g_prof.on = on != 0;
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_len,
+                                 int64_t* launches, double* total_ms, double* flops, double* bytes,
+                                 int32_t* n_entries) {
+  if (!n_entries) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  try {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    std::vector<std::string> keys;
+    std::vector<int64_t> cnt;
+    std::vector<double> ms, fl, by;
+    for (auto& r : g_prof.recs) {
+      check(hipEventSynchronize(r.b), "hipEventSynchronize");
+      float t = 0.0f;
+      check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+      size_t k = std::find(keys.begin(), keys.end(), std::string(r.name)) - keys.begin();
+      if (k == keys.size()) {
+        keys.push_back(r.name);
+        cnt.push_back(0);
+        ms.push_back(0);
+        fl.push_back(0);
+        by.push_back(0);
+      }
+      cnt[k]++;
+      ms[k] += t;
+      fl[k] += r.flops;
+      by[k] += r.bytes;
+    }
+    *n_entries = (int32_t)keys.size();
+    for (int32_t k = 0; k < (int32_t)keys.size() && k < max_entries; ++k) {
+      if (names && name_len > 0) {
+        std::strncpy(names + (size_t)k * name_len, keys[k].c_str(), name_len - 1);
+        names[(size_t)k * name_len + name_len - 1] = 0;
+      }
+      if (launches) launches[k] = cnt[k];
+      if (total_ms) total_ms[k] = ms[k];
+      if (flops) flops[k] = fl[k];
+      if (bytes) bytes[k] = by[k];
+    }
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
 }
 
 gmrf_status gmrf_probe_mma(const double* A, const double* B, int32_t K, double* T) {

@@ -1,0 +1,466 @@
+// Kernels of the batched Laplace step: Q(theta) assembly, likelihood
+// derivatives, A^T W A scatter (gather form), sparse products, reductions and
+// the adjoint trace contractions. Every kernel works on `B` theta-points laid
+// out one after another (row b of a [B x len] array).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gmrf {
+
+// Likelihood families / links (crates/inla_stats/src/inference.rs:16-21,170-311).
+enum { FAM_GAUSSIAN = 0, FAM_POISSON = 1, FAM_BINOMIAL = 2, FAM_NBINOMIAL = 3 };
+enum { LINK_IDENTITY = 0, LINK_LOG = 1, LINK_LOGIT = 2 };
+
+// Lanczos log-gamma (g = 7, n = 9; the method of inference.rs:1896-1920).
+__device__ double lanczos_lgamma(double z) {
+  const double PI = 3.14159265358979323846;
+  double refl = 0.0, sgn = 1.0;
+  if (z < 0.5) {
+    refl = log(PI) - log(sin(PI * z));
+    z = 1.0 - z;
+    sgn = -1.0;
+  }
+  const double c[8] = {676.5203681218851,     -1259.1392167224028,  771.32342877765313,
+                       -176.61502916214059,   12.507343278686905,   -0.13857109526572012,
+                       9.9843695780195716e-6, 1.5056327351493116e-7};
+  const double z1 = z - 1.0;
+  double x = 0.99999999999980993;
+  for (int i = 0; i < 8; ++i) x += c[i] / (z1 + (double)i + 1.0);
+  const double t = z1 + 7.5;
+  const double v = 0.5 * log(2.0 * PI) + (z1 + 0.5) * log(t) - t + log(x);
+  return sgn < 0 ? refl - v : v;
+}
+
+// link_forward (inference.rs:1641-1664): mu, dmu/deta, d2mu/deta2, plus d3mu.
+__device__ __forceinline__ void link_fwd(int link, double eta, double& mu, double& d1, double& d2,
+                                         double& d3) {
+  if (link == LINK_IDENTITY) {
+    mu = eta; d1 = 1.0; d2 = 0.0; d3 = 0.0;
+  } else if (link == LINK_LOG) {
+    mu = exp(eta); d1 = mu; d2 = mu; d3 = mu;
+  } else {
+    double p;
+    if (eta >= 0.0) {
+      const double z = exp(-eta);
+      p = 1.0 / (1.0 + z);
+    } else {
+      const double z = exp(eta);
+      p = z / (1.0 + z);
+    }
+    mu = p;
+    d1 = p * (1.0 - p);
+    d2 = d1 * (1.0 - 2.0 * p);
+    d3 = d1 * (1.0 - 6.0 * p + 6.0 * p * p);
+  }
+}
+
+// Per observation: logp, grad = dlogp/deta, w = -hess (the Newton weight the
+// reference uses: Fisher for Poisson / binomial, observed otherwise), dw/deta,
+// and for the Gaussian precision hyperparameter d(logp)/dlog(prec) (dl_par).
+// ok = 0 when the reference would return an error or a non-finite logp.
+struct LikOut {
+  double logp, grad, w, dw, dl_par;
+  int ok;
+};
+
+__device__ LikOut eval_obs(int fam, int link, double eta, double y, double a, double b, double lpar) {
+  LikOut o{0, 0, 0, 0, 0, 1};
+  if (!isfinite(eta)) {
+    o.ok = 0;
+    return o;
+  }
+  double mu, d1, d2, d3;
+  link_fwd(link, eta, mu, d1, d2, d3);
+  if (fam == FAM_GAUSSIAN) {  // 170-186; precision = lpar * a
+    const double prec = lpar * a;
+    const double r = mu - y;
+    o.logp = -0.9189385332046728 + 0.5 * log(prec) - 0.5 * prec * r * r;
+    o.grad = -prec * r * d1;
+    o.w = prec * (d1 * d1 + r * d2);
+    o.dw = prec * (3.0 * d1 * d2 + r * d3);
+    o.dl_par = 0.5 - 0.5 * prec * r * r;
+    if (!(prec > 0.0) || !isfinite(prec) || !isfinite(y)) o.ok = 0;
+  } else if (fam == FAM_POISSON) {  // 188-223; a = exposure
+    const double m = a * mu, dm = a * d1;
+    if (m > 0.0) {
+      o.logp = y * log(m) - m - lanczos_lgamma(y + 1.0);
+      o.grad = (y / m - 1.0) * dm;
+      o.w = dm * dm / m;
+      // d/deta (dm^2 / m) with dm = a d1, d(dm)/deta = a d2
+      o.dw = (2.0 * dm * a * d2 * m - dm * dm * dm) / (m * m);
+    } else {
+      o.logp = (y == 0.0) ? 0.0 : -INFINITY;
+      o.grad = -dm;
+      o.w = fabs(a * d2);
+      o.dw = 0.0;
+    }
+    if (a < 0.0 || !isfinite(a) || y < 0.0 || !isfinite(y)) o.ok = 0;
+  } else if (fam == FAM_BINOMIAL) {  // 225-267; a = trials
+    const double n = a, p = mu;
+    const double lc = lanczos_lgamma(n + 1.0) - lanczos_lgamma(y + 1.0) - lanczos_lgamma(n - y + 1.0);
+    if (p == 0.0)
+      o.logp = (y == 0.0) ? lc : -INFINITY;
+    else if (p == 1.0)
+      o.logp = (y == n) ? lc : -INFINITY;
+    else
+      o.logp = lc + y * log(p) + (n - y) * log(1.0 - p);
+    const double g = (p > 0.0 && p < 1.0) ? y / p - (n - y) / (1.0 - p) : 0.0;
+    o.grad = g * d1;
+    o.w = n * d1;
+    o.dw = n * d2;
+    if (n < 0.0 || y < 0.0 || y > n || !isfinite(n) || !isfinite(y) || p < 0.0 || p > 1.0) o.ok = 0;
+  } else {  // 269-311; a = exposure, b = size
+    const double m = a * mu, dm = a * d1, d2m = a * d2, d3m = a * d3, r = b, mr = m + r;
+    if (!(m > 0.0) || !(r > 0.0) || y < 0.0) {
+      o.ok = 0;
+      return o;
+    }
+    o.logp = lanczos_lgamma(y + r) - lanczos_lgamma(r) - lanczos_lgamma(y + 1.0) + r * log(r / mr) +
+             y * log(m / mr);
+    const double dl = y / m - (y + r) / mr;
+    const double d2l = -y / (m * m) + (y + r) / (mr * mr);
+    const double d3l = 2.0 * y / (m * m * m) - 2.0 * (y + r) / (mr * mr * mr);
+    o.grad = dl * dm;
+    const double hess = d2l * dm * dm + dl * d2m;
+    o.w = -hess;
+    o.dw = -(d3l * dm * dm * dm + 3.0 * d2l * dm * d2m + dl * d3m);
+  }
+  if (!isfinite(o.logp)) o.ok = 0;
+  return o;
+}
+
+__global__ void k_lik(int B, int64_t nobs, int fam, int link, const double* __restrict__ eta,
+                      const double* __restrict__ y, const double* __restrict__ pa,
+                      const double* __restrict__ pb, const double* __restrict__ lpar,
+                      double* __restrict__ logp, double* __restrict__ grad, double* __restrict__ w,
+                      double* __restrict__ dw, double* __restrict__ dlpar, int32_t* __restrict__ bad) {
+  const int64_t tot = (int64_t)B * nobs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / nobs);
+    const int64_t i = e - (int64_t)b * nobs;
+    const LikOut o = eval_obs(fam, link, eta[e], y[i], pa[i], pb ? pb[i] : 0.0, lpar ? lpar[b] : 1.0);
+    if (logp) logp[e] = o.logp;
+    if (grad) grad[e] = o.grad;
+    if (w) w[e] = o.w;
+    if (dw) dw[e] = o.dw;
+    if (dlpar) dlpar[e] = o.dl_par;
+    if (!o.ok && bad) bad[b] = 1;  // benign race: every writer stores 1
+  }
+}
+
+// vals[b][e] = sum_k coefs[b][k] comps[k][e]
+__global__ void k_affine(int B, int K, int64_t nnz, const double* __restrict__ coefs,
+                         const double* __restrict__ comps, double* __restrict__ out) {
+  const int64_t tot = (int64_t)B * nnz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / nnz);
+    const int64_t q = e - (int64_t)b * nnz;
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s += coefs[b * K + k] * comps[(int64_t)k * nnz + q];
+    out[e] = s;
+  }
+}
+
+// out[b][diag_idx[i]] += add (prior jitter on the stored diagonal)
+__global__ void k_add_diag(int B, int64_t n, int64_t nnz, const int64_t* __restrict__ diag_idx,
+                           double add, double* __restrict__ out) {
+  const int64_t tot = (int64_t)B * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / n);
+    const int64_t i = e - (int64_t)b * n;
+    out[(int64_t)b * nnz + diag_idx[i]] += add;
+  }
+}
+
+// Q_post = Q + A^T diag(w) A on the fixed pattern: every CSC entry gathers its
+// observation products (ptr/obs/coef built once per model), no atomics.
+__global__ void k_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* __restrict__ ptr,
+                           const int32_t* __restrict__ obs, const double* __restrict__ coef,
+                           const double* __restrict__ w, const double* __restrict__ base,
+                           double* __restrict__ out) {
+  const int64_t tot = (int64_t)B * nnz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / nnz);
+    const int64_t q = e - (int64_t)b * nnz;
+    double s = base[e];
+    const double* wb = w + (int64_t)b * nobs;
+    for (int64_t t = ptr[q]; t < ptr[q + 1]; ++t) s += coef[t] * wb[obs[t]];
+    out[e] = s;
+  }
+}
+
+// y[b][o] = alpha * sum_{p in [ptr[o], ptr[o+1])} vals[b*vld + p] x[b][idx[p]] + beta * y[b][o]
+__global__ void k_gather_spmv(int B, int64_t nout, const int64_t* __restrict__ ptr,
+                              const int32_t* __restrict__ idx, const double* __restrict__ vals,
+                              int64_t vld, const double* __restrict__ x, int64_t ldx, double alpha,
+                              double beta, double* __restrict__ y, int64_t ldy) {
+  const int64_t tot = (int64_t)B * nout;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / nout);
+    const int64_t o = e - (int64_t)b * nout;
+    const double* vb = vals + (int64_t)b * vld;
+    const double* xb = x + (int64_t)b * ldx;
+    double s = 0.0;
+    for (int64_t p = ptr[o]; p < ptr[o + 1]; ++p) s += vb[p] * xb[idx[p]];
+    double* yo = y + (int64_t)b * ldy + o;
+    *yo = alpha * s + (beta == 0.0 ? 0.0 : beta * *yo);
+  }
+}
+
+// out[b][i] = a[b][i] + alpha[b] * c[b][i]
+__global__ void k_axpy_b(int B, int64_t len, const double* __restrict__ a,
+                         const double* __restrict__ alpha, const double* __restrict__ c,
+                         double* __restrict__ out) {
+  const int64_t tot = (int64_t)B * len;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / len);
+    out[e] = a[e] + alpha[b] * c[e];
+  }
+}
+
+// Per-row reductions over [B x len]: op 0 max|x|, 1 sum x, 2 sum x*y, 3 max|x - y|
+__global__ __launch_bounds__(256) void k_rowreduce(int64_t len, int op, const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  const double* xb = x + (int64_t)b * len;
+  const double* yb = y ? y + (int64_t)b * len : nullptr;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < len; i += 256) {
+    const double v = xb[i];
+    if (op == 0)
+      acc = isfinite(v) ? fmax(acc, fabs(v)) : INFINITY;
+    else if (op == 1)
+      acc += v;
+    else if (op == 2)
+      acc += v * yb[i];
+    else
+      acc = fmax(acc, fabs(v - yb[i]));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      red[threadIdx.x] = (op == 0 || op == 3) ? fmax(red[threadIdx.x], red[threadIdx.x + o])
+                                              : red[threadIdx.x] + red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = red[0];
+}
+
+// Adjoint contractions, one block per (b, k):
+//   tr[b][k]   = sum_e comps[k][e] * Sigma_b[slot_full[e]]     (= tr(Sigma_b Q_k))
+//   quad[b][k] = sum_e comps[k][e] * x_b[row_e] * z_b[col_e]   (= x_b^T Q_k z_b)
+__global__ __launch_bounds__(256) void k_contract(int K, int64_t nnz, const double* __restrict__ comps,
+                                                  const double* __restrict__ S, int64_t l_size,
+                                                  const int64_t* __restrict__ slot_full,
+                                                  const int32_t* __restrict__ qrow,
+                                                  const int32_t* __restrict__ qcol,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ z, int64_t n,
+                                                  double* __restrict__ tr, double* __restrict__ quad) {
+  __shared__ double r1[256], r2[256];
+  const int k = blockIdx.x, b = blockIdx.y;
+  const double* ck = comps + (int64_t)k * nnz;
+  const double* Sb = S ? S + (int64_t)b * l_size : nullptr;
+  const double* xb = x ? x + (int64_t)b * n : nullptr;
+  const double* zb = z ? z + (int64_t)b * n : nullptr;
+  double a1 = 0.0, a2 = 0.0;
+  for (int64_t e = threadIdx.x; e < nnz; e += 256) {
+    const double c = ck[e];
+    if (c == 0.0) continue;
+    if (Sb) {
+      const int64_t sl = slot_full[e];
+      if (sl >= 0) a1 += c * Sb[sl];
+    }
+    if (xb) a2 += c * xb[qrow[e]] * zb[qcol[e]];
+  }
+  r1[threadIdx.x] = a1;
+  r2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      r1[threadIdx.x] += r1[threadIdx.x + o];
+      r2[threadIdx.x] += r2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (tr) tr[b * K + k] = r1[0];
+    if (quad) quad[b * K + k] = r2[0];
+  }
+}
+
+// v[b][i] = (A Sigma_b A^T)_ii = sum over the row's entry pairs of A_ia A_ib Sigma_b[ja, jb]
+__global__ void k_vdiag(int B, int64_t nobs, const int64_t* __restrict__ vptr,
+                        const int64_t* __restrict__ vslot, const double* __restrict__ vcoef,
+                        const double* __restrict__ S, int64_t l_size, double* __restrict__ v) {
+  const int64_t tot = (int64_t)B * nobs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / nobs);
+    const int64_t i = e - (int64_t)b * nobs;
+    const double* Sb = S + (int64_t)b * l_size;
+    double s = 0.0;
+    for (int64_t t = vptr[i]; t < vptr[i + 1]; ++t) s += vcoef[t] * Sb[vslot[t]];
+    v[e] = s;
+  }
+}
+
+// Constraint projection pieces: r[b][k] = sum_j ca[k][j] x[b][j]; x[b][j] -= sum_k ca[k][j] lam[b][k]
+__global__ __launch_bounds__(256) void k_constr_dot(int kc, int64_t n, const double* __restrict__ ca,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ r) {
+  __shared__ double red[256];
+  const int k = blockIdx.x, b = blockIdx.y;
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) acc += ca[(int64_t)k * n + j] * x[(int64_t)b * n + j];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) r[b * kc + k] = red[0];
+}
+
+__global__ void k_constr_apply(int B, int kc, int64_t n, const double* __restrict__ ca,
+                               const double* __restrict__ lam, double* __restrict__ x) {
+  const int64_t tot = (int64_t)B * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / n);
+    const int64_t j = e - (int64_t)b * n;
+    double corr = 0.0;
+    for (int k = 0; k < kc; ++k) corr += ca[(int64_t)k * n + j] * lam[b * kc + k];
+    x[e] -= corr;
+  }
+}
+
+// out = a .* b (flat)
+__global__ void k_mul(int64_t tot, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = a[e] * b[e];
+}
+
+// x[b] <- keep[b] ? xt[b] : x[b]
+__global__ void k_blend(int B, int64_t len, const double* __restrict__ keep,
+                        const double* __restrict__ xt, double* __restrict__ x) {
+  const int64_t tot = (int64_t)B * len;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / len);
+    if (keep[b] != 0.0) x[e] = xt[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline int grid_for(int64_t tot) {
+  int64_t g = (tot + 255) / 256;
+  if (g > 65535) g = 65535;
+  return (int)(g < 1 ? 1 : g);
+}
+
+hipError_t lap_lik(int B, int64_t nobs, int fam, int link, const double* eta, const double* y,
+                   const double* pa, const double* pb, const double* lpar, double* logp, double* grad,
+                   double* w, double* dw, double* dlpar, int32_t* bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_lik, dim3(grid_for((int64_t)B * nobs)), dim3(256), 0, st, B, nobs, fam, link,
+                     eta, y, pa, pb, lpar, logp, grad, w, dw, dlpar, bad);
+  return hipGetLastError();
+}
+
+hipError_t lap_affine(int B, int K, int64_t nnz, const double* coefs, const double* comps,
+                      double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_affine, dim3(grid_for((int64_t)B * nnz)), dim3(256), 0, st, B, K, nnz, coefs,
+                     comps, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_add_diag(int B, int64_t n, int64_t nnz, const int64_t* diag_idx, double add,
+                        double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_diag, dim3(grid_for((int64_t)B * n)), dim3(256), 0, st, B, n, nnz,
+                     diag_idx, add, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* ptr, const int32_t* obs,
+                        const double* coef, const double* w, const double* base, double* out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_add_atwa, dim3(grid_for((int64_t)B * nnz)), dim3(256), 0, st, B, nnz, nobs,
+                     ptr, obs, coef, w, base, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_spmv(int B, int64_t nout, const int64_t* ptr, const int32_t* idx, const double* vals,
+                    int64_t vld, const double* x, int64_t ldx, double alpha, double beta, double* y,
+                    int64_t ldy, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_spmv, dim3(grid_for((int64_t)B * nout)), dim3(256), 0, st, B, nout,
+                     ptr, idx, vals, vld, x, ldx, alpha, beta, y, ldy);
+  return hipGetLastError();
+}
+
+hipError_t lap_axpy(int B, int64_t len, const double* a, const double* alpha, const double* c,
+                    double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_b, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, a, alpha,
+                     c, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_rowreduce(int B, int64_t len, int op, const double* x, const double* y, double* out,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_rowreduce, dim3(B), dim3(256), 0, st, len, op, x, y, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_contract(int B, int K, int64_t nnz, const double* comps, const double* S,
+                        int64_t l_size, const int64_t* slot_full, const int32_t* qrow,
+                        const int32_t* qcol, const double* x, const double* z, int64_t n, double* tr,
+                        double* quad, hipStream_t st) {
+  hipLaunchKernelGGL(k_contract, dim3(K, B), dim3(256), 0, st, K, nnz, comps, S, l_size, slot_full,
+                     qrow, qcol, x, z, n, tr, quad);
+  return hipGetLastError();
+}
+
+hipError_t lap_vdiag(int B, int64_t nobs, const int64_t* vptr, const int64_t* vslot,
+                     const double* vcoef, const double* S, int64_t l_size, double* v,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_vdiag, dim3(grid_for((int64_t)B * nobs)), dim3(256), 0, st, B, nobs, vptr,
+                     vslot, vcoef, S, l_size, v);
+  return hipGetLastError();
+}
+
+hipError_t lap_constr_dot(int B, int kc, int64_t n, const double* ca, const double* x, double* r,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_constr_dot, dim3(kc, B), dim3(256), 0, st, kc, n, ca, x, r);
+  return hipGetLastError();
+}
+
+hipError_t lap_constr_apply(int B, int kc, int64_t n, const double* ca, const double* lam, double* x,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_constr_apply, dim3(grid_for((int64_t)B * n)), dim3(256), 0, st, B, kc, n, ca,
+                     lam, x);
+  return hipGetLastError();
+}
+
+hipError_t lap_mul(int64_t tot, const double* a, const double* b, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(tot)), dim3(256), 0, st, tot, a, b, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_blend(int B, int64_t len, const double* keep, const double* xt, double* x,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_blend, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, keep, xt, x);
+  return hipGetLastError();
+}
+
+}  // namespace gmrf

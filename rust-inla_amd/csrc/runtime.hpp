@@ -1,0 +1,432 @@
+// Internal runtime structures shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gmrf_c.h"
+#include "plan.hpp"
+#include "symbolic.hpp"
+
+namespace gmrf {
+hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
+                               int64_t, double*, double*, hipStream_t);
+hipError_t launch_factor_large(const DevPlan&, const int32_t*, int, int, const double*, int64_t,
+                               double*, double*, hipStream_t);
+hipError_t launch_logdet(const DevPlan&, int, const double*, double*, int32_t*, hipStream_t);
+hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t);
+hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
+hipError_t launch_diag_scale(const DevPlan&, int, const double*, double*, hipStream_t);
+hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
+                            double*, hipStream_t);
+hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
+                            hipStream_t);
+hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
+                               double*, double*, hipStream_t);
+hipError_t launch_selinv_large(const DevPlan&, const int32_t*, int, int, const double*, double*,
+                               double*, double*, hipStream_t);
+hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
+hipError_t launch_pattern_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
+hipError_t launch_mma_probe(const double*, const double*, int, double*, hipStream_t);
+}  // namespace gmrf
+
+using namespace gmrf;
+
+namespace gmrf_rt {
+
+extern thread_local std::string g_err;
+extern int g_device;
+
+inline gmrf_status fail(gmrf_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const char* what)
+      : std::runtime_error(std::string(what) + ": " + hipGetErrorString(c)), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(e, what);
+}
+
+inline gmrf_status from_exception() {
+  try {
+    throw;
+  } catch (const HipError& e) {
+    if (e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation)
+      return fail(GMRF_OUT_OF_MEMORY, e.what());
+    return fail(GMRF_DEVICE_ERROR, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(GMRF_OUT_OF_MEMORY, "out of memory during sparse factorization");
+  } catch (const std::exception& e) {
+    return fail(GMRF_INVALID_ARGUMENT, e.what());
+  } catch (...) {
+    return fail(GMRF_DEVICE_ERROR, "unknown error");
+  }
+}
+
+// Optional per-launch timing: HIP events around each launch on the launching
+// stream, tagged with the launch's algorithmic flops and bytes.
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+struct Profiler {
+  bool on = false;
+  std::mutex mu;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void clear() {
+    for (auto& r : recs) {
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+};
+extern Profiler g_prof;
+
+// Run `launch` (returning hipError_t) with profiling when enabled.
+template <typename F>
+inline void prof_launch(const char* name, double flops, double bytes, hipStream_t st, F launch) {
+  if (!g_prof.on) {
+    check(launch(), name);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  ProfRec r{name, g_prof.get(), g_prof.get(), flops, bytes};
+  check(hipEventRecord(r.a, st), "hipEventRecord");
+  check(launch(), name);
+  check(hipEventRecord(r.b, st), "hipEventRecord");
+  g_prof.recs.push_back(r);
+}
+
+// Device allocation owned by a vector of raw pointers freed in the destructor.
+struct DevArena {
+  std::vector<void*> ptrs;
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v) {
+    T* d = alloc<T>(v.size());
+    if (!v.empty()) check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return d;
+  }
+};
+
+struct LevelInfo {
+  int64_t sb, se;     // small fronts [sb, se) in small list
+  int64_t lb, le;     // large fronts [lb, le) in large list
+  int64_t ab, ae;     // all fronts [ab, ae) in all list
+  int32_t maxm_small, maxm_all;
+  // algorithmic work of the level's launches (per matrix)
+  double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol = 0;
+  double by_small = 0, by_large = 0;
+};
+
+}  // namespace gmrf_rt
+
+using namespace gmrf_rt;
+
+// Symbolic analysis plus its device plan; shared by batches and solvers.
+struct SymData {
+  Symbolic S;
+  int device = 0;
+  DevArena arena;
+  DevPlan plan{};
+  std::vector<LevelInfo> levels;
+  int32_t* d_small = nullptr;
+  int32_t* d_large = nullptr;
+  int32_t* d_all = nullptr;
+  int order_req = 0;
+  std::vector<int64_t> given;  // given permutation (cache key)
+  std::mutex dev_mu;
+  bool dev_built = false;
+
+  // Upload the plan on first use (symbolic analysis itself needs no device).
+  void ensure_device() {
+    std::lock_guard<std::mutex> lk(dev_mu);
+    if (dev_built) return;
+    build_device();
+    dev_built = true;
+  }
+
+  void build_device() {
+    check(hipSetDevice(device), "hipSetDevice");
+    const int64_t ns = S.nsuper, n = S.n;
+    if (n >= (int64_t)1 << 31 || S.nnz_full >= (int64_t)1 << 31)
+      throw std::runtime_error("pattern too large for 32-bit device indices");
+    std::vector<int32_t> sn_start(ns + 1), sn_r(ns), sn_parent(ns), child_ptr(ns + 1),
+        child_idx(S.child_idx.size()), rows(S.sn_rows.size()), perm(n);
+    for (int64_t s = 0; s <= ns; ++s) sn_start[s] = (int32_t)S.sn_start[s];
+    for (int64_t s = 0; s < ns; ++s) {
+      sn_r[s] = (int32_t)(S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s]);
+      sn_parent[s] = (int32_t)S.sn_parent[s];
+    }
+    for (int64_t s = 0; s <= ns; ++s) child_ptr[s] = (int32_t)S.child_ptr[s];
+    for (size_t q = 0; q < S.child_idx.size(); ++q) child_idx[q] = (int32_t)S.child_idx[q];
+    for (size_t q = 0; q < S.sn_rows.size(); ++q) rows[q] = (int32_t)S.sn_rows[q];
+    for (int64_t k = 0; k < n; ++k) perm[k] = (int32_t)S.perm[k];
+    // input entries per supernode
+    std::vector<int64_t> col2s(n);
+    for (int64_t s = 0; s < ns; ++s)
+      for (int64_t j = S.sn_start[s]; j < S.sn_start[s + 1]; ++j) col2s[j] = s;
+    std::vector<int64_t> fa_ptr(ns + 1, 0);
+    std::vector<int64_t> esn(S.nnz_full, -1);
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t p = S.indptr[c]; p < S.indptr[c + 1]; ++p) {
+        if (S.amap[p] < 0) continue;
+        int64_t fj = std::min(S.iperm[S.indices[p]], S.iperm[c]);
+        esn[p] = col2s[fj];
+        fa_ptr[esn[p] + 1]++;
+      }
+    for (int64_t s = 0; s < ns; ++s) fa_ptr[s + 1] += fa_ptr[s];
+    std::vector<int32_t> fa_csc(fa_ptr[ns]), fa_slot(fa_ptr[ns]);
+    {
+      std::vector<int64_t> fill(fa_ptr.begin(), fa_ptr.end() - 1);
+      for (int64_t p = 0; p < S.nnz_full; ++p) {
+        if (esn[p] < 0) continue;
+        int64_t s = esn[p], q = fill[s]++;
+        fa_csc[q] = (int32_t)p;
+        int64_t local = S.amap[p] - S.l_off[s];
+        if (local < 0 || local >= ((int64_t)1 << 31)) throw std::runtime_error("front too large");
+        fa_slot[q] = (int32_t)local;
+      }
+    }
+    plan.n = (int32_t)n;
+    plan.nsuper = (int32_t)ns;
+    plan.nnz = S.nnz_full;
+    plan.l_size = S.l_size;
+    plan.u_size = S.u_size;
+    plan.v_size = S.v_size;
+    plan.sn_start = arena.upload(sn_start);
+    plan.sn_r = arena.upload(sn_r);
+    plan.sn_parent = arena.upload(sn_parent);
+    std::vector<int64_t> loff(S.l_off.begin(), S.l_off.begin() + ns);
+    std::vector<int64_t> uoff(S.u_off.begin(), S.u_off.begin() + ns);
+    std::vector<int64_t> voff(S.v_off.begin(), S.v_off.begin() + ns);
+    plan.l_off = arena.upload(loff);
+    plan.u_off = arena.upload(uoff);
+    plan.v_off = arena.upload(voff);
+    plan.rows_ptr = arena.upload(S.sn_rows_ptr);
+    plan.rows = arena.upload(rows);
+    plan.rel = arena.upload(S.rel);
+    plan.child_ptr = arena.upload(child_ptr);
+    plan.child_idx = arena.upload(child_idx);
+    plan.fa_ptr = arena.upload(fa_ptr);
+    plan.fa_csc = arena.upload(fa_csc);
+    plan.fa_slot = arena.upload(fa_slot);
+    plan.perm = arena.upload(perm);
+    plan.diag_slot = arena.upload(S.diag_slot);
+    plan.slot_full = arena.upload(S.slot_full);
+    // level classes
+    std::vector<int32_t> small, large, all;
+    levels.resize(S.nlevels);
+    for (int l = 0; l < S.nlevels; ++l) {
+      LevelInfo& L = levels[l];
+      L.sb = (int64_t)small.size();
+      L.lb = (int64_t)large.size();
+      L.ab = (int64_t)all.size();
+      L.maxm_small = 0;
+      L.maxm_all = 0;
+      for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+        int64_t s = S.level_sn[q];
+        int32_t m = (int32_t)((S.sn_start[s + 1] - S.sn_start[s]) + sn_r[s]);
+        all.push_back((int32_t)s);
+        L.maxm_all = std::max(L.maxm_all, m);
+        double fl = 0, sel = 0, sol = 0;
+        for (int64_t j = S.sn_start[s]; j < S.sn_start[s + 1]; ++j) {
+          const double rj = (double)(S.colcount[j] - 1);
+          fl += rj * (rj + 3.0);
+          sel += 2.0 * rj * (rj + 1.0) + 1.0;
+          sol += 4.0 * rj + 1.0;
+        }
+        const int64_t w = S.sn_start[s + 1] - S.sn_start[s];
+        const double by = 8.0 * (double)m * (double)w + 12.0 * (double)(fa_ptr[s + 1] - fa_ptr[s]);
+        L.sol += sol;
+        if (m <= kSmallFront) {
+          small.push_back((int32_t)s);
+          L.maxm_small = std::max(L.maxm_small, m);
+          L.fl_small += fl;
+          L.sel_small += sel;
+          L.by_small += by;
+        } else {
+          large.push_back((int32_t)s);
+          L.fl_large += fl;
+          L.sel_large += sel;
+          L.by_large += by;
+        }
+      }
+      L.se = (int64_t)small.size();
+      L.le = (int64_t)large.size();
+      L.ae = (int64_t)all.size();
+    }
+    d_small = arena.upload(small);
+    d_large = arena.upload(large);
+    d_all = arena.upload(all);
+  }
+};
+
+struct gmrf_symbolic {
+  std::shared_ptr<SymData> d;
+};
+
+
+// Numeric workspace for up to `cap` matrices of one pattern.
+struct gmrf_batch {
+  std::shared_ptr<SymData> sym;
+  int cap = 0;
+  hipStream_t stream = nullptr;  // stream in use (own or caller's)
+  hipStream_t own = nullptr;
+  DevArena arena;
+  double *L = nullptr, *U = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *V = nullptr;
+  double* d_logdet = nullptr;
+  int32_t* d_status = nullptr;
+
+  gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
+    check(hipSetDevice(sym->device), "hipSetDevice");
+    sym->ensure_device();
+    check(hipStreamCreateWithFlags(&own, hipStreamNonBlocking), "hipStreamCreate");
+    stream = own;
+    const Symbolic& Sy = sym->S;
+    L = arena.alloc<double>((size_t)cap * Sy.l_size);
+    U = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.u_size, 1));
+    S = arena.alloc<double>((size_t)cap * Sy.l_size);
+    T = arena.alloc<double>((size_t)cap * Sy.l_size);
+    X = arena.alloc<double>((size_t)cap * Sy.n);
+    V = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.v_size, 1));
+    d_logdet = arena.alloc<double>(cap);
+    d_status = arena.alloc<int32_t>(cap);
+  }
+  ~gmrf_batch() {
+    if (own) {
+      (void)hipStreamSynchronize(own);
+      (void)hipStreamDestroy(own);
+    }
+  }
+
+  // Matrices [first, first + count) of the batch.
+  void factorize(int count, const double* vals, int64_t ldv, int first = 0) {
+    const DevPlan& P = sym->plan;
+    double* Lf = L + (int64_t)first * P.l_size;
+    double* Uf = U + (int64_t)first * P.u_size;
+    for (const LevelInfo& lv : sym->levels) {
+      if (lv.se > lv.sb)
+        prof_launch("k_factor_small", lv.fl_small * count, lv.by_small * count, stream, [&] {
+          return launch_factor_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
+                                     count, vals, ldv, Lf, Uf, stream);
+        });
+      if (lv.le > lv.lb)
+        prof_launch("k_factor_large", lv.fl_large * count, lv.by_large * count, stream, [&] {
+          return launch_factor_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, vals, ldv,
+                                     Lf, Uf, stream);
+        });
+    }
+    check(launch_logdet(P, count, Lf, d_logdet + first, d_status + first, stream), "logdet");
+  }
+
+  void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0) {
+    const DevPlan& P = sym->plan;
+    const double* Lf = L + (int64_t)first * P.l_size;
+    double* Xf = X + (int64_t)first * P.n;
+    double* Vf = V + (int64_t)first * std::max<int64_t>(P.v_size, 1);
+    check(launch_gather_perm(P, count, rhs, ldr, Xf, stream), "gather");
+    for (const LevelInfo& lv : sym->levels)
+      prof_launch("k_solve_fwd", 0.5 * lv.sol * count, 0.0, stream, [&] {
+        return launch_solve_fwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, Lf,
+                                Xf, Vf, stream);
+      });
+    check(launch_diag_scale(P, count, Lf, Xf, stream), "diag_scale");
+    for (size_t l = sym->levels.size(); l-- > 0;) {
+      const LevelInfo& lv = sym->levels[l];
+      prof_launch("k_solve_bwd", 0.5 * lv.sol * count, 0.0, stream, [&] {
+        return launch_solve_bwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, Lf,
+                                Xf, stream);
+      });
+    }
+    check(launch_scatter_perm(P, count, Xf, x, ldx, stream), "scatter");
+  }
+
+  void selinv(int count, int first = 0) {
+    const DevPlan& P = sym->plan;
+    const double* Lf = L + (int64_t)first * P.l_size;
+    double* Sf = S + (int64_t)first * P.l_size;
+    double* Uf = U + (int64_t)first * P.u_size;
+    double* Tf = T + (int64_t)first * P.l_size;
+    for (size_t l = sym->levels.size(); l-- > 0;) {
+      const LevelInfo& lv = sym->levels[l];
+      if (lv.le > lv.lb)
+        prof_launch("k_selinv_large", lv.sel_large * count, lv.by_large * count, stream, [&] {
+          return launch_selinv_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, Lf, Sf, Uf,
+                                     Tf, stream);
+        });
+      if (lv.se > lv.sb)
+        prof_launch("k_selinv_small", lv.sel_small * count, lv.by_small * count, stream, [&] {
+          return launch_selinv_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
+                                     count, Lf, Sf, Uf, stream);
+        });
+    }
+  }
+};
+
+// InlaSolver mirror: one matrix, host pointers, double-buffered so that a failed
+// factorize leaves the previous factor usable (solver.rs:70-74 keeps it).
+struct gmrf_solver {
+  struct Slot {
+    std::unique_ptr<gmrf_batch> bt;
+    double logdet = 0.0;
+    bool have_selinv = false;
+  };
+  Slot slot[2];
+  int cur = -1;
+  double* d_vals = nullptr;
+  size_t d_vals_cap = 0;
+  double* d_vec = nullptr;
+  size_t d_vec_cap = 0;
+  ~gmrf_solver() {
+    if (d_vals) (void)hipFree(d_vals);
+    if (d_vec) (void)hipFree(d_vec);
+  }
+  void ensure(double*& p, size_t& cap, size_t need) {
+    if (cap >= need) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    check(hipMalloc(&p, need * sizeof(double)), "hipMalloc");
+    cap = need;
+  }
+};
+
+
+std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
+                                        int order, const int64_t* given);

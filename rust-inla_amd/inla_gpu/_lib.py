@@ -53,6 +53,49 @@ class SymbolicStats(C.Structure):
     ]
 
 
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("indptr", _pi64),
+        ("indices", _pi64),
+        ("n_comp", C.c_int32),
+        ("comps", _pdbl),
+        ("n_obs", C.c_int64),
+        ("a_indptr", _pi64),
+        ("a_indices", _pi64),
+        ("a_values", _pdbl),
+        ("family", C.c_int32),
+        ("link", C.c_int32),
+        ("y", _pdbl),
+        ("obs_a", _pdbl),
+        ("obs_b", _pdbl),
+        ("n_constr", C.c_int32),
+        ("constr_a", _pdbl),
+        ("constr_e", _pdbl),
+        ("capacity", C.c_int32),
+    ]
+
+
+class EvalArgs(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32),
+        ("n_theta", C.c_int32),
+        ("coefs", C.c_void_p),
+        ("lpar", C.c_void_p),
+        ("dcoefs", C.c_void_p),
+        ("dlog_lpar", C.c_void_p),
+        ("max_iter", C.c_int32),
+        ("tol", C.c_double),
+        ("mlik", C.c_void_p),
+        ("x", C.c_void_p),
+        ("var", C.c_void_p),
+        ("grad", C.c_void_p),
+        ("status", C.POINTER(C.c_int32)),
+        ("n_iter", C.POINTER(C.c_int32)),
+    ]
+
+
+
 # name -> (restype, argtypes); every symbol declared in include/gmrf_c.h
 SIGNATURES = {
     "gmrf_last_error": (C.c_char_p, []),
@@ -85,6 +128,13 @@ SIGNATURES = {
     "gmrf_solver_log_abs_det": (_i32, [_vp, _pdbl]),
     "gmrf_solver_n": (_i64, [_vp]),
     "gmrf_probe_mma": (_i32, [_pdbl, _pdbl, _i32, _pdbl]),
+    "gmrf_model_create": (_i32, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
+    "gmrf_model_destroy": (None, [_vp]),
+    "gmrf_model_eval": (_i32, [_vp, C.POINTER(EvalArgs)]),
+    "gmrf_model_stream": (_vp, [_vp]),
+    "gmrf_model_set_stream": (_i32, [_vp, _vp]),
+    "gmrf_profile_enable": (_i32, [_i32]),
+    "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _pi32]),
 }
 
 
@@ -107,6 +157,27 @@ def _load():
 
 
 lib = _load()
+
+
+def profile_enable(on: bool) -> None:
+    if lib.gmrf_profile_enable(1 if on else 0) != GMRF_OK:
+        raise RuntimeError(last_error())
+
+
+def profile_summary() -> dict:
+    """{kernel: dict(launches, ms, flops, bytes)} of the recorded launches."""
+    cap, nl = 64, 64
+    names = C.create_string_buffer(cap * nl)
+    la = (C.c_int64 * cap)()
+    ms, fl, by = (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)()
+    ne = C.c_int32(0)
+    if lib.gmrf_profile_summary(cap, names, nl, la, ms, fl, by, C.byref(ne)) != GMRF_OK:
+        raise RuntimeError(last_error())
+    out = {}
+    for k in range(min(ne.value, cap)):
+        nm = names.raw[k * nl:(k + 1) * nl].split(b"\0", 1)[0].decode()
+        out[nm] = {"launches": int(la[k]), "ms": ms[k], "flops": fl[k], "bytes": by[k]}
+    return out
 
 
 def last_error() -> str:

@@ -1,0 +1,191 @@
+"""Batched Laplace evaluation over theta-points (gmrf_model_* in include/gmrf_c.h).
+
+``LaplaceModel`` holds one latent Gaussian model on the device: an affine
+precision Q(theta) = sum_k c_k(theta) Q_k, a projector A, observations and a
+likelihood family. ``LaplaceModel.eval`` runs, for a batch of theta-points,
+the per-point work of crates/inla_stats/src/inference.rs:776-975 (prior
+factorisation, Newton loop, Laplace marginal likelihood), the marginal
+variances (inference.rs:1371) and, when coefficient derivatives are given, the
+gradient of the marginal likelihood from the selected inverses.
+
+``LogMarginal`` exposes that gradient to torch.autograd.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib as L
+from ._lib import EvalArgs, ModelDesc
+from .solver import InvalidArgument, as_csc, check
+
+FAMILY = {"gaussian": 0, "poisson": 1, "binomial": 2, "nbinomial": 3}
+LINK = {"identity": 0, "log": 1, "logit": 2}
+
+_p64 = C.POINTER(C.c_int64)
+_pd = C.POINTER(C.c_double)
+
+
+def _arr64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _arrd(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dev(t, shape=None):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(np.asarray(t, dtype=np.float64))
+    t = t.to(device="cuda", dtype=torch.float64).contiguous()
+    if shape is not None:
+        t = t.reshape(shape)
+    return t
+
+
+class LaplaceModel:
+    """Latent Gaussian model with affine Q(theta) = sum_k c_k Q_k, resident on the GPU."""
+
+    def __init__(self, n, indptr, indices, components, y, family="gaussian", link=None, A=None,
+                 obs_a=None, obs_b=None, constraints=None, capacity=16):
+        link = link or {"gaussian": "identity", "poisson": "log", "binomial": "logit",
+                        "nbinomial": "log"}[family]
+        self.n = int(n)
+        self.indptr = _arr64(indptr)
+        self.indices = _arr64(indices)
+        comps = _arrd(components)
+        if comps.ndim != 2 or comps.shape[1] != len(self.indices):
+            raise InvalidArgument("components must be K x nnz")
+        self.K = comps.shape[0]
+        self.comps = comps
+        self.y = _arrd(y)
+        self.n_obs = len(self.y)
+        self.family, self.link = family, link
+        self.capacity = int(capacity)
+        keep = [self.indptr, self.indices, comps, self.y]
+        d = ModelDesc()
+        d.n = self.n
+        d.indptr = self.indptr.ctypes.data_as(_p64)
+        d.indices = self.indices.ctypes.data_as(_p64)
+        d.n_comp = self.K
+        d.comps = comps.ctypes.data_as(_pd)
+        d.n_obs = self.n_obs
+        if A is not None:
+            A = as_csc(A)
+            if A.shape != (self.n_obs, self.n):
+                raise InvalidArgument("A must be n_obs x n")
+            self.A = A
+            ap, ai, ax = _arr64(A.indptr), _arr64(A.indices), _arrd(A.data)
+            keep += [ap, ai, ax]
+            d.a_indptr, d.a_indices, d.a_values = ap.ctypes.data_as(_p64), ai.ctypes.data_as(_p64), ax.ctypes.data_as(_pd)
+        else:
+            self.A = None
+        d.family = FAMILY[family]
+        d.link = LINK[link]
+        d.y = self.y.ctypes.data_as(_pd)
+        if obs_a is not None:
+            oa = _arrd(np.broadcast_to(obs_a, (self.n_obs,)))
+            keep.append(oa)
+            d.obs_a = oa.ctypes.data_as(_pd)
+        if obs_b is not None:
+            ob = _arrd(np.broadcast_to(obs_b, (self.n_obs,)))
+            keep.append(ob)
+            d.obs_b = ob.ctypes.data_as(_pd)
+        if constraints is not None:
+            ca, ce = constraints
+            ca = _arrd(ca).reshape(-1, self.n)
+            ce = _arrd(ce)
+            keep += [ca, ce]
+            d.n_constr = ca.shape[0]
+            d.constr_a = ca.ctypes.data_as(_pd)
+            d.constr_e = ce.ctypes.data_as(_pd)
+        d.capacity = self.capacity
+        h = C.c_void_p()
+        check(L.lib.gmrf_model_create(C.byref(d), C.byref(h)))
+        self._h = h
+        del keep
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.lib.gmrf_model_destroy(h)
+            self._h = None
+
+    def eval(self, coefs, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
+             max_iter=200, tol=1e-5):
+        """Evaluate a batch of theta-points.
+
+        coefs: (B, K); lpar: (B,) Gaussian precision multipliers; dcoefs: (B, m, K);
+        dlog_lpar: (B, m). Returns a dict of device tensors (mlik, x, var, grad) plus
+        numpy status / n_iter.
+        """
+        c = _dev(coefs)
+        if c.dim() == 1:
+            c = c.unsqueeze(0)
+        B = c.shape[0]
+        if c.shape[1] != self.K:
+            raise InvalidArgument("coefs must be B x K")
+        if B > self.capacity:
+            raise InvalidArgument("batch exceeds model capacity")
+        lp = _dev(lpar, (B,)) if lpar is not None else None
+        dc = _dev(dcoefs) if dcoefs is not None else None
+        m = 0
+        if dc is not None:
+            dc = dc.reshape(B, -1, self.K)
+            m = dc.shape[1]
+        dl = _dev(dlog_lpar, (B, m)) if dlog_lpar is not None else None
+        out = {"mlik": torch.empty(B, dtype=torch.float64, device="cuda")}
+        if want_x:
+            out["x"] = torch.empty((B, self.n), dtype=torch.float64, device="cuda")
+        if want_var:
+            out["var"] = torch.empty((B, self.n), dtype=torch.float64, device="cuda")
+        if dc is not None:
+            out["grad"] = torch.empty((B, m), dtype=torch.float64, device="cuda")
+        status = np.zeros(B, dtype=np.int32)
+        niter = np.zeros(B, dtype=np.int32)
+        a = EvalArgs()
+        a.count, a.n_theta = B, m
+        a.coefs = c.data_ptr()
+        a.lpar = lp.data_ptr() if lp is not None else None
+        a.dcoefs = dc.data_ptr() if dc is not None else None
+        a.dlog_lpar = dl.data_ptr() if dl is not None else None
+        a.max_iter, a.tol = int(max_iter), float(tol)
+        a.mlik = out["mlik"].data_ptr()
+        a.x = out["x"].data_ptr() if want_x else None
+        a.var = out["var"].data_ptr() if want_var else None
+        a.grad = out["grad"].data_ptr() if dc is not None else None
+        a.status = status.ctypes.data_as(C.POINTER(C.c_int32))
+        a.n_iter = niter.ctypes.data_as(C.POINTER(C.c_int32))
+        check(L.lib.gmrf_model_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        check(L.lib.gmrf_model_eval(self._h, C.byref(a)))
+        out["status"] = status
+        out["n_iter"] = niter
+        return out
+
+
+class LogMarginal(torch.autograd.Function):
+    """mlik(theta) for a batch of theta-points with the adjoint gradient.
+
+    ``param_fn(theta_np) -> (coefs (B,K), dcoefs (B,m,K), lpar (B,), dlog_lpar (B,m))``
+    maps theta to the affine coefficients of Q(theta) and the likelihood
+    precision.
+    """
+
+    @staticmethod
+    def forward(ctx, theta, model, param_fn):
+        th = theta.detach().cpu().numpy().reshape(-1, theta.shape[-1])
+        coefs, dcoefs, lpar, dlog = param_fn(th)
+        r = model.eval(coefs, lpar=lpar, dcoefs=dcoefs, dlog_lpar=dlog, want_x=False, want_var=False)
+        ctx.save_for_backward(r["grad"].to(theta.device))
+        return r["mlik"].to(theta.device).reshape(theta.shape[:-1])
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout.reshape(-1, 1) * grad, None, None

@@ -239,3 +239,85 @@ def sample_gmrf(Q: sp.csc_matrix, rng: np.random.Generator) -> np.ndarray:
     x = np.empty(n)
     x[lu.perm_c] = y
     return x
+
+
+# --------------------------------------------------------------------------
+# Priors used by the benchmark configurations (crates/inla_stats/src/priors.rs)
+# --------------------------------------------------------------------------
+def log_gamma_fn(z):
+    return math.lgamma(z)
+
+
+def loggamma_prior(theta, shape=1.0, rate=5e-5):
+    """log pi(theta) and d/dtheta for Gamma(shape, rate) on exp(theta) (priors.rs:836-855)."""
+    e = math.exp(theta)
+    return shape * math.log(rate) - math.lgamma(shape) + shape * theta - rate * e, shape - rate * e
+
+
+def pc_spde_prior(log_tau, log_kappa, range0=0.1, alpha_r=0.5, sigma0=1.0, alpha_s=0.01, d=2.0):
+    """PC prior of the 2-D SPDE in (log tau, log kappa) (priors.rs:809-834) and its gradient."""
+    lam1 = -math.log(alpha_r) * range0 ** (d / 2)
+    lam2 = -math.log(alpha_s) / sigma0
+    log_rho = 0.5 * math.log(8.0) - log_kappa
+    log_sigma = -0.5 * math.log(4 * math.pi) - log_tau - log_kappa
+    dh = d / 2
+    e1 = math.exp(-dh * log_rho)
+    s_rho = math.log(lam1 * dh) - dh * log_rho - lam1 * e1
+    ds_rho = -dh + lam1 * dh * e1              # d/dlog_rho
+    es = math.exp(log_sigma)
+    s_sig = math.log(lam2) + log_sigma - lam2 * es
+    ds_sig = 1.0 - lam2 * es                   # d/dlog_sigma
+    # dlog_rho/dlog_tau = 0, dlog_rho/dlog_kappa = -1; dlog_sigma/d(tau, kappa) = (-1, -1)
+    return s_rho + s_sig, np.array([-ds_sig, -ds_rho - ds_sig])
+
+
+@dataclass
+class SpdeGaussianWorkload:
+    """SPDE alpha=2 on an nx x nx unit-square lattice, Gaussian observations.
+
+    Internal theta = (log prec_eps, log tau, log kappa).
+    """
+
+    gmrf: AffineGmrf
+    A: sp.csc_matrix
+    y: np.ndarray
+    theta_true: np.ndarray
+    nx: int
+
+    @property
+    def n(self):
+        return self.gmrf.n
+
+    def params(self, thetas):
+        """(coefs (B,3), dcoefs (B,3,3), lpar (B,), dlog_lpar (B,3)) for theta rows."""
+        thetas = np.atleast_2d(thetas)
+        B = len(thetas)
+        coefs = np.zeros((B, 3))
+        dco = np.zeros((B, 3, 3))
+        lpar = np.exp(thetas[:, 0])
+        dl = np.zeros((B, 3))
+        dl[:, 0] = 1.0
+        for b, th in enumerate(thetas):
+            c, dc = spde_coefs(th[1], th[2])
+            coefs[b] = c
+            dco[b, 1:] = dc
+        return coefs, dco, lpar, dl
+
+    def log_prior(self, theta):
+        lp0, g0 = loggamma_prior(theta[0])
+        lp1, g12 = pc_spde_prior(theta[1], theta[2])
+        return lp0 + lp1, np.r_[g0, g12]
+
+
+def spde_gaussian_workload(nx=142, n_obs=50_000, seed=20241203, range_=0.3, sigma=1.0, sigma_eps=0.1):
+    """The n ~ 20k SPDE Gaussian configuration: true range 0.3, sigma 1, sigma_eps 0.1."""
+    g, _, _ = spde_components(nx, nx)
+    rng = np.random.default_rng(seed)
+    kappa = math.sqrt(8.0) / range_
+    tau = 1.0 / (math.sqrt(4 * math.pi) * kappa * sigma)
+    c, _ = spde_coefs(math.log(tau), math.log(kappa))
+    x = sample_gmrf(g.csc(c), rng)
+    A = lattice_projector(nx, nx, rng.uniform(size=(n_obs, 2)))
+    y = A @ x + sigma_eps * rng.standard_normal(n_obs)
+    theta = np.array([math.log(1.0 / sigma_eps ** 2), math.log(tau), math.log(kappa)])
+    return SpdeGaussianWorkload(g, A, y, theta, nx)

@@ -1,0 +1,167 @@
+"""Batched Laplace evaluation on the GPU vs the oracle's Newton loop at fixed theta.
+
+Bars: Newton mode 1e-10 relative (max norm), marginal log-likelihood 1e-8
+absolute, marginal variances 1e-9 relative; the adjoint gradient against central
+finite differences (step 1e-4) of the oracle's marginal likelihood at 1e-5
+relative (the FD truncation error dominates).
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import matrices as M
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+MLIK_ATOL = 1e-8
+MODE_RTOL = 1e-10
+VAR_RTOL = 1e-9
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def spde_setup(nx, nobs, seed, family):
+    from inla_gpu import workloads as W
+
+    g, _, _ = W.spde_components(nx, nx)
+    rng = np.random.default_rng(seed)
+    A = W.lattice_projector(nx, nx, rng.uniform(size=(nobs, 2)))
+    c, _ = W.spde_coefs(-0.5, 1.2)
+    x = W.sample_gmrf(g.csc(c) + sp.identity(g.n) * 1e-9, rng)
+    eta = A @ x
+    if family == "gaussian":
+        y = eta + 0.3 * rng.standard_normal(nobs)
+        pa = np.ones(nobs)
+    elif family == "poisson":
+        y = rng.poisson(np.exp(0.5 + eta)).astype(float)
+        pa = np.ones(nobs)
+    elif family == "binomial":
+        y = rng.binomial(10, 1 / (1 + np.exp(-eta))).astype(float)
+        pa = np.full(nobs, 10.0)
+    else:
+        y = rng.negative_binomial(5, 5 / (5 + np.exp(0.5 + eta))).astype(float)
+        pa = np.ones(nobs)
+    return g, A, y, pa
+
+
+def spde_params(theta_batch, gaussian):
+    """theta = (log prec, log tau, log kappa) for Gaussian, else (log tau, log kappa)."""
+    from inla_gpu import workloads as W
+
+    coefs, dco, lp, dl = [], [], [], []
+    for th in np.atleast_2d(theta_batch):
+        off = 1 if gaussian else 0
+        c, dc = W.spde_coefs(th[off], th[off + 1])
+        d = np.zeros((len(th), 3))
+        d[off:off + 2] = dc
+        coefs.append(c)
+        dco.append(d)
+        lp.append(np.exp(th[0]) if gaussian else 1.0)
+        e = np.zeros(len(th))
+        if gaussian:
+            e[0] = 1.0
+        dl.append(e)
+    return np.array(coefs), np.array(dco), np.array(lp), np.array(dl)
+
+
+def oracle_mlik(g, A, y, pa, family, th, gaussian, pb=None, constraints=None, keep_post=False):
+    c, _, lp, _ = spde_params(th, gaussian)
+    q = g.csc(c[0])
+    return orc.find_mode(q, family, {"gaussian": "identity", "poisson": "log", "binomial": "logit",
+                                     "nbinomial": "log"}[family], y, pa * (lp[0] if gaussian else 1.0),
+                         pb=pb, A=A, constraints=constraints, keep_post=keep_post)
+
+
+@pytest.mark.parametrize("family", ["gaussian", "poisson", "binomial", "nbinomial"])
+def test_laplace_matches_oracle(family):
+    from inla_gpu.laplace import LaplaceModel
+
+    gaussian = family == "gaussian"
+    g, A, y, pa = spde_setup(18, 700, 7, family)
+    pb = np.full(len(y), 5.0) if family == "nbinomial" else None
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, y, family=family, A=A, obs_a=pa,
+                         obs_b=pb, capacity=4)
+    thetas = np.array([[2.0, -0.5, 1.2], [2.5, -0.3, 1.0], [1.5, -0.8, 1.4]]) if gaussian else \
+        np.array([[-0.5, 1.2], [-0.3, 1.0], [-0.8, 1.4]])
+    coefs, dco, lp, dl = spde_params(thetas, gaussian)
+    r = model.eval(coefs, lpar=lp, dcoefs=dco, dlog_lpar=dl)
+    assert (r["status"] == 0).all(), r["status"]
+    for b, th in enumerate(thetas):
+        ref = oracle_mlik(g, A, y, pa, family, th, gaussian, pb=pb, keep_post=True)
+        assert abs(r["mlik"][b].item() - ref.mlik) < MLIK_ATOL, (b, r["mlik"][b].item(), ref.mlik)
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < MODE_RTOL
+        post = ref.post_factor()
+        assert _rel(r["var"][b].cpu().numpy(), post.takahashi_diag()) < VAR_RTOL
+        if not gaussian:
+            assert r["n_iter"][b] >= 2
+
+
+@pytest.mark.parametrize("family", ["gaussian", "poisson", "binomial"])
+def test_gradient_matches_central_fd(family):
+    from inla_gpu.laplace import LaplaceModel
+
+    gaussian = family == "gaussian"
+    g, A, y, pa = spde_setup(14, 400, 3, family)
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, y, family=family, A=A, obs_a=pa,
+                         capacity=2)
+    th = np.array([2.0, -0.5, 1.2]) if gaussian else np.array([-0.5, 1.2])
+    coefs, dco, lp, dl = spde_params(th, gaussian)
+    r = model.eval(coefs, lpar=lp, dcoefs=dco, dlog_lpar=dl, tol=1e-12)
+    grad = r["grad"][0].cpu().numpy()
+    h = 1e-4
+    fd = np.zeros_like(th)
+    for l in range(len(th)):
+        e = np.zeros_like(th)
+        e[l] = h
+        fp = oracle_mlik(g, A, y, pa, family, th + e, gaussian)
+        fm = oracle_mlik(g, A, y, pa, family, th - e, gaussian)
+        fd[l] = (fp.mlik - fm.mlik) / (2 * h)
+    assert np.max(np.abs(grad - fd) / np.maximum(np.abs(fd), 1.0)) < 1e-5, (grad, fd)
+
+
+def test_gaussian_mlik_closed_form():
+    # Gaussian likelihood: the Laplace mlik is exact, log N(y; 0, A Q^-1 A^T + I/prec)
+    from inla_gpu.laplace import LaplaceModel
+
+    g, A, y, pa = spde_setup(8, 90, 11, "gaussian")
+    th = np.array([1.0, -0.2, 1.1])
+    coefs, _, lp, _ = spde_params(th, True)
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, y, A=A, obs_a=pa, capacity=1)
+    r = model.eval(coefs, lpar=lp)
+    Q = g.csc(coefs[0]).toarray() + 1e-12 * np.eye(g.n)
+    Ad = A.toarray()
+    S = Ad @ np.linalg.solve(Q, Ad.T) + np.eye(len(y)) / lp[0]
+    sign, logdet = np.linalg.slogdet(S)
+    exact = -0.5 * (len(y) * np.log(2 * np.pi) + logdet + y @ np.linalg.solve(S, y))
+    assert abs(r["mlik"][0].item() - exact) < 1e-7
+
+
+def test_constrained_besag_matches_oracle():
+    # Besag ICAR + sum-to-zero, kappa A_c^T A_c augmentation (constraints.rs:275-313)
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    nr = 7
+    adj = W.rook_adjacency(nr, nr)
+    R = W.besag_structure(adj)
+    n = R.shape[0]
+    kappa = 3.059023205018258e6
+    ca = np.full((1, n), 1.0 / np.sqrt(n))
+    Kc = sp.csc_matrix(kappa * ca.T @ ca)
+    indptr, indices = W.union_pattern([R, Kc])
+    comps = np.stack([W.values_on_pattern(R, indptr, indices), W.values_on_pattern(Kc, indptr, indices)])
+    rng = np.random.default_rng(4)
+    y = rng.poisson(np.exp(0.5 + 0.3 * rng.standard_normal(n))).astype(float)
+    model = LaplaceModel(n, indptr, indices, comps, y, family="poisson", constraints=(ca, [0.0]),
+                         capacity=2)
+    taus = [1.0, 3.0]
+    r = model.eval(np.array([[t, 1.0] for t in taus]))
+    assert (r["status"] == 0).all()
+    for b, tau in enumerate(taus):
+        ref = orc.find_mode(R * tau, "poisson", "log", y, np.ones(n), constraints=(ca, [0.0]))
+        assert abs(r["mlik"][b].item() - ref.mlik) < 1e-7, (r["mlik"][b].item(), ref.mlik)
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
