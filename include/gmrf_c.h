@@ -132,10 +132,10 @@ gmrf_status gmrf_batch_selinv_pattern(gmrf_batch* bt, int32_t count, double* sig
 gmrf_status gmrf_batch_sync(gmrf_batch* bt);
 /* The batch's HIP stream (hipStream_t) for callers that enqueue around it. */
 void* gmrf_batch_stream(gmrf_batch* bt);
-/* Enqueue subsequent batch work on the caller's stream (hipStream_t, not owned;
- * NULL restores the batch's own stream). Keeps ordering with the caller's work
- * without host synchronisation. */
-gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream);
+/* use_given != 0: enqueue subsequent batch work on `stream` (hipStream_t, not
+ * owned; NULL is the default stream), keeping order with the caller's work
+ * without host synchronisation. use_given == 0: back to the batch's own stream. */
+gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream, int32_t use_given);
 
 /* ---- InlaSolver mirror (host pointers) --------------------------------------
  * gmrf_solver_factorize = InlaSolver::factorize (solver.rs:70-74): symbolic
@@ -217,7 +217,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* desc, gmrf_model** out);
 void gmrf_model_destroy(gmrf_model* model);
 gmrf_status gmrf_model_eval(gmrf_model* model, gmrf_eval_args* args);
 void* gmrf_model_stream(gmrf_model* model);
-gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream);
+gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_given);
 
 /* ---- diagnostics -------------------------------------------------------------
  * Per-launch timing of the level kernels with HIP events on the launching

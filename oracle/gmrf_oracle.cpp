@@ -532,10 +532,10 @@ int orc_takahashi_diag(void* hp, double* out) {
 // positions inside the filled pattern; returns 2 if one is outside.
 int orc_selinv_entries(void* hp, i64 nq, const i64* rows, const i64* cols, double* out) {
   Handle* h = (Handle*)hp;
-  double tmp;
   if (!h->have_s) {
-    int st = orc_takahashi_diag(hp, &tmp) ;
-    (void)st;
+    std::vector<double> tmp(h->F.n);
+    int st = orc_takahashi_diag(hp, tmp.data());
+    if (st) return st;
   }
   for (i64 q = 0; q < nq; ++q) {
     i64 a = h->F.iperm[rows[q]], b = h->F.iperm[cols[q]];

@@ -281,9 +281,9 @@ gmrf_status gmrf_batch_sync(gmrf_batch* bt) {
 
 void* gmrf_batch_stream(gmrf_batch* bt) { return bt ? (void*)bt->stream : nullptr; }
 
-gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream) {
+gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream, int32_t use_given) {
   if (!bt) return fail(GMRF_INVALID_ARGUMENT, "null argument");
-  bt->stream = stream ? (hipStream_t)stream : bt->own;
+  bt->stream = use_given ? (hipStream_t)stream : bt->own;
   return GMRF_OK;
 }
 

@@ -340,9 +340,9 @@ void gmrf_model_destroy(gmrf_model* M) { delete M; }
 
 void* gmrf_model_stream(gmrf_model* M) { return M ? (void*)M->bt->stream : nullptr; }
 
-gmrf_status gmrf_model_set_stream(gmrf_model* M, void* stream) {
+gmrf_status gmrf_model_set_stream(gmrf_model* M, void* stream, int32_t use_given) {
   if (!M) return fail(GMRF_INVALID_ARGUMENT, "null argument");
-  return gmrf_batch_set_stream(M->bt.get(), stream);
+  return gmrf_batch_set_stream(M->bt.get(), stream, use_given);
 }
 
 gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {

@@ -119,7 +119,7 @@ SIGNATURES = {
     "gmrf_batch_selinv_pattern": (_i32, [_vp, _i32, _vp, _i64]),
     "gmrf_batch_sync": (_i32, [_vp]),
     "gmrf_batch_stream": (_vp, [_vp]),
-    "gmrf_batch_set_stream": (_i32, [_vp, _vp]),
+    "gmrf_batch_set_stream": (_i32, [_vp, _vp, _i32]),
     "gmrf_solver_create": (_i32, [C.POINTER(_vp)]),
     "gmrf_solver_destroy": (None, [_vp]),
     "gmrf_solver_factorize": (_i32, [_vp, _i64, _pi64, _pi64, _pdbl]),
@@ -132,7 +132,7 @@ SIGNATURES = {
     "gmrf_model_destroy": (None, [_vp]),
     "gmrf_model_eval": (_i32, [_vp, C.POINTER(EvalArgs)]),
     "gmrf_model_stream": (_vp, [_vp]),
-    "gmrf_model_set_stream": (_i32, [_vp, _vp]),
+    "gmrf_model_set_stream": (_i32, [_vp, _vp, _i32]),
     "gmrf_profile_enable": (_i32, [_i32]),
     "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _pi32]),
 }

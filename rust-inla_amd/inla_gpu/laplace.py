@@ -162,7 +162,7 @@ class LaplaceModel:
         a.grad = out["grad"].data_ptr() if dc is not None else None
         a.status = status.ctypes.data_as(C.POINTER(C.c_int32))
         a.n_iter = niter.ctypes.data_as(C.POINTER(C.c_int32))
-        check(L.lib.gmrf_model_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        check(L.lib.gmrf_model_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream), 1))
         check(L.lib.gmrf_model_eval(self._h, C.byref(a)))
         out["status"] = status
         out["n_iter"] = niter

@@ -173,7 +173,7 @@ class Batch:
     def _stream(self):
         import torch
 
-        check(L.lib.gmrf_batch_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        check(L.lib.gmrf_batch_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream), 1))
 
     @staticmethod
     def _dev(t, cols, name):
