@@ -146,7 +146,7 @@ def main():
     avg_s = d["ms"] * 1e-3 / d["launches"]
     fl = d["flops"] / d["launches"]
     achieved = fl / avg_s
-    bound = "mfma" if name in ("k_factor_large", "k_selinv_large") else "hbm"
+    bound = "mfma" if name in ("k_fl_diag", "k_fl_trsm", "k_fl_update", "k_sl_gemm", "k_sl_pp", "k_sl_lhat") else "hbm"
     if bound == "mfma":
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": None, "kernel": name}

@@ -177,109 +177,167 @@ __global__ __launch_bounds__(64) void k_factor_small(DevPlan P, const int32_t* _
     for (int i = j + t; i < r; i += 64) Ub[(int64_t)j * r + i] = F[(w + i) * ld + (w + j)];
 }
 
-// Large fronts: one 256-thread workgroup per (front, matrix), front in global
-// memory (L block + U workspace). Right-looking over 32-wide panels: factor the
-// panel's diagonal block in LDS, solve the rows below (TRSM), then subtract
-// L_panel D L_panel^T from the trailing lower triangle with MFMA tiles.
-__global__ __launch_bounds__(256) void k_factor_large(DevPlan P, const int32_t* __restrict__ list,
-                                                      const double* __restrict__ vals, int64_t ldv,
-                                                      double* __restrict__ L,
-                                                      double* __restrict__ U) {
-  __shared__ double Dg[kPanel][kPanel + 1];
-  __shared__ double dpiv[kPanel];
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
-  const int s = list[blockIdx.x];
+// ---------------------------------------------------------------------------
+// Large fronts (m > 64), tile-parallel. Each launch covers every large front of
+// one level for every matrix of the batch; a task is an int32 triple
+// (supernode, a, b) whose meaning depends on the kernel.
+// ---------------------------------------------------------------------------
+
+// Assembly of one 64x64 tile (tile row a, tile column b, a >= b) of a front:
+// input entries through the dense L-slot map plus every child's update through
+// the child's inverse row map (children in fixed order: deterministic).
+__global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
+                                                     const double* __restrict__ vals, int64_t ldv,
+                                                     double* __restrict__ L,
+                                                     double* __restrict__ U) {
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1], J = tk[2];
   const int b = blockIdx.y;
-  const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
   const double* vb = vals + (int64_t)b * ldv;
-  // ---- assemble
-  for (int64_t idx = tid; idx < (int64_t)m * w; idx += 256) Lb[idx] = 0.0;
-  for (int64_t idx = tid; idx < (int64_t)r * r; idx += 256) Ub[idx] = 0.0;
+  const int32_t* am = P.dense_amap + P.l_off[s];
+  const double* Um = U + (int64_t)b * P.u_size;
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+    const int i = I * kTile + (idx & (kTile - 1)), j = J * kTile + (idx / kTile);
+    if (i >= m || j >= m || i < j) continue;
+    double v = 0.0;
+    if (j < w) {
+      const int e = am[(int64_t)j * m + i];
+      if (e >= 0) v = vb[e];
+    }
+    for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
+      const int32_t* inv = P.cinv + P.cinv_off[cq];
+      const int ci = inv[i], cj = inv[j];
+      if (ci >= 0 && cj >= 0) {
+        const int c = P.child_idx[cq];
+        const int rc = P.sn_r[c];
+        v += Um[P.u_off[c] + (int64_t)cj * rc + ci];
+      }
+    }
+    *front_at(Lb, Ub, m, w, r, i, j) = v;
+  }
+}
+
+// Diagonal block of panel step `step` (columns [64 step, 64 step + nb)), one
+// task per front: LDL^T of the nb x nb block in LDS with 256 threads, and, in the
+// same sweep, Z = L11^-1 accumulated from the Gauss transforms
+// Z <- (I - l_k e_k^T) Z. The factorised block goes back to L; Z is kept in the
+// per-panel Z workspace for the panel solve and for selected inversion.
+__global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __restrict__ tasks,
+                                                 int step, double* __restrict__ L,
+                                                 double* __restrict__ Z) {
+  __shared__ double D[kTile][kTile + 1];   // D[i][j], lower
+  __shared__ double Zs[kTile][kTile + 1];  // Zs[i][j], lower, unit diagonal
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int c0 = step * kTile, nb = min(kTile, w - c0);
+  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int j = idx / nb, i = idx - j * nb;
+    D[i][j] = (i >= j) ? Lb[(int64_t)(c0 + j) * m + c0 + i] : 0.0;
+    Zs[i][j] = (i == j) ? 1.0 : 0.0;
+  }
   __syncthreads();
-  for (int64_t q = P.fa_ptr[s] + tid; q < P.fa_ptr[s + 1]; q += 256) Lb[P.fa_slot[q]] = vb[P.fa_csc[q]];
-  __syncthreads();
-  for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
-    const int c = P.child_idx[cq];
-    const int rc = P.sn_r[c];
-    const double* Uc = U + (int64_t)b * P.u_size + P.u_off[c];
-    const int32_t* relc = P.rel + P.rows_ptr[c];
-    const int64_t tot = (int64_t)rc * rc;
-    for (int64_t e = tid; e < tot; e += 256) {
-      const int j = (int)(e / rc), i = (int)(e - (int64_t)j * rc);
-      if (i < j) continue;
-      *front_at(Lb, Ub, m, w, r, relc[i], relc[j]) += Uc[e];
+  for (int k = 0; k < nb; ++k) {
+    const double dk = D[k][k];
+    for (int i = k + 1 + tid; i < nb; i += 256) D[i][k] /= dk;
+    __syncthreads();
+    // trailing update D[i][j] -= l_ik d_k l_jk (k < j <= i) and Z[i][q] -= l_ik Z[k][q] (q <= k < i)
+    const int rest = nb - k - 1;
+    for (int idx = tid; idx < rest * rest; idx += 256) {
+      const int ii = idx % rest, jj = idx / rest;
+      if (ii >= jj) {
+        const int i = k + 1 + ii, j = k + 1 + jj;
+        D[i][j] -= D[i][k] * dk * D[j][k];
+      }
+    }
+    for (int idx = tid; idx < rest * (k + 1); idx += 256) {
+      const int i = k + 1 + idx % rest, q = idx / rest;
+      Zs[i][q] -= D[i][k] * Zs[k][q];
     }
     __syncthreads();
   }
-  // ---- panels
-  for (int c0 = 0; c0 < w; c0 += kPanel) {
-    const int nb = min(kPanel, w - c0);
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-      const int j = idx / nb, i = idx - j * nb;
-      Dg[i][j] = (i >= j) ? Lb[(int64_t)(c0 + j) * m + c0 + i] : 0.0;
-    }
-    __syncthreads();
-    for (int k = 0; k < nb; ++k) {
-      const double dk = Dg[k][k];
-      if (tid > k && tid < nb) Dg[tid][k] /= dk;
-      __syncthreads();
-      if (tid > k && tid < nb) {
-        const double y = Dg[tid][k] * dk;
-        for (int j = k + 1; j <= tid; ++j) Dg[tid][j] -= y * Dg[j][k];
-      }
-      __syncthreads();
-    }
-    if (tid < nb) dpiv[tid] = Dg[tid][tid];
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-      const int j = idx / nb, i = idx - j * nb;
-      Lb[(int64_t)(c0 + j) * m + c0 + i] = (i >= j) ? Dg[i][j] : 0.0;
-    }
-    __syncthreads();
-    // rows below the diagonal block: y L11^T = a_i, l_i = y / d
-    for (int i = c0 + nb + tid; i < m; i += 256) {
-      double y[kPanel];
-#pragma unroll
-      for (int p = 0; p < kPanel; ++p) {
-        if (p < nb) {
-          double v = Lb[(int64_t)(c0 + p) * m + i];
-          for (int q = 0; q < p; ++q) v -= y[q] * Dg[p][q];
-          y[p] = v;
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < kPanel; ++p)
-        if (p < nb) Lb[(int64_t)(c0 + p) * m + i] = y[p] / dpiv[p];
-    }
-    __syncthreads();
-    // trailing update on [t0, m)^2, lower triangle, 64x64 tiles
-    const int t0 = c0 + nb;
-    const int nt = (m - t0 + 63) / 64;
-    for (int tj = 0; tj < nt; ++tj)
-      for (int ti = tj; ti < nt; ++ti) {
-        const int R0 = t0 + ti * 64, C0 = t0 + tj * 64;
-        TileAcc acc;
-        tile_zero(acc);
-        mma_tile(
-            [&](int ii, int p) {
-              const int i = R0 + ii;
-              return (i < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + i] * dpiv[p] : 0.0;
-            },
-            [&](int jj, int p) {
-              const int j = C0 + jj;
-              return (j < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + j] : 0.0;
-            },
-            nb, As, Bs, acc);
-        tile_foreach(acc, [&](int ii, int jj, double v) {
-          const int i = R0 + ii, j = C0 + jj;
-          if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) -= v;
-        });
-        __syncthreads();
-      }
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int j = idx / nb, i = idx - j * nb;
+    Lb[(int64_t)(c0 + j) * m + c0 + i] = (i >= j) ? D[i][j] : 0.0;
+    Zb[j * kTile + i] = (i >= j) ? Zs[i][j] : 0.0;
   }
+}
+
+// Panel solve as a product: task (s, I) computes 64 rows below the diagonal block,
+// L21[i][p] = sum_q A21[i][q] Z[p][q] / d_p  (L21 = A21 L11^-T D^-1), in place.
+__global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __restrict__ tasks,
+                                                 int step, double* __restrict__ L,
+                                                 const double* __restrict__ Z) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double dinv[kTile];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int c0 = step * kTile, nb = min(kTile, w - c0);
+  const int R0 = c0 + nb + I * kTile;
+  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
+  if (threadIdx.x < nb) dinv[threadIdx.x] = 1.0 / Lb[(int64_t)(c0 + threadIdx.x) * m + c0 + threadIdx.x];
+  __syncthreads();
+  TileAcc acc;
+  tile_zero(acc);
+  mma_tile(
+      [&](int ii, int q) {
+        const int i = R0 + ii;
+        return (i < m && q < nb) ? Lb[(int64_t)(c0 + q) * m + i] : 0.0;
+      },
+      [&](int pp, int q) { return (pp < nb && q < nb) ? Zb[q * kTile + pp] * dinv[pp] : 0.0; }, nb, As,
+      Bs, acc);
+  tile_foreach(acc, [&](int ii, int pp, double v) {
+    const int i = R0 + ii;
+    if (i < m && pp < nb) Lb[(int64_t)(c0 + pp) * m + i] = v;
+  });
+}
+
+// Trailing update of panel step `step`: task (s, I, J), I >= J, is the 64x64
+// tile at rows t0 + 64 I, columns t0 + 64 J of the trailing lower triangle,
+// t0 = 64 step + nb:  C -= L_I D L_J^T on FP64 MFMA.
+__global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __restrict__ tasks,
+                                                   int step, double* __restrict__ L,
+                                                   double* __restrict__ U) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double dpiv[kTile];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1], J = tk[2];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int c0 = step * kTile, nb = min(kTile, w - c0), t0 = c0 + nb;
+  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
+  const int R0 = t0 + I * kTile, C0 = t0 + J * kTile;
+  if (threadIdx.x < nb) dpiv[threadIdx.x] = Lb[(int64_t)(c0 + threadIdx.x) * m + c0 + threadIdx.x];
+  __syncthreads();
+  TileAcc acc;
+  tile_zero(acc);
+  mma_tile(
+      [&](int ii, int p) {
+        const int i = R0 + ii;
+        return (i < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + i] * dpiv[p] : 0.0;
+      },
+      [&](int jj, int p) {
+        const int j = C0 + jj;
+        return (j < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + j] : 0.0;
+      },
+      nb, As, Bs, acc);
+  tile_foreach(acc, [&](int ii, int jj, double v) {
+    const int i = R0 + ii, j = C0 + jj;
+    if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) -= v;
+  });
 }
 
 // log|Q| = sum log|D_j| per matrix, plus the pivot status: an exact zero pivot
@@ -426,9 +484,11 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
 // evaluated densely inside the front).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double sigma_parent(const DevPlan& P, const double* Sm, const double* Um,
-                                               int p, int pi, int pj) {
+                                               const double* Wm, int p, int pi, int pj) {
   const int wp = P.sn_start[p + 1] - P.sn_start[p];
-  if (pj < wp) return Sm[P.l_off[p] + (int64_t)pj * (wp + P.sn_r[p]) + pi];
+  const int mp = wp + P.sn_r[p];
+  if (P.is_large[p]) return Wm[P.w_off[p] + (int64_t)pj * mp + pi];
+  if (pj < wp) return Sm[P.l_off[p] + (int64_t)pj * mp + pi];
   const int rp = P.sn_r[p];
   return Um[P.u_off[p] + (int64_t)(pj - wp) * rp + (pi - wp)];
 }
@@ -436,7 +496,8 @@ __device__ __forceinline__ double sigma_parent(const DevPlan& P, const double* S
 __global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* __restrict__ list,
                                                      const double* __restrict__ L,
                                                      double* __restrict__ S,
-                                                     double* __restrict__ U) {
+                                                     double* __restrict__ U,
+                                                     const double* __restrict__ W) {
   extern __shared__ double sh[];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
@@ -448,6 +509,7 @@ __global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* _
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Sm = S + (int64_t)b * P.l_size;
   double* Um = U + (int64_t)b * P.u_size;
+  const double* Wm = W + (int64_t)b * P.w_size;
   if (r > 0) {
     const int p = P.sn_parent[s];
     const int32_t* rl = P.rel + P.rows_ptr[s];
@@ -455,7 +517,7 @@ __global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* _
     for (int j = 0; j < r; ++j) {
       const int pj = rl[j];
       for (int i = j + t; i < r; i += 64) {
-        const double v = sigma_parent(P, Sm, Um, p, rl[i], pj);
+        const double v = sigma_parent(P, Sm, Um, Wm, p, rl[i], pj);
         Sg[(w + i) * ld + w + j] = v;
         Sg[(w + j) * ld + w + i] = v;
         Us[(int64_t)j * r + i] = v;
@@ -484,135 +546,176 @@ __global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* _
     for (int i = t; i < m; i += 64) Sb[(int64_t)j * m + i] = (i >= j) ? Sg[i * ld + j] : 0.0;
 }
 
-// Large fronts: 256 threads; columns processed in 32-wide panels from the right.
-// For panel P=[c0,c1) with B=[c1,m): Lh = L[B,P] L_PP^-1 (into T),
-// Sigma[B,P] = -Sigma[B,B] Lh (MFMA tiles), Sigma[P,P] = L_PP^-T D^-1 L_PP^-1 - Lh^T Sigma[B,P].
-__global__ __launch_bounds__(256) void k_selinv_large(DevPlan P, const int32_t* __restrict__ list,
-                                                      const double* __restrict__ L,
-                                                      double* __restrict__ S,
-                                                      double* __restrict__ U,
-                                                      double* __restrict__ T) {
-  __shared__ double Dg[kPanel][kPanel + 1];  // L_PP, then Z = L_PP^-1
-  __shared__ double dinv[kPanel];
+// ---------------------------------------------------------------------------
+// Selected inversion of large fronts. Each large front keeps its whole Sigma
+// front (m x m, both triangles) in the W workspace: Sigma_{R R} is gathered
+// from the parent first, then panels of 64 columns are processed right to
+// left. For panel P = [c0, c1) and B = [c1, m):
+//   Lh = L[B,P] L_PP^-1,  Sigma[B,P] = -Sigma[B,B] Lh,
+//   Sigma[P,P] = L_PP^-T D_P^-1 L_PP^-1 - Lh^T Sigma[B,P]
+// (the block form of the Takahashi recurrence).
+// ---------------------------------------------------------------------------
+
+// Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into W (both triangles).
+__global__ __launch_bounds__(256) void k_sl_gather(DevPlan P, const int32_t* __restrict__ tasks,
+                                                   const double* __restrict__ S,
+                                                   const double* __restrict__ U,
+                                                   double* __restrict__ W) {
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1], J = tk[2];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int p = P.sn_parent[s];
+  const int32_t* rl = P.rel + P.rows_ptr[s];
+  const double* Sm = S + (int64_t)b * P.l_size;
+  const double* Um = U + (int64_t)b * P.u_size;
+  double* Wm = W + (int64_t)b * P.w_size;
+  double* Ws = Wm + P.w_off[s];
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+    const int i = I * kTile + (idx & (kTile - 1)), j = J * kTile + (idx / kTile);
+    if (i >= r || j >= r || i < j) continue;
+    const double v = sigma_parent(P, Sm, Um, Wm, p, rl[i], rl[j]);
+    Ws[(int64_t)(w + j) * m + (w + i)] = v;
+    Ws[(int64_t)(w + i) * m + (w + j)] = v;
+  }
+}
+
+// Selected-inversion panels reuse the factorisation's 64-column partition,
+// walked right to left: selinv step k handles panel index np - 1 - k.
+__device__ __forceinline__ void sel_panel(int w, int step, int& c0, int& c1, int& pk) {
+  const int np = (w + kTile - 1) / kTile;
+  pk = np - 1 - step;
+  c0 = pk * kTile;
+  c1 = min(w, c0 + kTile);
+}
+
+// Lh rows of panel P, as a product with the stored Z = L_PP^-1: task (s, I)
+// computes rows c1 + 64 I .. of Lh[i][p] = sum_q L[i][q] Z[q][p] into T.
+__global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __restrict__ tasks,
+                                                 int step, const double* __restrict__ L,
+                                                 const double* __restrict__ Z,
+                                                 double* __restrict__ T) {
   __shared__ double As[64][kPanel + 1];
   __shared__ double Bs[64][kPanel + 1];
-  const int s = list[blockIdx.x];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int R0 = c1 + I * kTile;
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
+  double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  TileAcc acc;
+  tile_zero(acc);
+  mma_tile(
+      [&](int ii, int q) {
+        const int i = R0 + ii;
+        return (i < m && q < nb) ? Lb[(int64_t)(c0 + q) * m + i] : 0.0;
+      },
+      [&](int pp, int q) { return (pp < nb && q < nb) ? Zb[pp * kTile + q] : 0.0; }, nb, As, Bs, acc);
+  tile_foreach(acc, [&](int ii, int pp, double v) {
+    const int i = R0 + ii;
+    if (i < m && pp < nb) Tb[(int64_t)(c0 + pp) * m + i] = v;
+  });
+}
+
+// Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
+// -sum_{k in B} W[i][k] Lh[k][p] (K = m - c1) and stores it into S (lower) and
+// both triangles of W.
+__global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
+                                                 int step, double* __restrict__ S,
+                                                 double* __restrict__ W,
+                                                 const double* __restrict__ T) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int R0 = c1 + I * kTile;
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  TileAcc acc;
+  tile_zero(acc);
+  mma_tile(
+      [&](int ii, int kk) {
+        const int i = R0 + ii;
+        return i < m ? Ws[(int64_t)(c1 + kk) * m + i] : 0.0;
+      },
+      [&](int jj, int kk) { return jj < nb ? Tb[(int64_t)(c0 + jj) * m + c1 + kk] : 0.0; }, m - c1,
+      As, Bs, acc);
+  tile_foreach(acc, [&](int ii, int jj, double v) {
+    const int i = R0 + ii;
+    if (i < m && jj < nb) {
+      Sb[(int64_t)(c0 + jj) * m + i] = -v;
+      Ws[(int64_t)(c0 + jj) * m + i] = -v;
+      Ws[(int64_t)i * m + c0 + jj] = -v;
+    }
+  });
+}
+
+// Sigma[P,P]: task (s) = Z^T D^-1 Z - Lh^T Sigma[B,P] (MFMA, K = m - c1);
+// lower part into S, both triangles into W.
+__global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restrict__ tasks,
+                                               int step, const double* __restrict__ L,
+                                               const double* __restrict__ Z,
+                                               double* __restrict__ S, double* __restrict__ W,
+                                               const double* __restrict__ T) {
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  __shared__ double dinv[kTile];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
-  double* Sm = S + (int64_t)b * P.l_size;
-  double* Um = U + (int64_t)b * P.u_size;
-  double* Sb = Sm + P.l_off[s];
-  double* Us = Um + P.u_off[s];
-  double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
-  if (r > 0) {
-    const int p = P.sn_parent[s];
-    const int32_t* rl = P.rel + P.rows_ptr[s];
-    const int64_t tot = (int64_t)r * r;
-    for (int64_t e = tid; e < tot; e += 256) {
-      const int j = (int)(e / r), i = (int)(e - (int64_t)j * r);
-      if (i >= j) Us[e] = sigma_parent(P, Sm, Um, p, rl[i], rl[j]);
-    }
-  }
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
+  double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
+  TileAcc acc;
+  tile_zero(acc);
+  if (m > c1)
+    mma_tile([&](int ii, int kk) { return ii < nb ? Tb[(int64_t)(c0 + ii) * m + c1 + kk] : 0.0; },
+             [&](int jj, int kk) { return jj < nb ? Ws[(int64_t)(c0 + jj) * m + c1 + kk] : 0.0; },
+             m - c1, As, Bs, acc);
   __syncthreads();
-  // symmetric lookup of Sigma inside this front for (i, k) both >= current c1
-  auto sig = [&](int i, int k) -> double {
-    const int a = i > k ? i : k, c = i > k ? k : i;
-    return c < w ? Sb[(int64_t)c * m + a] : Us[(int64_t)(c - w) * r + (a - w)];
-  };
-  for (int c1 = w; c1 > 0;) {
-    const int c0 = c1 > kPanel ? c1 - kPanel : 0;
-    const int nb = c1 - c0;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-      const int j = idx / nb, i = idx - j * nb;
-      Dg[i][j] = (i > j) ? Lb[(int64_t)(c0 + j) * m + c0 + i] : (i == j ? 1.0 : 0.0);
-    }
-    if (tid < nb) dinv[tid] = 1.0 / Lb[(int64_t)(c0 + tid) * m + c0 + tid];
-    __syncthreads();
-    // Lh rows: x L_PP = l_i  (x_p = l_ip - sum_{q>p} x_q L_PP[q][p])
-    for (int i = c1 + tid; i < m; i += 256) {
-      double x[kPanel];
-#pragma unroll
-      for (int pp = kPanel - 1; pp >= 0; --pp) {
-        x[pp] = 0.0;
-        if (pp < nb) {
-          double v = Lb[(int64_t)(c0 + pp) * m + i];
-#pragma unroll
-          for (int q = pp + 1; q < kPanel; ++q)
-            if (q < nb) v -= x[q] * Dg[q][pp];
-          x[pp] = v;
-        }
-      }
-#pragma unroll
-      for (int pp = 0; pp < kPanel; ++pp)
-        if (pp < nb) Tb[(int64_t)(c0 + pp) * m + i] = x[pp];
-    }
-    __syncthreads();
-    // Sigma[B, P] = -Sigma[B, B] Lh
-    const int nB = m - c1;
-    const int nt = (nB + 63) / 64;
-    for (int ti = 0; ti < nt; ++ti) {
-      const int R0 = c1 + ti * 64;
-      TileAcc acc;
-      tile_zero(acc);
-      mma_tile([&](int ii, int kk) {
-                 const int i = R0 + ii;
-                 return i < m ? sig(i, c1 + kk) : 0.0;
-               },
-               [&](int jj, int kk) {
-                 return jj < nb ? Tb[(int64_t)(c0 + jj) * m + c1 + kk] : 0.0;
-               },
-               nB, As, Bs, acc);
-      tile_foreach(acc, [&](int ii, int jj, double v) {
-        const int i = R0 + ii;
-        if (i < m && jj < nb) Sb[(int64_t)(c0 + jj) * m + i] = -v;
-      });
-      __syncthreads();
-    }
-    // Z = L_PP^-1 (unit lower), column q by thread q; Dg is overwritten only
-    // after every thread has finished reading L_PP.
-    {
-      const int q = tid;
-      double z[kPanel];
-#pragma unroll
-      for (int i = 0; i < kPanel; ++i) {
-        z[i] = 0.0;
-        if (q < nb && i < nb && i >= q) {
-          double v = (i == q) ? 1.0 : 0.0;
-#pragma unroll
-          for (int k = 0; k < i; ++k) v -= Dg[i][k] * z[k];  // z[k] = 0 for k < q
-          z[i] = v;
-        }
-      }
-      __syncthreads();
-      if (q < nb) {
-#pragma unroll
-        for (int i = 0; i < kPanel; ++i)
-          if (i < nb) Dg[i][q] = z[i];
-      }
-    }
-    __syncthreads();
-    // Sigma[P,P] lower: sum_k Z[k][p] Z[k][q] / d_k - sum_{i in B} Lh[i][p] Sigma[i][q]
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-      const int q = idx / nb, pp = idx - q * nb;
-      if (pp < q) continue;
+  // Zt[q][k] = Z[k][q] in the LDS freed by the tile engine
+  double (*Zt)[kTile + 1] = (double (*)[kTile + 1])sh;
+  for (int idx = tid; idx < kTile * kTile; idx += 256) {
+    const int q = idx / kTile, k = idx - q * kTile;
+    Zt[q][k] = (q < nb && k < nb) ? Zb[q * kTile + k] : 0.0;
+  }
+  if (tid < nb) dinv[tid] = 1.0 / Lb[(int64_t)(c0 + tid) * m + c0 + tid];
+  __syncthreads();
+  tile_foreach(acc, [&](int ii, int jj, double v) {
+    // tile row ii = p, tile column jj = q; lower part p >= q
+    if (ii < nb && jj < nb && ii >= jj) {
       double g = 0.0;
-      for (int k = pp; k < nb; ++k) g += Dg[k][pp] * Dg[k][q] * dinv[k];
-      double h = 0.0;
-      const double* lp = Tb + (int64_t)(c0 + pp) * m;
-      const double* sq = Sb + (int64_t)(c0 + q) * m;
-      for (int i = c1; i < m; ++i) h += lp[i] * sq[i];
-      Sb[(int64_t)(c0 + q) * m + c0 + pp] = g - h;
+      for (int k = ii; k < nb; ++k) g += Zt[ii][k] * Zt[jj][k] * dinv[k];
+      const double val = g - v;
+      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = val;
+      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = val;
+      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = val;
     }
-    __syncthreads();
-    c1 = c0;
-  }
-  // zero the strictly upper part of the diagonal block (unused, kept clean)
-  for (int idx = tid; idx < w * w; idx += 256) {
-    const int j = idx / w, i = idx - j * w;
-    if (i < j) Sb[(int64_t)j * m + i] = 0.0;
-  }
+  });
+  if (c0 == 0)  // strictly upper part of the diagonal block: unused, kept zero
+    for (int idx = tid; idx < w * w; idx += 256) {
+      const int j = idx / w, i = idx - j * w;
+      if (i < j) Sb[(int64_t)j * m + i] = 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -657,11 +760,31 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
   return hipGetLastError();
 }
 
-hipError_t launch_factor_large(const DevPlan& P, const int32_t* list, int count, int B,
-                               const double* vals, int64_t ldv, double* L, double* U,
-                               hipStream_t st) {
+hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int B,
+                              const double* vals, int64_t ldv, double* L, double* U, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_factor_large, dim3(count, B), dim3(256), 0, st, P, list, vals, ldv, L, U);
+  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, vals, ldv, L, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_fl_diag(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                          double* L, double* Z, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fl_diag, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_fl_trsm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                          double* L, const double* Z, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fl_trsm, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_fl_update(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                            double* L, double* U, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fl_update, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, U);
   return hipGetLastError();
 }
 
@@ -707,17 +830,40 @@ hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, in
 }
 
 hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                               const double* L, double* S, double* U, hipStream_t st) {
+                               const double* L, double* S, double* U, const double* W,
+                               hipStream_t st) {
   if (count <= 0) return hipSuccess;
   size_t sh = ((size_t)maxm * (maxm + 1) + maxm) * sizeof(double);
-  hipLaunchKernelGGL(k_selinv_small, dim3(count, B), dim3(64), sh, st, P, list, L, S, U);
+  hipLaunchKernelGGL(k_selinv_small, dim3(count, B), dim3(64), sh, st, P, list, L, S, U, W);
   return hipGetLastError();
 }
 
-hipError_t launch_selinv_large(const DevPlan& P, const int32_t* list, int count, int B,
-                               const double* L, double* S, double* U, double* T, hipStream_t st) {
+hipError_t launch_sl_gather(const DevPlan& P, const int32_t* tasks, int count, int B,
+                            const double* S, const double* U, double* W, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_selinv_large, dim3(count, B), dim3(256), 0, st, P, list, L, S, U, T);
+  hipLaunchKernelGGL(k_sl_gather, dim3(count, B), dim3(256), 0, st, P, tasks, S, U, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                          const double* L, const double* Z, double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_lhat, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                          double* S, double* W, const double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_pp(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                        const double* L, const double* Z, double* S, double* W, const double* T,
+                        hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_pp, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, S, W, T);
   return hipGetLastError();
 }
 

@@ -256,25 +256,31 @@ __global__ __launch_bounds__(256) void k_rowreduce(int64_t len, int op, const do
   if (threadIdx.x == 0) out[b] = red[0];
 }
 
-// Adjoint contractions, one block per (b, k):
+// Adjoint contractions over the CSC pattern, two passes (deterministic order):
 //   tr[b][k]   = sum_e comps[k][e] * Sigma_b[slot_full[e]]     (= tr(Sigma_b Q_k))
 //   quad[b][k] = sum_e comps[k][e] * x_b[row_e] * z_b[col_e]   (= x_b^T Q_k z_b)
-__global__ __launch_bounds__(256) void k_contract(int K, int64_t nnz, const double* __restrict__ comps,
-                                                  const double* __restrict__ S, int64_t l_size,
-                                                  const int64_t* __restrict__ slot_full,
-                                                  const int32_t* __restrict__ qrow,
-                                                  const int32_t* __restrict__ qcol,
-                                                  const double* __restrict__ x,
-                                                  const double* __restrict__ z, int64_t n,
-                                                  double* __restrict__ tr, double* __restrict__ quad) {
+// Pass 1: block (chunk, k, b) reduces one chunk of the entries into part[].
+constexpr int kContractChunks = 64;
+
+__global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
+                                                       const double* __restrict__ comps,
+                                                       const double* __restrict__ S, int64_t l_size,
+                                                       const int64_t* __restrict__ slot_full,
+                                                       const int32_t* __restrict__ qrow,
+                                                       const int32_t* __restrict__ qcol,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ z, int64_t n,
+                                                       double* __restrict__ part) {
   __shared__ double r1[256], r2[256];
-  const int k = blockIdx.x, b = blockIdx.y;
+  const int ch = blockIdx.x, k = blockIdx.y, b = blockIdx.z;
+  const int64_t len = (nnz + kContractChunks - 1) / kContractChunks;
+  const int64_t e0 = ch * len, e1 = min(nnz, e0 + len);
   const double* ck = comps + (int64_t)k * nnz;
   const double* Sb = S ? S + (int64_t)b * l_size : nullptr;
   const double* xb = x ? x + (int64_t)b * n : nullptr;
   const double* zb = z ? z + (int64_t)b * n : nullptr;
   double a1 = 0.0, a2 = 0.0;
-  for (int64_t e = threadIdx.x; e < nnz; e += 256) {
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
     const double c = ck[e];
     if (c == 0.0) continue;
     if (Sb) {
@@ -294,9 +300,24 @@ __global__ __launch_bounds__(256) void k_contract(int K, int64_t nnz, const doub
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if (tr) tr[b * K + k] = r1[0];
-    if (quad) quad[b * K + k] = r2[0];
+    const int64_t q = ((int64_t)b * K + k) * kContractChunks + ch;
+    part[2 * q] = r1[0];
+    part[2 * q + 1] = r2[0];
   }
+}
+
+// Pass 2: one thread per (b, k) sums its chunks in order.
+__global__ void k_contract_sum(int BK, const double* __restrict__ part, double* __restrict__ tr,
+                               double* __restrict__ quad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= BK) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int ch = 0; ch < kContractChunks; ++ch) {
+    a1 += part[2 * ((int64_t)t * kContractChunks + ch)];
+    a2 += part[2 * ((int64_t)t * kContractChunks + ch) + 1];
+  }
+  if (tr) tr[t] = a1;
+  if (quad) quad[t] = a2;
 }
 
 // v[b][i] = (A Sigma_b A^T)_ii = sum over the row's entry pairs of A_ia A_ib Sigma_b[ja, jb]
@@ -425,9 +446,12 @@ hipError_t lap_rowreduce(int B, int64_t len, int op, const double* x, const doub
 hipError_t lap_contract(int B, int K, int64_t nnz, const double* comps, const double* S,
                         int64_t l_size, const int64_t* slot_full, const int32_t* qrow,
                         const int32_t* qcol, const double* x, const double* z, int64_t n, double* tr,
-                        double* quad, hipStream_t st) {
-  hipLaunchKernelGGL(k_contract, dim3(K, B), dim3(256), 0, st, K, nnz, comps, S, l_size, slot_full,
-                     qrow, qcol, x, z, n, tr, quad);
+                        double* quad, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_contract_part, dim3(kContractChunks, K, B), dim3(256), 0, st, K, nnz, comps, S,
+                     l_size, slot_full, qrow, qcol, x, z, n, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_contract_sum, dim3((B * K + 63) / 64), dim3(64), 0, st, B * K, part, tr, quad);
   return hipGetLastError();
 }
 

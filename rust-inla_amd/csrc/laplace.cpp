@@ -29,7 +29,7 @@ hipError_t lap_axpy(int, int64_t, const double*, const double*, const double*, d
 hipError_t lap_rowreduce(int, int64_t, int, const double*, const double*, double*, hipStream_t);
 hipError_t lap_contract(int, int, int64_t, const double*, const double*, int64_t, const int64_t*,
                         const int32_t*, const int32_t*, const double*, const double*, int64_t,
-                        double*, double*, hipStream_t);
+                        double*, double*, double*, hipStream_t);
 hipError_t lap_vdiag(int, int64_t, const int64_t*, const int64_t*, const double*, const double*,
                      int64_t, double*, hipStream_t);
 hipError_t lap_constr_dot(int, int, int64_t, const double*, const double*, double*, hipStream_t);
@@ -78,7 +78,7 @@ struct gmrf_model {
          *v = nullptr, *u = nullptr;
   double *red = nullptr, *alpha = nullptr, *cres = nullptr;
   int32_t* bad = nullptr;
-  double *tr_post = nullptr, *tr_prior = nullptr, *qx = nullptr, *ql = nullptr;
+  double *tr_post = nullptr, *tr_prior = nullptr, *qx = nullptr, *ql = nullptr, *cpart = nullptr;
   hipStream_t stream() const { return bt->stream; }
 };
 
@@ -329,6 +329,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->tr_prior = M->arena.alloc<double>(C * M->K);
     M->qx = M->arena.alloc<double>(C * M->K);
     M->ql = M->arena.alloc<double>(C * M->K);
+    M->cpart = M->arena.alloc<double>(C * M->K * 64 * 2);  // contraction partial sums
     *out = M.release();
     return GMRF_OK;
   } catch (...) {
@@ -520,10 +521,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       const DevPlan& P = M->sym->plan;
       // tr(Sigma_post Q_k), x*^T Q_k x*, tr(Sigma_prior Q_k)
       check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col,
-                         M->x, M->x, n, M->tr_post, M->qx, st),
+                         M->x, M->x, n, M->tr_post, M->qx, M->cpart, st),
             "contract post");
       check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)cap * P.l_size, P.l_size, P.slot_full,
-                         nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, st),
+                         nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st),
             "contract prior");
       // v_i = (A Sigma_post A^T)_ii
       check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
@@ -535,7 +536,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
               "A^T u");
         bt->solve(B, M->rhs, n, M->lam, n, 0);
         check(lap_contract(B, K, nnz, M->comps, nullptr, 0, nullptr, M->q_row, M->q_col, M->lam, M->x,
-                           n, nullptr, M->ql, st),
+                           n, nullptr, M->ql, M->cpart, st),
               "lambda^T Q_k x");
         d2h(hql.data(), M->ql, hql.size() * sizeof(double), st);
       }

@@ -31,19 +31,20 @@ struct DevPlan {
   const int32_t* perm;       // [n] original index of factor position k
   const int64_t* diag_slot;  // [n] L-layout slot of D_k
   const int64_t* slot_full;  // [nnz] L-layout slot of the (symmetric) CSC entry
-};
 
-// Host-side level lists for one launch class: supernode ids on the device plus
-// per-level [begin, end) into that array and the largest front of the level.
-struct LevelClass {
-  int32_t* d_list = nullptr;  // device
-  std::int64_t* begin = nullptr;
-  std::int64_t* end = nullptr;
-  int32_t* maxm = nullptr;
-  int32_t* maxw = nullptr;
+  // Large fronts (m > kSmallFront): tile-parallel kernels.
+  const int32_t* is_large;    // [nsuper]
+  const int64_t* w_off;       // [nsuper] dense m x m Sigma front (large fronts), -1 otherwise
+  int64_t w_size;             // doubles of the Sigma-front workspace per matrix
+  const int64_t* z_off;       // [nsuper] per-panel 64 x 64 blocks Z = L11^-1 (large fronts)
+  int64_t z_size;             // doubles of the Z workspace per matrix
+  const int32_t* dense_amap;  // [l_size] CSC entry assembled into each L slot, -1 if none
+  const int64_t* cinv_off;    // [n children] per (front, child) slot: offset into cinv
+  const int32_t* cinv;        // parent front position -> child row index, -1 if absent
 };
 
 constexpr int kSmallFront = 64;  // fronts with m <= 64 run one wavefront each, in LDS
-constexpr int kPanel = 32;       // panel width of the large-front kernels
+constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
+constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 
 }  // namespace gmrf

@@ -18,8 +18,22 @@
 namespace gmrf {
 hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                int64_t, double*, double*, hipStream_t);
-hipError_t launch_factor_large(const DevPlan&, const int32_t*, int, int, const double*, int64_t,
-                               double*, double*, hipStream_t);
+hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, const double*, int64_t,
+                              double*, double*, hipStream_t);
+hipError_t launch_fl_diag(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+                          hipStream_t);
+hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
+                          hipStream_t);
+hipError_t launch_fl_update(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+                            hipStream_t);
+hipError_t launch_sl_gather(const DevPlan&, const int32_t*, int, int, const double*, const double*,
+                            double*, hipStream_t);
+hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const double*,
+                          const double*, double*, hipStream_t);
+hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+                          const double*, hipStream_t);
+hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const double*,
+                        const double*, double*, double*, const double*, hipStream_t);
 hipError_t launch_logdet(const DevPlan&, int, const double*, double*, int32_t*, hipStream_t);
 hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t);
 hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
@@ -29,9 +43,7 @@ hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const
 hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
                             hipStream_t);
 hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
-                               double*, double*, hipStream_t);
-hipError_t launch_selinv_large(const DevPlan&, const int32_t*, int, int, const double*, double*,
-                               double*, double*, hipStream_t);
+                               double*, double*, const double*, hipStream_t);
 hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
 hipError_t launch_pattern_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
 hipError_t launch_mma_probe(const double*, const double*, int, double*, hipStream_t);
@@ -152,6 +164,14 @@ struct LevelInfo {
   // algorithmic work of the level's launches (per matrix)
   double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol = 0;
   double by_small = 0, by_large = 0;
+  // large-front task ranges (triples in SymData::d_tasks)
+  struct Range {
+    int64_t off = 0, cnt = 0;
+    double flops = 0, bytes = 0;
+  };
+  Range asm_t, gather;
+  std::vector<Range> fdiag, ftrsm, fupd;   // per factor panel step
+  std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
 };
 
 }  // namespace gmrf_rt
@@ -168,6 +188,7 @@ struct SymData {
   int32_t* d_small = nullptr;
   int32_t* d_large = nullptr;
   int32_t* d_all = nullptr;
+  int32_t* d_tasks = nullptr;  // large-front task triples
   int order_req = 0;
   std::vector<int64_t> given;  // given permutation (cache key)
   std::mutex dev_mu;
@@ -294,6 +315,207 @@ struct SymData {
     d_small = arena.upload(small);
     d_large = arena.upload(large);
     d_all = arena.upload(all);
+    build_large_plan(large, fa_ptr);
+  }
+
+  // Task lists, inverse child maps, dense input map and Sigma-front offsets of
+  // the large fronts.
+  void build_large_plan(const std::vector<int32_t>& large, const std::vector<int64_t>& fa_ptr) {
+    const int64_t ns = S.nsuper;
+    auto mw = [&](int64_t s, int64_t& w, int64_t& r, int64_t& m) {
+      w = S.sn_start[s + 1] - S.sn_start[s];
+      r = S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s];
+      m = w + r;
+    };
+    std::vector<int32_t> is_large(ns, 0);
+    std::vector<int64_t> w_off(ns, -1), z_off(ns, -1);
+    int64_t wsz = 0, zsz = 0;
+    for (int32_t s : large) {
+      int64_t w, r, m;
+      mw(s, w, r, m);
+      is_large[s] = 1;
+      w_off[s] = wsz;
+      wsz += m * m;
+      z_off[s] = zsz;
+      zsz += ((w + kTile - 1) / kTile) * kTile * kTile;
+    }
+    std::vector<int32_t> dense_amap(S.l_size, -1);
+    for (int64_t p = 0; p < S.nnz_full; ++p)
+      if (S.amap[p] >= 0) dense_amap[S.amap[p]] = (int32_t)p;
+    std::vector<int64_t> cinv_off(std::max<int64_t>((int64_t)S.child_idx.size(), 1), -1);
+    std::vector<int32_t> cinv;
+    for (int32_t pf : large) {
+      int64_t w, r, m;
+      mw(pf, w, r, m);
+      for (int64_t q = S.child_ptr[pf]; q < S.child_ptr[pf + 1]; ++q) {
+        const int64_t c = S.child_idx[q];
+        cinv_off[q] = (int64_t)cinv.size();
+        cinv.resize(cinv.size() + m, -1);
+        int32_t* inv = cinv.data() + cinv_off[q];
+        for (int64_t i = S.sn_rows_ptr[c]; i < S.sn_rows_ptr[c + 1]; ++i)
+          inv[S.rel[i]] = (int32_t)(i - S.sn_rows_ptr[c]);
+      }
+    }
+    plan.is_large = arena.upload(is_large);
+    plan.w_off = arena.upload(w_off);
+    plan.w_size = std::max<int64_t>(wsz, 1);
+    plan.z_off = arena.upload(z_off);
+    plan.z_size = std::max<int64_t>(zsz, 1);
+    plan.dense_amap = arena.upload(dense_amap);
+    plan.cinv_off = arena.upload(cinv_off);
+    plan.cinv = arena.upload(cinv.empty() ? std::vector<int32_t>(1, -1) : cinv);
+    // structural work per front (for the per-launch roofline accounting)
+    auto struct_work = [&](int64_t s, double& fl, double& sel) {
+      fl = sel = 0.0;
+      for (int64_t j = S.sn_start[s]; j < S.sn_start[s + 1]; ++j) {
+        const double rj = (double)(S.colcount[j] - 1);
+        fl += rj * (rj + 3.0);
+        sel += 2.0 * rj * (rj + 1.0) + 1.0;
+      }
+    };
+    std::vector<int32_t> tk;
+    auto add = [&](int64_t s, int64_t a, int64_t b) {
+      tk.push_back((int32_t)s);
+      tk.push_back((int32_t)a);
+      tk.push_back((int32_t)b);
+    };
+    auto ceil64 = [](int64_t x) { return (x + kTile - 1) / kTile; };
+    for (LevelInfo& L : levels) {
+      std::vector<int64_t> fronts;
+      for (int64_t q = L.lb; q < L.le; ++q) fronts.push_back(large[q]);
+      if (fronts.empty()) continue;
+      int64_t maxsteps = 0;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        maxsteps = std::max(maxsteps, ceil64(w));
+      }
+      L.fdiag.resize(maxsteps);
+      L.ftrsm.resize(maxsteps);
+      L.fupd.resize(maxsteps);
+      L.lhat.resize(maxsteps);
+      L.sgemm.resize(maxsteps);
+      L.spp.resize(maxsteps);
+      // dense work shares of each front per launch kind and step
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        double F, G;
+        struct_work(s, F, G);
+        double Dfac = 0, Dsel = 0;
+        std::vector<double> dd(maxsteps, 0), dt(maxsteps, 0), du(maxsteps, 0), dl(maxsteps, 0),
+            dg(maxsteps, 0), dq(maxsteps, 0);
+        const int64_t np = (w + kTile - 1) / kTile;
+        for (int64_t k = 0; k < np; ++k) {
+          const double c0 = (double)(k * kTile), nb = (double)std::min<int64_t>(kTile, w - k * kTile);
+          const double T = (double)m - c0 - nb;
+          dd[k] = nb * nb * nb / 3.0;
+          dt[k] = nb * nb * T;
+          du[k] = nb * T * (T + 1.0) / 2.0;
+          Dfac += dd[k] + dt[k] + du[k];
+          // selinv step k handles panel np - 1 - k
+          const int64_t pk = np - 1 - k, sc0 = pk * kTile, sc1 = std::min<int64_t>(w, sc0 + kTile);
+          const double nbs = (double)(sc1 - sc0), nB = (double)(m - sc1);
+          dl[k] = nbs * nbs * nB / 2.0;
+          dg[k] = nB * nB * nbs;
+          dq[k] = nbs * nbs * nB + nbs * nbs * nbs / 6.0;
+          Dsel += dl[k] + dg[k] + dq[k];
+        }
+        for (int64_t k = 0; k < maxsteps; ++k) {
+          if (Dfac > 0) {
+            L.fdiag[k].flops += F * dd[k] / Dfac;
+            L.ftrsm[k].flops += F * dt[k] / Dfac;
+            L.fupd[k].flops += F * du[k] / Dfac;
+          }
+          if (Dsel > 0) {
+            L.lhat[k].flops += G * dl[k] / Dsel;
+            L.sgemm[k].flops += G * dg[k] / Dsel;
+            L.spp[k].flops += G * dq[k] / Dsel;
+          }
+        }
+        L.asm_t.bytes += 8.0 * (double)(m * w) + 12.0 * (double)(fa_ptr[s + 1] - fa_ptr[s]);
+      }
+      L.asm_t.off = (int64_t)tk.size() / 3;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        const int64_t nt = ceil64(m);
+        for (int64_t J = 0; J < nt; ++J)
+          for (int64_t I = J; I < nt; ++I) add(s, I, J);
+      }
+      L.asm_t.cnt = (int64_t)tk.size() / 3 - L.asm_t.off;
+      for (int64_t k = 0; k < maxsteps; ++k) {
+        L.fdiag[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t w, r, m;
+          mw(s, w, r, m);
+          if (w > k * kTile) add(s, 0, 0);
+        }
+        L.fdiag[k].cnt = (int64_t)tk.size() / 3 - L.fdiag[k].off;
+        L.ftrsm[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t w, r, m;
+          mw(s, w, r, m);
+          if (w <= k * kTile) continue;
+          const int64_t c0 = k * kTile, nb = std::min<int64_t>(kTile, w - c0);
+          for (int64_t I = 0; I < ceil64(m - c0 - nb); ++I) add(s, I, 0);
+        }
+        L.ftrsm[k].cnt = (int64_t)tk.size() / 3 - L.ftrsm[k].off;
+        L.fupd[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t w, r, m;
+          mw(s, w, r, m);
+          if (w <= k * kTile) continue;
+          const int64_t c0 = k * kTile, nb = std::min<int64_t>(kTile, w - c0), T = m - c0 - nb;
+          const int64_t nt = ceil64(T);
+          for (int64_t J = 0; J < nt; ++J)
+            for (int64_t I = J; I < nt; ++I) add(s, I, J);
+        }
+        L.fupd[k].cnt = (int64_t)tk.size() / 3 - L.fupd[k].off;
+      }
+      L.gather.off = (int64_t)tk.size() / 3;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        const int64_t nt = ceil64(r);
+        for (int64_t J = 0; J < nt; ++J)
+          for (int64_t I = J; I < nt; ++I) add(s, I, J);
+      }
+      L.gather.cnt = (int64_t)tk.size() / 3 - L.gather.off;
+      for (int64_t k = 0; k < maxsteps; ++k) {
+        auto rows_of = [&](int64_t s, int64_t& nB) {
+          int64_t w, r, m;
+          mw(s, w, r, m);
+          const int64_t np = (w + kTile - 1) / kTile;
+          if (k >= np) return false;
+          const int64_t c1 = std::min<int64_t>(w, (np - 1 - k) * kTile + kTile);
+          nB = m - c1;
+          return true;
+        };
+        L.lhat[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t nB;
+          if (!rows_of(s, nB)) continue;
+          for (int64_t rb = 0; rb < ceil64(nB); ++rb) add(s, rb, 0);
+        }
+        L.lhat[k].cnt = (int64_t)tk.size() / 3 - L.lhat[k].off;
+        L.sgemm[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t nB;
+          if (!rows_of(s, nB)) continue;
+          for (int64_t rb = 0; rb < ceil64(nB); ++rb) add(s, rb, 0);
+        }
+        L.sgemm[k].cnt = (int64_t)tk.size() / 3 - L.sgemm[k].off;
+        L.spp[k].off = (int64_t)tk.size() / 3;
+        for (int64_t s : fronts) {
+          int64_t nB;
+          if (!rows_of(s, nB)) continue;
+          add(s, 0, 0);
+        }
+        L.spp[k].cnt = (int64_t)tk.size() / 3 - L.spp[k].off;
+      }
+    }
+    d_tasks = arena.upload(tk.empty() ? std::vector<int32_t>(3, 0) : tk);
   }
 };
 
@@ -309,7 +531,8 @@ struct gmrf_batch {
   hipStream_t stream = nullptr;  // stream in use (own or caller's)
   hipStream_t own = nullptr;
   DevArena arena;
-  double *L = nullptr, *U = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *V = nullptr;
+  double *L = nullptr, *U = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *V = nullptr,
+         *W = nullptr, *Z = nullptr;
   double* d_logdet = nullptr;
   int32_t* d_status = nullptr;
 
@@ -325,6 +548,8 @@ struct gmrf_batch {
     T = arena.alloc<double>((size_t)cap * Sy.l_size);
     X = arena.alloc<double>((size_t)cap * Sy.n);
     V = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.v_size, 1));
+    W = arena.alloc<double>((size_t)cap * sym->plan.w_size);
+    Z = arena.alloc<double>((size_t)cap * sym->plan.z_size);
     d_logdet = arena.alloc<double>(cap);
     d_status = arena.alloc<int32_t>(cap);
   }
@@ -340,17 +565,34 @@ struct gmrf_batch {
     const DevPlan& P = sym->plan;
     double* Lf = L + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
+    double* Zf = Z + (int64_t)first * P.z_size;
     for (const LevelInfo& lv : sym->levels) {
       if (lv.se > lv.sb)
         prof_launch("k_factor_small", lv.fl_small * count, lv.by_small * count, stream, [&] {
           return launch_factor_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
                                      count, vals, ldv, Lf, Uf, stream);
         });
-      if (lv.le > lv.lb)
-        prof_launch("k_factor_large", lv.fl_large * count, lv.by_large * count, stream, [&] {
-          return launch_factor_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, vals, ldv,
-                                     Lf, Uf, stream);
+      if (lv.le > lv.lb) {
+        const int32_t* tk = sym->d_tasks;
+        prof_launch("k_fl_assemble", 0.0, lv.asm_t.bytes * count, stream, [&] {
+          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, count, vals, ldv, Lf,
+                                    Uf, stream);
         });
+        for (size_t k = 0; k < lv.fdiag.size(); ++k) {
+          prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
+            return launch_fl_diag(P, tk + 3 * lv.fdiag[k].off, (int)lv.fdiag[k].cnt, (int)k, count,
+                                  Lf, Zf, stream);
+          });
+          prof_launch("k_fl_trsm", lv.ftrsm[k].flops * count, 0.0, stream, [&] {
+            return launch_fl_trsm(P, tk + 3 * lv.ftrsm[k].off, (int)lv.ftrsm[k].cnt, (int)k, count,
+                                  Lf, Zf, stream);
+          });
+          prof_launch("k_fl_update", lv.fupd[k].flops * count, 0.0, stream, [&] {
+            return launch_fl_update(P, tk + 3 * lv.fupd[k].off, (int)lv.fupd[k].cnt, (int)k, count,
+                                    Lf, Uf, stream);
+          });
+        }
+      }
     }
     check(launch_logdet(P, count, Lf, d_logdet + first, d_status + first, stream), "logdet");
   }
@@ -383,17 +625,35 @@ struct gmrf_batch {
     double* Sf = S + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
     double* Tf = T + (int64_t)first * P.l_size;
+    double* Wf = W + (int64_t)first * P.w_size;
+    const double* Zf = Z + (int64_t)first * P.z_size;
+    const int32_t* tk = sym->d_tasks;
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
-      if (lv.le > lv.lb)
-        prof_launch("k_selinv_large", lv.sel_large * count, lv.by_large * count, stream, [&] {
-          return launch_selinv_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, Lf, Sf, Uf,
-                                     Tf, stream);
+      if (lv.le > lv.lb) {
+        prof_launch("k_sl_gather", 0.0, 0.0, stream, [&] {
+          return launch_sl_gather(P, tk + 3 * lv.gather.off, (int)lv.gather.cnt, count, Sf, Uf, Wf,
+                                  stream);
         });
+        for (size_t k = 0; k < lv.lhat.size(); ++k) {
+          prof_launch("k_sl_lhat", lv.lhat[k].flops * count, 0.0, stream, [&] {
+            return launch_sl_lhat(P, tk + 3 * lv.lhat[k].off, (int)lv.lhat[k].cnt, (int)k, count, Lf,
+                                  Zf, Tf, stream);
+          });
+          prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, 0.0, stream, [&] {
+            return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
+                                  Sf, Wf, Tf, stream);
+          });
+          prof_launch("k_sl_pp", lv.spp[k].flops * count, 0.0, stream, [&] {
+            return launch_sl_pp(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Lf, Zf,
+                                Sf, Wf, Tf, stream);
+          });
+        }
+      }
       if (lv.se > lv.sb)
         prof_launch("k_selinv_small", lv.sel_small * count, lv.by_small * count, stream, [&] {
           return launch_selinv_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
-                                     count, Lf, Sf, Uf, stream);
+                                     count, Lf, Sf, Uf, Wf, stream);
         });
     }
   }
