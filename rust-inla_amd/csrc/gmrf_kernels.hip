@@ -25,6 +25,12 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 #define GMRF_ST_SINGULAR 1
 #define GMRF_ST_ZERO_PIVOT 2
 
+// Compiler-only fence after a round of independent loads: the loads stay issued
+// above it, so the compiler cannot sink them into the conditional code that
+// uses them (which would wait for each load separately). No instruction is
+// emitted; the hardware waits only where a value is first used.
+#define ISSUE_FENCE() asm volatile("" ::: "memory")
+
 __device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -49,20 +55,53 @@ struct TileAcc {
   d4 c[2][2];  // [bj][bi]
 };
 
-template <typename FA, typename FB>
+// AK / BK: the operand is contiguous along k in memory, so consecutive lanes
+// load consecutive k (otherwise consecutive rows). The accessors are called with
+// k < K only and must load unconditionally from a valid (clamped) address and
+// select zero afterwards: a branch around a load serialises the memory latency.
+// The next K chunk is fetched into registers while the current one is on MFMA.
+template <bool AK = false, bool BK = false, typename FA, typename FB>
 __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPanel + 1],
                                          double (*Bs)[kPanel + 1], TileAcc& acc) {
+  constexpr int NL = 64 * kPanel / 256;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int qi = wv & 1, qj = wv >> 1;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += kPanel) {
+  double ra[NL], rb[NL];
+  auto fetch = [&](int k0) {
     const int kc = min(kPanel, K - k0);
-    for (int idx = tid; idx < 64 * kPanel; idx += 256) {
-      const int ii = idx & 63, kk = idx >> 6;
-      As[ii][kk] = kk < kc ? fa(ii, k0 + kk) : 0.0;
-      Bs[ii][kk] = kk < kc ? fb(ii, k0 + kk) : 0.0;
+#pragma unroll
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + 256 * it;
+      {
+        const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
+        const double v = fa(ii, k0 + min(kk, kc - 1));
+        ra[it] = kk < kc ? v : 0.0;
+      }
+      {
+        const int ii = BK ? idx / kPanel : (idx & 63), kk = BK ? idx % kPanel : (idx >> 6);
+        const double v = fb(ii, k0 + min(kk, kc - 1));
+        rb[it] = kk < kc ? v : 0.0;
+      }
+    }
+    ISSUE_FENCE();
+  };
+  if (K > 0) fetch(0);
+  for (int k0 = 0; k0 < K; k0 += kPanel) {
+#pragma unroll
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + 256 * it;
+      {
+        const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
+        As[ii][kk] = ra[it];
+      }
+      {
+        const int ii = BK ? idx / kPanel : (idx & 63), kk = BK ? idx % kPanel : (idx >> 6);
+        Bs[ii][kk] = rb[it];
+      }
     }
     __syncthreads();
+    if (k0 + kPanel < K) fetch(k0 + kPanel);
 #pragma unroll
     for (int kk = 0; kk < kPanel; kk += 4) {
       const double a0 = Bs[qj * 32 + lr][kk + lk];
@@ -113,68 +152,288 @@ __global__ __launch_bounds__(256) void k_mma_probe(const double* A, const double
   TileAcc acc;
   tile_zero(acc);
   mma_tile([&](int i, int k) { return A[(int64_t)k * 64 + i]; },
-           [&](int j, int k) { return B[(int64_t)k * 64 + j]; }, K, As, Bs, acc);
+           [&](int j, int k) { return B[(int64_t)k * 64 + j]; }, K, As, Bs, acc);  // k < K
   tile_foreach(acc, [&](int i, int j, double v) { T[(int64_t)j * 64 + i] = v; });
+}
+
+// ---------------------------------------------------------------------------
+// Blocked dense LDL^T of a front of at most 64 rows held in LDS.
+//
+// The front sits row-major in LDS (F[i * LD + j], lower triangle). Columns are
+// eliminated in blocks of 16: one wave factors the 16 x 16 diagonal block in
+// registers (shuffles only, no barrier), then all waves form the panel
+// W = A21 Z11^T (Z11 = L11^-1), L21 = W D^-1 and the trailing update
+// A22 -= W L21^T as 16 x 16 x 16 FP64 MFMA products. Optionally the inverse
+// Z = L^-1 of the whole block is built alongside (block Gauss transforms):
+// Z[k][j] <- Z11 Z[k][j] for j < k and Z[i][j] -= L[i][k] Z[k][j] for i > k.
+// ---------------------------------------------------------------------------
+
+// acc(16 x 16) += A(16 x 16) B(16 x 16) on one wave; a(i, q) and b(q, j) must
+// load unconditionally (clamped addresses) and mask afterwards. acc[v] holds
+// row 4 v + lane / 16, column lane % 16.
+template <typename FA, typename FB>
+__device__ __forceinline__ d4 wgemm16(FA a, FB b, d4 acc) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  double av[4], bv[4];
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) {
+    av[s4] = a(r, 4 * s4 + q);
+    bv[s4] = b(4 * s4 + q, r);
+  }
+  ISSUE_FENCE();
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) acc = mfma16(av[s4], bv[s4], acc);
+  return acc;
+}
+
+// One wave: LDL^T of the kn x kn (kn <= 16) diagonal block at (k0, k0) of F in
+// place (unit L below the diagonal, D on it) and Zk = L_kk^-1 (16 x 17, unit
+// lower, identity beyond kn). Lane (i = lane % 16, c = lane / 16) holds
+// columns c + 4t; column k and row k of Z travel by shuffles.
+template <int LD>
+__device__ __forceinline__ void ldlt_base16(double* F, int k0, int kn, double* Zk) {
+  const int lane = threadIdx.x & 63, i = lane & 15, c = lane >> 4;
+  double d[4], z[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = c + 4 * t;
+    const double fv = F[(k0 + min(i, kn - 1)) * LD + k0 + min(j, kn - 1)];
+    d[t] = (i < kn && j < kn) ? (j <= i ? fv : 0.0) : (i == j ? 1.0 : 0.0);
+    z[t] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int k = 0; k < kn; ++k) {
+    const int tk = k >> 2, src = 16 * (k & 3);
+    double v = d[0];
+#pragma unroll
+    for (int t = 1; t < 4; ++t) v = (t == tk) ? d[t] : v;
+    const double ci = __shfl(v, src + i, 64);  // D[i][k]
+    const double dk = __shfl(v, src + k, 64);
+    double cj[4], zk[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      cj[t] = __shfl(v, src + c + 4 * t, 64);  // D[j][k]
+      zk[t] = __shfl(z[t], 16 * c + k, 64);    // Z[k][j]
+    }
+    const bool act = i > k && i < kn;
+    const double lik = ci / dk;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j = c + 4 * t;
+      const double nd = d[t] - lik * cj[t];  // D_ij -= l_ik d_k l_jk
+      const double nz = z[t] - lik * zk[t];
+      d[t] = act ? ((j > k && j <= i) ? nd : (j == k ? lik : d[t])) : d[t];
+      z[t] = (act && j <= k) ? nz : z[t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = c + 4 * t;
+    if (i < kn && j <= i) F[(k0 + i) * LD + k0 + j] = d[t];
+    Zk[i * 17 + j] = z[t];
+  }
+}
+
+// Blocked LDL^T of the first ncols columns of an m x m front in F (m <= MB;
+// the calling workgroup has MB / 16 waves). With WITHZ (ncols == m), Zs (same
+// layout as F) receives Z = L^-1. Zk (16 x 17) and Wb (MB x 17) are scratch.
+template <int MB, bool WITHZ>
+__device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zs, double* Zk,
+                                           double* Wb) {
+  constexpr int LD = MB + 1, NW = MB / 16;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int vr = lane >> 4, vc = lane & 15;  // accumulator element (4 v + vr, vc)
+  if (WITHZ) {
+    for (int idx = threadIdx.x; idx < MB * LD; idx += 64 * NW) Zs[idx] = 0.0;
+  }
+  for (int k0 = 0; k0 < ncols; k0 += 16) {
+    const int kn = min(16, ncols - k0), t0 = k0 + kn, kb = k0 >> 4;
+    if (wv == 0) ldlt_base16<LD>(F, k0, kn, Zk);
+    __syncthreads();
+    // panel rows below the block; Z[kb][jb] <- Zk Z[kb][jb] (jb < kb); Z[kb][kb] = Zk
+    const int nrb = (m - t0 + 15) >> 4;
+    const int ntask2 = nrb + (WITHZ ? kb + 1 : 0);
+    for (int task = wv; task < ntask2; task += NW) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      if (task < nrb) {
+        const int r0 = t0 + 16 * task;
+        acc = wgemm16(
+            [&](int ii, int q) {
+              const double x = F[min(r0 + ii, m - 1) * LD + k0 + min(q, kn - 1)];
+              return (r0 + ii < m && q < kn) ? x : 0.0;
+            },
+            [&](int q, int pp) {
+              const double x = Zk[pp * 17 + q];
+              return q < kn ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ii = 4 * v + vr, pp = vc;
+          if (r0 + ii < m && pp < kn) {
+            Wb[(r0 - t0 + ii) * 17 + pp] = acc[v];
+            F[(r0 + ii) * LD + k0 + pp] = acc[v] / F[(k0 + pp) * LD + k0 + pp];
+          }
+        }
+      } else if (task < nrb + kb) {
+        const int jb = task - nrb;
+        acc = wgemm16(
+            [&](int ii, int q) {
+              const double x = Zk[ii * 17 + q];
+              return q < kn ? x : 0.0;
+            },
+            [&](int q, int jj) {
+              const double x = Zs[(k0 + min(q, kn - 1)) * LD + 16 * jb + jj];
+              return q < kn ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ii = 4 * v + vr;
+          if (ii < kn) Zs[(k0 + ii) * LD + 16 * jb + vc] = acc[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ii = 4 * v + vr;
+          if (ii < kn && vc < kn) Zs[(k0 + ii) * LD + k0 + vc] = Zk[ii * 17 + vc];
+        }
+      }
+    }
+    __syncthreads();
+    // trailing update A22 -= W L21^T (lower blocks) and Z[ib][jb] -= L[ib][kb] Z[kb][jb]
+    const int ntb = nrb * (nrb + 1) / 2;
+    const int ntask3 = ntb + (WITHZ ? nrb * (kb + 1) : 0);
+    for (int task = wv; task < ntask3; task += NW) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      if (task < ntb) {
+        int ib = 0, rem = task;
+        while (rem > ib) {
+          rem -= ib + 1;
+          ++ib;
+        }
+        const int jb = rem;
+        const int R = t0 + 16 * ib, C = t0 + 16 * jb;
+        acc = wgemm16(
+            [&](int ii, int pp) {
+              const double x = Wb[(16 * ib + ii) * 17 + min(pp, kn - 1)];
+              return pp < kn ? x : 0.0;
+            },
+            [&](int pp, int jj) {
+              const double x = F[min(C + jj, m - 1) * LD + k0 + min(pp, kn - 1)];
+              return (pp < kn && C + jj < m) ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = R + 4 * v + vr, j = C + vc;
+          if (i < m && j < m && i >= j) F[i * LD + j] -= acc[v];
+        }
+      } else {
+        const int t2 = task - ntb, ib = t2 / (kb + 1), jb = t2 - ib * (kb + 1);
+        const int R = t0 + 16 * ib;
+        acc = wgemm16(
+            [&](int ii, int pp) {
+              const double x = F[min(R + ii, m - 1) * LD + k0 + min(pp, kn - 1)];
+              return pp < kn ? x : 0.0;
+            },
+            [&](int pp, int jj) {
+              const double x = Zs[(k0 + min(pp, kn - 1)) * LD + 16 * jb + jj];
+              return pp < kn ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = R + 4 * v + vr;
+          if (i < m) Zs[i * LD + 16 * jb + vc] -= acc[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Factorisation
 // ---------------------------------------------------------------------------
 
-// Small fronts: one wavefront, front in LDS row-major (ld = m + 1); lane t owns
-// row t. Assembles the input entries and the children's updates, eliminates the
-// w supernode columns, writes the L block and the r x r update.
-__global__ __launch_bounds__(64) void k_factor_small(DevPlan P, const int32_t* __restrict__ list,
-                                                     const double* __restrict__ vals, int64_t ldv,
-                                                     double* __restrict__ L,
-                                                     double* __restrict__ U) {
-  extern __shared__ double F[];
+// Small fronts (m <= MB, MB in {16, 32, 64}), 4 MB threads per front. The front
+// is assembled in LDS (row-major, ld = MB + 1) from the input entries and the
+// children's updates, every global load issued before its use; the w supernode
+// columns are then eliminated with the blocked LDL^T above, and the L block and
+// the lower r x r update are written out.
+template <int MB>
+__global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_t* __restrict__ list,
+                                                         const double* __restrict__ vals, int64_t ldv,
+                                                         double* __restrict__ L,
+                                                         double* __restrict__ U) {
+  constexpr int NT = 4 * MB, LD = MB + 1;
+  __shared__ double F[MB * LD];
+  __shared__ double Zk[16 * 17];
+  __shared__ double Wb[MB * 17];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
-  const int t = threadIdx.x;
+  const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const int ld = m + 1;
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
+  const double* Um = U + (int64_t)b * P.u_size;
   const double* vb = vals + (int64_t)b * ldv;
-  for (int idx = t; idx < m * ld; idx += 64) F[idx] = 0.0;
+  for (int idx = tid; idx < m * LD; idx += NT) F[idx] = 0.0;
   __syncthreads();
-  for (int64_t q = P.fa_ptr[s] + t; q < P.fa_ptr[s + 1]; q += 64) {
-    const int slot = P.fa_slot[q];
-    const int j = slot / m, i = slot - j * m;
-    F[i * ld + j] = vb[P.fa_csc[q]];
+  const int64_t q0 = P.fa_ptr[s], q1 = P.fa_ptr[s + 1];
+  for (int64_t qb = q0; qb < q1; qb += 4 * NT) {
+    int slot[4];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t q = min(qb + tid + NT * u, q1 - 1);
+      slot[u] = P.fa_slot[q];
+      v[u] = vb[P.fa_csc[q]];
+    }
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (qb + tid + NT * u < q1) {
+        const int j = slot[u] / m, i = slot[u] - j * m;
+        F[i * LD + j] = v[u];
+      }
   }
   __syncthreads();
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int c = P.child_idx[cq];
     const int rc = P.sn_r[c];
-    const double* Uc = U + (int64_t)b * P.u_size + P.u_off[c];
+    const double* Uc = Um + P.u_off[c];
     const int32_t* relc = P.rel + P.rows_ptr[c];
-    for (int j = 0; j < rc; ++j) {
-      const int pj = relc[j];
-      for (int i = j + t; i < rc; i += 64) F[relc[i] * ld + pj] += Uc[(int64_t)j * rc + i];
+    const int ne = rc * rc;
+    for (int e0 = 0; e0 < ne; e0 += 4 * NT) {
+      double v[4];
+      int ri[4], rj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + tid + NT * u, ne - 1);
+        const int j = e / rc, i = e - j * rc;
+        v[u] = Uc[e];
+        ri[u] = relc[i];
+        rj[u] = relc[j];
+      }
+      ISSUE_FENCE();
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + tid + NT * u;
+        const int j = e / rc, i = e - j * rc;
+        if (e < ne && i >= j) F[ri[u] * LD + rj[u]] += v[u];
+      }
     }
     __syncthreads();
   }
-  for (int k = 0; k < w; ++k) {
-    const double dk = F[k * ld + k];
-    double lik = 0.0;
-    if (t > k && t < m) {
-      lik = F[t * ld + k] / dk;
-      F[t * ld + k] = lik;
-    }
-    __syncthreads();
-    if (t > k && t < m) {
-      const double y = lik * dk;
-      double* row = F + t * ld;
-      for (int j = k + 1; j <= t; ++j) row[j] -= y * F[j * ld + k];
-    }
-    __syncthreads();
+  front_ldlt<MB, false>(F, m, w, nullptr, Zk, Wb);
+  for (int idx = tid; idx < m * m; idx += NT) {
+    const int j = idx / m, i = idx - j * m;
+    if (j < w)
+      Lb[(int64_t)j * m + i] = (j <= i) ? F[i * LD + j] : 0.0;
+    else if (i >= j)
+      Ub[(int64_t)(j - w) * r + (i - w)] = F[i * LD + j];
   }
-  for (int j = 0; j < w; ++j)
-    for (int i = t; i < m; i += 64) Lb[(int64_t)j * m + i] = (i >= j) ? F[i * ld + j] : 0.0;
-  for (int j = 0; j < r; ++j)
-    for (int i = j + t; i < r; i += 64) Ub[(int64_t)j * r + i] = F[(w + i) * ld + (w + j)];
 }
 
 // ---------------------------------------------------------------------------
@@ -183,9 +442,11 @@ __global__ __launch_bounds__(64) void k_factor_small(DevPlan P, const int32_t* _
 // (supernode, a, b) whose meaning depends on the kernel.
 // ---------------------------------------------------------------------------
 
-// Assembly of one 64x64 tile (tile row a, tile column b, a >= b) of a front:
+// Assembly of one 64x64 tile (tile row I, tile column J, I >= J) of a front:
 // input entries through the dense L-slot map plus every child's update through
-// the child's inverse row map (children in fixed order: deterministic).
+// the child's inverse row map (children in fixed order: deterministic). Thread
+// (row i = tid % 64) owns tile columns tid / 64 + 4u, u < 16; each round of
+// loads covers all 16 elements before any is used.
 __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
                                                      const double* __restrict__ vals, int64_t ldv,
                                                      double* __restrict__ L,
@@ -199,37 +460,64 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
   const double* vb = vals + (int64_t)b * ldv;
   const int32_t* am = P.dense_amap + P.l_off[s];
   const double* Um = U + (int64_t)b * P.u_size;
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
-    const int i = I * kTile + (idx & (kTile - 1)), j = J * kTile + (idx / kTile);
-    if (i >= m || j >= m || i < j) continue;
-    double v = 0.0;
-    if (j < w) {
-      const int e = am[(int64_t)j * m + i];
-      if (e >= 0) v = vb[e];
+  const int i = I * kTile + (threadIdx.x & 63), jb = J * kTile + (threadIdx.x >> 6);
+  const int ic = min(i, m - 1);
+  double v[16];
+  {
+    int e[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int j = jb + 4 * u;
+      e[u] = am[(int64_t)min(j, w - 1) * m + ic];
     }
-    for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
-      const int32_t* inv = P.cinv + P.cinv_off[cq];
-      const int ci = inv[i], cj = inv[j];
-      if (ci >= 0 && cj >= 0) {
-        const int c = P.child_idx[cq];
-        const int rc = P.sn_r[c];
-        v += Um[P.u_off[c] + (int64_t)cj * rc + ci];
-      }
+    double x[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = vb[max(e[u], 0)];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int j = jb + 4 * u;
+      v[u] = (j < w && e[u] >= 0) ? x[u] : 0.0;
     }
-    *front_at(Lb, Ub, m, w, r, i, j) = v;
+  }
+  for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
+    const int32_t* inv = P.cinv + P.cinv_off[cq];
+    const int c = P.child_idx[cq];
+    const int rc = P.sn_r[c];
+    const double* Uc = Um + P.u_off[c];
+    const int ci = inv[ic];
+    int cj[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) cj[u] = inv[min(jb + 4 * u, m - 1)];
+    double x[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const bool ok = ci >= 0 && cj[u] >= 0;
+      x[u] = Uc[ok ? (int64_t)cj[u] * rc + ci : 0];
+    }
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] += (ci >= 0 && cj[u] >= 0) ? x[u] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int j = jb + 4 * u;
+    if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) = v[u];
   }
 }
 
 // Diagonal block of panel step `step` (columns [64 step, 64 step + nb)), one
-// task per front: LDL^T of the nb x nb block in LDS with 256 threads, and, in the
-// same sweep, Z = L11^-1 accumulated from the Gauss transforms
-// Z <- (I - l_k e_k^T) Z. The factorised block goes back to L; Z is kept in the
-// per-panel Z workspace for the panel solve and for selected inversion.
+// task per front: blocked LDL^T of the nb x nb block in LDS and Z = L11^-1. The
+// factorised block goes back to L; Z is kept in the per-panel Z workspace
+// (Zb[q * 64 + i] = Z[i][q]) for the panel solves and selected inversion.
 __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, double* __restrict__ L,
                                                  double* __restrict__ Z) {
-  __shared__ double D[kTile][kTile + 1];   // D[i][j], lower
-  __shared__ double Zs[kTile][kTile + 1];  // Zs[i][j], lower, unit diagonal
+  constexpr int LD = kTile + 1;
+  __shared__ double F[kTile * LD];
+  __shared__ double Zs[kTile * LD];
+  __shared__ double Zk[16 * 17];
+  __shared__ double Wb[kTile * 17];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0];
   const int b = blockIdx.y;
@@ -238,35 +526,27 @@ __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __res
   const int c0 = step * kTile, nb = min(kTile, w - c0);
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int j = idx / nb, i = idx - j * nb;
-    D[i][j] = (i >= j) ? Lb[(int64_t)(c0 + j) * m + c0 + i] : 0.0;
-    Zs[i][j] = (i == j) ? 1.0 : 0.0;
+  const int i = tid & 63, c = tid >> 6;
+  {
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) v[t] = Lb[(int64_t)(c0 + min(c + 4 * t, nb - 1)) * m + c0 + min(i, nb - 1)];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int j = c + 4 * t;
+      F[i * LD + j] = (i < nb && j <= i) ? v[t] : 0.0;
+    }
   }
   __syncthreads();
-  for (int k = 0; k < nb; ++k) {
-    const double dk = D[k][k];
-    for (int i = k + 1 + tid; i < nb; i += 256) D[i][k] /= dk;
-    __syncthreads();
-    // trailing update D[i][j] -= l_ik d_k l_jk (k < j <= i) and Z[i][q] -= l_ik Z[k][q] (q <= k < i)
-    const int rest = nb - k - 1;
-    for (int idx = tid; idx < rest * rest; idx += 256) {
-      const int ii = idx % rest, jj = idx / rest;
-      if (ii >= jj) {
-        const int i = k + 1 + ii, j = k + 1 + jj;
-        D[i][j] -= D[i][k] * dk * D[j][k];
-      }
+  front_ldlt<kTile, true>(F, nb, nb, Zs, Zk, Wb);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int j = c + 4 * t;
+    if (i < nb && j < nb) {
+      Lb[(int64_t)(c0 + j) * m + c0 + i] = (j <= i) ? F[i * LD + j] : 0.0;
+      Zb[j * kTile + i] = (j <= i) ? Zs[i * LD + j] : 0.0;
     }
-    for (int idx = tid; idx < rest * (k + 1); idx += 256) {
-      const int i = k + 1 + idx % rest, q = idx / rest;
-      Zs[i][q] -= D[i][k] * Zs[k][q];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int j = idx / nb, i = idx - j * nb;
-    Lb[(int64_t)(c0 + j) * m + c0 + i] = (i >= j) ? D[i][j] : 0.0;
-    Zb[j * kTile + i] = (i >= j) ? Zs[i][j] : 0.0;
   }
 }
 
@@ -293,10 +573,14 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
   mma_tile(
       [&](int ii, int q) {
         const int i = R0 + ii;
-        return (i < m && q < nb) ? Lb[(int64_t)(c0 + q) * m + i] : 0.0;
+        const double v = Lb[(int64_t)(c0 + q) * m + min(i, m - 1)];
+        return i < m ? v : 0.0;
       },
-      [&](int pp, int q) { return (pp < nb && q < nb) ? Zb[q * kTile + pp] * dinv[pp] : 0.0; }, nb, As,
-      Bs, acc);
+      [&](int pp, int q) {
+        const double v = Zb[q * kTile + pp] * dinv[pp];
+        return pp < nb ? v : 0.0;
+      },
+      nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int pp, double v) {
     const int i = R0 + ii;
     if (i < m && pp < nb) Lb[(int64_t)(c0 + pp) * m + i] = v;
@@ -327,11 +611,13 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
   mma_tile(
       [&](int ii, int p) {
         const int i = R0 + ii;
-        return (i < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + i] * dpiv[p] : 0.0;
+        const double v = Lb[(int64_t)(c0 + p) * m + min(i, m - 1)] * dpiv[p];
+        return i < m ? v : 0.0;
       },
       [&](int jj, int p) {
         const int j = C0 + jj;
-        return (j < m && p < nb) ? Lb[(int64_t)(c0 + p) * m + j] : 0.0;
+        const double v = Lb[(int64_t)(c0 + p) * m + min(j, m - 1)];
+        return j < m ? v : 0.0;
       },
       nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int jj, double v) {
@@ -343,34 +629,50 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
 // log|Q| = sum log|D_j| per matrix, plus the pivot status: an exact zero pivot
 // (the numeric phase fails: NotPositiveDefinite) or |D_j| < 1e-14 / non-finite
 // (Singular), as crates/inla_math/src/sparse_ldlt.rs:104-149,217-230.
-__global__ __launch_bounds__(256) void k_logdet(DevPlan P, const double* __restrict__ L,
-                                                double* __restrict__ logdet,
-                                                int32_t* __restrict__ status) {
-  __shared__ double red[256];
-  __shared__ int flags[256];
-  const int b = blockIdx.x;
+__global__ __launch_bounds__(1024) void k_logdet(DevPlan P, const double* __restrict__ L,
+                                                 double* __restrict__ logdet,
+                                                 int32_t* __restrict__ status) {
+  __shared__ double red[16];
+  __shared__ int flags[16];
+  const int b = blockIdx.x, tid = threadIdx.x;
   const double* Lm = L + (int64_t)b * P.l_size;
   double acc = 0.0;
   int fl = 0;
-  for (int j = threadIdx.x; j < P.n; j += 256) {
-    const double d = Lm[P.diag_slot[j]];
-    if (d == 0.0) fl |= GMRF_ST_ZERO_PIVOT;
-    if (!(fabs(d) >= 1e-14) || !isfinite(d)) fl |= GMRF_ST_SINGULAR;
-    acc += log(fabs(d));
+  for (int j0 = 0; j0 < P.n; j0 += 8 * 1024) {
+    int64_t sl[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl[u] = P.diag_slot[min(j0 + tid + 1024 * u, P.n - 1)];
+    double d[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[u] = Lm[sl[u]];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + tid + 1024 * u < P.n) {
+        if (d[u] == 0.0) fl |= GMRF_ST_ZERO_PIVOT;
+        if (!(fabs(d[u]) >= 1e-14) || !isfinite(d[u])) fl |= GMRF_ST_SINGULAR;
+        acc += log(fabs(d[u]));
+      }
   }
-  red[threadIdx.x] = acc;
-  flags[threadIdx.x] = fl;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_xor(acc, o, 64);
+    fl |= __shfl_xor(fl, o, 64);
+  }
+  if ((tid & 63) == 0) {
+    red[tid >> 6] = acc;
+    flags[tid >> 6] = fl;
+  }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      red[threadIdx.x] += red[threadIdx.x + o];
-      flags[threadIdx.x] |= flags[threadIdx.x + o];
+  if (tid == 0) {
+    double t = 0.0;
+    int f = 0;
+    for (int q = 0; q < 16; ++q) {
+      t += red[q];
+      f |= flags[q];
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    logdet[b] = red[0];
-    status[b] = flags[0];
+    logdet[b] = t;
+    status[b] = f;
   }
 }
 
@@ -474,6 +776,143 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
   for (int j = tid; j < w; j += blockDim.x) Xb[a0 + j] = t[j];
 }
 
+// Large fronts, blocked by the factorisation's 64-column panels with the stored
+// Z = L_PP^-1: y_P = Z f_P, then f_B -= L[B,P] y_P for the rows below. Loads are
+// issued in groups of 16 before use.
+__global__ __launch_bounds__(256) void k_solve_fwd_large(DevPlan P, const int32_t* __restrict__ list,
+                                                         const double* __restrict__ L,
+                                                         const double* __restrict__ Z,
+                                                         double* __restrict__ X,
+                                                         double* __restrict__ V) {
+  extern __shared__ double f[];
+  __shared__ double part[4][kTile];
+  __shared__ double y[kTile];
+  const int s = list[blockIdx.x];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
+  double* Xb = X + (int64_t)b * P.n;
+  for (int i = tid; i < m; i += 256) f[i] = i < w ? Xb[a0 + i] : 0.0;
+  __syncthreads();
+  for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
+    const int c = P.child_idx[cq];
+    const int rc = P.sn_r[c];
+    const double* Vc = V + (int64_t)b * P.v_size + P.v_off[c];
+    const int32_t* relc = P.rel + P.rows_ptr[c];
+    for (int i = tid; i < rc; i += 256) f[relc[i]] += Vc[i];
+    __syncthreads();
+  }
+  const int i = tid & 63, pt = tid >> 6;
+  for (int c0 = 0; c0 < w; c0 += kTile) {
+    const int nb = min(kTile, w - c0), c1 = c0 + nb;
+    const double* Zp = Zb + (int64_t)c0 * kTile;
+    double zv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) zv[u] = Zp[(16 * pt + u) * kTile + i];
+    ISSUE_FENCE();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int q = 16 * pt + u;
+      const double prod = zv[u] * f[c0 + min(q, nb - 1)];
+      acc += (q < nb && i < nb) ? prod : 0.0;
+    }
+    part[pt][i] = acc;
+    __syncthreads();
+    if (tid < nb) {
+      const double v = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+      y[tid] = v;
+      f[c0 + tid] = v;
+    }
+    __syncthreads();
+    for (int row = c1 + tid; row < m; row += 256) {
+      double a = 0.0;
+      for (int q0 = 0; q0 < nb; q0 += 16) {
+        double lv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(q0 + u, nb - 1)) * m + row];
+        ISSUE_FENCE();
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a += (q0 + u < nb) ? lv[u] * y[min(q0 + u, nb - 1)] : 0.0;
+      }
+      f[row] -= a;
+    }
+    __syncthreads();
+  }
+  double* Vs = V + (int64_t)b * P.v_size + P.v_off[s];
+  for (int k = tid; k < m; k += 256) {
+    if (k < w)
+      Xb[a0 + k] = f[k];
+    else
+      Vs[k - w] = f[k];
+  }
+}
+
+// Backward on large fronts, panels right to left:
+// t_P = z_P - L[B,P]^T x_B, x_P = Z^T t_P. Wave wv owns panel columns
+// q = wv + 4u (u < 16) and keeps their 16 partial dot products in registers.
+__global__ __launch_bounds__(256) void k_solve_bwd_large(DevPlan P, const int32_t* __restrict__ list,
+                                                         const double* __restrict__ L,
+                                                         const double* __restrict__ Z,
+                                                         double* __restrict__ X) {
+  extern __shared__ double xf[];
+  __shared__ double tq[kTile];
+  const int s = list[blockIdx.x];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
+  double* Xb = X + (int64_t)b * P.n;
+  const int32_t* rows = P.rows + P.rows_ptr[s];
+  for (int k = tid; k < m; k += 256) xf[k] = k < w ? Xb[a0 + k] : Xb[rows[k - w]];
+  __syncthreads();
+  const int np = (w + kTile - 1) / kTile;
+  for (int pk = np - 1; pk >= 0; --pk) {
+    const int c0 = pk * kTile, nb = min(kTile, w - c0), c1 = c0 + nb;
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    for (int k = c1 + lane; k < m + 63; k += 64) {
+      const int kc = min(k, m - 1);
+      const double xk = k < m ? xf[kc] : 0.0;
+      double lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(wv + 4 * u, nb - 1)) * m + kc];
+      ISSUE_FENCE();
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[u] += lv[u] * xk;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      double a = acc[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      const int q = wv + 4 * u;
+      if (lane == 0 && q < nb) tq[q] = xf[c0 + q] - a;
+    }
+    __syncthreads();
+    const double* Zp = Zb + (int64_t)pk * kTile * kTile;
+    double zv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) zv[u] = Zp[(wv + 4 * u) * kTile + lane];
+    ISSUE_FENCE();
+    const double tl = lane < nb ? tq[lane] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      double a = lane < nb ? zv[u] * tl : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      const int q = wv + 4 * u;
+      if (lane == 0 && q < nb) xf[c0 + q] = a;
+    }
+    __syncthreads();
+  }
+  for (int k = tid; k < w; k += 256) Xb[a0 + k] = xf[k];
+}
+
 // ---------------------------------------------------------------------------
 // Selected inversion (roots to leaves). For supernode s with parent p, the
 // entries Sigma_{R_s R_s} are gathered from p's front (p's S block and p's
@@ -483,67 +922,143 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
 // 1973; the same recurrence as crates/inla_math/src/sparse_ldlt.rs:376-419,
 // evaluated densely inside the front).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sigma_parent(const DevPlan& P, const double* Sm, const double* Um,
-                                               const double* Wm, int p, int pi, int pj) {
-  const int wp = P.sn_start[p + 1] - P.sn_start[p];
-  const int mp = wp + P.sn_r[p];
-  if (P.is_large[p]) return Wm[P.w_off[p] + (int64_t)pj * mp + pi];
-  if (pj < wp) return Sm[P.l_off[p] + (int64_t)pj * mp + pi];
-  const int rp = P.sn_r[p];
-  return Um[P.u_off[p] + (int64_t)(pj - wp) * rp + (pi - wp)];
-}
 
-__global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* __restrict__ list,
-                                                     const double* __restrict__ L,
-                                                     double* __restrict__ S,
-                                                     double* __restrict__ U,
-                                                     const double* __restrict__ W) {
-  extern __shared__ double sh[];
+// Small fronts (m <= MB), 4 MB threads: thread (row i = tid % MB, class c)
+// holds Sigma[i][c + 4t] (both triangles) in registers, starting from
+// Sigma_{R R} gathered from the parent; the L block sits in LDS (column k
+// contiguous). Column k (right to left): x_i = sum_{l > k} Sigma[i][l] L[l][k]
+// from per-class partials, Sigma[i][k] = Sigma[k][i] = -x_i and
+// Sigma[k][k] = 1 / D_k + sum_l L[l][k] x_l; one barrier per column.
+template <int MB>
+__global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_t* __restrict__ list,
+                                                         const double* __restrict__ L,
+                                                         double* __restrict__ S,
+                                                         double* __restrict__ U,
+                                                         const double* __restrict__ W) {
+  constexpr int NT = 4 * MB, NJ = MB / 4, NW = NT / 64;
+  __shared__ double Ls[MB * MB];
+  __shared__ double part[2][4][MB];
+  __shared__ double wsum[2][NW];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
-  const int t = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const int ld = m + 1;
-  double* Sg = sh;             // m x (m+1)
-  double* Lc = sh + m * ld;    // m
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Sm = S + (int64_t)b * P.l_size;
   double* Um = U + (int64_t)b * P.u_size;
   const double* Wm = W + (int64_t)b * P.w_size;
+  const int i = tid % MB, c = tid / MB;
+  // L block into LDS
+  const int nl = m * w;
+  for (int e0 = 0; e0 < nl; e0 += 8 * NT) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = Lb[min(e0 + tid + NT * u, nl - 1)];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + tid + NT * u;
+      const int k = e / m, l = e - k * m;
+      if (e < nl) Ls[k * MB + l] = v[u];
+    }
+  }
+  // Sigma_{R R} from the parent front (its S block and Sigma_RR workspace, or
+  // its dense Sigma front when the parent is large); lower part also kept in
+  // this front's U workspace for the children.
+  double sg[NJ];
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) sg[t] = 0.0;
   if (r > 0) {
     const int p = P.sn_parent[s];
+    const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
+    const bool plarge = P.is_large[p] != 0;
+    const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off[p];
     const int32_t* rl = P.rel + P.rows_ptr[s];
-    double* Us = Um + P.u_off[s];
-    for (int j = 0; j < r; ++j) {
-      const int pj = rl[j];
-      for (int i = j + t; i < r; i += 64) {
-        const double v = sigma_parent(P, Sm, Um, Wm, p, rl[i], pj);
-        Sg[(w + i) * ld + w + j] = v;
-        Sg[(w + j) * ld + w + i] = v;
-        Us[(int64_t)j * r + i] = v;
+    const int ri = rl[min(max(i - w, 0), r - 1)];
+    int rj[NJ];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) rj[t] = rl[min(max(c + 4 * t - w, 0), r - 1)];
+    double v[NJ];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const int pi = max(ri, rj[t]), pj = min(ri, rj[t]);
+      const double* base;
+      int64_t off;
+      if (plarge) {
+        base = Wm;
+        off = woff + (int64_t)pj * mp + pi;
+      } else if (pj < wp) {
+        base = Sm;
+        off = loff + (int64_t)pj * mp + pi;
+      } else {
+        base = Um;
+        off = uoff + (int64_t)(pj - wp) * rp + (pi - wp);
       }
+      v[t] = base[off];
+    }
+    ISSUE_FENCE();
+    double* Us = Um + P.u_off[s];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const int j = c + 4 * t;
+      const bool ok = i >= w && i < m && j >= w && j < m;
+      sg[t] = ok ? v[t] : 0.0;
+      if (ok && i >= j) Us[(int64_t)(j - w) * r + (i - w)] = v[t];
     }
   }
   __syncthreads();
   for (int k = w - 1; k >= 0; --k) {
-    if (t > k && t < m) Lc[t] = Lb[(int64_t)k * m + t];
-    __syncthreads();
-    if (t > k && t < m) {
-      double acc = 0.0;
-      const double* row = Sg + t * ld;
-      for (int l = k + 1; l < m; ++l) acc += row[l] * Lc[l];
-      Sg[t * ld + k] = -acc;
-      Sg[k * ld + t] = -acc;
+    const int buf = k & 1;
+    const double* Lk = Ls + k * MB;
+    double lk[NJ];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) lk[t] = Lk[c + 4 * t];
+    const double li = Lk[i];
+    ISSUE_FENCE();
+    double pacc = 0.0;
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const int l = c + 4 * t;
+      const double prod = sg[t] * lk[t];
+      pacc += (l > k && l < m) ? prod : 0.0;
     }
+    part[buf][c][i] = pacc;
+    double qp = (i > k && i < m) ? li * pacc : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) qp += __shfl_xor(qp, o, 64);
+    if (lane == 0) wsum[buf][wv] = qp;
     __syncthreads();
-    double prod = (t > k && t < m) ? Lc[t] * Sg[t * ld + k] : 0.0;
-    for (int o = 32; o > 0; o >>= 1) prod += __shfl_xor(prod, o, 64);
-    if (t == 0) Sg[k * ld + k] = 1.0 / Lb[(int64_t)k * m + k] - prod;
-    __syncthreads();
+    const double xi = part[buf][0][i] + part[buf][1][i] + part[buf][2][i] + part[buf][3][i];
+    if (i == k) {
+      double tot = 0.0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) tot += wsum[buf][q];
+      const double skk = 1.0 / Lk[k] + tot;
+      double xj[NJ];
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) {
+        const int j = c + 4 * t;
+        xj[t] = part[buf][0][j] + part[buf][1][j] + part[buf][2][j] + part[buf][3][j];
+      }
+      ISSUE_FENCE();
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) {
+        const int j = c + 4 * t;
+        sg[t] = (j > k && j < m) ? -xj[t] : (j == k ? skk : sg[t]);
+      }
+    } else if (i > k && i < m) {
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) sg[t] = (c + 4 * t == k) ? -xi : sg[t];
+    }
   }
   double* Sb = Sm + P.l_off[s];
-  for (int j = 0; j < w; ++j)
-    for (int i = t; i < m; i += 64) Sb[(int64_t)j * m + i] = (i >= j) ? Sg[i * ld + j] : 0.0;
+  if (i < m) {
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const int j = c + 4 * t;
+      if (j < w) Sb[(int64_t)j * m + i] = (j <= i) ? sg[t] : 0.0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,27 +1071,63 @@ __global__ __launch_bounds__(64) void k_selinv_small(DevPlan P, const int32_t* _
 // (the block form of the Takahashi recurrence).
 // ---------------------------------------------------------------------------
 
-// Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into W (both triangles).
+// Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into W (both triangles; the
+// transposed copy is staged through LDS so both stores are coalesced).
 __global__ __launch_bounds__(256) void k_sl_gather(DevPlan P, const int32_t* __restrict__ tasks,
                                                    const double* __restrict__ S,
                                                    const double* __restrict__ U,
                                                    double* __restrict__ W) {
+  __shared__ double Ct[kTile][kTile + 1];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1], J = tk[2];
   const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const int p = P.sn_parent[s];
+  const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
+  const bool plarge = P.is_large[p] != 0;
+  const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off[p];
   const int32_t* rl = P.rel + P.rows_ptr[s];
   const double* Sm = S + (int64_t)b * P.l_size;
   const double* Um = U + (int64_t)b * P.u_size;
   double* Wm = W + (int64_t)b * P.w_size;
   double* Ws = Wm + P.w_off[s];
+  const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
+  const int i = I * kTile + ii;
+  const int ri = rl[min(i, r - 1)];
+  int rj[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) rj[u] = rl[min(J * kTile + jb + 4 * u, r - 1)];
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int pi = max(ri, rj[u]), pj = min(ri, rj[u]);
+    const double* base;
+    int64_t off;
+    if (plarge) {
+      base = Wm;
+      off = woff + (int64_t)pj * mp + pi;
+    } else if (pj < wp) {
+      base = Sm;
+      off = loff + (int64_t)pj * mp + pi;
+    } else {
+      base = Um;
+      off = uoff + (int64_t)(pj - wp) * rp + (pi - wp);
+    }
+    v[u] = base[off];
+  }
+  ISSUE_FENCE();
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int jj = jb + 4 * u, j = J * kTile + jj;
+    Ct[jj][ii] = v[u];
+    if (i < r && j < r && i >= j) Ws[(int64_t)(w + j) * m + (w + i)] = v[u];
+  }
+  __syncthreads();
+  // W[w + j][w + i] for the same (i >= j): consecutive lanes take consecutive j
   for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
-    const int i = I * kTile + (idx & (kTile - 1)), j = J * kTile + (idx / kTile);
-    if (i >= r || j >= r || i < j) continue;
-    const double v = sigma_parent(P, Sm, Um, Wm, p, rl[i], rl[j]);
-    Ws[(int64_t)(w + j) * m + (w + i)] = v;
-    Ws[(int64_t)(w + i) * m + (w + j)] = v;
+    const int jj = idx & 63, i2 = idx >> 6;
+    const int iq = I * kTile + i2, jq = J * kTile + jj;
+    if (iq < r && jq < r && iq >= jq) Ws[(int64_t)(w + iq) * m + (w + jq)] = Ct[jj][i2];
   }
 }
 
@@ -610,12 +1161,17 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
   double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
   TileAcc acc;
   tile_zero(acc);
-  mma_tile(
+  mma_tile<false, true>(
       [&](int ii, int q) {
         const int i = R0 + ii;
-        return (i < m && q < nb) ? Lb[(int64_t)(c0 + q) * m + i] : 0.0;
+        const double v = Lb[(int64_t)(c0 + q) * m + min(i, m - 1)];
+        return i < m ? v : 0.0;
       },
-      [&](int pp, int q) { return (pp < nb && q < nb) ? Zb[pp * kTile + q] : 0.0; }, nb, As, Bs, acc);
+      [&](int pp, int q) {
+        const double v = Zb[pp * kTile + q];
+        return pp < nb ? v : 0.0;
+      },
+      nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int pp, double v) {
     const int i = R0 + ii;
     if (i < m && pp < nb) Tb[(int64_t)(c0 + pp) * m + i] = v;
@@ -624,13 +1180,15 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
 
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
 // -sum_{k in B} W[i][k] Lh[k][p] (K = m - c1) and stores it into S (lower) and
-// both triangles of W.
+// both triangles of W (the transposed copy staged through LDS so that both
+// stores are coalesced).
 __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, double* __restrict__ S,
                                                  double* __restrict__ W,
                                                  const double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
+  double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1];
   const int b = blockIdx.y;
@@ -644,31 +1202,47 @@ __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __res
   const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
   TileAcc acc;
   tile_zero(acc);
-  mma_tile(
+  mma_tile<false, true>(
       [&](int ii, int kk) {
         const int i = R0 + ii;
-        return i < m ? Ws[(int64_t)(c1 + kk) * m + i] : 0.0;
+        const double v = Ws[(int64_t)(c1 + kk) * m + min(i, m - 1)];
+        return i < m ? v : 0.0;
       },
-      [&](int jj, int kk) { return jj < nb ? Tb[(int64_t)(c0 + jj) * m + c1 + kk] : 0.0; }, m - c1,
-      As, Bs, acc);
+      [&](int jj, int kk) {
+        const double v = Tb[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk];
+        return jj < nb ? v : 0.0;
+      },
+      m - c1, As, Bs, acc);
+  // the tile engine ends with a barrier, so its LDS can hold the tile now
+  double (*Ct)[kTile + 1] = (double (*)[kTile + 1])sh;
   tile_foreach(acc, [&](int ii, int jj, double v) {
     const int i = R0 + ii;
+    Ct[jj][ii] = -v;
     if (i < m && jj < nb) {
       Sb[(int64_t)(c0 + jj) * m + i] = -v;
       Ws[(int64_t)(c0 + jj) * m + i] = -v;
-      Ws[(int64_t)i * m + c0 + jj] = -v;
     }
   });
+  __syncthreads();
+  // W[c0 + jj][i]: consecutive lanes take consecutive jj
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+    const int jj = idx & 63, ii = idx >> 6;
+    const int i = R0 + ii;
+    if (i < m && jj < nb) Ws[(int64_t)i * m + c0 + jj] = Ct[jj][ii];
+  }
 }
 
-// Sigma[P,P]: task (s) = Z^T D^-1 Z - Lh^T Sigma[B,P] (MFMA, K = m - c1);
-// lower part into S, both triangles into W.
+// Sigma[P,P]: task (s) = Z^T D^-1 Z - Lh^T Sigma[B,P], both products on MFMA
+// (K = nb, then K = m - c1); lower part into S, both triangles into W. The
+// strict upper triangle of S's diagonal blocks is not part of the layout's
+// contract and is left as is.
 __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restrict__ tasks,
                                                int step, const double* __restrict__ L,
                                                const double* __restrict__ Z,
                                                double* __restrict__ S, double* __restrict__ W,
                                                const double* __restrict__ T) {
-  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
   __shared__ double dinv[kTile];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0];
@@ -683,39 +1257,43 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
-  double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
-  double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
+  if (tid < kTile) {
+    const int q = min(tid, nb - 1);
+    dinv[tid] = 1.0 / Lb[(int64_t)(c0 + q) * m + c0 + q];
+  }
+  __syncthreads();
   TileAcc acc;
   tile_zero(acc);
+  // (Z^T D^-1 Z)[p][q] = sum_k Z[k][p] Z[k][q] / d_k, Zb[q * 64 + k] = Z[k][q]
+  mma_tile<true, true>(
+      [&](int pp, int k) {
+        const double x = Zb[pp * kTile + k] * dinv[k];
+        return pp < nb ? x : 0.0;
+      },
+      [&](int qq, int k) {
+        const double x = Zb[qq * kTile + k];
+        return qq < nb ? x : 0.0;
+      },
+      nb, As, Bs, acc);
   if (m > c1)
-    mma_tile([&](int ii, int kk) { return ii < nb ? Tb[(int64_t)(c0 + ii) * m + c1 + kk] : 0.0; },
-             [&](int jj, int kk) { return jj < nb ? Ws[(int64_t)(c0 + jj) * m + c1 + kk] : 0.0; },
-             m - c1, As, Bs, acc);
-  __syncthreads();
-  // Zt[q][k] = Z[k][q] in the LDS freed by the tile engine
-  double (*Zt)[kTile + 1] = (double (*)[kTile + 1])sh;
-  for (int idx = tid; idx < kTile * kTile; idx += 256) {
-    const int q = idx / kTile, k = idx - q * kTile;
-    Zt[q][k] = (q < nb && k < nb) ? Zb[q * kTile + k] : 0.0;
-  }
-  if (tid < nb) dinv[tid] = 1.0 / Lb[(int64_t)(c0 + tid) * m + c0 + tid];
-  __syncthreads();
+    mma_tile<true, true>(
+        [&](int ii, int kk) {
+          const double v = Tb[(int64_t)(c0 + min(ii, nb - 1)) * m + c1 + kk];
+          return ii < nb ? -v : 0.0;
+        },
+        [&](int jj, int kk) {
+          const double v = Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk];
+          return jj < nb ? v : 0.0;
+        },
+        m - c1, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int jj, double v) {
     // tile row ii = p, tile column jj = q; lower part p >= q
     if (ii < nb && jj < nb && ii >= jj) {
-      double g = 0.0;
-      for (int k = ii; k < nb; ++k) g += Zt[ii][k] * Zt[jj][k] * dinv[k];
-      const double val = g - v;
-      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = val;
-      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = val;
-      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = val;
+      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
     }
   });
-  if (c0 == 0)  // strictly upper part of the diagonal block: unused, kept zero
-    for (int idx = tid; idx < w * w; idx += 256) {
-      const int j = idx / w, i = idx - j * w;
-      if (i < j) Sb[(int64_t)j * m + i] = 0.0;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -755,8 +1333,14 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
                                const double* vals, int64_t ldv, double* L, double* U,
                                hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  size_t sh = (size_t)maxm * (maxm + 1) * sizeof(double);
-  hipLaunchKernelGGL(k_factor_small, dim3(count, B), dim3(64), sh, st, P, list, vals, ldv, L, U);
+  if (maxm <= 16)
+    hipLaunchKernelGGL(k_factor_small<16>, dim3(count, B), dim3(64), 0, st, P, list, vals, ldv, L, U);
+  else if (maxm <= 32)
+    hipLaunchKernelGGL(k_factor_small<32>, dim3(count, B), dim3(128), 0, st, P, list, vals, ldv, L, U);
+  else if (maxm <= 64)
+    hipLaunchKernelGGL(k_factor_small<64>, dim3(count, B), dim3(256), 0, st, P, list, vals, ldv, L, U);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -790,7 +1374,7 @@ hipError_t launch_fl_update(const DevPlan& P, const int32_t* tasks, int count, i
 
 hipError_t launch_logdet(const DevPlan& P, int B, const double* L, double* logdet, int32_t* status,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_logdet, dim3(B), dim3(256), 0, st, P, L, logdet, status);
+  hipLaunchKernelGGL(k_logdet, dim3(B), dim3(1024), 0, st, P, L, logdet, status);
   return hipGetLastError();
 }
 
@@ -829,12 +1413,46 @@ hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, in
   return hipGetLastError();
 }
 
+// Dynamic LDS beyond 64 KiB has to be requested per kernel (gfx950: 160 KiB per CU).
+static hipError_t lds_limit(const void* fn, size_t bytes) {
+  if (bytes > 160 * 1024) return hipErrorInvalidValue;
+  if (bytes <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_solve_fwd_large(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
+                                  const double* L, const double* Z, double* X, double* V,
+                                  hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const size_t sh = (size_t)maxm * sizeof(double);
+  hipError_t e = lds_limit((const void*)k_solve_fwd_large, sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_solve_fwd_large, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve_bwd_large(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
+                                  const double* L, const double* Z, double* X, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const size_t sh = (size_t)maxm * sizeof(double);
+  hipError_t e = lds_limit((const void*)k_solve_bwd_large, sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_solve_bwd_large, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X);
+  return hipGetLastError();
+}
+
 hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
                                const double* L, double* S, double* U, const double* W,
                                hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  size_t sh = ((size_t)maxm * (maxm + 1) + maxm) * sizeof(double);
-  hipLaunchKernelGGL(k_selinv_small, dim3(count, B), dim3(64), sh, st, P, list, L, S, U, W);
+  if (maxm <= 16)
+    hipLaunchKernelGGL(k_selinv_small<16>, dim3(count, B), dim3(64), 0, st, P, list, L, S, U, W);
+  else if (maxm <= 32)
+    hipLaunchKernelGGL(k_selinv_small<32>, dim3(count, B), dim3(128), 0, st, P, list, L, S, U, W);
+  else if (maxm <= 64)
+    hipLaunchKernelGGL(k_selinv_small<64>, dim3(count, B), dim3(256), 0, st, P, list, L, S, U, W);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

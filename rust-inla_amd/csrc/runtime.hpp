@@ -42,6 +42,10 @@ hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const
                             double*, hipStream_t);
 hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
                             hipStream_t);
+hipError_t launch_solve_fwd_large(const DevPlan&, const int32_t*, int, int, int, const double*,
+                                  const double*, double*, double*, hipStream_t);
+hipError_t launch_solve_bwd_large(const DevPlan&, const int32_t*, int, int, int, const double*,
+                                  const double*, double*, hipStream_t);
 hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                double*, double*, const double*, hipStream_t);
 hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
@@ -157,12 +161,14 @@ struct DevArena {
 };
 
 struct LevelInfo {
-  int64_t sb, se;     // small fronts [sb, se) in small list
+  int64_t sb, se;     // small fronts [sb, se) in small list, ordered by size class
+  int64_t cls_b[4];   // class q (m <= 16 << q) covers [cls_b[q], cls_b[q + 1])
+  double fl_cls[3] = {0, 0, 0}, sel_cls[3] = {0, 0, 0}, by_cls[3] = {0, 0, 0};
   int64_t lb, le;     // large fronts [lb, le) in large list
   int64_t ab, ae;     // all fronts [ab, ae) in all list
-  int32_t maxm_small, maxm_all;
+  int32_t maxm_small, maxm_large, maxm_all;
   // algorithmic work of the level's launches (per matrix)
-  double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol = 0;
+  double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol_small = 0, sol_large = 0;
   double by_small = 0, by_large = 0;
   // large-front task ranges (triples in SymData::d_tasks)
   struct Range {
@@ -279,7 +285,9 @@ struct SymData {
       L.lb = (int64_t)large.size();
       L.ab = (int64_t)all.size();
       L.maxm_small = 0;
+      L.maxm_large = 0;
       L.maxm_all = 0;
+      std::vector<int32_t> cls[3];
       for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
         int64_t s = S.level_sn[q];
         int32_t m = (int32_t)((S.sn_start[s + 1] - S.sn_start[s]) + sn_r[s]);
@@ -294,20 +302,31 @@ struct SymData {
         }
         const int64_t w = S.sn_start[s + 1] - S.sn_start[s];
         const double by = 8.0 * (double)m * (double)w + 12.0 * (double)(fa_ptr[s + 1] - fa_ptr[s]);
-        L.sol += sol;
         if (m <= kSmallFront) {
-          small.push_back((int32_t)s);
+          const int c = m <= 16 ? 0 : (m <= 32 ? 1 : 2);
+          cls[c].push_back((int32_t)s);
+          L.sol_small += sol;
           L.maxm_small = std::max(L.maxm_small, m);
           L.fl_small += fl;
           L.sel_small += sel;
           L.by_small += by;
+          L.fl_cls[c] += fl;
+          L.sel_cls[c] += sel;
+          L.by_cls[c] += by;
         } else {
           large.push_back((int32_t)s);
+          L.sol_large += sol;
+          L.maxm_large = std::max(L.maxm_large, m);
           L.fl_large += fl;
           L.sel_large += sel;
           L.by_large += by;
         }
       }
+      for (int c = 0; c < 3; ++c) {
+        L.cls_b[c] = (int64_t)small.size();
+        small.insert(small.end(), cls[c].begin(), cls[c].end());
+      }
+      L.cls_b[3] = (int64_t)small.size();
       L.se = (int64_t)small.size();
       L.le = (int64_t)large.size();
       L.ae = (int64_t)all.size();
@@ -567,11 +586,12 @@ struct gmrf_batch {
     double* Uf = U + (int64_t)first * P.u_size;
     double* Zf = Z + (int64_t)first * P.z_size;
     for (const LevelInfo& lv : sym->levels) {
-      if (lv.se > lv.sb)
-        prof_launch("k_factor_small", lv.fl_small * count, lv.by_small * count, stream, [&] {
-          return launch_factor_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
-                                     count, vals, ldv, Lf, Uf, stream);
-        });
+      for (int c = 0; c < 3; ++c)
+        if (lv.cls_b[c + 1] > lv.cls_b[c])
+          prof_launch("k_factor_small", lv.fl_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
+            return launch_factor_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
+                                       16 << c, count, vals, ldv, Lf, Uf, stream);
+          });
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
         prof_launch("k_fl_assemble", 0.0, lv.asm_t.bytes * count, stream, [&] {
@@ -600,21 +620,35 @@ struct gmrf_batch {
   void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0) {
     const DevPlan& P = sym->plan;
     const double* Lf = L + (int64_t)first * P.l_size;
+    const double* Zf = Z + (int64_t)first * P.z_size;
     double* Xf = X + (int64_t)first * P.n;
     double* Vf = V + (int64_t)first * std::max<int64_t>(P.v_size, 1);
     check(launch_gather_perm(P, count, rhs, ldr, Xf, stream), "gather");
-    for (const LevelInfo& lv : sym->levels)
-      prof_launch("k_solve_fwd", 0.5 * lv.sol * count, 0.0, stream, [&] {
-        return launch_solve_fwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, Lf,
-                                Xf, Vf, stream);
-      });
+    for (const LevelInfo& lv : sym->levels) {
+      if (lv.se > lv.sb)
+        prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count, 0.0, stream, [&] {
+          return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
+                                  Lf, Xf, Vf, stream);
+        });
+      if (lv.le > lv.lb)
+        prof_launch("k_solve_fwd_large", 0.5 * lv.sol_large * count, 0.0, stream, [&] {
+          return launch_solve_fwd_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxm_large,
+                                        count, Lf, Zf, Xf, Vf, stream);
+        });
+    }
     check(launch_diag_scale(P, count, Lf, Xf, stream), "diag_scale");
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
-      prof_launch("k_solve_bwd", 0.5 * lv.sol * count, 0.0, stream, [&] {
-        return launch_solve_bwd(P, sym->d_all + lv.ab, (int)(lv.ae - lv.ab), lv.maxm_all, count, Lf,
-                                Xf, stream);
-      });
+      if (lv.le > lv.lb)
+        prof_launch("k_solve_bwd_large", 0.5 * lv.sol_large * count, 0.0, stream, [&] {
+          return launch_solve_bwd_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxm_large,
+                                        count, Lf, Zf, Xf, stream);
+        });
+      if (lv.se > lv.sb)
+        prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count, 0.0, stream, [&] {
+          return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
+                                  Lf, Xf, stream);
+        });
     }
     check(launch_scatter_perm(P, count, Xf, x, ldx, stream), "scatter");
   }
@@ -650,11 +684,12 @@ struct gmrf_batch {
           });
         }
       }
-      if (lv.se > lv.sb)
-        prof_launch("k_selinv_small", lv.sel_small * count, lv.by_small * count, stream, [&] {
-          return launch_selinv_small(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small,
-                                     count, Lf, Sf, Uf, Wf, stream);
-        });
+      for (int c = 0; c < 3; ++c)
+        if (lv.cls_b[c + 1] > lv.cls_b[c])
+          prof_launch("k_selinv_small", lv.sel_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
+            return launch_selinv_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
+                                       16 << c, count, Lf, Sf, Uf, Wf, stream);
+          });
     }
   }
 };
