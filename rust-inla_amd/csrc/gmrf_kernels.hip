@@ -923,32 +923,34 @@ __global__ __launch_bounds__(256) void k_solve_bwd_large(DevPlan P, const int32_
 // evaluated densely inside the front).
 // ---------------------------------------------------------------------------
 
-// Small fronts (m <= MB), 4 MB threads: thread (row i = tid % MB, class c)
-// holds Sigma[i][c + 4t] (both triangles) in registers, starting from
-// Sigma_{R R} gathered from the parent; the L block sits in LDS (column k
-// contiguous). Column k (right to left): x_i = sum_{l > k} Sigma[i][l] L[l][k]
-// from per-class partials, Sigma[i][k] = Sigma[k][i] = -x_i and
-// Sigma[k][k] = 1 / D_k + sum_l L[l][k] x_l; one barrier per column.
+// Small fronts (m <= MB), 4 MB threads, blocked by 16 columns from the right.
+// The whole Sigma front (both triangles) and the L block sit in LDS; Sigma_{R R}
+// is gathered from the parent first. For block P = [c0, c1) and B = [c1, m):
+//   Zk = L_PP^-1 (one wave), Lh = L[B,P] Zk, Sigma[B,P] = -Sigma[B,B] Lh,
+//   Sigma[P,P] = Zk^T D_P^-1 Zk - Lh^T Sigma[B,P]
+// (the block form of the Takahashi recurrence), the products on FP64 MFMA.
 template <int MB>
 __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_t* __restrict__ list,
                                                          const double* __restrict__ L,
                                                          double* __restrict__ S,
                                                          double* __restrict__ U,
                                                          const double* __restrict__ W) {
-  constexpr int NT = 4 * MB, NJ = MB / 4, NW = NT / 64;
-  __shared__ double Ls[MB * MB];
-  __shared__ double part[2][4][MB];
-  __shared__ double wsum[2][NW];
+  constexpr int NT = 4 * MB, NJ = MB / 4, LD = MB + 1, NW = MB / 16;
+  __shared__ double Sg[MB * LD];
+  __shared__ double Ls[MB * LD];
+  __shared__ double Zk[16 * 17];
+  __shared__ double Lh[MB * 17];
+  __shared__ double dinv[16];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int vr = lane >> 4, vc = lane & 15;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Sm = S + (int64_t)b * P.l_size;
   double* Um = U + (int64_t)b * P.u_size;
   const double* Wm = W + (int64_t)b * P.w_size;
-  const int i = tid % MB, c = tid / MB;
-  // L block into LDS
+  // L block into LDS, row-major
   const int nl = m * w;
   for (int e0 = 0; e0 < nl; e0 += 8 * NT) {
     double v[8];
@@ -958,17 +960,15 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = e0 + tid + NT * u;
-      const int k = e / m, l = e - k * m;
-      if (e < nl) Ls[k * MB + l] = v[u];
+      const int k = e / m, i = e - k * m;
+      if (e < nl) Ls[i * LD + k] = v[u];
     }
   }
   // Sigma_{R R} from the parent front (its S block and Sigma_RR workspace, or
-  // its dense Sigma front when the parent is large); lower part also kept in
-  // this front's U workspace for the children.
-  double sg[NJ];
-#pragma unroll
-  for (int t = 0; t < NJ; ++t) sg[t] = 0.0;
+  // its dense Sigma front when the parent is large); the lower part is also
+  // kept in this front's U workspace for the children.
   if (r > 0) {
+    const int i = tid % MB, c = tid / MB;
     const int p = P.sn_parent[s];
     const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
     const bool plarge = P.is_large[p] != 0;
@@ -1001,63 +1001,121 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
 #pragma unroll
     for (int t = 0; t < NJ; ++t) {
       const int j = c + 4 * t;
-      const bool ok = i >= w && i < m && j >= w && j < m;
-      sg[t] = ok ? v[t] : 0.0;
-      if (ok && i >= j) Us[(int64_t)(j - w) * r + (i - w)] = v[t];
+      if (i >= w && i < m && j >= w && j < m) {
+        Sg[i * LD + j] = v[t];
+        if (i >= j) Us[(int64_t)(j - w) * r + (i - w)] = v[t];
+      }
     }
   }
   __syncthreads();
-  for (int k = w - 1; k >= 0; --k) {
-    const int buf = k & 1;
-    const double* Lk = Ls + k * MB;
-    double lk[NJ];
+  for (int c0 = ((w - 1) >> 4) << 4; c0 >= 0; c0 -= 16) {
+    const int c1 = min(w, c0 + 16), kn = c1 - c0, nB = m - c1;
+    const int nrb = (nB + 15) >> 4;
+    if (wv == 0) {
+      // Zk = L_PP^-1 by Gauss transforms; lane (i, c) holds Zk[i][c + 4t]
+      const int i = lane & 15, c = lane >> 4;
+      double z[4];
 #pragma unroll
-    for (int t = 0; t < NJ; ++t) lk[t] = Lk[c + 4 * t];
-    const double li = Lk[i];
-    ISSUE_FENCE();
-    double pacc = 0.0;
+      for (int t = 0; t < 4; ++t) z[t] = (i == c + 4 * t) ? 1.0 : 0.0;
+      for (int k = 0; k < kn; ++k) {
+        const double lik = Ls[(c0 + min(i, kn - 1)) * LD + c0 + k];
+        double zk[4];
 #pragma unroll
-    for (int t = 0; t < NJ; ++t) {
-      const int l = c + 4 * t;
-      const double prod = sg[t] * lk[t];
-      pacc += (l > k && l < m) ? prod : 0.0;
+        for (int t = 0; t < 4; ++t) zk[t] = __shfl(z[t], 16 * c + k, 64);
+        const bool act = i > k && i < kn;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const double nz = z[t] - lik * zk[t];
+          z[t] = (act && c + 4 * t <= k) ? nz : z[t];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) Zk[i * 17 + c + 4 * t] = z[t];
+      if (lane < 16) {
+        const int q = c0 + min(lane, kn - 1);
+        dinv[lane] = 1.0 / Ls[q * LD + q];
+      }
     }
-    part[buf][c][i] = pacc;
-    double qp = (i > k && i < m) ? li * pacc : 0.0;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) qp += __shfl_xor(qp, o, 64);
-    if (lane == 0) wsum[buf][wv] = qp;
     __syncthreads();
-    const double xi = part[buf][0][i] + part[buf][1][i] + part[buf][2][i] + part[buf][3][i];
-    if (i == k) {
-      double tot = 0.0;
+    // Lh = L[B,P] Zk
+    for (int task = wv; task < nrb; task += NW) {
+      const int r0 = 16 * task;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      acc = wgemm16(
+          [&](int ii, int q) {
+            const double x = Ls[(c1 + min(r0 + ii, nB - 1)) * LD + c0 + min(q, kn - 1)];
+            return (r0 + ii < nB && q < kn) ? x : 0.0;
+          },
+          [&](int q, int pp) {
+            const double x = Zk[q * 17 + pp];
+            return q < kn ? x : 0.0;
+          },
+          acc);
 #pragma unroll
-      for (int q = 0; q < NW; ++q) tot += wsum[buf][q];
-      const double skk = 1.0 / Lk[k] + tot;
-      double xj[NJ];
-#pragma unroll
-      for (int t = 0; t < NJ; ++t) {
-        const int j = c + 4 * t;
-        xj[t] = part[buf][0][j] + part[buf][1][j] + part[buf][2][j] + part[buf][3][j];
-      }
-      ISSUE_FENCE();
-#pragma unroll
-      for (int t = 0; t < NJ; ++t) {
-        const int j = c + 4 * t;
-        sg[t] = (j > k && j < m) ? -xj[t] : (j == k ? skk : sg[t]);
-      }
-    } else if (i > k && i < m) {
-#pragma unroll
-      for (int t = 0; t < NJ; ++t) sg[t] = (c + 4 * t == k) ? -xi : sg[t];
+      for (int v = 0; v < 4; ++v) Lh[(r0 + 4 * v + vr) * 17 + vc] = acc[v];
     }
+    __syncthreads();
+    // Sigma[B,P] = -Sigma[B,B] Lh, both triangles
+    for (int task = wv; task < nrb; task += NW) {
+      const int r0 = 16 * task;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < nB; k0 += 16)
+        acc = wgemm16(
+            [&](int ii, int kk) {
+              const double x = Sg[(c1 + min(r0 + ii, nB - 1)) * LD + c1 + min(k0 + kk, nB - 1)];
+              return (r0 + ii < nB && k0 + kk < nB) ? x : 0.0;
+            },
+            [&](int kk, int pp) {
+              const double x = Lh[min(k0 + kk, nB - 1) * 17 + pp];
+              return k0 + kk < nB ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + 4 * v + vr;
+        if (i < nB && vc < kn) {
+          Sg[(c1 + i) * LD + c0 + vc] = -acc[v];
+          Sg[(c0 + vc) * LD + c1 + i] = -acc[v];
+        }
+      }
+    }
+    __syncthreads();
+    // Sigma[P,P] = Zk^T D^-1 Zk - Lh^T Sigma[B,P]
+    if (wv == 0) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      acc = wgemm16(
+          [&](int pp, int k) {
+            const double x = Zk[k * 17 + pp] * dinv[k];
+            return k < kn ? x : 0.0;
+          },
+          [&](int k, int qq) {
+            const double x = Zk[k * 17 + qq];
+            return k < kn ? x : 0.0;
+          },
+          acc);
+      for (int k0 = 0; k0 < nB; k0 += 16)
+        acc = wgemm16(
+            [&](int pp, int ii) {
+              const double x = Lh[min(k0 + ii, nB - 1) * 17 + pp];
+              return k0 + ii < nB ? -x : 0.0;
+            },
+            [&](int ii, int qq) {
+              const double x = Sg[(c1 + min(k0 + ii, nB - 1)) * LD + c0 + qq];
+              return k0 + ii < nB ? x : 0.0;
+            },
+            acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int pp = 4 * v + vr;
+        if (pp < kn && vc < kn) Sg[(c0 + pp) * LD + c0 + vc] = acc[v];
+      }
+    }
+    __syncthreads();
   }
   double* Sb = Sm + P.l_off[s];
-  if (i < m) {
-#pragma unroll
-    for (int t = 0; t < NJ; ++t) {
-      const int j = c + 4 * t;
-      if (j < w) Sb[(int64_t)j * m + i] = (j <= i) ? sg[t] : 0.0;
-    }
+  for (int idx = tid; idx < m * w; idx += NT) {
+    const int j = idx / m, i = idx - j * m;
+    Sb[(int64_t)j * m + i] = (i >= j) ? Sg[i * LD + j] : 0.0;
   }
 }
 

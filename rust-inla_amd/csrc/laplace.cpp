@@ -49,7 +49,7 @@ struct gmrf_model {
   int64_t n = 0, nnz = 0, nobs = 0;
   int K = 0, fam = 0, link = 0, cap = 0, kc = 0;
   DevArena arena;
-  std::unique_ptr<gmrf_batch> bt;  // slots [0, cap) posterior, [cap, 2 cap) prior
+  std::unique_ptr<gmrf_batch> bt;  // for B points: slots [0, B) posterior, [B, 2B) prior
   // pattern of Q
   double* comps = nullptr;
   int64_t* q_ptr = nullptr;
@@ -355,16 +355,17 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     const hipStream_t st = M->stream();
     const int64_t n = M->n, nnz = M->nnz, m = M->nobs;
-    const int K = M->K, cap = M->cap;
+    const int K = M->K;
     gmrf_batch* bt = M->bt.get();
-    // ---- Q(theta) and the prior factorisation
+    // ---- Q(theta); the prior matrices occupy batch slots [B, 2B) right after the
+    // posterior ones, so the first Newton factorisation covers both in one call
     check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st), "affine");
-    double* vprior_f = M->vals + (size_t)cap * nnz;
+    double* vprior_f = M->vals + (size_t)B * nnz;
     check(hipMemcpyAsync(vprior_f, M->vprior, (size_t)B * nnz * sizeof(double),
                          hipMemcpyDeviceToDevice, st),
           "copy prior");
     if (M->kc == 0) check(lap_add_diag(B, n, nnz, M->diag_idx, kPriorJitter, vprior_f, st), "jitter");
-    bt->factorize(B, vprior_f, nnz, cap);
+    bool prior_pending = true;
     // ---- Newton (inference.rs:842-942)
     check(hipMemsetAsync(M->x, 0, (size_t)B * n * sizeof(double), st), "zero x");
     std::vector<int> state(B, 0);  // 0 active, 1 confirming, 2 done, 3 failed
@@ -387,7 +388,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
             "A^T g");
       check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, -1.0, 1.0, M->rhs, n, st),
             "- Q x");
-      bt->factorize(B, M->vals, nnz, 0);
+      bt->factorize(prior_pending ? 2 * B : B, M->vals, nnz, 0);
+      prior_pending = false;
       bt->solve(B, M->rhs, n, M->step, n, 0);
       check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, M->red, st), "max step");
       d2h(hred.data(), M->red, B * sizeof(double), st);
@@ -396,6 +398,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     if (gauss_id) {
       if (a->max_iter == 0) {
         for (int b = 0; b < B; ++b) status[b] = GMRF_SOLVE_FAILURE, state[b] = 3;
+        bt->factorize(B, vprior_f, nnz, B);
+        prior_pending = false;
       } else {
         newton_system();
         for (int b = 0; b < B; ++b) {
@@ -481,13 +485,17 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         if (state[b] < 2 && status[b] == GMRF_OK) status[b] = GMRF_SOLVE_FAILURE, state[b] = 3;
       if (!project(M, B, M->x)) throw std::runtime_error("project_constraints: singular A A^T");
     }
+    if (prior_pending) {  // no Newton system was formed
+      bt->factorize(B, vprior_f, nnz, B);
+      prior_pending = false;
+    }
     // ---- log-determinants and factor status
-    std::vector<double> ld(2 * cap);
-    std::vector<int32_t> fst(2 * cap);
-    d2h(ld.data(), bt->d_logdet, 2 * cap * sizeof(double), st);
-    d2h(fst.data(), bt->d_status, 2 * cap * sizeof(int32_t), st);
+    std::vector<double> ld(2 * B);
+    std::vector<int32_t> fst(2 * B);
+    d2h(ld.data(), bt->d_logdet, 2 * B * sizeof(double), st);
+    d2h(fst.data(), bt->d_status, 2 * B * sizeof(int32_t), st);
     for (int b = 0; b < B; ++b) {
-      const int32_t fp = fst[cap + b], fq = fst[b];
+      const int32_t fp = fst[B + b], fq = fst[b];
       auto code = [](int32_t f) {
         return (f & 2) ? GMRF_NOT_POSITIVE_DEFINITE : (f & 1) ? GMRF_SINGULAR : GMRF_OK;
       };
@@ -505,25 +513,27 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     d2h(hred.data(), M->red, 2 * B * sizeof(double), st);
     std::vector<double> mlik(B);
     for (int b = 0; b < B; ++b) {
-      mlik[b] = hred[B + b] - 0.5 * hred[b] + 0.5 * ld[cap + b] - 0.5 * ld[b];
+      mlik[b] = hred[B + b] - 0.5 * hred[b] + 0.5 * ld[B + b] - 0.5 * ld[b];
       if (status[b] != GMRF_OK) mlik[b] = -INFINITY;
     }
     h2d(a->mlik, mlik.data(), B * sizeof(double), st);
     if (a->x) check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
     // ---- selected inversion, variances
-    const bool need_post = a->var != nullptr || want_grad;
-    if (need_post) bt->selinv(B, 0);
+    // posterior (and, for the gradient, prior) selected inverses in one call
+    if (want_grad)
+      bt->selinv(2 * B, 0);
+    else if (a->var)
+      bt->selinv(B, 0);
     if (a->var) check(launch_diag_out(M->sym->plan, B, bt->S, a->var, n, st), "var");
     // ---- gradient
     if (want_grad) {
       const int mt = a->n_theta;
-      bt->selinv(B, cap);
       const DevPlan& P = M->sym->plan;
       // tr(Sigma_post Q_k), x*^T Q_k x*, tr(Sigma_prior Q_k)
       check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col,
                          M->x, M->x, n, M->tr_post, M->qx, M->cpart, st),
             "contract post");
-      check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)cap * P.l_size, P.l_size, P.slot_full,
+      check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full,
                          nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st),
             "contract prior");
       // v_i = (A Sigma_post A^T)_ii
