@@ -137,6 +137,34 @@ void* gmrf_batch_stream(gmrf_batch* bt);
  * without host synchronisation. use_given == 0: back to the batch's own stream. */
 gmrf_status gmrf_batch_set_stream(gmrf_batch* bt, void* stream, int32_t use_given);
 
+/* ---- heterogeneous batches (several patterns, one launch sequence) ----------
+ * The reference factorises independent models one solver at a time
+ * (inference.rs:1336-1337); gmrf_factorize_multi of the drop-in contract is
+ * this group. k patterns (full symmetric CSC each) are analysed separately,
+ * with the ordering chooser of ordering.rs:8-162 per pattern, and factorised
+ * as their block-diagonal union: every level launch covers all k models.
+ * Values, right-hand sides and outputs use the union layout: the k CSC value
+ * arrays (or vectors) concatenated in pattern order. */
+typedef struct gmrf_multi gmrf_multi;
+gmrf_status gmrf_multi_analyze(int32_t k, const int64_t* n, const int64_t* const* indptr,
+                               const int64_t* const* indices, int32_t capacity, gmrf_multi** out);
+void gmrf_multi_free(gmrf_multi* m);
+/* n_off[k + 1], nnz_off[k + 1]: block offsets in the union vectors / values;
+ * order_kind[k]: the chooser branch of each pattern (0 AMD, 1 time-major, 2 RCM). */
+gmrf_status gmrf_multi_info(const gmrf_multi* m, int64_t* n_off, int64_t* nnz_off,
+                            int32_t* order_kind);
+/* A new handle on the union analysis (free with gmrf_symbolic_free). */
+gmrf_symbolic* gmrf_multi_symbolic(gmrf_multi* m);
+/* The union's batch, owned by m: use gmrf_batch_solve / _selinv / _selinv_diag /
+ * _selinv_pattern / _set_stream on it; do not destroy it. */
+gmrf_batch* gmrf_multi_batch(gmrf_multi* m);
+/* Numeric LDL^T of `count` union matrices (device values, ld_values >= total
+ * nnz) plus the per-block log|Q_q| and status. */
+gmrf_status gmrf_multi_factorize(gmrf_multi* m, int32_t count, const double* values,
+                                 int64_t ld_values);
+/* logdet[b * k + q], status[b * k + q] (host or device pointers); synchronises. */
+gmrf_status gmrf_multi_logdet(gmrf_multi* m, int32_t count, double* logdet, int32_t* status);
+
 /* ---- InlaSolver mirror (host pointers) --------------------------------------
  * gmrf_solver_factorize = InlaSolver::factorize (solver.rs:70-74): symbolic
  * analysis on the first call or when the pattern changes, numeric otherwise.

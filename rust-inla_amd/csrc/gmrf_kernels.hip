@@ -101,9 +101,11 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
       }
     }
     __syncthreads();
+    const int kc = min(kPanel, K - k0);
     if (k0 + kPanel < K) fetch(k0 + kPanel);
 #pragma unroll
     for (int kk = 0; kk < kPanel; kk += 4) {
+      if (kk >= kc) break;  // uniform: short last chunk
       const double a0 = Bs[qj * 32 + lr][kk + lk];
       const double a1 = Bs[qj * 32 + 16 + lr][kk + lk];
       const double b0 = As[qi * 32 + lr][kk + lk];
@@ -629,26 +631,36 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
 // log|Q| = sum log|D_j| per matrix, plus the pivot status: an exact zero pivot
 // (the numeric phase fails: NotPositiveDefinite) or |D_j| < 1e-14 / non-finite
 // (Singular), as crates/inla_math/src/sparse_ldlt.rs:104-149,217-230.
-__global__ __launch_bounds__(1024) void k_logdet(DevPlan P, const double* __restrict__ L,
-                                                 double* __restrict__ logdet,
-                                                 int32_t* __restrict__ status) {
-  __shared__ double red[16];
-  __shared__ int flags[16];
-  const int b = blockIdx.x, tid = threadIdx.x;
+// Grid (kLogdetChunks, B): each block reduces one chunk of the pivots; the last
+// block of a matrix to finish (device-scope counter) sums the chunk partials in
+// chunk order, writes the result and resets the counter for the next launch.
+constexpr int kLogdetChunks = 16;
+
+__global__ __launch_bounds__(256) void k_logdet(DevPlan P, const double* __restrict__ L,
+                                                double* __restrict__ part, uint32_t* __restrict__ counter,
+                                                double* __restrict__ logdet,
+                                                int32_t* __restrict__ status) {
+  __shared__ double red[4];
+  __shared__ int flags[4];
+  __shared__ int last;
+  const int ch = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   const double* Lm = L + (int64_t)b * P.l_size;
+  const int len = (P.n + kLogdetChunks - 1) / kLogdetChunks;
+  const int j0 = ch * len, j1 = min(P.n, j0 + len);
   double acc = 0.0;
   int fl = 0;
-  for (int j0 = 0; j0 < P.n; j0 += 8 * 1024) {
-    int64_t sl[8];
+  for (int jb = j0; jb < j1; jb += 4 * 256) {
+    int64_t sl[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) sl[u] = P.diag_slot[min(j0 + tid + 1024 * u, P.n - 1)];
-    double d[8];
+    for (int u = 0; u < 4; ++u) sl[u] = P.diag_slot[min(jb + tid + 256 * u, P.n - 1)];
+    ISSUE_FENCE();
+    double d[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) d[u] = Lm[sl[u]];
+    for (int u = 0; u < 4; ++u) d[u] = Lm[sl[u]];
     ISSUE_FENCE();
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (j0 + tid + 1024 * u < P.n) {
+    for (int u = 0; u < 4; ++u)
+      if (jb + tid + 256 * u < j1) {
         if (d[u] == 0.0) fl |= GMRF_ST_ZERO_PIVOT;
         if (!(fabs(d[u]) >= 1e-14) || !isfinite(d[u])) fl |= GMRF_ST_SINGULAR;
         acc += log(fabs(d[u]));
@@ -665,14 +677,73 @@ __global__ __launch_bounds__(1024) void k_logdet(DevPlan P, const double* __rest
   }
   __syncthreads();
   if (tid == 0) {
+    double* pb = part + ((int64_t)b * kLogdetChunks + ch) * 2;
+    pb[0] = red[0] + red[1] + red[2] + red[3];
+    pb[1] = (double)(flags[0] | flags[1] | flags[2] | flags[3]);
+    __threadfence();
+    last = atomicAdd(&counter[b], 1u) == kLogdetChunks - 1;
+  }
+  __syncthreads();
+  if (last && tid == 0) {
+    __threadfence();
+    const volatile double* pv = part + (int64_t)b * kLogdetChunks * 2;
     double t = 0.0;
     int f = 0;
-    for (int q = 0; q < 16; ++q) {
-      t += red[q];
-      f |= flags[q];
+    for (int q = 0; q < kLogdetChunks; ++q) {
+      t += pv[2 * q];
+      f |= (int)pv[2 * q + 1];
     }
     logdet[b] = t;
     status[b] = f;
+    counter[b] = 0u;
+  }
+}
+
+// Per-block log|D| of a block-diagonal union (heterogeneous batch): block q of
+// matrix b sums the pivots at its factor positions pos[ptr[q] .. ptr[q + 1]) in
+// index order; grid (k, B), one workgroup per (block, matrix).
+__global__ __launch_bounds__(256) void k_logdet_seg(DevPlan P, const double* __restrict__ L,
+                                                    const int64_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ pos, int k,
+                                                    double* __restrict__ logdet,
+                                                    int32_t* __restrict__ status) {
+  __shared__ double red[4];
+  __shared__ int flags[4];
+  const int q = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const double* Lm = L + (int64_t)b * P.l_size;
+  const int64_t p0 = ptr[q], p1 = ptr[q + 1];
+  double acc = 0.0;
+  int fl = 0;
+  for (int64_t jb = p0; jb < p1; jb += 4 * 256) {
+    int32_t j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = pos[min(jb + tid + 256 * u, p1 - 1)];
+    ISSUE_FENCE();
+    double d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d[u] = Lm[P.diag_slot[j[u]]];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (jb + tid + 256 * u < p1) {
+        if (d[u] == 0.0) fl |= GMRF_ST_ZERO_PIVOT;
+        if (!(fabs(d[u]) >= 1e-14) || !isfinite(d[u])) fl |= GMRF_ST_SINGULAR;
+        acc += log(fabs(d[u]));
+      }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_xor(acc, o, 64);
+    fl |= __shfl_xor(fl, o, 64);
+  }
+  if ((tid & 63) == 0) {
+    red[tid >> 6] = acc;
+    flags[tid >> 6] = fl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    logdet[(int64_t)b * k + q] = red[0] + red[1] + red[2] + red[3];
+    status[(int64_t)b * k + q] = flags[0] | flags[1] | flags[2] | flags[3];
   }
 }
 
@@ -777,15 +848,19 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
 }
 
 // Large fronts, blocked by the factorisation's 64-column panels with the stored
-// Z = L_PP^-1: y_P = Z f_P, then f_B -= L[B,P] y_P for the rows below. Loads are
-// issued in groups of 16 before use.
-__global__ __launch_bounds__(256) void k_solve_fwd_large(DevPlan P, const int32_t* __restrict__ list,
-                                                         const double* __restrict__ L,
-                                                         const double* __restrict__ Z,
-                                                         double* __restrict__ X,
-                                                         double* __restrict__ V) {
+// Z = L_PP^-1: y_P = Z f_P, then f_B -= L[B,P] y_P for the rows below. 1024
+// threads; both products split their K dimension over waves so that each panel
+// costs one or two rounds of loads, reduced through LDS.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_solve_fwd_large(DevPlan P, const int32_t* __restrict__ list,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ Z,
+                                                        double* __restrict__ X,
+                                                        double* __restrict__ V) {
+  constexpr int NQ = NT / 64, QW = 64 / NQ, NR = NT / 4;  // y: NQ parts of QW columns; rows: NR per round
   extern __shared__ double f[];
-  __shared__ double part[4][kTile];
+  __shared__ double part[NQ][kTile];
+  __shared__ double rp[4][NR];
   __shared__ double y[kTile];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
@@ -794,55 +869,64 @@ __global__ __launch_bounds__(256) void k_solve_fwd_large(DevPlan P, const int32_
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  for (int i = tid; i < m; i += 256) f[i] = i < w ? Xb[a0 + i] : 0.0;
+  for (int i = tid; i < m; i += NT) f[i] = i < w ? Xb[a0 + i] : 0.0;
   __syncthreads();
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int c = P.child_idx[cq];
     const int rc = P.sn_r[c];
     const double* Vc = V + (int64_t)b * P.v_size + P.v_off[c];
     const int32_t* relc = P.rel + P.rows_ptr[c];
-    for (int i = tid; i < rc; i += 256) f[relc[i]] += Vc[i];
+    for (int i = tid; i < rc; i += NT) f[relc[i]] += Vc[i];
     __syncthreads();
   }
-  const int i = tid & 63, pt = tid >> 6;
+  const int i = tid & 63, pt = tid >> 6;   // y: row i, q in [QW pt, QW pt + QW)
+  const int rr = tid % NR, qp = tid / NR;  // rows below: row c1 + rr (+NR k), q in [16 qp, 16 qp + 16)
   for (int c0 = 0; c0 < w; c0 += kTile) {
     const int nb = min(kTile, w - c0), c1 = c0 + nb;
     const double* Zp = Zb + (int64_t)c0 * kTile;
-    double zv[16];
+    {
+      double zv[QW];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) zv[u] = Zp[(16 * pt + u) * kTile + i];
-    ISSUE_FENCE();
-    double acc = 0.0;
+      for (int u = 0; u < QW; ++u) zv[u] = Zp[(QW * pt + u) * kTile + i];
+      ISSUE_FENCE();
+      double acc = 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int q = 16 * pt + u;
-      const double prod = zv[u] * f[c0 + min(q, nb - 1)];
-      acc += (q < nb && i < nb) ? prod : 0.0;
+      for (int u = 0; u < QW; ++u) {
+        const int q = QW * pt + u;
+        const double prod = zv[u] * f[c0 + min(q, nb - 1)];
+        acc += (q < nb && i < nb) ? prod : 0.0;
+      }
+      part[pt][i] = acc;
     }
-    part[pt][i] = acc;
     __syncthreads();
     if (tid < nb) {
-      const double v = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v += part[q][tid];
       y[tid] = v;
       f[c0 + tid] = v;
     }
     __syncthreads();
-    for (int row = c1 + tid; row < m; row += 256) {
+    for (int row0 = c1; row0 < m; row0 += NR) {
+      const int row = min(row0 + rr, m - 1);
+      double lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(16 * qp + u, nb - 1)) * m + row];
+      ISSUE_FENCE();
       double a = 0.0;
-      for (int q0 = 0; q0 < nb; q0 += 16) {
-        double lv[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(q0 + u, nb - 1)) * m + row];
-        ISSUE_FENCE();
-#pragma unroll
-        for (int u = 0; u < 16; ++u) a += (q0 + u < nb) ? lv[u] * y[min(q0 + u, nb - 1)] : 0.0;
+      for (int u = 0; u < 16; ++u) {
+        const int q = 16 * qp + u;
+        a += q < nb ? lv[u] * y[min(q, nb - 1)] : 0.0;
       }
-      f[row] -= a;
+      rp[qp][rr] = a;
+      __syncthreads();
+      if (tid < NR && row0 + tid < m) f[row0 + tid] -= rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
+      __syncthreads();
     }
-    __syncthreads();
   }
   double* Vs = V + (int64_t)b * P.v_size + P.v_off[s];
-  for (int k = tid; k < m; k += 256) {
+  for (int k = tid; k < m; k += NT) {
     if (k < w)
       Xb[a0 + k] = f[k];
     else
@@ -851,23 +935,27 @@ __global__ __launch_bounds__(256) void k_solve_fwd_large(DevPlan P, const int32_
 }
 
 // Backward on large fronts, panels right to left:
-// t_P = z_P - L[B,P]^T x_B, x_P = Z^T t_P. Wave wv owns panel columns
-// q = wv + 4u (u < 16) and keeps their 16 partial dot products in registers.
-__global__ __launch_bounds__(256) void k_solve_bwd_large(DevPlan P, const int32_t* __restrict__ list,
-                                                         const double* __restrict__ L,
-                                                         const double* __restrict__ Z,
-                                                         double* __restrict__ X) {
+// t_P = z_P - L[B,P]^T x_B, x_P = Z^T t_P. Wave wv covers panel columns
+// q = (wv & 3) + 4u (u < 16) over rows c1 + 64 (wv >> 2) + lane (+NT / 4 k).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_solve_bwd_large(DevPlan P, const int32_t* __restrict__ list,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ Z,
+                                                        double* __restrict__ X) {
+  constexpr int NW = NT / 64, NK = NW / 4;
   extern __shared__ double xf[];
+  __shared__ double red[NK][kTile];
   __shared__ double tq[kTile];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int qg = wv & 3, kg = wv >> 2;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
   const int32_t* rows = P.rows + P.rows_ptr[s];
-  for (int k = tid; k < m; k += 256) xf[k] = k < w ? Xb[a0 + k] : Xb[rows[k - w]];
+  for (int k = tid; k < m; k += NT) xf[k] = k < w ? Xb[a0 + k] : Xb[rows[k - w]];
   __syncthreads();
   const int np = (w + kTile - 1) / kTile;
   for (int pk = np - 1; pk >= 0; --pk) {
@@ -875,12 +963,12 @@ __global__ __launch_bounds__(256) void k_solve_bwd_large(DevPlan P, const int32_
     double acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    for (int k = c1 + lane; k < m + 63; k += 64) {
+    for (int k = c1 + 64 * kg + lane; k < m + 63; k += 64 * NK) {
       const int kc = min(k, m - 1);
       const double xk = k < m ? xf[kc] : 0.0;
       double lv[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(wv + 4 * u, nb - 1)) * m + kc];
+      for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(qg + 4 * u, nb - 1)) * m + kc];
       ISSUE_FENCE();
 #pragma unroll
       for (int u = 0; u < 16; ++u) acc[u] += lv[u] * xk;
@@ -890,27 +978,33 @@ __global__ __launch_bounds__(256) void k_solve_bwd_large(DevPlan P, const int32_
       double a = acc[u];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      const int q = wv + 4 * u;
-      if (lane == 0 && q < nb) tq[q] = xf[c0 + q] - a;
+      if (lane == 0) red[kg][qg + 4 * u] = a;
+    }
+    __syncthreads();
+    if (tid < nb) {
+      double a = 0.0;
+#pragma unroll
+      for (int q = 0; q < NK; ++q) a += red[q][tid];
+      tq[tid] = xf[c0 + tid] - a;
     }
     __syncthreads();
     const double* Zp = Zb + (int64_t)pk * kTile * kTile;
-    double zv[16];
+    double zv[64 / NW];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) zv[u] = Zp[(wv + 4 * u) * kTile + lane];
+    for (int u = 0; u < 64 / NW; ++u) zv[u] = Zp[(wv + NW * u) * kTile + lane];
     ISSUE_FENCE();
     const double tl = lane < nb ? tq[lane] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 64 / NW; ++u) {
       double a = lane < nb ? zv[u] * tl : 0.0;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      const int q = wv + 4 * u;
+      const int q = wv + NW * u;
       if (lane == 0 && q < nb) xf[c0 + q] = a;
     }
     __syncthreads();
   }
-  for (int k = tid; k < w; k += 256) Xb[a0 + k] = xf[k];
+  for (int k = tid; k < w; k += NT) Xb[a0 + k] = xf[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1430,9 +1524,18 @@ hipError_t launch_fl_update(const DevPlan& P, const int32_t* tasks, int count, i
   return hipGetLastError();
 }
 
-hipError_t launch_logdet(const DevPlan& P, int B, const double* L, double* logdet, int32_t* status,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_logdet, dim3(B), dim3(1024), 0, st, P, L, logdet, status);
+hipError_t launch_logdet(const DevPlan& P, int B, const double* L, double* part, uint32_t* counter,
+                         double* logdet, int32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_logdet, dim3(kLogdetChunks, B), dim3(256), 0, st, P, L, part, counter, logdet,
+                     status);
+  return hipGetLastError();
+}
+
+hipError_t launch_logdet_seg(const DevPlan& P, int B, const double* L, const int64_t* ptr,
+                             const int32_t* pos, int k, double* logdet, int32_t* status,
+                             hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_logdet_seg, dim3(k, B), dim3(256), 0, st, P, L, ptr, pos, k, logdet, status);
   return hipGetLastError();
 }
 
@@ -1483,9 +1586,16 @@ hipError_t launch_solve_fwd_large(const DevPlan& P, const int32_t* list, int cou
                                   hipStream_t st) {
   if (count <= 0) return hipSuccess;
   const size_t sh = (size_t)maxm * sizeof(double);
-  hipError_t e = lds_limit((const void*)k_solve_fwd_large, sh);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_solve_fwd_large, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
+  // few fronts: 1024 threads each for latency; many: 256 for occupancy
+  if ((int64_t)count * B < 256) {
+    hipError_t e = lds_limit((const void*)k_solve_fwd_large<1024>, sh);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve_fwd_large<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, V);
+  } else {
+    hipError_t e = lds_limit((const void*)k_solve_fwd_large<256>, sh);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve_fwd_large<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
+  }
   return hipGetLastError();
 }
 
@@ -1493,9 +1603,15 @@ hipError_t launch_solve_bwd_large(const DevPlan& P, const int32_t* list, int cou
                                   const double* L, const double* Z, double* X, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   const size_t sh = (size_t)maxm * sizeof(double);
-  hipError_t e = lds_limit((const void*)k_solve_bwd_large, sh);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_solve_bwd_large, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X);
+  if ((int64_t)count * B < 256) {
+    hipError_t e = lds_limit((const void*)k_solve_bwd_large<1024>, sh);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve_bwd_large<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X);
+  } else {
+    hipError_t e = lds_limit((const void*)k_solve_bwd_large<256>, sh);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve_bwd_large<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X);
+  }
   return hipGetLastError();
 }
 

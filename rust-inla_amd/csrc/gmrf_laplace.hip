@@ -8,6 +8,38 @@
 
 namespace gmrf {
 
+// Compiler-only fence after a round of independent loads: keeps them issued
+// ahead of the conditional code that consumes them (no instruction emitted).
+#define LAP_ISSUE_FENCE() asm volatile("" ::: "memory")
+
+// sum_{t in [t0, t1)} c[t] * x[idx[t]] with the index and value loads issued in
+// rounds of 8 (clamped addresses, masked terms). idx and c are never empty
+// arrays (allocations hold at least one element).
+template <typename IT>
+__device__ __forceinline__ double gather_dot(int64_t t0, int64_t t1, const IT* __restrict__ idx,
+                                             const double* __restrict__ c,
+                                             const double* __restrict__ x) {
+  double s = 0.0;
+  const int64_t tl = t1 > t0 ? t1 - 1 : 0;
+  for (int64_t tb = t0; tb < t1; tb += 8) {
+    IT ix[8];
+    double cv[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t t = min(tb + u, tl);
+      ix[u] = idx[t];
+      cv[u] = c[t];
+    }
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = x[ix[u]];
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (tb + u < t1) ? cv[u] * xv[u] : 0.0;
+  }
+  return s;
+}
+
 // Likelihood families / links (crates/inla_stats/src/inference.rs:16-21,170-311).
 enum { FAM_GAUSSIAN = 0, FAM_POISSON = 1, FAM_BINOMIAL = 2, FAM_NBINOMIAL = 3 };
 enum { LINK_IDENTITY = 0, LINK_LOG = 1, LINK_LOGIT = 2 };
@@ -187,10 +219,8 @@ __global__ void k_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* __re
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / nnz);
     const int64_t q = e - (int64_t)b * nnz;
-    double s = base[e];
     const double* wb = w + (int64_t)b * nobs;
-    for (int64_t t = ptr[q]; t < ptr[q + 1]; ++t) s += coef[t] * wb[obs[t]];
-    out[e] = s;
+    out[e] = base[e] + gather_dot(ptr[q], ptr[q + 1], obs, coef, wb);
   }
 }
 
@@ -206,8 +236,7 @@ __global__ void k_gather_spmv(int B, int64_t nout, const int64_t* __restrict__ p
     const int64_t o = e - (int64_t)b * nout;
     const double* vb = vals + (int64_t)b * vld;
     const double* xb = x + (int64_t)b * ldx;
-    double s = 0.0;
-    for (int64_t p = ptr[o]; p < ptr[o + 1]; ++p) s += vb[p] * xb[idx[p]];
+    const double s = gather_dot(ptr[o], ptr[o + 1], idx, vb, xb);
     double* yo = y + (int64_t)b * ldy + o;
     *yo = alpha * s + (beta == 0.0 ? 0.0 : beta * *yo);
   }
@@ -225,35 +254,52 @@ __global__ void k_axpy_b(int B, int64_t len, const double* __restrict__ a,
   }
 }
 
-// Per-row reductions over [B x len]: op 0 max|x|, 1 sum x, 2 sum x*y, 3 max|x - y|
-__global__ __launch_bounds__(256) void k_rowreduce(int64_t len, int op, const double* __restrict__ x,
-                                                   const double* __restrict__ y,
-                                                   double* __restrict__ out) {
-  __shared__ double red[256];
-  const int b = blockIdx.x;
+// Per-row reductions over [B x len]: op 0 max|x|, 1 sum x, 2 sum x*y, 3 max|x - y|.
+// One 1024-thread block per row, 8 loads per thread per round; fixed order.
+__device__ __forceinline__ double red_combine(int op, double a, double v) {
+  return (op == 0 || op == 3) ? fmax(a, v) : a + v;
+}
+
+__global__ __launch_bounds__(1024) void k_rowreduce(int64_t len, int op, const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    double* __restrict__ out) {
+  __shared__ double red[16];
+  const int b = blockIdx.x, tid = threadIdx.x;
   const double* xb = x + (int64_t)b * len;
-  const double* yb = y ? y + (int64_t)b * len : nullptr;
+  const double* yb = y ? y + (int64_t)b * len : xb;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < len; i += 256) {
-    const double v = xb[i];
-    if (op == 0)
-      acc = isfinite(v) ? fmax(acc, fabs(v)) : INFINITY;
-    else if (op == 1)
-      acc += v;
-    else if (op == 2)
-      acc += v * yb[i];
-    else
-      acc = fmax(acc, fabs(v - yb[i]));
+  for (int64_t i0 = 0; i0 < len; i0 += 8 * 1024) {
+    double xv[8], yv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t i = min(i0 + tid + 1024 * u, len - 1);
+      xv[u] = xb[i];
+      yv[u] = yb[i];
+    }
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i0 + tid + 1024 * u >= len) continue;
+      const double v = xv[u];
+      if (op == 0)
+        acc = isfinite(v) && isfinite(acc) ? fmax(acc, fabs(v)) : INFINITY;
+      else if (op == 1)
+        acc += v;
+      else if (op == 2)
+        acc += v * yv[u];
+      else
+        acc = fmax(acc, fabs(v - yv[u]));
+    }
   }
-  red[threadIdx.x] = acc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc = red_combine(op, acc, __shfl_xor(acc, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      red[threadIdx.x] = (op == 0 || op == 3) ? fmax(red[threadIdx.x], red[threadIdx.x + o])
-                                              : red[threadIdx.x] + red[threadIdx.x + o];
-    __syncthreads();
+  if (tid == 0) {
+    double t = red[0];
+    for (int q = 1; q < 16; ++q) t = red_combine(op, t, red[q]);
+    out[b] = t;
   }
-  if (threadIdx.x == 0) out[b] = red[0];
 }
 
 // Adjoint contractions over the CSC pattern, two passes (deterministic order):
@@ -280,14 +326,32 @@ __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
   const double* xb = x ? x + (int64_t)b * n : nullptr;
   const double* zb = z ? z + (int64_t)b * n : nullptr;
   double a1 = 0.0, a2 = 0.0;
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
-    const double c = ck[e];
-    if (c == 0.0) continue;
-    if (Sb) {
-      const int64_t sl = slot_full[e];
-      if (sl >= 0) a1 += c * Sb[sl];
+  for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += 4 * 256) {
+    double cv[4], sv[4], xv[4], zv[4];
+    int64_t sl[4];
+    int32_t qr[4], qc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t e = min(eb + 256 * u, e1 - 1);
+      cv[u] = ck[e];
+      sl[u] = Sb ? slot_full[e] : 0;
+      qr[u] = xb ? qrow[e] : 0;
+      qc[u] = xb ? qcol[e] : 0;
     }
-    if (xb) a2 += c * xb[qrow[e]] * zb[qcol[e]];
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sv[u] = Sb ? Sb[max(sl[u], (int64_t)0)] : 0.0;
+      xv[u] = xb ? xb[qr[u]] : 0.0;
+      zv[u] = xb ? zb[qc[u]] : 0.0;
+    }
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (eb + 256 * u >= e1) continue;
+      if (sl[u] >= 0) a1 += cv[u] * sv[u];
+      a2 += cv[u] * xv[u] * zv[u];
+    }
   }
   r1[threadIdx.x] = a1;
   r2[threadIdx.x] = a2;
@@ -330,9 +394,7 @@ __global__ void k_vdiag(int B, int64_t nobs, const int64_t* __restrict__ vptr,
     const int b = (int)(e / nobs);
     const int64_t i = e - (int64_t)b * nobs;
     const double* Sb = S + (int64_t)b * l_size;
-    double s = 0.0;
-    for (int64_t t = vptr[i]; t < vptr[i + 1]; ++t) s += vcoef[t] * Sb[vslot[t]];
-    v[e] = s;
+    v[e] = gather_dot(vptr[i], vptr[i + 1], vslot, vcoef, Sb);
   }
 }
 
@@ -439,7 +501,7 @@ hipError_t lap_axpy(int B, int64_t len, const double* a, const double* alpha, co
 
 hipError_t lap_rowreduce(int B, int64_t len, int op, const double* x, const double* y, double* out,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_rowreduce, dim3(B), dim3(256), 0, st, len, op, x, y, out);
+  hipLaunchKernelGGL(k_rowreduce, dim3(B), dim3(1024), 0, st, len, op, x, y, out);
   return hipGetLastError();
 }
 

@@ -34,7 +34,10 @@ hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*
                           const double*, hipStream_t);
 hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const double*,
                         const double*, double*, double*, const double*, hipStream_t);
-hipError_t launch_logdet(const DevPlan&, int, const double*, double*, int32_t*, hipStream_t);
+hipError_t launch_logdet(const DevPlan&, int, const double*, double*, uint32_t*, double*, int32_t*,
+                         hipStream_t);
+hipError_t launch_logdet_seg(const DevPlan&, int, const double*, const int64_t*, const int32_t*, int,
+                             double*, int32_t*, hipStream_t);
 hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t);
 hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
 hipError_t launch_diag_scale(const DevPlan&, int, const double*, double*, hipStream_t);
@@ -554,6 +557,8 @@ struct gmrf_batch {
          *W = nullptr, *Z = nullptr;
   double* d_logdet = nullptr;
   int32_t* d_status = nullptr;
+  double* d_ldpart = nullptr;     // per-matrix chunk partials of log|D|
+  uint32_t* d_ldcount = nullptr;  // per-matrix chunk counters (zero between launches)
 
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
@@ -571,6 +576,9 @@ struct gmrf_batch {
     Z = arena.alloc<double>((size_t)cap * sym->plan.z_size);
     d_logdet = arena.alloc<double>(cap);
     d_status = arena.alloc<int32_t>(cap);
+    d_ldpart = arena.alloc<double>((size_t)cap * 2 * 16);
+    d_ldcount = arena.alloc<uint32_t>(cap);
+    check(hipMemset(d_ldcount, 0, (size_t)cap * sizeof(uint32_t)), "hipMemset");
   }
   ~gmrf_batch() {
     if (own) {
@@ -614,7 +622,10 @@ struct gmrf_batch {
         }
       }
     }
-    check(launch_logdet(P, count, Lf, d_logdet + first, d_status + first, stream), "logdet");
+    prof_launch("k_logdet", 0.0, 16.0 * P.n * count, stream, [&] {
+      return launch_logdet(P, count, Lf, d_ldpart + (size_t)first * 2 * 16, d_ldcount + first,
+                           d_logdet + first, d_status + first, stream);
+    });
   }
 
   void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0) {

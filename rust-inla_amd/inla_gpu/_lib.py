@@ -120,6 +120,13 @@ SIGNATURES = {
     "gmrf_batch_sync": (_i32, [_vp]),
     "gmrf_batch_stream": (_vp, [_vp]),
     "gmrf_batch_set_stream": (_i32, [_vp, _vp, _i32]),
+    "gmrf_multi_analyze": (_i32, [_i32, _pi64, C.POINTER(_pi64), C.POINTER(_pi64), _i32, C.POINTER(_vp)]),
+    "gmrf_multi_free": (None, [_vp]),
+    "gmrf_multi_info": (_i32, [_vp, _pi64, _pi64, _pi32]),
+    "gmrf_multi_symbolic": (_vp, [_vp]),
+    "gmrf_multi_batch": (_vp, [_vp]),
+    "gmrf_multi_factorize": (_i32, [_vp, _i32, _vp, _i64]),
+    "gmrf_multi_logdet": (_i32, [_vp, _i32, _vp, _vp]),
     "gmrf_solver_create": (_i32, [C.POINTER(_vp)]),
     "gmrf_solver_destroy": (None, [_vp]),
     "gmrf_solver_factorize": (_i32, [_vp, _i64, _pi64, _pi64, _pdbl]),

@@ -1,0 +1,249 @@
+"""End-to-end fit over theta on the GPU: mode search, Hessian, CCD nodes, mixture.
+
+Two drivers of the same pipeline, both on the batched Laplace evaluator:
+
+* ``fit_batched``: BFGS on the adjoint gradient of the log posterior, with the
+  line-search step lengths of an iteration evaluated as one batch; the
+  finite-difference Hessian stencil as one batch; all CCD nodes as one batch.
+* ``fit_plain``: the reference's sequence one theta-point at a time:
+  Nelder-Mead (crates/inla_math/src/hyper_opt.rs:15-185 with step 0.1,
+  200 iterations, tol 1e-6 as crates/inla_stats/src/inference.rs:1258), the
+  Hessian stencil point by point (hyper_opt.rs:197-259, step from
+  inference.rs:1264) and the CCD nodes one by one (inference.rs:1296-1397).
+
+Both end in the mixture of inference.rs:1419-1468.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import ccd
+
+
+@dataclass
+class FitResult:
+    mode: np.ndarray
+    hessian: np.ndarray
+    thetas: np.ndarray
+    weights: np.ndarray
+    log_post: np.ndarray
+    mean: np.ndarray
+    var: np.ndarray
+    mlik: float
+    n_evals: int = 0
+    n_points: int = 0
+    seconds: dict = field(default_factory=dict)
+
+
+class Objective:
+    """log posterior g(theta) = mlik(theta) + log pi(theta) on a LaplaceModel.
+
+    ``params(thetas) -> (coefs, dcoefs, lpar, dlog_lpar)`` and
+    ``log_prior(theta) -> (value, gradient)`` describe the model's theta.
+    Counts calls (batched calls) and theta-points.
+    """
+
+    def __init__(self, model, params, log_prior, max_iter=200, tol=1e-5):
+        self.model, self.params, self.log_prior = model, params, log_prior
+        self.max_iter, self.tol = max_iter, tol
+        self.calls = 0
+        self.points = 0
+
+    def _chunks(self, thetas):
+        cap = self.model.capacity
+        for i in range(0, len(thetas), cap):
+            yield thetas[i:i + cap]
+
+    def values(self, thetas, grad=False, want_x=False, want_var=False):
+        """g (and its gradient) at each row of thetas, in batches of the model capacity."""
+        thetas = np.atleast_2d(np.asarray(thetas, dtype=float))
+        out_g, out_d, out_x, out_v = [], [], [], []
+        for th in self._chunks(thetas):
+            c, dc, lp, dl = self.params(th)
+            r = self.model.eval(c, lpar=lp, dcoefs=dc if grad else None, dlog_lpar=dl if grad else None,
+                                want_x=want_x, want_var=want_var, max_iter=self.max_iter, tol=self.tol)
+            self.calls += 1
+            self.points += len(th)
+            ml = r["mlik"].cpu().numpy()
+            pri = [self.log_prior(t) for t in th]
+            out_g.append(ml + np.array([p[0] for p in pri]))
+            if grad:
+                out_d.append(r["grad"].cpu().numpy() + np.array([p[1] for p in pri]))
+            if want_x:
+                out_x.append(r["x"].cpu().numpy())
+            if want_var:
+                out_v.append(r["var"].cpu().numpy())
+        res = {"g": np.concatenate(out_g)}
+        if grad:
+            res["grad"] = np.concatenate(out_d)
+        if want_x:
+            res["x"] = np.concatenate(out_x)
+        if want_var:
+            res["var"] = np.concatenate(out_v)
+        return res
+
+
+def bfgs_mode(obj: Objective, theta0, max_iter=100, gtol=1e-3, xtol=1e-6, n_steps=4, max_step=1.0):
+    """Maximise g with BFGS on the adjoint gradient. Each iteration evaluates the
+    step lengths 1, 1/2, ..., 2^-(n_steps-1) along the search direction in one
+    batch and takes the longest one that satisfies the Armijo condition. Steps
+    are capped at ``max_step`` in every coordinate. Returns (theta, iterations)."""
+    x = np.asarray(theta0, dtype=float).copy()
+    m = len(x)
+    r = obj.values(x[None], grad=True)
+    f, g = -r["g"][0], -r["grad"][0]  # minimise f = -g
+    H = np.eye(m)
+    it = 0
+    while it < max_iter:
+        if np.max(np.abs(g)) < gtol:
+            break
+        it += 1
+        p = -H @ g
+        if g @ p >= 0:  # not a descent direction: restart from steepest descent
+            H = np.eye(m)
+            p = -g
+        big = np.max(np.abs(p))
+        if big > max_step:  # theta lives on a log / logit scale
+            p = p * (max_step / big)
+        ts = 0.5 ** np.arange(n_steps)
+        accepted = None
+        while accepted is None:
+            trial = x[None, :] + ts[:, None] * p[None, :]
+            rr = obj.values(trial, grad=True)
+            fv, gv = -rr["g"], -rr["grad"]
+            for k, t in enumerate(ts):
+                if np.isfinite(fv[k]) and fv[k] <= f + 1e-4 * t * (g @ p):
+                    accepted = (t, fv[k], gv[k])
+                    break
+            if accepted is None:
+                ts = ts * 0.5 ** n_steps
+                if ts[0] < 1e-12:
+                    return x, it
+        t, fn, gn = accepted
+        s = t * p
+        yk = gn - g
+        sy = s @ yk
+        if sy > 1e-12 * max(1.0, np.linalg.norm(s) * np.linalg.norm(yk)):
+            if it == 1:
+                H = np.eye(m) * (sy / (yk @ yk))
+            rho = 1.0 / sy
+            I = np.eye(m)
+            H = (I - rho * np.outer(s, yk)) @ H @ (I - rho * np.outer(yk, s)) + rho * np.outer(s, s)
+        x, f, g = x + s, fn, gn
+        if np.max(np.abs(s)) < xtol:
+            break
+    return x, it
+
+
+def nelder_mead_mode(obj: Objective, theta0, step=0.1, max_iter=200, tol=1e-6):
+    """Minimise -g with the downhill simplex of hyper_opt.rs:15-185, one point per call."""
+    def f(th):
+        v = -obj.values(np.asarray(th)[None])["g"][0]
+        if not np.isfinite(v):
+            raise RuntimeError("objective is not finite")
+        return v
+
+    m = len(theta0)
+    verts = [np.asarray(theta0, dtype=float).copy()]
+    fv = [f(verts[0])]
+    for i in range(m):
+        v = verts[0].copy()
+        v[i] += step
+        verts.append(v)
+        fv.append(f(v))
+    for _ in range(max_iter):
+        order = sorted(range(m + 1), key=lambda i: fv[i])
+        verts = [verts[i] for i in order]
+        fv = [fv[i] for i in order]
+        if np.std(fv) < tol:  # population std, as the reference
+            return verts[0]
+        cen = np.mean(verts[:m], axis=0)
+        refl = cen + (cen - verts[m])
+        fr = f(refl)
+        if fv[0] <= fr < fv[m - 1]:
+            verts[m], fv[m] = refl, fr
+            continue
+        if fr < fv[0]:
+            exp_ = cen + 2.0 * (refl - cen)
+            fe = f(exp_)
+            if fe < fr:
+                verts[m], fv[m] = exp_, fe
+            else:
+                verts[m], fv[m] = refl, fr
+            continue
+        if fr < fv[m]:
+            con = cen + 0.5 * (refl - cen)
+            fc = f(con)
+            if fc < fr:
+                verts[m], fv[m] = con, fc
+                continue
+        else:
+            con = cen + 0.5 * (verts[m] - cen)
+            fc = f(con)
+            if fc < fv[m]:
+                verts[m], fv[m] = con, fc
+                continue
+        for i in range(1, m + 1):
+            verts[i] = verts[0] + 0.5 * (verts[i] - verts[0])
+            fv[i] = f(verts[i])
+    return verts[0]
+
+
+def _hessian_step(mode):
+    return max(1e-4, 0.05 * float(np.max(np.abs(mode))))
+
+
+def _nodes(mode, H, f0=1.0):
+    lam, V = ccd.scaling_from_hessian(H)
+    z, w = ccd.ccd_design(len(mode), f0)
+    thetas = np.array([ccd.theta_from_z(mode, lam, V, zz) for zz in z])
+    return thetas, w
+
+
+def _finish(mode, H, thetas, w, r, n_evals, n_points, secs):
+    wts, mean, var, mlik = ccd.aggregate(w, r["g"], r["x"], r["var"])
+    return FitResult(mode, H, thetas, wts, r["g"], mean, var, mlik, n_evals, n_points, secs)
+
+
+def fit_batched(obj: Objective, theta0, **bfgs_kw) -> FitResult:
+    """Mode by batched BFGS, Hessian stencil and CCD nodes as single batches."""
+    secs = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mode, _ = bfgs_mode(obj, theta0, **bfgs_kw)
+    t1 = time.perf_counter()
+    h = _hessian_step(mode)
+    pts = ccd.hessian_points(mode, h)
+    H = ccd.hessian_from_values(obj.values(pts)["g"], len(mode), h)
+    t2 = time.perf_counter()
+    thetas, w = _nodes(mode, H)
+    r = obj.values(thetas, want_x=True, want_var=True)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    secs.update(mode=t1 - t0, hessian=t2 - t1, nodes=t3 - t2, total=t3 - t0)
+    return _finish(mode, H, thetas, w, r, obj.calls, obj.points, secs)
+
+
+def fit_plain(obj: Objective, theta0) -> FitResult:
+    """The reference's sequence, one theta-point per evaluation."""
+    secs = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mode = nelder_mead_mode(obj, theta0)
+    t1 = time.perf_counter()
+    h = _hessian_step(mode)
+    pts = ccd.hessian_points(mode, h)
+    H = ccd.hessian_from_values(np.concatenate([obj.values(p[None])["g"] for p in pts]), len(mode), h)
+    t2 = time.perf_counter()
+    thetas, w = _nodes(mode, H)
+    parts = [obj.values(t[None], want_x=True, want_var=True) for t in thetas]
+    r = {k: np.concatenate([p[k] for p in parts]) for k in ("g", "x", "var")}
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    secs.update(mode=t1 - t0, hessian=t2 - t1, nodes=t3 - t2, total=t3 - t0)
+    return _finish(mode, H, thetas, w, r, obj.calls, obj.points, secs)

@@ -246,6 +246,68 @@ class Batch:
         return out
 
 
+class MultiBatch(Batch):
+    """Heterogeneous batch: k matrices with different patterns factorised as
+    their block-diagonal union in one launch sequence (gmrf_multi_* in
+    include/gmrf_c.h). Values, right-hand sides and outputs use the union
+    layout (the per-pattern arrays concatenated in order); ``logdet`` returns
+    one log-determinant per pattern."""
+
+    def __init__(self, mats, capacity: int = 1):
+        mats = [as_csc(q) for q in mats]
+        self.k = len(mats)
+        if self.k < 1:
+            raise InvalidArgument("need at least one pattern")
+        self._keep = [(np.ascontiguousarray(q.indptr, dtype=np.int64),
+                       np.ascontiguousarray(q.indices, dtype=np.int64)) for q in mats]
+        ns = np.array([q.shape[0] for q in mats], dtype=np.int64)
+        pi = C.POINTER(C.c_int64)
+        ip = (pi * self.k)(*[_i64p(a) for a, _ in self._keep])
+        ix = (pi * self.k)(*[_i64p(b) for _, b in self._keep])
+        h = C.c_void_p()
+        check(L.lib.gmrf_multi_analyze(self.k, _i64p(ns), ip, ix, int(capacity), C.byref(h)))
+        self._m = h
+        self.n_off = np.empty(self.k + 1, dtype=np.int64)
+        self.nnz_off = np.empty(self.k + 1, dtype=np.int64)
+        self.order_kind = np.empty(self.k, dtype=np.int32)
+        check(L.lib.gmrf_multi_info(h, _i64p(self.n_off), _i64p(self.nnz_off),
+                                    self.order_kind.ctypes.data_as(C.POINTER(C.c_int32))))
+        self.capacity = int(capacity)
+        self.n = int(self.n_off[-1])
+        self.nnz = int(self.nnz_off[-1])
+        self.sym = None
+        self._h = C.c_void_p(L.lib.gmrf_multi_batch(h))  # owned by the multi handle
+
+    def __del__(self):
+        m = getattr(self, "_m", None)
+        if m:
+            self._h = None
+            L.lib.gmrf_multi_free(m)
+            self._m = None
+
+    def factorize(self, values):
+        v = self._dev(values, self.nnz, "values")
+        self._stream()
+        check(L.lib.gmrf_multi_factorize(self._m, v.shape[0], C.c_void_p(v.data_ptr()), self.nnz))
+        self.count = v.shape[0]
+        return self
+
+    def logdet(self, count=None):
+        """(log|Q_q| per matrix and pattern, status), numpy arrays of shape (count, k)."""
+        import torch
+
+        count = self.count if count is None else count
+        torch.cuda.current_stream().synchronize()
+        ld = np.empty((count, self.k))
+        st = np.empty((count, self.k), dtype=np.int32)
+        check(L.lib.gmrf_multi_logdet(self._m, count, C.c_void_p(ld.ctypes.data), C.c_void_p(st.ctypes.data)))
+        return ld, st
+
+    def split(self, t):
+        """Per-pattern views of a union-layout vector tensor (last axis n)."""
+        return [t[..., self.n_off[q]:self.n_off[q + 1]] for q in range(self.k)]
+
+
 class HipSolver:
     """InlaSolver on the GPU (drop-in for FaerCpuSolver, solver.rs:48-99)."""
 

@@ -321,3 +321,113 @@ def spde_gaussian_workload(nx=142, n_obs=50_000, seed=20241203, range_=0.3, sigm
     y = A @ x + sigma_eps * rng.standard_normal(n_obs)
     theta = np.array([math.log(1.0 / sigma_eps ** 2), math.log(tau), math.log(kappa)])
     return SpdeGaussianWorkload(g, A, y, theta, nx)
+
+
+# --------------------------------------------------------------------------
+# BYM2 + Poisson on random planar graphs (heterogeneous batch)
+# --------------------------------------------------------------------------
+def delaunay_adjacency(npts: int, rng: np.random.Generator):
+    """Neighbour lists of the Delaunay triangulation of npts uniform points in the unit square."""
+    from scipy.spatial import Delaunay
+
+    pts = rng.uniform(size=(npts, 2))
+    tri = Delaunay(pts)
+    nb = [set() for _ in range(npts)]
+    for a, b, c in tri.simplices:
+        nb[a].update((b, c))
+        nb[b].update((a, c))
+        nb[c].update((a, b))
+    return [sorted(s) for s in nb]
+
+
+def grounded(R: sp.csc_matrix) -> sp.csc_matrix:
+    """R without its last row and column (positive definite for a connected graph)."""
+    n = R.shape[0]
+    G = sp.csc_matrix(R[: n - 1, : n - 1])
+    G.sort_indices()
+    return G
+
+
+def scale_factors(structs, solver_batch=None):
+    """scale_model factors exp(mean log diag(R^+)) (crates/inla_math/src/design.rs:151-195)
+    of connected-graph structure matrices, without the dense eigen-decomposition.
+
+    For a connected Laplacian-like R with null space 1, R^+ = (I - 11^T/n) G (I - 11^T/n)
+    where G is the inverse of the grounded matrix (last node removed, padded with 0):
+    diag(R^+)_i = G_ii - 2 (G 1)_i / n + 1^T G 1 / n^2. G's diagonal comes from one
+    selected inversion and G 1 from one solve, all k graphs in one heterogeneous batch
+    on the GPU."""
+    import torch
+
+    from .solver import MultiBatch
+
+    gs = [grounded(R) for R in structs]
+    mb = MultiBatch(gs, capacity=1) if solver_batch is None else solver_batch
+    vals = torch.as_tensor(np.concatenate([g.data for g in gs]), dtype=torch.float64, device="cuda")
+    mb.factorize(vals[None])
+    ones = torch.ones((1, mb.n), dtype=torch.float64, device="cuda")
+    s_all = mb.solve(ones)
+    mb.selinv()
+    d_all = mb.selinv_diag()
+    ld, st = mb.logdet()
+    if np.any(st != 0):
+        raise ValueError("scale_factors: a grounded structure matrix is singular (disconnected graph?)")
+    out = []
+    for q, R in enumerate(structs):
+        n = R.shape[0]
+        a, b = mb.n_off[q], mb.n_off[q + 1]
+        s = np.r_[s_all[0, a:b].cpu().numpy(), 0.0]
+        d = np.r_[d_all[0, a:b].cpu().numpy(), 0.0]
+        tot = s.sum()
+        dg = d - 2.0 * s / n + tot / n ** 2
+        out.append(math.exp(np.mean(np.log(dg))))
+    return np.array(out)
+
+
+@dataclass
+class Bym2Model:
+    """One BYM2 (combined-field, besag.rs:192-219) + Poisson areal model: n regions,
+    Q(theta) = tau (1 - phi) I + tau phi Q*, Q* the scaled Besag structure."""
+
+    adj: list
+    R: sp.csc_matrix
+    scale: float
+    y: np.ndarray
+    indptr: np.ndarray = None
+    indices: np.ndarray = None
+    components: np.ndarray = None
+
+    def __post_init__(self):
+        n = self.R.shape[0]
+        I = sp.identity(n, format="csc")
+        self.indptr, self.indices = union_pattern([I, self.R])
+        self.components = np.vstack([values_on_pattern(I, self.indptr, self.indices),
+                                     values_on_pattern(self.R * self.scale, self.indptr, self.indices)])
+
+    @property
+    def n(self):
+        return self.R.shape[0]
+
+    def coefs(self, tau, phi):
+        return np.array([tau * (1.0 - phi), tau * phi])
+
+    def values(self, tau, phi):
+        return self.coefs(tau, phi) @ self.components
+
+
+def bym2_hetero_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.5, intercept=1.0):
+    """k independent BYM2 + Poisson models on Delaunay-neighbour graphs with sizes
+    uniform in [nmin, nmax]; u drawn from the model's Q, y ~ Poisson(exp(intercept + u))."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=k)
+    adjs = [delaunay_adjacency(int(s), rng) for s in sizes]
+    Rs = [besag_structure(a) for a in adjs]
+    scales = scale_factors(Rs)
+    models = []
+    for a, R, c in zip(adjs, Rs, scales):
+        m = Bym2Model(a, R, float(c), np.zeros(R.shape[0]))
+        Q = sp.csc_matrix((m.values(tau, phi), m.indices, m.indptr), shape=(m.n, m.n))
+        u = sample_gmrf(Q, rng)
+        m.y = rng.poisson(np.exp(intercept + u)).astype(float)
+        models.append(m)
+    return models
