@@ -340,47 +340,76 @@ def delaunay_adjacency(npts: int, rng: np.random.Generator):
     return [sorted(s) for s in nb]
 
 
-def grounded(R: sp.csc_matrix) -> sp.csc_matrix:
-    """R without its last row and column (positive definite for a connected graph)."""
+def rw2_structure(n: int) -> sp.csc_matrix:
+    """Second-order random-walk structure D2^T D2 (unit precision, equal spacing)."""
+    D = sp.diags([np.ones(n - 2), -2.0 * np.ones(n - 2), np.ones(n - 2)], [0, 1, 2], shape=(n - 2, n))
+    R = sp.csc_matrix(D.T @ D)
+    R.sort_indices()
+    return R
+
+
+def grounded(R: sp.csc_matrix, k: int = 1) -> sp.csc_matrix:
+    """R without its last k rows and columns (positive definite when the null space of R
+    has dimension k and no null vector vanishes on the last k nodes)."""
     n = R.shape[0]
-    G = sp.csc_matrix(R[: n - 1, : n - 1])
+    G = sp.csc_matrix(R[: n - k, : n - k])
     G.sort_indices()
     return G
 
 
-def scale_factors(structs, solver_batch=None):
-    """scale_model factors exp(mean log diag(R^+)) (crates/inla_math/src/design.rs:151-195)
-    of connected-graph structure matrices, without the dense eigen-decomposition.
+def ginv_diag_from_grounded(gdiag, gN, N):
+    """diag(R^+) from the grounded inverse G (zero-padded g-inverse of R):
+    R^+ = P G P with P = I - N N^T, N an orthonormal basis of the null space, so
+    diag(R^+)_i = G_ii - 2 sum_a N_ia (G N)_ia + sum_ab N_ia (N^T G N)_ab N_ib."""
+    NtGN = N.T @ gN
+    return gdiag - 2.0 * np.sum(N * gN, axis=1) + np.einsum("ia,ab,ib->i", N, NtGN, N)
 
-    For a connected Laplacian-like R with null space 1, R^+ = (I - 11^T/n) G (I - 11^T/n)
-    where G is the inverse of the grounded matrix (last node removed, padded with 0):
-    diag(R^+)_i = G_ii - 2 (G 1)_i / n + 1^T G 1 / n^2. G's diagonal comes from one
-    selected inversion and G 1 from one solve, all k graphs in one heterogeneous batch
-    on the GPU."""
+
+def null_basis(n: int, kind: str = "constant"):
+    """Orthonormal null-space basis: constants (Besag on a connected graph) or
+    constants + linear trend (RW2)."""
+    cols = [np.ones(n)] if kind == "constant" else [np.ones(n), np.arange(n, dtype=float)]
+    Nq, _ = np.linalg.qr(np.column_stack(cols))
+    return Nq
+
+
+def scale_factors(structs, kinds=None):
+    """scale_model factors exp(mean log diag(R^+)) (crates/inla_math/src/design.rs:151-195)
+    of intrinsic structure matrices, without the dense eigen-decomposition.
+
+    R^+ = P G P where G is the inverse of the grounded matrix (the last k nodes
+    removed, zero-padded: a generalised inverse of R) and P projects out the
+    k-dimensional null space. G's diagonal comes from one selected inversion and
+    G N from k solves, all structures in one heterogeneous batch on the GPU."""
     import torch
 
     from .solver import MultiBatch
 
-    gs = [grounded(R) for R in structs]
-    mb = MultiBatch(gs, capacity=1) if solver_batch is None else solver_batch
+    kinds = kinds or ["constant"] * len(structs)
+    Ns = [null_basis(R.shape[0], kd) for R, kd in zip(structs, kinds)]
+    gs = [grounded(R, N.shape[1]) for R, N in zip(structs, Ns)]
+    kmax = max(N.shape[1] for N in Ns)
+    mb = MultiBatch(gs, capacity=1)
     vals = torch.as_tensor(np.concatenate([g.data for g in gs]), dtype=torch.float64, device="cuda")
     mb.factorize(vals[None])
-    ones = torch.ones((1, mb.n), dtype=torch.float64, device="cuda")
-    s_all = mb.solve(ones)
-    mb.selinv()
-    d_all = mb.selinv_diag()
     ld, st = mb.logdet()
     if np.any(st != 0):
-        raise ValueError("scale_factors: a grounded structure matrix is singular (disconnected graph?)")
-    out = []
-    for q, R in enumerate(structs):
-        n = R.shape[0]
+        raise ValueError("scale_factors: a grounded structure matrix is singular")
+    rhs = np.zeros((kmax, mb.n))
+    for q, N in enumerate(Ns):
         a, b = mb.n_off[q], mb.n_off[q + 1]
-        s = np.r_[s_all[0, a:b].cpu().numpy(), 0.0]
-        d = np.r_[d_all[0, a:b].cpu().numpy(), 0.0]
-        tot = s.sum()
-        dg = d - 2.0 * s / n + tot / n ** 2
-        out.append(math.exp(np.mean(np.log(dg))))
+        rhs[: N.shape[1], a:b] = N[: b - a].T
+    sol = np.stack([mb.solve(torch.as_tensor(rhs[a][None], device="cuda"))[0].cpu().numpy()
+                    for a in range(kmax)])
+    mb.selinv()
+    d_all = mb.selinv_diag()[0].cpu().numpy()
+    out = []
+    for q, (R, N) in enumerate(zip(structs, Ns)):
+        n, k = R.shape[0], N.shape[1]
+        a, b = mb.n_off[q], mb.n_off[q + 1]
+        gdiag = np.r_[d_all[a:b], np.zeros(k)]
+        gN = np.vstack([sol[:k, a:b].T, np.zeros((k, k))])
+        out.append(math.exp(np.mean(np.log(ginv_diag_from_grounded(gdiag, gN, N)))))
     return np.array(out)
 
 
