@@ -444,9 +444,11 @@ class Bym2Model:
         return self.coefs(tau, phi) @ self.components
 
 
-def bym2_hetero_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.5, intercept=1.0):
+def bym2_hetero_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.5, intercept=1.0,
+                      sample=True):
     """k independent BYM2 + Poisson models on Delaunay-neighbour graphs with sizes
-    uniform in [nmin, nmax]; u drawn from the model's Q, y ~ Poisson(exp(intercept + u))."""
+    uniform in [nmin, nmax]; u drawn from the model's Q, y ~ Poisson(exp(intercept + u))
+    (with sample=False the counts stay zero: precision matrices only)."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(nmin, nmax + 1, size=k)
     adjs = [delaunay_adjacency(int(s), rng) for s in sizes]
@@ -455,8 +457,65 @@ def bym2_hetero_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.
     models = []
     for a, R, c in zip(adjs, Rs, scales):
         m = Bym2Model(a, R, float(c), np.zeros(R.shape[0]))
-        Q = sp.csc_matrix((m.values(tau, phi), m.indices, m.indptr), shape=(m.n, m.n))
-        u = sample_gmrf(Q, rng)
-        m.y = rng.poisson(np.exp(intercept + u)).astype(float)
+        if sample:
+            Q = sp.csc_matrix((m.values(tau, phi), m.indices, m.indptr), shape=(m.n, m.n))
+            u = sample_gmrf(Q, rng)
+            m.y = rng.poisson(np.exp(intercept + u)).astype(float)
         models.append(m)
     return models
+
+
+@dataclass
+class SpdePoissonWorkload:
+    """SPDE alpha=2 on an nx x nx lattice plus an intercept, Poisson counts:
+    latent x = (beta0, u), eta = beta0 + A u, y ~ Poisson(exp(eta)).
+    Internal theta = (log tau, log kappa); beta0 has a fixed prior precision."""
+
+    gmrf: AffineGmrf
+    A: sp.csc_matrix
+    y: np.ndarray
+    theta_true: np.ndarray
+    nx: int
+    int_prec: float
+
+    @property
+    def n(self):
+        return self.gmrf.n
+
+    def params(self, thetas):
+        """(coefs (B,4), dcoefs (B,2,4), lpar None, dlog_lpar None)."""
+        thetas = np.atleast_2d(thetas)
+        B = len(thetas)
+        coefs = np.zeros((B, 4))
+        dco = np.zeros((B, 2, 4))
+        for b, th in enumerate(thetas):
+            c, dc = spde_coefs(th[0], th[1])
+            coefs[b] = np.r_[self.int_prec, c]
+            dco[b, :, 1:] = dc
+        return coefs, dco, None, None
+
+    def log_prior(self, theta):
+        return pc_spde_prior(theta[0], theta[1])
+
+
+def spde_poisson_workload(nx=142, n_obs=50_000, seed=20241205, range_=0.3, sigma=1.0, intercept=1.0,
+                          int_prec=1e-3):
+    """The batched Poisson configuration on the SPDE mesh: y ~ Poisson(exp(1 + u))."""
+    g, _, _ = spde_components(nx, nx)
+    rng = np.random.default_rng(seed)
+    kappa = math.sqrt(8.0) / range_
+    tau = 1.0 / (math.sqrt(4 * math.pi) * kappa * sigma)
+    c, _ = spde_coefs(math.log(tau), math.log(kappa))
+    u = sample_gmrf(g.csc(c), rng)
+    Au = lattice_projector(nx, nx, rng.uniform(size=(n_obs, 2)))
+    y = rng.poisson(np.exp(intercept + Au @ u)).astype(float)
+    n = g.n + 1
+    A = sp.csc_matrix(sp.hstack([sp.csc_matrix(np.ones((n_obs, 1))), Au]))
+    A.sort_indices()
+    E0 = sp.csc_matrix(([1.0], ([0], [0])), shape=(n, n))
+    comps_q = [sp.block_diag([sp.csc_matrix((1, 1)), g.csc(e)], format="csc") for e in np.eye(3)]
+    indptr, indices = union_pattern([E0] + comps_q + [sp.csc_matrix(A.T @ A)])
+    comps = np.stack([values_on_pattern(E0, indptr, indices)] +
+                     [values_on_pattern(q, indptr, indices) for q in comps_q])
+    gm = AffineGmrf(n, indptr, indices, comps, ["intercept", "C0", "G1", "G1C0invG1"])
+    return SpdePoissonWorkload(gm, A, y, np.array([math.log(tau), math.log(kappa)]), nx, int_prec)

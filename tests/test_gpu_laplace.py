@@ -191,3 +191,24 @@ def test_fit_batched_and_plain_agree():
     ob = F.Objective(model, wl.params, wl.log_prior)
     gr = ob.values(rb.mode[None], grad=True)["grad"][0]
     assert np.max(np.abs(gr)) < 1e-3
+
+
+def test_spde_poisson_with_intercept_matches_oracle():
+    """The batched Poisson configuration (intercept + SPDE, dense intercept row in
+    Q_post) at small size: mode, mlik and variances against the oracle."""
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.spde_poisson_workload(nx=12, n_obs=300, seed=9)
+    g = wl.gmrf
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.2, -0.1], [-0.15, 0.1]])
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="poisson", A=wl.A,
+                         capacity=len(thetas))
+    c, _, _, _ = wl.params(thetas)
+    r = model.eval(c)
+    assert np.all(r["status"] == 0)
+    for b in range(len(thetas)):
+        ref = orc.find_mode(g.csc(c[b]), "poisson", "log", wl.y, np.ones(len(wl.y)), A=wl.A, keep_post=True)
+        assert abs(r["mlik"][b].item() - ref.mlik) < MLIK_ATOL * max(1.0, abs(ref.mlik))
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
+        assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < VAR_RTOL
