@@ -188,6 +188,18 @@ __device__ __forceinline__ d4 wgemm16(FA a, FB b, d4 acc) {
   return acc;
 }
 
+// 1/d from the hardware reciprocal estimate refined by two Newton-Raphson steps
+// x <- x + x (1 - d x) (each step squares the relative error); within about one
+// ulp of the correctly rounded quotient, with a shorter dependent chain than the
+// IEEE division sequence.
+__device__ __forceinline__ double recip_nr(double d) {
+  double x = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, x, 1.0);
+  x = fma(x, e, x);
+  e = fma(-d, x, 1.0);
+  return fma(x, e, x);
+}
+
 // One wave: LDL^T of the kn x kn (kn <= 16) diagonal block at (k0, k0) of F in
 // place (unit L below the diagonal, D on it) and Zk = L_kk^-1 (16 x 17, unit
 // lower, identity beyond kn). Lane (i = lane % 16, c = lane / 16) holds
@@ -217,7 +229,7 @@ __device__ __forceinline__ void ldlt_base16(double* F, int k0, int kn, double* Z
       zk[t] = __shfl(z[t], 16 * c + k, 64);    // Z[k][j]
     }
     const bool act = i > k && i < kn;
-    const double lik = ci / dk;
+    const double lik = ci * recip_nr(dk);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int j = c + 4 * t;
@@ -247,10 +259,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
   if (WITHZ) {
     for (int idx = threadIdx.x; idx < MB * LD; idx += 64 * NW) Zs[idx] = 0.0;
   }
+  if (wv == 0 && ncols > 0) ldlt_base16<LD>(F, 0, min(16, ncols), Zk);
+  __syncthreads();
   for (int k0 = 0; k0 < ncols; k0 += 16) {
     const int kn = min(16, ncols - k0), t0 = k0 + kn, kb = k0 >> 4;
-    if (wv == 0) ldlt_base16<LD>(F, k0, kn, Zk);
-    __syncthreads();
     // panel rows below the block; Z[kb][jb] <- Zk Z[kb][jb] (jb < kb); Z[kb][kb] = Zk
     const int nrb = (m - t0 + 15) >> 4;
     const int ntask2 = nrb + (WITHZ ? kb + 1 : 0);
@@ -302,10 +314,17 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
       }
     }
     __syncthreads();
-    // trailing update A22 -= W L21^T (lower blocks) and Z[ib][jb] -= L[ib][kb] Z[kb][jb]
+    // trailing update A22 -= W L21^T (lower blocks) and Z[ib][jb] -= L[ib][kb] Z[kb][jb].
+    // Look-ahead: task 0 is the next diagonal block; wave 0 takes only that one
+    // (when there are other waves) and then factors the next block right away,
+    // while the other waves finish the remaining tasks.
     const int ntb = nrb * (nrb + 1) / 2;
     const int ntask3 = ntb + (WITHZ ? nrb * (kb + 1) : 0);
-    for (int task = wv; task < ntask3; task += NW) {
+    const int k1 = k0 + 16;
+    const int t_first = NW == 1 ? 0 : (wv == 0 ? 0 : wv);
+    const int t_last = NW == 1 ? ntask3 : (wv == 0 ? min(1, ntask3) : ntask3);
+    const int t_step = NW == 1 ? 1 : (wv == 0 ? 1 : NW - 1);
+    for (int task = t_first; task < t_last; task += t_step) {
       d4 acc = {0.0, 0.0, 0.0, 0.0};
       if (task < ntb) {
         int ib = 0, rem = task;
@@ -350,6 +369,7 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         }
       }
     }
+    if (wv == 0 && k1 < ncols) ldlt_base16<LD>(F, k1, min(16, ncols - k1), Zk);
     __syncthreads();
   }
 }
