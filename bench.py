@@ -228,6 +228,66 @@ def hetero_compare(k=64, reps=3):
             "work": "LDL^T + selected-inverse diagonal + one solve per model"}
 
 
+def c4_solver_line(nr=100, T=20, probes=6):
+    """Full-size Besag x AR1 (100 x 100 rook lattice x T = 20, n = 200,000, the
+    reference's chooser picks RCM: a chain of dense fronts): factorise, solve and
+    selected-inverse diagonal of Q_post = Q(tau=2, rho=0.7) + diag(2.5) (binomial(10)
+    weights at eta = 0), at batch 1, with size-independent checks: log|2Q| - log|Q|
+    = n log 2, the solve residual, and diagonal entries against solves of e_i."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+
+    from inla_gpu import workloads as W
+    from inla_gpu.solver import Batch, Symbolic
+
+    wl = W.besag_ar1_workload(nr=nr, T=T, sample=False)
+    c, _, _, _ = wl.params(wl.theta_true[None])
+    Q = sp.csc_matrix(wl.gmrf.csc(c[0]) + 2.5 * sp.identity(wl.n))
+    Q.sort_indices()
+    t0 = time.perf_counter()
+    sym = Symbolic.from_csc(Q)
+    st = sym.stats()
+    bt = Batch(sym, 1)
+    t_setup = time.perf_counter() - t0
+    v = torch.as_tensor(Q.data[None], device="cuda")
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(wl.n)
+    bd = torch.as_tensor(b[None], device="cuda")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - a
+
+    _, tf = timed(lambda: bt.factorize(v))
+    ld1, st1 = bt.logdet()
+    x, ts = timed(lambda: bt.solve(bd))
+    _, tsel = timed(lambda: bt.selinv())
+    dg = bt.selinv_diag()[0].cpu().numpy()
+    xr = x[0].cpu().numpy()
+    resid = float(np.linalg.norm(Q @ xr - b) / np.linalg.norm(b))
+    idx = rng.choice(wl.n, probes, replace=False)
+    dmax = 0.0
+    for i in idx:
+        e = torch.zeros((1, wl.n), dtype=torch.float64, device="cuda")
+        e[0, i] = 1.0
+        dmax = max(dmax, abs(bt.solve(e)[0, i].item() - dg[i]) / dg[i])
+    bt.factorize(2.0 * v)
+    ld2, _ = bt.logdet()
+    return {"workload": f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7",
+            "order_kind": int(st["order_kind"]), "nsuper": int(st["nsuper"]), "levels": int(st["nlevels"]),
+            "nnz_l": int(st["nnz_l"]), "max_front": int(st["max_front"]),
+            "flops_ldlt": st["flops_ldlt"], "flops_selinv": st["flops_selinv"],
+            "setup_s": t_setup, "factorize_s": tf, "solve_s": ts, "selinv_s": tsel,
+            "ldlt_tflops": st["flops_ldlt"] / tf / 1e12, "selinv_tflops": st["flops_selinv"] / tsel / 1e12,
+            "status": int(st1[0]), "logdet": float(ld1[0]),
+            "logdet_scaling_err": float(abs((ld2[0] - ld1[0]) - wl.n * math.log(2.0))),
+            "solve_rel_residual": resid, "selinv_diag_vs_solves_rel": dmax}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +297,8 @@ def main():
     ap.add_argument("--no-fit", action="store_true", help="skip the plain-path and end-to-end fit lines")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Poisson and heterogeneous-batch lines")
+    ap.add_argument("--c4", action="store_true",
+                    help="add the full-size Besag x AR1 solver line (about 80 GB of HBM)")
     ap.add_argument("--nx", type=int, default=142)
     ap.add_argument("--n-obs", type=int, default=50_000)
     args = ap.parse_args()
@@ -368,6 +430,10 @@ def main():
         log("== heterogeneous batch: 64 BYM2 models")
         res["hetero"] = hetero_compare()
         log(json.dumps(res["hetero"]))
+    if args.c4:
+        log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
+        res["c4"] = c4_solver_line()
+        log(json.dumps(res["c4"]))
     if not args.no_cpu_baseline:
         log("== cpu baseline (oracle, 1 thread, 1 theta-point)")
         res["cpu_baseline"] = cpu_baseline(wl, wl.theta_true, 25)

@@ -519,3 +519,139 @@ def spde_poisson_workload(nx=142, n_obs=50_000, seed=20241205, range_=0.3, sigma
                      [values_on_pattern(q, indptr, indices) for q in comps_q])
     gm = AffineGmrf(n, indptr, indices, comps, ["intercept", "C0", "G1", "G1C0invG1"])
     return SpdePoissonWorkload(gm, A, y, np.array([math.log(tau), math.log(kappa)]), nx, int_prec)
+
+
+# --------------------------------------------------------------------------
+# Space x time configurations (Kronecker precisions, group-major layout)
+# --------------------------------------------------------------------------
+def ar1_pieces(T: int):
+    """(I, M_inner, O) with the reference AR1 precision (crates/inla_stats/src/ar1.rs:9-63)
+    tau * (I + rho^2 M_inner - rho O): diagonal 1 at the ends and 1 + rho^2 inside,
+    off-diagonal -rho."""
+    I = sp.identity(T, format="csc")
+    inner = np.ones(T)
+    inner[0] = inner[-1] = 0.0
+    M = sp.diags(inner, format="csc")
+    O = sp.csc_matrix(sp.diags([np.ones(T - 1), np.ones(T - 1)], [-1, 1]))
+    return I, M, O
+
+
+def rho_of(theta_rho):
+    """rho = 2 / (1 + exp(-theta)) - 1, clamped to +-0.999 (structured.rs:353), and drho/dtheta."""
+    rho = 2.0 / (1.0 + math.exp(-theta_rho)) - 1.0
+    rc = min(max(rho, -0.999), 0.999)
+    return rc, (0.5 * (1.0 - rho * rho) if rc == rho else 0.0)
+
+
+@dataclass
+class KronWorkload:
+    """A Kronecker-structured latent model with an affine precision, observations,
+    a likelihood family and optional linear constraints."""
+
+    gmrf: AffineGmrf
+    A: object
+    y: np.ndarray
+    obs_a: np.ndarray
+    obs_b: object
+    family: str
+    constraints: object
+    theta_true: np.ndarray
+    params_fn: object
+
+    @property
+    def n(self):
+        return self.gmrf.n
+
+    def params(self, thetas):
+        return self.params_fn(np.atleast_2d(thetas))
+
+
+def besag_ar1_workload(nr=100, T=20, seed=20241206, tau=2.0, rho=0.7, trials=10, scale=None,
+                       sample=True):
+    """Besag (rook lattice) x AR1 group model, group-major (time outer), binomial(trials)
+    logit, one observation per cell, 20 sum-to-zero rows (one per time point):
+    Q = tau (R_ar1(rho) x Q*_besag), Q* the scaled Besag structure. With sample=False
+    the counts stay zero (precision only)."""
+    adj = rook_adjacency(nr, nr)
+    R = besag_structure(adj)
+    if scale is None:
+        scale = float(scale_factors([R])[0])
+    Qs = sp.csc_matrix(R * scale)
+    I, M, O = ar1_pieces(T)
+    mats = [sp.kron(I, Qs, format="csc"), sp.kron(M, Qs, format="csc"), sp.kron(O, Qs, format="csc")]
+    indptr, indices = union_pattern(mats)
+    comps = np.stack([values_on_pattern(m_, indptr, indices) for m_ in mats])
+    n = nr * nr * T
+    g = AffineGmrf(n, indptr, indices, comps, ["I x Q*", "M x Q*", "O x Q*"])
+    rng = np.random.default_rng(seed)
+    th_rho = math.log((1 + rho) / (1 - rho))  # inverse of rho_of
+    c_true = tau * np.array([1.0, rho * rho, -rho])
+    ns = nr * nr
+    ca = np.zeros((T, n))
+    for t in range(T):
+        ca[t, t * ns:(t + 1) * ns] = 1.0
+    y = np.zeros(n)
+    if sample:
+        # x ~ N(0, Q^-1) restricted to the constraints (sample, then project)
+        x = sample_gmrf(sp.csc_matrix(g.csc(c_true) + sp.identity(n) * 1e-6), rng)
+        x = x - ca.T @ np.linalg.solve(ca @ ca.T, ca @ x)
+        y = rng.binomial(trials, 1.0 / (1.0 + np.exp(-x))).astype(float)
+
+    def params(thetas):
+        B = len(thetas)
+        coefs = np.zeros((B, 3))
+        dco = np.zeros((B, 2, 3))
+        for b, th in enumerate(thetas):
+            t_ = math.exp(th[0])
+            r_, dr = rho_of(th[1])
+            coefs[b] = t_ * np.array([1.0, r_ * r_, -r_])
+            dco[b, 0] = coefs[b]
+            dco[b, 1] = t_ * np.array([0.0, 2.0 * r_, -1.0]) * dr
+        return coefs, dco, None, None
+
+    return KronWorkload(g, None, y, np.full(n, float(trials)), None, "binomial", (ca, np.zeros(T)),
+                        np.array([math.log(tau), th_rho]), params)
+
+
+def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3):
+    """SPDE alpha=2 (nx x nx lattice) x RW2 over T time steps, group-major; negative
+    binomial (size) observations uniform in space x time through the barycentric
+    projector of their time slice: Q = Q_spde(tau, kappa) x (R_rw2 + eps I)."""
+    g_s, _, _ = spde_components(nx, nx)
+    Rt = sp.csc_matrix(rw2_structure(T) + eps * sp.identity(T))
+    mats = [sp.kron(Rt, sp.csc_matrix((g_s.components[k], g_s.indices, g_s.indptr), shape=(g_s.n, g_s.n)),
+                    format="csc") for k in range(3)]
+    indptr, indices = union_pattern(mats)
+    comps = np.stack([values_on_pattern(m_, indptr, indices) for m_ in mats])
+    n = g_s.n * T
+    g = AffineGmrf(n, indptr, indices, comps, ["Rt x C0", "Rt x G1", "Rt x G1C0invG1"])
+    rng = np.random.default_rng(seed)
+    tslot = rng.integers(0, T, size=n_obs)
+    As = lattice_projector(nx, nx, rng.uniform(size=(n_obs, 2)))
+    As = sp.csr_matrix(As)
+    rows, cols, vals = [], [], []
+    for i in range(n_obs):
+        a, b = As.indptr[i], As.indptr[i + 1]
+        rows += [i] * (b - a)
+        cols += list(tslot[i] * g_s.n + As.indices[a:b])
+        vals += list(As.data[a:b])
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(n_obs, n))
+    A.sort_indices()
+    # the pattern must contain A^T A (neighbours within a triangle of one time slice: already in G1)
+    th = np.array([math.log(0.5), math.log(3.0)])
+    c, _ = spde_coefs(th[0], th[1])
+    x = sample_gmrf(g.csc(c), rng)
+    mu = np.exp(0.5 + A @ x)
+    y = rng.negative_binomial(size, size / (size + mu)).astype(float)
+
+    def params(thetas):
+        B = len(thetas)
+        coefs = np.zeros((B, 3))
+        dco = np.zeros((B, 2, 3))
+        for b, t_ in enumerate(thetas):
+            cc, dc = spde_coefs(t_[0], t_[1])
+            coefs[b] = cc
+            dco[b] = dc
+        return coefs, dco, None, None
+
+    return KronWorkload(g, A, y, np.full(n_obs, math.exp(0.5)), np.full(n_obs, size), "nbinomial", None, th, params)

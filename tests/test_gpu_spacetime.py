@@ -1,0 +1,94 @@
+"""Space x time configurations at test size against the oracle's Newton loop:
+Besag x AR1 group model, binomial(10), one sum-to-zero row per time point
+(kappa A_c^T A_c augmentation, constraints.rs:275-313); SPDE x RW2, negative
+binomial (size 5) through the space-time projector. Bars as the other Laplace
+tests: mlik 1e-8 absolute (scaled by max(1, |mlik|)), mode 1e-9 relative,
+variances 1e-9 relative."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+KAPPA = 3.059023205018258e6  # the oracle's (and reference's) augmentation constant
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _besag_ar1_model(nr=6, T=4):
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.besag_ar1_workload(nr=nr, T=T, seed=13)
+    ca_raw, ce = wl.constraints
+    ns = nr * nr
+    ca = ca_raw / math.sqrt(ns)
+    Kc = sp.csc_matrix(KAPPA * ca.T @ ca)
+    base = [wl.gmrf.csc(e) for e in np.eye(3)]
+    indptr, indices = W.union_pattern(base + [Kc])
+    comps = np.stack([W.values_on_pattern(m, indptr, indices) for m in base + [Kc]])
+    model = LaplaceModel(wl.n, indptr, indices, comps, wl.y, family="binomial", obs_a=wl.obs_a,
+                         constraints=(ca, ce), capacity=3)
+    return wl, model, ca, ce
+
+
+def test_besag_ar1_binomial_constrained_matches_oracle():
+    wl, model, ca, ce = _besag_ar1_model()
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.3, -0.2], [-0.2, 0.4]])
+    c3, _, _, _ = wl.params(thetas)
+    r = model.eval(np.c_[c3, np.ones(len(thetas))])
+    assert np.all(r["status"] == 0)
+    for b in range(len(thetas)):
+        ref = orc.find_mode(wl.gmrf.csc(c3[b]), "binomial", "logit", wl.y, wl.obs_a, constraints=(ca, ce),
+                            keep_post=True)
+        assert abs(r["mlik"][b].item() - ref.mlik) < 1e-8 * max(1.0, abs(ref.mlik))
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
+        # every time slice sums to zero at the mode
+        assert np.max(np.abs(ca @ r["x"][b].cpu().numpy())) < 1e-8
+
+
+def test_besag_ar1_gradient_matches_central_fd():
+    wl, model, ca, ce = _besag_ar1_model()
+    th = wl.theta_true + np.array([0.1, -0.1])
+    c3, dc3, _, _ = wl.params(th[None])
+    r = model.eval(np.c_[c3, [1.0]], dcoefs=np.concatenate([dc3, np.zeros((1, 2, 1))], axis=2))
+    h = 1e-4
+    fd = []
+    for l in range(2):
+        vals = []
+        for s in (h, -h):
+            t = th.copy()
+            t[l] += s
+            c, _, _, _ = wl.params(t[None])
+            vals.append(orc.find_mode(wl.gmrf.csc(c[0]), "binomial", "logit", wl.y, wl.obs_a,
+                                      constraints=(ca, ce)).mlik)
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    g = r["grad"][0].cpu().numpy()
+    assert _rel(g, np.array(fd)) < 1e-5, (g, fd)
+
+
+def test_spde_rw2_negbin_matches_oracle():
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.spde_rw2_workload(nx=8, T=4, n_obs=400, seed=17)
+    g = wl.gmrf
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.2, -0.15]])
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="nbinomial", A=wl.A,
+                         obs_a=wl.obs_a, obs_b=wl.obs_b, capacity=len(thetas))
+    c, _, _, _ = wl.params(thetas)
+    r = model.eval(c)
+    assert np.all(r["status"] == 0)
+    for b in range(len(thetas)):
+        ref = orc.find_mode(g.csc(c[b]), "nbinomial", "log", wl.y, wl.obs_a, pb=wl.obs_b, A=wl.A,
+                            keep_post=True)
+        assert abs(r["mlik"][b].item() - ref.mlik) < 1e-8 * max(1.0, abs(ref.mlik))
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
+        assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < 1e-9
