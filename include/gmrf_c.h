@@ -67,7 +67,7 @@ typedef struct gmrf_symbolic_stats {
   int64_t nsuper;
   int64_t nnz_l;         /* structural nnz(L) incl. the diagonal */
   int64_t l_size;        /* doubles of the supernodal L storage (incl. relaxation zeros) */
-  int64_t u_size;        /* doubles of the update workspace */
+  int64_t u_size;        /* doubles of the update workspace (pooled by front lifetime) */
   int64_t max_front;
   double flops_ldlt;     /* sum_j r_j (r_j + 3) */
   double flops_selinv;   /* sum_j 2 r_j (r_j + 1) + n */

@@ -53,6 +53,7 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
                      : order == GMRF_ORDER_GIVEN   ? OrderRequest::Given
                                                    : OrderRequest::Chooser;
   d->S = analyze(n, indptr, indices, req, given);
+  d->pools = plan_pools(d->S);
   d->device = g_device;
   d->order_req = order;
   if (order == GMRF_ORDER_GIVEN) d->given.assign(given, given + n);
@@ -104,7 +105,7 @@ gmrf_status gmrf_symbolic_get_stats(const gmrf_symbolic* sym, gmrf_symbolic_stat
   o->nsuper = S.nsuper;
   o->nnz_l = S.nnz_l;
   o->l_size = S.l_size;
-  o->u_size = S.u_size;
+  o->u_size = sym->d->pools.u_size;
   o->max_front = S.max_front;
   o->flops_ldlt = S.flops_ldlt;
   o->flops_selinv = S.flops_selinv;

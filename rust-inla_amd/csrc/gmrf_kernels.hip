@@ -3,8 +3,9 @@
 // Storage (per matrix of a batch, offsets in doubles):
 //   L block of supernode s: m x w column-major (ld = m) at l_off[s]; rows are the
 //     w supernode columns followed by R_s. D sits on the diagonal, unit-L below.
-//   U workspace of s: r x r column-major at u_off[s]; holds the Schur update
-//     during the factorisation and Sigma_{R_s R_s} during selected inversion.
+//   U workspace of s: r x r column-major; holds the Schur update during the
+//     factorisation (at u_off[s]) and Sigma_{R_s R_s} during selected inversion
+//     (at u_off_sel[s]); both offset maps share one pool sized by lifetimes.
 //   S (selected inverse) uses the L layout.
 // Batches are laid out matrix after matrix; blockIdx.y selects the matrix.
 //
@@ -465,12 +466,11 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
 // ---------------------------------------------------------------------------
 
 // Assembly of one 64x64 tile (tile row I, tile column J, I >= J) of a front:
-// input entries through the dense L-slot map plus every child's update through
-// the child's inverse row map (children in fixed order: deterministic). Thread
-// (row i = tid % 64) owns tile columns tid / 64 + 4u, u < 16; each round of
-// loads covers all 16 elements before any is used.
+// the sum of every child's update through the child's inverse row map (children
+// in fixed order: deterministic); the input entries are added afterwards by
+// k_fl_input. Thread (row i = tid % 64) owns tile columns tid / 64 + 4u, u < 16;
+// each round of loads covers all 16 elements before any is used.
 __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
-                                                     const double* __restrict__ vals, int64_t ldv,
                                                      double* __restrict__ L,
                                                      double* __restrict__ U) {
   const int32_t* tk = tasks + 3 * blockIdx.x;
@@ -479,29 +479,12 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
-  const double* vb = vals + (int64_t)b * ldv;
-  const int32_t* am = P.dense_amap + P.l_off[s];
   const double* Um = U + (int64_t)b * P.u_size;
   const int i = I * kTile + (threadIdx.x & 63), jb = J * kTile + (threadIdx.x >> 6);
   const int ic = min(i, m - 1);
   double v[16];
-  {
-    int e[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int j = jb + 4 * u;
-      e[u] = am[(int64_t)min(j, w - 1) * m + ic];
-    }
-    double x[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) x[u] = vb[max(e[u], 0)];
-    ISSUE_FENCE();
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int j = jb + 4 * u;
-      v[u] = (j < w && e[u] >= 0) ? x[u] : 0.0;
-    }
-  }
+  for (int u = 0; u < 16; ++u) v[u] = 0.0;
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int32_t* inv = P.cinv + P.cinv_off[cq];
     const int c = P.child_idx[cq];
@@ -525,6 +508,33 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
   for (int u = 0; u < 16; ++u) {
     const int j = jb + 4 * u;
     if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) = v[u];
+  }
+}
+
+// Input entries of the large fronts of a level, added after the children's
+// updates: one workgroup per (front, matrix); every L slot receives at most one
+// entry (the lower-triangle map of the input pattern), so the adds do not race.
+__global__ __launch_bounds__(256) void k_fl_input(DevPlan P, const int32_t* __restrict__ list,
+                                                  const double* __restrict__ vals, int64_t ldv,
+                                                  double* __restrict__ L) {
+  const int s = list[blockIdx.x];
+  const int b = blockIdx.y;
+  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* vb = vals + (int64_t)b * ldv;
+  const int64_t q0 = P.fa_ptr[s], q1 = P.fa_ptr[s + 1];
+  for (int64_t qb = q0; qb < q1; qb += 4 * 256) {
+    int slot[4];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t q = min(qb + threadIdx.x + 256 * u, q1 - 1);
+      slot[u] = P.fa_slot[q];
+      v[u] = vb[P.fa_csc[q]];
+    }
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (qb + threadIdx.x + 256 * u < q1) Lb[slot[u]] += v[u];
   }
 }
 
@@ -1086,7 +1096,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
     const int p = P.sn_parent[s];
     const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
     const bool plarge = P.is_large[p] != 0;
-    const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off[p];
+    const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off_sel[p];
     const int32_t* rl = P.rel + P.rows_ptr[s];
     const int ri = rl[min(max(i - w, 0), r - 1)];
     int rj[NJ];
@@ -1111,7 +1121,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       v[t] = base[off];
     }
     ISSUE_FENCE();
-    double* Us = Um + P.u_off[s];
+    double* Us = Um + P.u_off_sel[s];
 #pragma unroll
     for (int t = 0; t < NJ; ++t) {
       const int j = c + 4 * t;
@@ -1257,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_sl_gather(DevPlan P, const int32_t* __r
   const int p = P.sn_parent[s];
   const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
   const bool plarge = P.is_large[p] != 0;
-  const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off[p];
+  const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off_sel[p];
   const int32_t* rl = P.rel + P.rows_ptr[s];
   const double* Sm = S + (int64_t)b * P.l_size;
   const double* Um = U + (int64_t)b * P.u_size;
@@ -1330,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
   const int R0 = c1 + I * kTile;
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
-  double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   TileAcc acc;
   tile_zero(acc);
   mma_tile<false, true>(
@@ -1346,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
       nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int pp, double v) {
     const int i = R0 + ii;
-    if (i < m && pp < nb) Tb[(int64_t)(c0 + pp) * m + i] = v;
+    if (i < m && pp < nb) Tb[(int64_t)pp * m + i] = v;
   });
 }
 
@@ -1371,7 +1381,7 @@ __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __res
   const int R0 = c1 + I * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
-  const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   TileAcc acc;
   tile_zero(acc);
   mma_tile<false, true>(
@@ -1381,7 +1391,7 @@ __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __res
         return i < m ? v : 0.0;
       },
       [&](int jj, int kk) {
-        const double v = Tb[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk];
+        const double v = Tb[(int64_t)min(jj, nb - 1) * m + c1 + kk];
         return jj < nb ? v : 0.0;
       },
       m - c1, As, Bs, acc);
@@ -1428,7 +1438,7 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
-  const double* Tb = T + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   if (tid < kTile) {
     const int q = min(tid, nb - 1);
     dinv[tid] = 1.0 / Lb[(int64_t)(c0 + q) * m + c0 + q];
@@ -1450,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   if (m > c1)
     mma_tile<true, true>(
         [&](int ii, int kk) {
-          const double v = Tb[(int64_t)(c0 + min(ii, nb - 1)) * m + c1 + kk];
+          const double v = Tb[(int64_t)min(ii, nb - 1) * m + c1 + kk];
           return ii < nb ? -v : 0.0;
         },
         [&](int jj, int kk) {
@@ -1516,10 +1526,17 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
   return hipGetLastError();
 }
 
-hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int B,
-                              const double* vals, int64_t ldv, double* L, double* U, hipStream_t st) {
+hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int B, double* L,
+                              double* U, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, vals, ldv, L, U);
+  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, L, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_fl_input(const DevPlan& P, const int32_t* list, int count, int B, const double* vals,
+                           int64_t ldv, double* L, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fl_input, dim3(count, B), dim3(256), 0, st, P, list, vals, ldv, L);
   return hipGetLastError();
 }
 

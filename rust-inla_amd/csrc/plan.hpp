@@ -18,7 +18,8 @@ struct DevPlan {
   const int32_t* sn_r;       // [nsuper] rows below the supernode
   const int32_t* sn_parent;  // [nsuper] -1 for roots
   const int64_t* l_off;      // [nsuper]
-  const int64_t* u_off;      // [nsuper]
+  const int64_t* u_off;      // [nsuper] update block during factorisation (pooled by lifetime)
+  const int64_t* u_off_sel;  // [nsuper] Sigma_RR block during selected inversion (pooled)
   const int64_t* v_off;      // [nsuper]
   const int64_t* rows_ptr;   // [nsuper+1]
   const int32_t* rows;       // R_s (factor indices)
@@ -34,11 +35,12 @@ struct DevPlan {
 
   // Large fronts (m > kSmallFront): tile-parallel kernels.
   const int32_t* is_large;    // [nsuper]
-  const int64_t* w_off;       // [nsuper] dense m x m Sigma front (large fronts), -1 otherwise
-  int64_t w_size;             // doubles of the Sigma-front workspace per matrix
+  const int64_t* w_off;       // [nsuper] dense m x m Sigma front (large fronts, pooled), -1 otherwise
+  int64_t w_size;             // doubles of the Sigma-front pool per matrix
+  const int64_t* t_off;       // [nsuper] m x 64 panel staging L_hat (large fronts, pooled)
+  int64_t t_size;             // doubles of the staging pool per matrix
   const int64_t* z_off;       // [nsuper] per-panel 64 x 64 blocks Z = L11^-1 (large fronts)
   int64_t z_size;             // doubles of the Z workspace per matrix
-  const int32_t* dense_amap;  // [l_size] CSC entry assembled into each L slot, -1 if none
   const int64_t* cinv_off;    // [n children] per (front, child) slot: offset into cinv
   const int32_t* cinv;        // parent front position -> child row index, -1 if absent
 };

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -18,8 +20,10 @@
 namespace gmrf {
 hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                int64_t, double*, double*, hipStream_t);
-hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, const double*, int64_t,
-                              double*, double*, hipStream_t);
+hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, double*, double*,
+                              hipStream_t);
+hipError_t launch_fl_input(const DevPlan&, const int32_t*, int, int, const double*, int64_t, double*,
+                           hipStream_t);
 hipError_t launch_fl_diag(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                           hipStream_t);
 hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
@@ -183,6 +187,131 @@ struct LevelInfo {
   std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
 };
 
+// First-fit allocator over one workspace, in doubles; freed ranges coalesce.
+class FirstFit {
+ public:
+  int64_t alloc(int64_t len) {
+    len = (len + 7) & ~int64_t(7);
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= len) {
+        const int64_t off = it->first, rest = it->second - len;
+        free_.erase(it);
+        if (rest > 0) free_[off + len] = rest;
+        return off;
+      }
+    if (!free_.empty()) {
+      auto last = std::prev(free_.end());
+      if (last->first + last->second == top_) {
+        const int64_t off = last->first;
+        free_.erase(last);
+        top_ = off + len;
+        return off;
+      }
+    }
+    const int64_t off = top_;
+    top_ += len;
+    return off;
+  }
+  void release(int64_t off, int64_t len) {
+    len = (len + 7) & ~int64_t(7);
+    auto it = free_.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && off + len == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  int64_t size() const { return top_; }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+  int64_t top_ = 0;
+};
+
+// Workspace offsets pooled by lifetime over the level schedule.
+//  - factorisation update of s (r x r): written at level(s), read at
+//    level(parent); freed once the parent's level has run;
+//  - Sigma_RR of a small front s (r x r, selected inversion, levels walked from
+//    the root): written at level(s), read by its children; freed after the
+//    lowest child level (its own level for a leaf); shares the update buffer,
+//    whose size is the larger of the two peaks;
+//  - dense Sigma front of a large front s (m x m): lifetime as Sigma_RR;
+//  - L_hat panel staging of a large front (m x 64): its own level only.
+struct Pools {
+  std::vector<int64_t> u_fac, u_sel, w_off, t_off;
+  int64_t u_size = 1, w_size = 1, t_size = 1;
+};
+
+inline Pools plan_pools(const Symbolic& S) {
+  const int64_t ns = S.nsuper;
+  Pools P;
+  P.u_fac.assign(ns, 0);
+  P.u_sel.assign(ns, 0);
+  P.w_off.assign(ns, -1);
+  P.t_off.assign(ns, -1);
+  auto r_of = [&](int64_t s) { return S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s]; };
+  auto m_of = [&](int64_t s) { return S.sn_start[s + 1] - S.sn_start[s] + r_of(s); };
+  auto large = [&](int64_t s) { return m_of(s) > kSmallFront; };
+  std::vector<int32_t> lev(ns, 0);
+  for (int l = 0; l < S.nlevels; ++l)
+    for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) lev[S.level_sn[q]] = l;
+  FirstFit uf;
+  for (int l = 0; l < S.nlevels; ++l) {
+    for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int64_t s = S.level_sn[q], r = r_of(s);
+      if (r > 0) P.u_fac[s] = uf.alloc(r * r);
+    }
+    for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int64_t s = S.level_sn[q];
+      for (int64_t cq = S.child_ptr[s]; cq < S.child_ptr[s + 1]; ++cq) {
+        const int64_t c = S.child_idx[cq], rc = r_of(c);
+        if (rc > 0) uf.release(P.u_fac[c], rc * rc);
+      }
+    }
+  }
+  std::vector<std::vector<int64_t>> free_after(S.nlevels);
+  for (int64_t s = 0; s < ns; ++s) {
+    int fl = lev[s];
+    for (int64_t cq = S.child_ptr[s]; cq < S.child_ptr[s + 1]; ++cq)
+      fl = std::min(fl, lev[S.child_idx[cq]]);
+    free_after[fl].push_back(s);
+  }
+  FirstFit us, wf;
+  int64_t tmax = 0;
+  for (int l = S.nlevels - 1; l >= 0; --l) {
+    int64_t tl = 0;
+    for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int64_t s = S.level_sn[q], r = r_of(s), m = m_of(s);
+      if (large(s)) {
+        P.w_off[s] = wf.alloc(m * m);
+        P.t_off[s] = tl;
+        tl += m * kTile;
+      } else if (r > 0) {
+        P.u_sel[s] = us.alloc(r * r);
+      }
+    }
+    tmax = std::max(tmax, tl);
+    for (int64_t s : free_after[l]) {
+      const int64_t r = r_of(s), m = m_of(s);
+      if (large(s))
+        wf.release(P.w_off[s], m * m);
+      else if (r > 0)
+        us.release(P.u_sel[s], r * r);
+    }
+  }
+  P.u_size = std::max<int64_t>({uf.size(), us.size(), 1});
+  P.w_size = std::max<int64_t>(wf.size(), 1);
+  P.t_size = std::max<int64_t>(tmax, 1);
+  return P;
+}
+
 }  // namespace gmrf_rt
 
 using namespace gmrf_rt;
@@ -190,6 +319,7 @@ using namespace gmrf_rt;
 // Symbolic analysis plus its device plan; shared by batches and solvers.
 struct SymData {
   Symbolic S;
+  Pools pools;  // lifetime-pooled workspace offsets (host copy)
   int device = 0;
   DevArena arena;
   DevPlan plan{};
@@ -257,16 +387,16 @@ struct SymData {
     plan.nsuper = (int32_t)ns;
     plan.nnz = S.nnz_full;
     plan.l_size = S.l_size;
-    plan.u_size = S.u_size;
+    plan.u_size = pools.u_size;
     plan.v_size = S.v_size;
     plan.sn_start = arena.upload(sn_start);
     plan.sn_r = arena.upload(sn_r);
     plan.sn_parent = arena.upload(sn_parent);
     std::vector<int64_t> loff(S.l_off.begin(), S.l_off.begin() + ns);
-    std::vector<int64_t> uoff(S.u_off.begin(), S.u_off.begin() + ns);
     std::vector<int64_t> voff(S.v_off.begin(), S.v_off.begin() + ns);
     plan.l_off = arena.upload(loff);
-    plan.u_off = arena.upload(uoff);
+    plan.u_off = arena.upload(pools.u_fac);
+    plan.u_off_sel = arena.upload(pools.u_sel);
     plan.v_off = arena.upload(voff);
     plan.rows_ptr = arena.upload(S.sn_rows_ptr);
     plan.rows = arena.upload(rows);
@@ -340,8 +470,7 @@ struct SymData {
     build_large_plan(large, fa_ptr);
   }
 
-  // Task lists, inverse child maps, dense input map and Sigma-front offsets of
-  // the large fronts.
+  // Task lists, inverse child maps and Z offsets of the large fronts.
   void build_large_plan(const std::vector<int32_t>& large, const std::vector<int64_t>& fa_ptr) {
     const int64_t ns = S.nsuper;
     auto mw = [&](int64_t s, int64_t& w, int64_t& r, int64_t& m) {
@@ -350,20 +479,15 @@ struct SymData {
       m = w + r;
     };
     std::vector<int32_t> is_large(ns, 0);
-    std::vector<int64_t> w_off(ns, -1), z_off(ns, -1);
-    int64_t wsz = 0, zsz = 0;
+    std::vector<int64_t> z_off(ns, -1);
+    int64_t zsz = 0;
     for (int32_t s : large) {
       int64_t w, r, m;
       mw(s, w, r, m);
       is_large[s] = 1;
-      w_off[s] = wsz;
-      wsz += m * m;
       z_off[s] = zsz;
       zsz += ((w + kTile - 1) / kTile) * kTile * kTile;
     }
-    std::vector<int32_t> dense_amap(S.l_size, -1);
-    for (int64_t p = 0; p < S.nnz_full; ++p)
-      if (S.amap[p] >= 0) dense_amap[S.amap[p]] = (int32_t)p;
     std::vector<int64_t> cinv_off(std::max<int64_t>((int64_t)S.child_idx.size(), 1), -1);
     std::vector<int32_t> cinv;
     for (int32_t pf : large) {
@@ -379,11 +503,12 @@ struct SymData {
       }
     }
     plan.is_large = arena.upload(is_large);
-    plan.w_off = arena.upload(w_off);
-    plan.w_size = std::max<int64_t>(wsz, 1);
+    plan.w_off = arena.upload(pools.w_off);
+    plan.w_size = pools.w_size;
+    plan.t_off = arena.upload(pools.t_off);
+    plan.t_size = pools.t_size;
     plan.z_off = arena.upload(z_off);
     plan.z_size = std::max<int64_t>(zsz, 1);
-    plan.dense_amap = arena.upload(dense_amap);
     plan.cinv_off = arena.upload(cinv_off);
     plan.cinv = arena.upload(cinv.empty() ? std::vector<int32_t>(1, -1) : cinv);
     // structural work per front (for the per-launch roofline accounting)
@@ -567,9 +692,9 @@ struct gmrf_batch {
     stream = own;
     const Symbolic& Sy = sym->S;
     L = arena.alloc<double>((size_t)cap * Sy.l_size);
-    U = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.u_size, 1));
+    U = arena.alloc<double>((size_t)cap * sym->plan.u_size);
     S = arena.alloc<double>((size_t)cap * Sy.l_size);
-    T = arena.alloc<double>((size_t)cap * Sy.l_size);
+    T = arena.alloc<double>((size_t)cap * sym->plan.t_size);
     X = arena.alloc<double>((size_t)cap * Sy.n);
     V = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.v_size, 1));
     W = arena.alloc<double>((size_t)cap * sym->plan.w_size);
@@ -603,8 +728,11 @@ struct gmrf_batch {
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
         prof_launch("k_fl_assemble", 0.0, lv.asm_t.bytes * count, stream, [&] {
-          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, count, vals, ldv, Lf,
-                                    Uf, stream);
+          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, count, Lf, Uf, stream);
+        });
+        prof_launch("k_fl_input", 0.0, 0.0, stream, [&] {
+          return launch_fl_input(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, vals, ldv, Lf,
+                                 stream);
         });
         for (size_t k = 0; k < lv.fdiag.size(); ++k) {
           prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
@@ -669,7 +797,7 @@ struct gmrf_batch {
     const double* Lf = L + (int64_t)first * P.l_size;
     double* Sf = S + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
-    double* Tf = T + (int64_t)first * P.l_size;
+    double* Tf = T + (int64_t)first * P.t_size;
     double* Wf = W + (int64_t)first * P.w_size;
     const double* Zf = Z + (int64_t)first * P.z_size;
     const int32_t* tk = sym->d_tasks;
