@@ -228,31 +228,27 @@ def hetero_compare(k=64, reps=3):
             "work": "LDL^T + selected-inverse diagonal + one solve per model"}
 
 
-def c4_solver_line(nr=100, T=20, probes=6):
-    """Full-size Besag x AR1 (100 x 100 rook lattice x T = 20, n = 200,000, the
-    reference's chooser picks RCM: a chain of dense fronts): factorise, solve and
-    selected-inverse diagonal of Q_post = Q(tau=2, rho=0.7) + diag(2.5) (binomial(10)
-    weights at eta = 0), at batch 1, with size-independent checks: log|2Q| - log|Q|
-    = n log 2, the solve residual, and diagonal entries against solves of e_i."""
+def big_solver_line(Q, label, probes=6):
+    """Full-size solver line at batch 1: factorise, solve and selected-inverse diagonal
+    of one posterior precision, with size-independent checks: log|2Q| - log|Q| = n log 2,
+    the solve residual, and diagonal entries against solves of e_i."""
     import numpy as np
-    import scipy.sparse as sp
     import torch
 
-    from inla_gpu import workloads as W
     from inla_gpu.solver import Batch, Symbolic
 
-    wl = W.besag_ar1_workload(nr=nr, T=T, sample=False)
-    c, _, _, _ = wl.params(wl.theta_true[None])
-    Q = sp.csc_matrix(wl.gmrf.csc(c[0]) + 2.5 * sp.identity(wl.n))
-    Q.sort_indices()
+    n = Q.shape[0]
     t0 = time.perf_counter()
     sym = Symbolic.from_csc(Q)
     st = sym.stats()
+    t_sym = time.perf_counter() - t0
     bt = Batch(sym, 1)
     t_setup = time.perf_counter() - t0
+    print(f"[{label}] analysed in {t_sym:.1f} s, workspaces in {t_setup - t_sym:.1f} s", file=sys.stderr,
+          flush=True)
     v = torch.as_tensor(Q.data[None], device="cuda")
     rng = np.random.default_rng(1)
-    b = rng.standard_normal(wl.n)
+    b = rng.standard_normal(n)
     bd = torch.as_tensor(b[None], device="cuda")
 
     def timed(fn):
@@ -264,28 +260,58 @@ def c4_solver_line(nr=100, T=20, probes=6):
 
     _, tf = timed(lambda: bt.factorize(v))
     ld1, st1 = bt.logdet()
+    print(f"[{label}] factorised in {tf:.2f} s", file=sys.stderr, flush=True)
     x, ts = timed(lambda: bt.solve(bd))
     _, tsel = timed(lambda: bt.selinv())
+    print(f"[{label}] selected inversion in {tsel:.2f} s", file=sys.stderr, flush=True)
     dg = bt.selinv_diag()[0].cpu().numpy()
     xr = x[0].cpu().numpy()
     resid = float(np.linalg.norm(Q @ xr - b) / np.linalg.norm(b))
-    idx = rng.choice(wl.n, probes, replace=False)
+    idx = rng.choice(n, probes, replace=False)
     dmax = 0.0
     for i in idx:
-        e = torch.zeros((1, wl.n), dtype=torch.float64, device="cuda")
+        e = torch.zeros((1, n), dtype=torch.float64, device="cuda")
         e[0, i] = 1.0
         dmax = max(dmax, abs(bt.solve(e)[0, i].item() - dg[i]) / dg[i])
     bt.factorize(2.0 * v)
     ld2, _ = bt.logdet()
-    return {"workload": f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7",
+    return {"workload": label, "n": int(n), "nnz_q": int(Q.nnz),
             "order_kind": int(st["order_kind"]), "nsuper": int(st["nsuper"]), "levels": int(st["nlevels"]),
-            "nnz_l": int(st["nnz_l"]), "max_front": int(st["max_front"]),
+            "nnz_l": int(st["nnz_l"]), "l_size": int(st["l_size"]), "max_front": int(st["max_front"]),
             "flops_ldlt": st["flops_ldlt"], "flops_selinv": st["flops_selinv"],
-            "setup_s": t_setup, "factorize_s": tf, "solve_s": ts, "selinv_s": tsel,
+            "analyse_s": t_sym, "setup_s": t_setup, "factorize_s": tf, "solve_s": ts, "selinv_s": tsel,
             "ldlt_tflops": st["flops_ldlt"] / tf / 1e12, "selinv_tflops": st["flops_selinv"] / tsel / 1e12,
             "status": int(st1[0]), "logdet": float(ld1[0]),
-            "logdet_scaling_err": float(abs((ld2[0] - ld1[0]) - wl.n * math.log(2.0))),
+            "logdet_scaling_err": float(abs((ld2[0] - ld1[0]) - n * math.log(2.0))),
             "solve_rel_residual": resid, "selinv_diag_vs_solves_rel": dmax}
+
+
+def c4_solver_line(nr=100, T=20, probes=6):
+    """Full-size Besag x AR1 (100 x 100 rook lattice x T = 20, n = 200,000; the
+    reference's chooser picks RCM: a chain of dense fronts): Q_post = Q(tau=2, rho=0.7)
+    + diag(2.5) (binomial(10) weights at eta = 0)."""
+    import scipy.sparse as sp
+
+    from inla_gpu import workloads as W
+
+    wl = W.besag_ar1_workload(nr=nr, T=T, sample=False)
+    c, _, _, _ = wl.params(wl.theta_true[None])
+    Q = sp.csc_matrix(wl.gmrf.csc(c[0]) + 2.5 * sp.identity(wl.n))
+    Q.sort_indices()
+    return big_solver_line(Q, f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7", probes)
+
+
+def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
+    """Full-size SPDE x RW2 (245 x 245 lattice x 24 time steps, n = 1,440,600; RCM chain of
+    fronts up to 13k wide, about 100 GB of L): Q_post with the negative binomial (size 5)
+    Fisher weights of 500,000 observations at eta = 0.5."""
+    from inla_gpu import workloads as W
+
+    t0 = time.perf_counter()
+    Q = W.spde_rw2_posterior_precision(nx=nx, T=T, n_obs=n_obs)
+    print(f"[c5] Q_post built in {time.perf_counter() - t0:.1f} s (nnz {Q.nnz})", file=sys.stderr, flush=True)
+    return big_solver_line(Q, f"spde {nx}x{nx} x rw2(T={T}), n={Q.shape[0]}, {n_obs} NB obs, "
+                              "Q_post at tau=0.5, kappa=3", probes)
 
 
 def main():
@@ -298,7 +324,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Poisson and heterogeneous-batch lines")
     ap.add_argument("--c4", action="store_true",
-                    help="add the full-size Besag x AR1 solver line (about 80 GB of HBM)")
+                    help="add the full-size Besag x AR1 solver line (about 5 GB of HBM)")
+    ap.add_argument("--c5", action="store_true",
+                    help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
     ap.add_argument("--nx", type=int, default=142)
     ap.add_argument("--n-obs", type=int, default=50_000)
     args = ap.parse_args()
@@ -434,6 +462,10 @@ def main():
         log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
         res["c4"] = c4_solver_line()
         log(json.dumps(res["c4"]))
+    if args.c5:
+        log("== full-size SPDE x RW2 (n = 1.44M): factorise, solve, selected inversion")
+        res["c5"] = c5_solver_line()
+        log(json.dumps(res["c5"]))
     if not args.no_cpu_baseline:
         log("== cpu baseline (oracle, 1 thread, 1 theta-point)")
         res["cpu_baseline"] = cpu_baseline(wl, wl.theta_true, 25)

@@ -655,3 +655,28 @@ def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=
         return coefs, dco, None, None
 
     return KronWorkload(g, A, y, np.full(n_obs, math.exp(0.5)), np.full(n_obs, size), "nbinomial", None, th, params)
+
+
+def spde_rw2_posterior_precision(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3,
+                                 log_tau=math.log(0.5), log_kappa=math.log(3.0), intercept=0.5):
+    """Q_post = Q_spde(tau, kappa) x (R_rw2 + eps I) + A^T W A of the SPDE x RW2 negative
+    binomial model at full size (n = nx^2 T), with the Fisher weights W = mu size / (size + mu)
+    at eta = intercept for every observation; A is the barycentric projector of uniform
+    points of a uniform time slice, built vectorised (no sampling of the field)."""
+    g_s, _, _ = spde_components(nx, nx)
+    c, _ = spde_coefs(log_tau, log_kappa)
+    Qs = g_s.csc(c)
+    Rt = sp.csc_matrix(rw2_structure(T) + eps * sp.identity(T))
+    rng = np.random.default_rng(seed)
+    tslot = rng.integers(0, T, size=n_obs)
+    As = sp.csr_matrix(lattice_projector(nx, nx, rng.uniform(size=(n_obs, 2))))
+    cnt = np.diff(As.indptr)
+    cols = As.indices.astype(np.int64) + np.repeat(tslot.astype(np.int64) * g_s.n, cnt)
+    n = g_s.n * T
+    A = sp.csr_matrix((As.data, cols, As.indptr), shape=(n_obs, n))
+    mu = math.exp(intercept)
+    w = mu * size / (size + mu)
+    Q = sp.csc_matrix(sp.kron(Rt, Qs, format="csc") + w * (A.T @ A))
+    Q.sum_duplicates()
+    Q.sort_indices()
+    return Q
