@@ -877,16 +877,20 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
   for (int j = tid; j < w; j += blockDim.x) Xb[a0 + j] = t[j];
 }
 
-// Large fronts, blocked by the factorisation's 64-column panels with the stored
-// Z = L_PP^-1: y_P = Z f_P, then f_B -= L[B,P] y_P for the rows below. 1024
-// threads; both products split their K dimension over waves so that each panel
-// costs one or two rounds of loads, reduced through LDS.
+// Large fronts, forward, in two launches per level. The triangular part (one
+// workgroup per front) solves the supernode's own w x w block, blocked by the
+// factorisation's 64-column panels with the stored Z = L_PP^-1: y_P = Z f_P, then
+// f_B -= L[B,P] y_P for the rows B below the panel inside the supernode. Both
+// products split their K dimension over waves so that each panel costs one or two
+// rounds of loads, reduced through LDS. Children's update entries that land in
+// the supernode's columns are added first; those landing in R_s are gathered by
+// k_solve_fwd_rows.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_solve_fwd_large(DevPlan P, const int32_t* __restrict__ list,
-                                                        const double* __restrict__ L,
-                                                        const double* __restrict__ Z,
-                                                        double* __restrict__ X,
-                                                        double* __restrict__ V) {
+__global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* __restrict__ list,
+                                                      const double* __restrict__ L,
+                                                      const double* __restrict__ Z,
+                                                      double* __restrict__ X,
+                                                      const double* __restrict__ V) {
   constexpr int NQ = NT / 64, QW = 64 / NQ, NR = NT / 4;  // y: NQ parts of QW columns; rows: NR per round
   extern __shared__ double f[];
   __shared__ double part[NQ][kTile];
@@ -895,18 +899,21 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_large(DevPlan P, const int32_t
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
-  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, m = w + P.sn_r[s];
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  for (int i = tid; i < m; i += NT) f[i] = i < w ? Xb[a0 + i] : 0.0;
+  for (int i = tid; i < w; i += NT) f[i] = Xb[a0 + i];
   __syncthreads();
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int c = P.child_idx[cq];
     const int rc = P.sn_r[c];
     const double* Vc = V + (int64_t)b * P.v_size + P.v_off[c];
     const int32_t* relc = P.rel + P.rows_ptr[c];
-    for (int i = tid; i < rc; i += NT) f[relc[i]] += Vc[i];
+    for (int i = tid; i < rc; i += NT) {
+      const int pi = relc[i];
+      if (pi < w) f[pi] += Vc[i];
+    }
     __syncthreads();
   }
   const int i = tid & 63, pt = tid >> 6;   // y: row i, q in [QW pt, QW pt + QW)
@@ -937,8 +944,8 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_large(DevPlan P, const int32_t
       f[c0 + tid] = v;
     }
     __syncthreads();
-    for (int row0 = c1; row0 < m; row0 += NR) {
-      const int row = min(row0 + rr, m - 1);
+    for (int row0 = c1; row0 < w; row0 += NR) {
+      const int row = min(row0 + rr, w - 1);
       double lv[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(16 * qp + u, nb - 1)) * m + row];
@@ -951,27 +958,112 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_large(DevPlan P, const int32_t
       }
       rp[qp][rr] = a;
       __syncthreads();
-      if (tid < NR && row0 + tid < m) f[row0 + tid] -= rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
+      if (tid < NR && row0 + tid < w) f[row0 + tid] -= rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
       __syncthreads();
     }
   }
-  double* Vs = V + (int64_t)b * P.v_size + P.v_off[s];
-  for (int k = tid; k < m; k += NT) {
-    if (k < w)
-      Xb[a0 + k] = f[k];
-    else
-      Vs[k - w] = f[k];
+  for (int k = tid; k < w; k += NT) Xb[a0 + k] = f[k];
+}
+
+// Forward, rows below a large supernode: task (s, chunk) forms the update entries
+// of rows 64 chunk .. of R_s, V_s[i] = sum_c V_c[cinv_c(w + i)] - sum_j L[w + i, j] y_j,
+// children in fixed order, the w columns split over the 4 waves (lane = row:
+// coalesced column segments), reduced through LDS in fixed order.
+__global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t* __restrict__ tasks,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ X,
+                                                        double* __restrict__ V) {
+  extern __shared__ double yv[];
+  __shared__ double red[4][kTile];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], ch = tk[1];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Xb = X + (int64_t)b * P.n;
+  for (int j = tid; j < w; j += 256) yv[j] = Xb[a0 + j];
+  __syncthreads();
+  const int i = ch * kTile + lane;
+  const int row = w + min(i, r - 1);
+  double acc = 0.0;
+  for (int j0 = wv; j0 < w; j0 += 64) {
+    double lv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)min(j0 + 4 * u, w - 1) * m + row];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int j = j0 + 4 * u;
+      acc += j < w ? lv[u] * yv[min(j, w - 1)] : 0.0;
+    }
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (tid < kTile && i < r) {
+    double c = 0.0;
+    for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
+      const int cc = P.child_idx[cq];
+      const int ci = P.cinv[P.cinv_off[cq] + w + i];
+      if (ci >= 0) c += V[(int64_t)b * P.v_size + P.v_off[cc] + ci];
+    }
+    V[(int64_t)b * P.v_size + P.v_off[s] + i] = c - (red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
   }
 }
 
-// Backward on large fronts, panels right to left:
-// t_P = z_P - L[B,P]^T x_B, x_P = Z^T t_P. Wave wv covers panel columns
+// Backward, rows below a large supernode: task (s, cc, part) forms the partial sums
+// t[part][q] = sum_{i in part} L[w + i, q] x_{R_s[i]} for the 64 columns q of chunk
+// cc, over rows 1024 part .. of R_s; wave wv covers columns 64 cc + 16 wv + u
+// (lane = row), reduced across lanes by shuffles. Partials go to the front's staging
+// area (parts x w), summed in fixed order by k_solve_bwd_tri.
+__global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t* __restrict__ tasks,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ X,
+                                                        double* __restrict__ T) {
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], cc = tk[1], part = tk[2];
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Xb = X + (int64_t)b * P.n;
+  const int32_t* rows = P.rows + P.rows_ptr[s];
+  const int q0 = cc * kTile + 16 * wv;
+  const int i1 = min(r, (part + 1) * kSolvePart);
+  double acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+  for (int i = part * kSolvePart + lane; i < i1 + 63; i += 64) {
+    const int ic = min(i, r - 1);
+    const double xi = Xb[rows[ic]];
+    double lv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)min(q0 + u, w - 1) * m + w + ic];
+    ISSUE_FENCE();
+    const double xs = i < i1 ? xi : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] += lv[u] * xs;
+  }
+  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s] + (int64_t)part * w;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    double a = acc[u];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (lane == 0 && q0 + u < w) Tb[q0 + u] = a;
+  }
+}
+
+// Backward, triangular part of a large front, panels right to left:
+// t_P = x_P - sum_parts (L[R,P]^T x_R) - L[B,P]^T x_B (B = rows below the panel
+// inside the supernode), x_P = Z^T t_P. Wave wv covers panel columns
 // q = (wv & 3) + 4u (u < 16) over rows c1 + 64 (wv >> 2) + lane (+NT / 4 k).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_solve_bwd_large(DevPlan P, const int32_t* __restrict__ list,
-                                                        const double* __restrict__ L,
-                                                        const double* __restrict__ Z,
-                                                        double* __restrict__ X) {
+__global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* __restrict__ list,
+                                                      const double* __restrict__ L,
+                                                      const double* __restrict__ Z,
+                                                      double* __restrict__ X,
+                                                      const double* __restrict__ T) {
   constexpr int NW = NT / 64, NK = NW / 4;
   extern __shared__ double xf[];
   __shared__ double red[NK][kTile];
@@ -984,8 +1076,13 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_large(DevPlan P, const int32_t
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  const int32_t* rows = P.rows + P.rows_ptr[s];
-  for (int k = tid; k < m; k += NT) xf[k] = k < w ? Xb[a0 + k] : Xb[rows[k - w]];
+  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  const int nparts = (r + kSolvePart - 1) / kSolvePart;
+  for (int k = tid; k < w; k += NT) {
+    double v = Xb[a0 + k];
+    for (int q = 0; q < nparts; ++q) v -= Tb[(int64_t)q * w + k];
+    xf[k] = v;
+  }
   __syncthreads();
   const int np = (w + kTile - 1) / kTile;
   for (int pk = np - 1; pk >= 0; --pk) {
@@ -993,9 +1090,9 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_large(DevPlan P, const int32_t
     double acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    for (int k = c1 + 64 * kg + lane; k < m + 63; k += 64 * NK) {
-      const int kc = min(k, m - 1);
-      const double xk = k < m ? xf[kc] : 0.0;
+    for (int k = c1 + 64 * kg + lane; k < w + 63; k += 64 * NK) {
+      const int kc = min(k, w - 1);
+      const double xk = k < w ? xf[kc] : 0.0;
       double lv[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(qg + 4 * u, nb - 1)) * m + kc];
@@ -1618,36 +1715,55 @@ static hipError_t lds_limit(const void* fn, size_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_solve_fwd_large(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                                  const double* L, const double* Z, double* X, double* V,
-                                  hipStream_t st) {
+// The triangular parts hold the supernode's w entries in LDS; few fronts get
+// 1024 threads each for latency, many get 256 for occupancy.
+hipError_t launch_solve_fwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
+                                const double* L, const double* Z, double* X, const double* V,
+                                hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  const size_t sh = (size_t)maxm * sizeof(double);
-  // few fronts: 1024 threads each for latency; many: 256 for occupancy
+  const size_t sh = (size_t)maxw * sizeof(double);
   if ((int64_t)count * B < 256) {
-    hipError_t e = lds_limit((const void*)k_solve_fwd_large<1024>, sh);
+    hipError_t e = lds_limit((const void*)k_solve_fwd_tri<1024>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_large<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, V);
+    hipLaunchKernelGGL(k_solve_fwd_tri<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, V);
   } else {
-    hipError_t e = lds_limit((const void*)k_solve_fwd_large<256>, sh);
+    hipError_t e = lds_limit((const void*)k_solve_fwd_tri<256>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_large<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
+    hipLaunchKernelGGL(k_solve_fwd_tri<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_solve_bwd_large(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                                  const double* L, const double* Z, double* X, hipStream_t st) {
+hipError_t launch_solve_fwd_rows(const DevPlan& P, const int32_t* tasks, int count, int maxw, int B,
+                                 const double* L, const double* X, double* V, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  const size_t sh = (size_t)maxm * sizeof(double);
+  const size_t sh = (size_t)maxw * sizeof(double);
+  hipError_t e = lds_limit((const void*)k_solve_fwd_rows, sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_solve_fwd_rows, dim3(count, B), dim3(256), sh, st, P, tasks, L, X, V);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve_bwd_rows(const DevPlan& P, const int32_t* tasks, int count, int B,
+                                 const double* L, const double* X, double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_solve_bwd_rows, dim3(count, B), dim3(256), 0, st, P, tasks, L, X, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve_bwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
+                                const double* L, const double* Z, double* X, const double* T,
+                                hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const size_t sh = (size_t)maxw * sizeof(double);
   if ((int64_t)count * B < 256) {
-    hipError_t e = lds_limit((const void*)k_solve_bwd_large<1024>, sh);
+    hipError_t e = lds_limit((const void*)k_solve_bwd_tri<1024>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_large<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X);
+    hipLaunchKernelGGL(k_solve_bwd_tri<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, T);
   } else {
-    hipError_t e = lds_limit((const void*)k_solve_bwd_large<256>, sh);
+    hipError_t e = lds_limit((const void*)k_solve_bwd_tri<256>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_large<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X);
+    hipLaunchKernelGGL(k_solve_bwd_tri<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, T);
   }
   return hipGetLastError();
 }

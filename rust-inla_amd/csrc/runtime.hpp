@@ -49,10 +49,14 @@ hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const
                             double*, hipStream_t);
 hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
                             hipStream_t);
-hipError_t launch_solve_fwd_large(const DevPlan&, const int32_t*, int, int, int, const double*,
-                                  const double*, double*, double*, hipStream_t);
-hipError_t launch_solve_bwd_large(const DevPlan&, const int32_t*, int, int, int, const double*,
-                                  const double*, double*, hipStream_t);
+hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, const double*,
+                                const double*, double*, const double*, hipStream_t);
+hipError_t launch_solve_fwd_rows(const DevPlan&, const int32_t*, int, int, int, const double*,
+                                 const double*, double*, hipStream_t);
+hipError_t launch_solve_bwd_rows(const DevPlan&, const int32_t*, int, int, const double*,
+                                 const double*, double*, hipStream_t);
+hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, const double*,
+                                const double*, double*, const double*, hipStream_t);
 hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                double*, double*, const double*, hipStream_t);
 hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
@@ -173,7 +177,7 @@ struct LevelInfo {
   double fl_cls[3] = {0, 0, 0}, sel_cls[3] = {0, 0, 0}, by_cls[3] = {0, 0, 0};
   int64_t lb, le;     // large fronts [lb, le) in large list
   int64_t ab, ae;     // all fronts [ab, ae) in all list
-  int32_t maxm_small, maxm_large, maxm_all;
+  int32_t maxm_small, maxm_large, maxm_all, maxw_large;
   // algorithmic work of the level's launches (per matrix)
   double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol_small = 0, sol_large = 0;
   double by_small = 0, by_large = 0;
@@ -183,6 +187,8 @@ struct LevelInfo {
     double flops = 0, bytes = 0;
   };
   Range asm_t, gather;
+  Range frow, brow;  // solve: rows below the large supernodes (forward, backward)
+  double sol_tri = 0;  // solve work of the large supernodes' own blocks (per matrix)
   std::vector<Range> fdiag, ftrsm, fupd;   // per factor panel step
   std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
 };
@@ -419,6 +425,7 @@ struct SymData {
       L.ab = (int64_t)all.size();
       L.maxm_small = 0;
       L.maxm_large = 0;
+      L.maxw_large = 0;
       L.maxm_all = 0;
       std::vector<int32_t> cls[3];
       for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
@@ -449,7 +456,9 @@ struct SymData {
         } else {
           large.push_back((int32_t)s);
           L.sol_large += sol;
+          L.sol_tri += 2.0 * (double)w * (double)w;
           L.maxm_large = std::max(L.maxm_large, m);
+          L.maxw_large = std::max(L.maxw_large, (int32_t)w);
           L.fl_large += fl;
           L.sel_large += sel;
           L.by_large += by;
@@ -629,6 +638,23 @@ struct SymData {
           for (int64_t I = J; I < nt; ++I) add(s, I, J);
       }
       L.gather.cnt = (int64_t)tk.size() / 3 - L.gather.off;
+      L.frow.off = (int64_t)tk.size() / 3;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        for (int64_t c = 0; c < ceil64(r); ++c) add(s, c, 0);
+      }
+      L.frow.cnt = (int64_t)tk.size() / 3 - L.frow.off;
+      L.brow.off = (int64_t)tk.size() / 3;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        const int64_t parts = (r + kSolvePart - 1) / kSolvePart;
+        if (parts * w > m * kTile) throw std::runtime_error("front too tall for the solve staging");
+        for (int64_t p = 0; p < parts; ++p)
+          for (int64_t c = 0; c < ceil64(w); ++c) add(s, c, p);
+      }
+      L.brow.cnt = (int64_t)tk.size() / 3 - L.brow.off;
       for (int64_t k = 0; k < maxsteps; ++k) {
         auto rows_of = [&](int64_t s, int64_t& nB) {
           int64_t w, r, m;
@@ -762,6 +788,8 @@ struct gmrf_batch {
     const double* Zf = Z + (int64_t)first * P.z_size;
     double* Xf = X + (int64_t)first * P.n;
     double* Vf = V + (int64_t)first * std::max<int64_t>(P.v_size, 1);
+    double* Tf = T + (int64_t)first * P.t_size;
+    const int32_t* tk = sym->d_tasks;
     check(launch_gather_perm(P, count, rhs, ldr, Xf, stream), "gather");
     for (const LevelInfo& lv : sym->levels) {
       if (lv.se > lv.sb)
@@ -769,20 +797,30 @@ struct gmrf_batch {
           return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
                                   Lf, Xf, Vf, stream);
         });
-      if (lv.le > lv.lb)
-        prof_launch("k_solve_fwd_large", 0.5 * lv.sol_large * count, 0.0, stream, [&] {
-          return launch_solve_fwd_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxm_large,
-                                        count, Lf, Zf, Xf, Vf, stream);
+      if (lv.le > lv.lb) {
+        prof_launch("k_solve_fwd_tri", 0.5 * lv.sol_tri * count, 0.0, stream, [&] {
+          return launch_solve_fwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
+                                      count, Lf, Zf, Xf, Vf, stream);
         });
+        prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count, 0.0, stream, [&] {
+          return launch_solve_fwd_rows(P, tk + 3 * lv.frow.off, (int)lv.frow.cnt, lv.maxw_large, count,
+                                       Lf, Xf, Vf, stream);
+        });
+      }
     }
     check(launch_diag_scale(P, count, Lf, Xf, stream), "diag_scale");
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
-      if (lv.le > lv.lb)
-        prof_launch("k_solve_bwd_large", 0.5 * lv.sol_large * count, 0.0, stream, [&] {
-          return launch_solve_bwd_large(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxm_large,
-                                        count, Lf, Zf, Xf, stream);
+      if (lv.le > lv.lb) {
+        prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count, 0.0, stream, [&] {
+          return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, Lf, Xf, Tf,
+                                       stream);
         });
+        prof_launch("k_solve_bwd_tri", 0.5 * lv.sol_tri * count, 0.0, stream, [&] {
+          return launch_solve_bwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
+                                      count, Lf, Zf, Xf, Tf, stream);
+        });
+      }
       if (lv.se > lv.sb)
         prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count, 0.0, stream, [&] {
           return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
