@@ -122,6 +122,12 @@ gmrf_status gmrf_batch_logdet_device(gmrf_batch* bt, int32_t count, double* logd
  * x may alias rhs. */
 gmrf_status gmrf_batch_solve(gmrf_batch* bt, int32_t count, const double* rhs, int64_t ld_rhs,
                              double* x, int64_t ld_x);
+/* nrhs right-hand sides per matrix in one launch sequence: vector j of matrix b
+ * is rhs + (b nrhs + j) ld_rhs, its solution x + (b nrhs + j) ld_x (device
+ * pointers, x may alias rhs). The reference solves one vector per call
+ * (sparse_ldlt.rs:243-265); several columns serve low-rank corrections. */
+gmrf_status gmrf_batch_solve_multi(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
+                                   int64_t ld_rhs, double* x, int64_t ld_x);
 /* Selected inversion on the supernodal pattern of L (Takahashi recurrence). */
 gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count);
 /* diag(Q_b^-1) in original order after gmrf_batch_selinv (sparse_ldlt.rs:421-431). */
@@ -220,6 +226,13 @@ typedef struct gmrf_model_desc {
   const double* constr_a;    /* k x n row-major (host) */
   const double* constr_e;    /* k */
   int32_t capacity;          /* max theta-points per gmrf_model_eval */
+  double constr_kappa;       /* 0: any kappa A_c^T A_c augmentation is one of the components
+                                (constraints.rs:275-313 forms it explicitly). > 0: the
+                                augmentation Q + kappa A_c^T A_c is applied to the prior and
+                                the posterior as an exact low-rank correction of the grounded
+                                precision, so long constraint rows do not densify the pattern
+                                (no reference counterpart; k <= 32) */
+  int32_t order;             /* GMRF_ORDER_CHOOSER (the reference's choice) or GMRF_ORDER_AMD */
 } gmrf_model_desc;
 
 typedef struct gmrf_eval_args {

@@ -232,6 +232,23 @@ gmrf_status gmrf_batch_solve(gmrf_batch* bt, int32_t count, const double* rhs, i
   }
 }
 
+gmrf_status gmrf_batch_solve_multi(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
+                                   int64_t ld_rhs, double* x, int64_t ld_x) {
+  if (!bt || !rhs || !x) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (count < 1 || count > bt->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds batch capacity");
+  if (nrhs < 1 || (int64_t)count * nrhs > 65535)
+    return fail(GMRF_INVALID_ARGUMENT, "nrhs must be >= 1 and count * nrhs <= 65535");
+  if (ld_rhs < bt->sym->S.n || ld_x < bt->sym->S.n)
+    return fail(GMRF_INVALID_ARGUMENT, "leading dimension < n");
+  try {
+    check(hipSetDevice(bt->sym->device), "hipSetDevice");
+    bt->solve(count, rhs, ld_rhs, x, ld_x, 0, nrhs);
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count) {
   if (!bt) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (count < 1 || count > bt->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds batch capacity");

@@ -778,15 +778,17 @@ __global__ __launch_bounds__(256) void k_logdet_seg(DevPlan P, const double* __r
 }
 
 // ---------------------------------------------------------------------------
-// Solves (one right-hand side per matrix)
+// Solves. A launch covers B * nr right-hand sides: vector v uses the factor of
+// matrix v / nr and its own slices of X, V and the staging area.
 // ---------------------------------------------------------------------------
 __global__ void k_gather_perm(DevPlan P, int B, const double* __restrict__ rhs, int64_t ldr,
-                              double* __restrict__ X) {
+                              int period, double* __restrict__ X) {
   const int64_t tot = (int64_t)B * P.n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / P.n), k = (int)(e - (int64_t)b * P.n);
-    X[e] = rhs[(int64_t)b * ldr + P.perm[k]];
+    const int v = period > 0 ? b % period : b;
+    X[e] = rhs[(int64_t)v * ldr + P.perm[k]];
   }
 }
 
@@ -800,27 +802,27 @@ __global__ void k_scatter_perm(DevPlan P, int B, const double* __restrict__ X,
   }
 }
 
-__global__ void k_diag_scale(DevPlan P, int B, const double* __restrict__ L,
+__global__ void k_diag_scale(DevPlan P, int B, int nr, const double* __restrict__ L,
                              double* __restrict__ X) {
   const int64_t tot = (int64_t)B * P.n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / P.n), k = (int)(e - (int64_t)b * P.n);
-    X[e] /= L[(int64_t)b * P.l_size + P.diag_slot[k]];
+    X[e] /= L[(int64_t)(b / nr) * P.l_size + P.diag_slot[k]];
   }
 }
 
 // Forward: f = [x_J; 0] + sum_c extend(v_c); y_J = L11^-1 f_J; v_s = f_R - L21 y_J.
-__global__ __launch_bounds__(128) void k_solve_fwd(DevPlan P, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(128) void k_solve_fwd(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                    const double* __restrict__ L,
                                                    double* __restrict__ X,
                                                    double* __restrict__ V) {
   extern __shared__ double f[];
   const int s = list[blockIdx.x];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   double* Xb = X + (int64_t)b * P.n;
   for (int i = tid; i < m; i += blockDim.x) f[i] = i < w ? Xb[a0 + i] : 0.0;
   __syncthreads();
@@ -847,15 +849,15 @@ __global__ __launch_bounds__(128) void k_solve_fwd(DevPlan P, const int32_t* __r
 }
 
 // Backward: x_J = L11^-T (z_J - L21^T x_R) with x_R gathered from final values.
-__global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                    const double* __restrict__ L,
                                                    double* __restrict__ X) {
   extern __shared__ double sh[];
   const int s = list[blockIdx.x];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   double* Xb = X + (int64_t)b * P.n;
   double* g = sh;      // r
   double* t = sh + r;  // w
@@ -886,7 +888,7 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
 // the supernode's columns are added first; those landing in R_s are gathered by
 // k_solve_fwd_rows.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                       const double* __restrict__ L,
                                                       const double* __restrict__ Z,
                                                       double* __restrict__ X,
@@ -897,11 +899,11 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
   __shared__ double rp[4][NR];
   __shared__ double y[kTile];
   const int s = list[blockIdx.x];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, m = w + P.sn_r[s];
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
-  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
   for (int i = tid; i < w; i += NT) f[i] = Xb[a0 + i];
   __syncthreads();
@@ -969,7 +971,7 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
 // of rows 64 chunk .. of R_s, V_s[i] = sum_c V_c[cinv_c(w + i)] - sum_j L[w + i, j] y_j,
 // children in fixed order, the w columns split over the 4 waves (lane = row:
 // coalesced column segments), reduced through LDS in fixed order.
-__global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t* __restrict__ tasks, int nr,
                                                         const double* __restrict__ L,
                                                         const double* __restrict__ X,
                                                         double* __restrict__ V) {
@@ -977,10 +979,10 @@ __global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t
   __shared__ double red[4][kTile];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], ch = tk[1];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   const double* Xb = X + (int64_t)b * P.n;
   for (int j = tid; j < w; j += 256) yv[j] = Xb[a0 + j];
   __syncthreads();
@@ -1016,16 +1018,16 @@ __global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t
 // cc, over rows 1024 part .. of R_s; wave wv covers columns 64 cc + 16 wv + u
 // (lane = row), reduced across lanes by shuffles. Partials go to the front's staging
 // area (parts x w), summed in fixed order by k_solve_bwd_tri.
-__global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t* __restrict__ tasks, int nr,
                                                         const double* __restrict__ L,
                                                         const double* __restrict__ X,
                                                         double* __restrict__ T) {
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], cc = tk[1], part = tk[2];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   const double* Xb = X + (int64_t)b * P.n;
   const int32_t* rows = P.rows + P.rows_ptr[s];
   const int q0 = cc * kTile + 16 * wv;
@@ -1059,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t
 // inside the supernode), x_P = Z^T t_P. Wave wv covers panel columns
 // q = (wv & 3) + 4u (u < 16) over rows c1 + 64 (wv >> 2) + lane (+NT / 4 k).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                       const double* __restrict__ L,
                                                       const double* __restrict__ Z,
                                                       double* __restrict__ X,
@@ -1069,12 +1071,12 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
   __shared__ double red[NK][kTile];
   __shared__ double tq[kTile];
   const int s = list[blockIdx.x];
-  const int b = blockIdx.y;
+  const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int qg = wv & 3, kg = wv >> 2;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
-  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s];
+  const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
   const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   const int nparts = (r + kSolvePart - 1) / kSolvePart;
@@ -1674,9 +1676,9 @@ hipError_t launch_logdet_seg(const DevPlan& P, int B, const double* L, const int
 }
 
 hipError_t launch_gather_perm(const DevPlan& P, int B, const double* rhs, int64_t ldr, double* X,
-                              hipStream_t st) {
+                              hipStream_t st, int period) {
   hipLaunchKernelGGL(k_gather_perm, dim3(grid1((int64_t)B * P.n)), dim3(256), 0, st, P, B, rhs, ldr,
-                     X);
+                     period, X);
   return hipGetLastError();
 }
 
@@ -1687,24 +1689,26 @@ hipError_t launch_scatter_perm(const DevPlan& P, int B, const double* X, double*
   return hipGetLastError();
 }
 
-hipError_t launch_diag_scale(const DevPlan& P, int B, const double* L, double* X, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_scale, dim3(grid1((int64_t)B * P.n)), dim3(256), 0, st, P, B, L, X);
+hipError_t launch_diag_scale(const DevPlan& P, int B, int nr, const double* L, double* X,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_scale, dim3(grid1((int64_t)B * nr * P.n)), dim3(256), 0, st, P, B * nr, nr,
+                     L, X);
   return hipGetLastError();
 }
 
 hipError_t launch_solve_fwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                            const double* L, double* X, double* V, hipStream_t st) {
+                            int nr, const double* L, double* X, double* V, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_fwd, dim3(count, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
-                     list, L, X, V);
+  hipLaunchKernelGGL(k_solve_fwd, dim3(count, B * nr), dim3(128), (size_t)maxm * sizeof(double), st, P,
+                     list, nr, L, X, V);
   return hipGetLastError();
 }
 
 hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                            const double* L, double* X, hipStream_t st) {
+                            int nr, const double* L, double* X, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_bwd, dim3(count, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
-                     list, L, X);
+  hipLaunchKernelGGL(k_solve_bwd, dim3(count, B * nr), dim3(128), (size_t)maxm * sizeof(double), st, P,
+                     list, nr, L, X);
   return hipGetLastError();
 }
 
@@ -1718,52 +1722,56 @@ static hipError_t lds_limit(const void* fn, size_t bytes) {
 // The triangular parts hold the supernode's w entries in LDS; few fronts get
 // 1024 threads each for latency, many get 256 for occupancy.
 hipError_t launch_solve_fwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
-                                const double* L, const double* Z, double* X, const double* V,
+                                int nr, const double* L, const double* Z, double* X, const double* V,
                                 hipStream_t st) {
   if (count <= 0) return hipSuccess;
   const size_t sh = (size_t)maxw * sizeof(double);
-  if ((int64_t)count * B < 256) {
+  if ((int64_t)count * B * nr < 256) {
     hipError_t e = lds_limit((const void*)k_solve_fwd_tri<1024>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_tri<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, V);
+    hipLaunchKernelGGL(k_solve_fwd_tri<1024>, dim3(count, B * nr), dim3(1024), sh, st, P, list, nr,
+                       L, Z, X, V);
   } else {
     hipError_t e = lds_limit((const void*)k_solve_fwd_tri<256>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_tri<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, V);
+    hipLaunchKernelGGL(k_solve_fwd_tri<256>, dim3(count, B * nr), dim3(256), sh, st, P, list, nr,
+                       L, Z, X, V);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_solve_fwd_rows(const DevPlan& P, const int32_t* tasks, int count, int maxw, int B,
-                                 const double* L, const double* X, double* V, hipStream_t st) {
+hipError_t launch_solve_fwd_rows(const DevPlan& P, const int32_t* tasks, int count, int maxw,
+                                 int B, int nr, const double* L, const double* X, double* V, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   const size_t sh = (size_t)maxw * sizeof(double);
   hipError_t e = lds_limit((const void*)k_solve_fwd_rows, sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_solve_fwd_rows, dim3(count, B), dim3(256), sh, st, P, tasks, L, X, V);
+  hipLaunchKernelGGL(k_solve_fwd_rows, dim3(count, B * nr), dim3(256), sh, st, P, tasks, nr, L, X, V);
   return hipGetLastError();
 }
 
-hipError_t launch_solve_bwd_rows(const DevPlan& P, const int32_t* tasks, int count, int B,
+hipError_t launch_solve_bwd_rows(const DevPlan& P, const int32_t* tasks, int count, int B, int nr,
                                  const double* L, const double* X, double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_bwd_rows, dim3(count, B), dim3(256), 0, st, P, tasks, L, X, T);
+  hipLaunchKernelGGL(k_solve_bwd_rows, dim3(count, B * nr), dim3(256), 0, st, P, tasks, nr, L, X, T);
   return hipGetLastError();
 }
 
 hipError_t launch_solve_bwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
-                                const double* L, const double* Z, double* X, const double* T,
+                                int nr, const double* L, const double* Z, double* X, const double* T,
                                 hipStream_t st) {
   if (count <= 0) return hipSuccess;
   const size_t sh = (size_t)maxw * sizeof(double);
-  if ((int64_t)count * B < 256) {
+  if ((int64_t)count * B * nr < 256) {
     hipError_t e = lds_limit((const void*)k_solve_bwd_tri<1024>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_tri<1024>, dim3(count, B), dim3(1024), sh, st, P, list, L, Z, X, T);
+    hipLaunchKernelGGL(k_solve_bwd_tri<1024>, dim3(count, B * nr), dim3(1024), sh, st, P, list, nr,
+                       L, Z, X, T);
   } else {
     hipError_t e = lds_limit((const void*)k_solve_bwd_tri<256>, sh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_tri<256>, dim3(count, B), dim3(256), sh, st, P, list, L, Z, X, T);
+    hipLaunchKernelGGL(k_solve_bwd_tri<256>, dim3(count, B * nr), dim3(256), sh, st, P, list, nr,
+                       L, Z, X, T);
   }
   return hipGetLastError();
 }

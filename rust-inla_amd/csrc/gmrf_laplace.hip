@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_constr_dot(int kc, int64_t n, const dou
 }
 
 __global__ void k_constr_apply(int B, int kc, int64_t n, const double* __restrict__ ca,
-                               const double* __restrict__ lam, double* __restrict__ x) {
+                               const double* __restrict__ lam, double scale, double* __restrict__ x) {
   const int64_t tot = (int64_t)B * n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -424,8 +424,139 @@ __global__ void k_constr_apply(int B, int kc, int64_t n, const double* __restric
     const int64_t j = e - (int64_t)b * n;
     double corr = 0.0;
     for (int k = 0; k < kc; ++k) corr += ca[(int64_t)k * n + j] * lam[b * kc + k];
-    x[e] -= corr;
+    x[e] -= scale * corr;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Low-rank constraint augmentation. With k constraint rows C, the augmented
+// precision Q + kappa C^T C (constraints.rs:275-313 forms it explicitly, which
+// makes the pattern dense for long rows) is written as Q_g + U E U^T with the
+// grounded Q_g = Q + sum_t alpha_t e_{p_t} e_{p_t}^T (same pattern; non-singular
+// when the constraints span the null space of Q), U = [C^T, e_p] (n x 2k) and
+// E = diag(kappa I, -alpha). Log-determinants, solves, diagonals and traces of
+// the augmented matrix follow from Y = Q_g^-1 U and K = E^-1 + U^T Y.
+// ---------------------------------------------------------------------------
+
+// dst_b[idx_t] += alpha_bt with alpha_bt = src_b[idx_t] (1 where that is not
+// positive); alpha is also written out (B x k).
+__global__ void k_ground(int B, int k, int64_t nnz, const int64_t* __restrict__ idx,
+                         const double* __restrict__ src, double* __restrict__ dst,
+                         double* __restrict__ alpha) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B * k) return;
+  const int b = e / k, t = e - b * k;
+  const int64_t q = (int64_t)b * nnz + idx[t];
+  const double v = src[q];
+  const double a = v > 0.0 ? v : 1.0;
+  dst[q] += a;
+  alpha[e] = a;
+}
+
+// Partial Gram products over one chunk of 2048 entries:
+// part[(b nchunk + c) RU RY + i RY + j] = sum_{x in chunk c} U_b[i][x] Y_b[j][x],
+// U_b[i] = U + b ldub + i ldu (ldub = 0: one U for all b), Y likewise. Operands
+// are staged through LDS 32 entries at a time; each thread owns up to 16 (i, j).
+constexpr int kGramChunk = 2048, kGramSub = 32;
+
+__global__ __launch_bounds__(256) void k_gram_part(int RU, int RY, int64_t n,
+                                                   const double* __restrict__ U, int64_t ldu,
+                                                   int64_t ldub, const double* __restrict__ Y,
+                                                   int64_t ldy, int64_t ldyb, int nchunk,
+                                                   double* __restrict__ part) {
+  __shared__ double us[64][kGramSub + 1];
+  __shared__ double ys[64][kGramSub + 1];
+  const int c = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int np = RU * RY;
+  const double* Ub = U + (int64_t)b * ldub;
+  const double* Yb = Y + (int64_t)b * ldyb;
+  double acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+  const int64_t x0 = (int64_t)c * kGramChunk, x1 = min(n, x0 + kGramChunk);
+  for (int64_t xs = x0; xs < x1; xs += kGramSub) {
+    for (int e = tid; e < (RU + RY) * kGramSub; e += 256) {
+      const int row = e / kGramSub, q = e - row * kGramSub;
+      const int64_t x = min(xs + q, x1 - 1);
+      const bool isu = row < RU;
+      const double* src = isu ? Ub + (int64_t)row * ldu : Yb + (int64_t)(row - RU) * ldy;
+      const double v = src[x];
+      const double vv = xs + q < x1 ? v : 0.0;
+      if (isu)
+        us[row][q] = vv;
+      else
+        ys[row - RU][q] = vv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int p = tid + 256 * u;
+      if (p < np) {
+        const int i = p / RY, j = p - (p / RY) * RY;
+        double a = acc[u];
+#pragma unroll 8
+        for (int q = 0; q < kGramSub; ++q) a += us[i][q] * ys[j][q];
+        acc[u] = a;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int p = tid + 256 * u;
+    if (p < np) part[((int64_t)b * nchunk + c) * np + p] = acc[u];
+  }
+}
+
+// out[b np + p] = sum_c part[(b nchunk + c) np + p], chunks in order.
+__global__ void k_gram_sum(int np, int nchunk, const double* __restrict__ part,
+                           double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (p >= np) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[((int64_t)b * nchunk + c) * np + p];
+  out[(int64_t)b * np + p] = s;
+}
+
+// x_b[i] -= sum_j Y_b[j][i] h_b[j]
+__global__ void k_lowrank_axpy(int B, int R, int64_t n, const double* __restrict__ Y, int64_t ldy,
+                               int64_t ldyb, const double* __restrict__ h, double* __restrict__ x,
+                               int64_t ldx) {
+  const int64_t tot = (int64_t)B * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / n);
+    const int64_t i = e - (int64_t)b * n;
+    const double* Yb = Y + (int64_t)b * ldyb + i;
+    double a = 0.0;
+    for (int j = 0; j < R; ++j) a += Yb[(int64_t)j * ldy] * h[b * R + j];
+    x[(int64_t)b * ldx + i] -= a;
+  }
+}
+
+// d_b[i] -= y^T Kinv_b y with y_j = Y_b[j][i]: the diagonal of Y Kinv Y^T.
+// 128 entries per workgroup; y (R x 128) and Kinv_b (R x R) in LDS.
+__global__ __launch_bounds__(128) void k_lowrank_diag(int R, int64_t n, const double* __restrict__ Y,
+                                                      int64_t ldy, int64_t ldyb,
+                                                      const double* __restrict__ Kinv,
+                                                      double* __restrict__ d, int64_t ldd) {
+  extern __shared__ double sm[];
+  double* ks = sm;            // R x R
+  double* yv = sm + R * R;    // R x 128
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 128 + tid;
+  const int64_t ic = min(i, n - 1);
+  for (int e = tid; e < R * R; e += 128) ks[e] = Kinv[(int64_t)b * R * R + e];
+  const double* Yb = Y + (int64_t)b * ldyb + ic;
+  for (int j = 0; j < R; ++j) yv[j * 128 + tid] = Yb[(int64_t)j * ldy];
+  __syncthreads();
+  double q = 0.0;
+  for (int j = 0; j < R; ++j) {
+    double s = 0.0;
+    for (int l = 0; l < R; ++l) s += ks[j * R + l] * yv[l * 128 + tid];
+    q += yv[j * 128 + tid] * s;
+  }
+  if (i < n) d[(int64_t)b * ldd + i] -= q;
 }
 
 // out = a .* b (flat)
@@ -532,9 +663,50 @@ hipError_t lap_constr_dot(int B, int kc, int64_t n, const double* ca, const doub
 }
 
 hipError_t lap_constr_apply(int B, int kc, int64_t n, const double* ca, const double* lam, double* x,
-                            hipStream_t st) {
+                            hipStream_t st, double scale) {
   hipLaunchKernelGGL(k_constr_apply, dim3(grid_for((int64_t)B * n)), dim3(256), 0, st, B, kc, n, ca,
-                     lam, x);
+                     lam, scale, x);
+  return hipGetLastError();
+}
+
+hipError_t lap_ground(int B, int k, int64_t nnz, const int64_t* idx, const double* src, double* dst,
+                      double* alpha, hipStream_t st) {
+  if (B * k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ground, dim3((B * k + 255) / 256), dim3(256), 0, st, B, k, nnz, idx, src, dst,
+                     alpha);
+  return hipGetLastError();
+}
+
+// out (B x RU x RY) = U_b Y_b^T over n entries; part holds B * ceil(n / 2048) * RU * RY doubles.
+hipError_t lap_gram(int B, int RU, int RY, int64_t n, const double* U, int64_t ldu, int64_t ldub,
+                    const double* Y, int64_t ldy, int64_t ldyb, double* part, double* out,
+                    hipStream_t st) {
+  if (RU < 1 || RY < 1 || RU > 64 || RY > 64) return hipErrorInvalidValue;
+  const int nchunk = (int)((n + kGramChunk - 1) / kGramChunk), np = RU * RY;
+  hipLaunchKernelGGL(k_gram_part, dim3(nchunk, B), dim3(256), 0, st, RU, RY, n, U, ldu, ldub, Y, ldy,
+                     ldyb, nchunk, part);
+  hipLaunchKernelGGL(k_gram_sum, dim3((np + 255) / 256, B), dim3(256), 0, st, np, nchunk, part, out);
+  return hipGetLastError();
+}
+
+hipError_t lap_lowrank_axpy(int B, int R, int64_t n, const double* Y, int64_t ldy, int64_t ldyb,
+                            const double* h, double* x, int64_t ldx, hipStream_t st) {
+  hipLaunchKernelGGL(k_lowrank_axpy, dim3(grid_for((int64_t)B * n)), dim3(256), 0, st, B, R, n, Y, ldy,
+                     ldyb, h, x, ldx);
+  return hipGetLastError();
+}
+
+hipError_t lap_lowrank_diag(int B, int R, int64_t n, const double* Y, int64_t ldy, int64_t ldyb,
+                            const double* Kinv, double* d, int64_t ldd, hipStream_t st) {
+  if (R < 1 || R > 64) return hipErrorInvalidValue;
+  const size_t sh = (size_t)(R * R + R * 128) * sizeof(double);
+  if (sh > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_lowrank_diag,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_lowrank_diag, dim3((unsigned)((n + 127) / 128), B), dim3(128), sh, st, R, n, Y,
+                     ldy, ldyb, Kinv, d, ldd);
   return hipGetLastError();
 }
 

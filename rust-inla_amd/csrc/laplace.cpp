@@ -36,7 +36,15 @@ hipError_t lap_constr_dot(int, int, int64_t, const double*, const double*, doubl
 hipError_t lap_mul(int64_t, const double*, const double*, double*, hipStream_t);
 hipError_t lap_blend(int, int64_t, const double*, const double*, double*, hipStream_t);
 hipError_t lap_constr_apply(int, int, int64_t, const double*, const double*, double*,
-                            hipStream_t);
+                            hipStream_t, double);
+hipError_t lap_ground(int, int, int64_t, const int64_t*, const double*, double*, double*,
+                      hipStream_t);
+hipError_t lap_gram(int, int, int, int64_t, const double*, int64_t, int64_t, const double*, int64_t,
+                    int64_t, double*, double*, hipStream_t);
+hipError_t lap_lowrank_axpy(int, int, int64_t, const double*, int64_t, int64_t, const double*,
+                            double*, int64_t, hipStream_t);
+hipError_t lap_lowrank_diag(int, int, int64_t, const double*, int64_t, int64_t, const double*,
+                            double*, int64_t, hipStream_t);
 }  // namespace gmrf
 
 namespace {
@@ -68,9 +76,30 @@ struct gmrf_model {
   double* v_coef = nullptr;
   // observations
   double *y = nullptr, *pa = nullptr, *pb = nullptr;
-  // constraints (projection; the kappa A^T A augmentation is one of the components)
+  // constraints (projection; the kappa A^T A augmentation is one of the components,
+  // or, with kappa > 0, applied as a low-rank correction: see LowRank)
   double* ca = nullptr;
-  std::vector<double> h_ca, h_ce;
+  std::vector<double> h_ca, h_ce, h_cct;  // h_cct = A_c A_c^T (k x k)
+  double kappa = 0.0;
+  // Low-rank augmentation: Q + kappa C^T C = Q_g + U E U^T with the grounded
+  // Q_g = Q + sum_t alpha_t e_{p_t} e_{p_t}^T, U = [C^T, e_p] (R = 2k columns),
+  // E = diag(kappa I, -alpha). Prior and posterior each keep Y = Q_g^-1 U and
+  // Kinv = (E^-1 + U^T Y)^-1; log|Q + kappa C^T C| = log|Q_g| + log|E| + log|K|.
+  struct LowRank {
+    int R = 0;
+    int64_t* gidx = nullptr;  // [k] CSC position of the grounded diagonal entries
+    double* U = nullptr;      // R x n
+    double *Ypr = nullptr, *Ypo = nullptr;  // cap x R x n
+    double* W = nullptr;      // cap x R x max(n, n_obs)
+    double* part = nullptr;   // Gram partials
+    double* gout = nullptr;   // cap x R x R
+    double *kinv_pr = nullptr, *kinv_po = nullptr;  // cap x R x R (device copies)
+    double* h = nullptr;      // cap x R
+    double* alpha = nullptr;  // cap x k
+    std::vector<double> hk_pr, hk_po;        // host Kinv, cap x R x R
+    std::vector<double> ld_pr, ld_po;        // log|E| + log|K| per point
+    std::vector<int32_t> ok_pr, ok_po;       // augmented matrix positive definite
+  } lr;
   // workspaces
   double *vprior = nullptr, *vals = nullptr;
   double *x = nullptr, *xt = nullptr, *step = nullptr, *rhs = nullptr, *tmpn = nullptr, *lam = nullptr;
@@ -83,6 +112,47 @@ struct gmrf_model {
 };
 
 namespace {
+
+// LU with partial pivoting of an R x R row-major matrix: inverse, log|det| and
+// the sign of det. False when a pivot is zero.
+bool lu_inverse(int R, std::vector<double> a, double* inv, double& logabs, int& sign) {
+  std::vector<int> piv(R);
+  logabs = 0.0;
+  sign = 1;
+  for (int i = 0; i < R; ++i) {
+    int p = i;
+    for (int r = i + 1; r < R; ++r)
+      if (std::fabs(a[r * R + i]) > std::fabs(a[p * R + i])) p = r;
+    if (a[p * R + i] == 0.0 || !std::isfinite(a[p * R + i])) return false;
+    piv[i] = p;
+    if (p != i) {
+      for (int c = 0; c < R; ++c) std::swap(a[i * R + c], a[p * R + c]);
+      sign = -sign;
+    }
+    const double d = a[i * R + i];
+    logabs += std::log(std::fabs(d));
+    if (d < 0) sign = -sign;
+    for (int r = i + 1; r < R; ++r) {
+      const double f = a[r * R + i] / d;
+      a[r * R + i] = f;
+      for (int c = i + 1; c < R; ++c) a[r * R + c] -= f * a[i * R + c];
+    }
+  }
+  // columns of the inverse: solve L U x = P e_j
+  for (int j = 0; j < R; ++j) {
+    std::vector<double> x(R, 0.0);
+    x[j] = 1.0;
+    for (int i = 0; i < R; ++i) std::swap(x[i], x[piv[i]]);
+    for (int i = 0; i < R; ++i)
+      for (int c = 0; c < i; ++c) x[i] -= a[i * R + c] * x[c];
+    for (int i = R - 1; i >= 0; --i) {
+      for (int c = i + 1; c < R; ++c) x[i] -= a[i * R + c] * x[c];
+      x[i] /= a[i * R + i];
+    }
+    for (int i = 0; i < R; ++i) inv[i * R + j] = x[i];
+  }
+  return true;
+}
 
 // position of (r, c) in CSC column c, -1 if absent
 int64_t csc_find(const std::vector<int64_t>& ip, const std::vector<int64_t>& ix, int64_t r,
@@ -135,13 +205,7 @@ bool project(gmrf_model* M, int B, double* xd) {
   check(lap_constr_dot(B, k, n, M->ca, xd, M->cres, M->stream()), "constr_dot");
   std::vector<double> r((size_t)B * k);
   d2h(r.data(), M->cres, r.size() * sizeof(double), M->stream());
-  std::vector<double> G((size_t)k * k);
-  for (int a = 0; a < k; ++a)
-    for (int b2 = 0; b2 < k; ++b2) {
-      double s = 0.0;
-      for (int64_t c = 0; c < n; ++c) s += M->h_ca[a * n + c] * M->h_ca[b2 * n + c];
-      G[a * k + b2] = s;
-    }
+  const std::vector<double>& G = M->h_cct;
   std::vector<double> lam((size_t)B * k);
   for (int b = 0; b < B; ++b) {
     std::vector<double> rb(k);
@@ -150,8 +214,91 @@ bool project(gmrf_model* M, int B, double* xd) {
     for (int a = 0; a < k; ++a) lam[b * k + a] = rb[a];
   }
   h2d(M->cres, lam.data(), lam.size() * sizeof(double), M->stream());
-  check(lap_constr_apply(B, k, n, M->ca, M->cres, xd, M->stream()), "constr_apply");
+  check(lap_constr_apply(B, k, n, M->ca, M->cres, xd, M->stream(), 1.0), "constr_apply");
   return true;
+}
+
+// Low-rank step for the augmented matrix of batch slots [first, first + B):
+// Y = Q_g^-1 U, K = E^-1 + U^T Y, Kinv, log|E| + log|K| and its sign check.
+void lowrank_factor(gmrf_model* M, int B, int first, bool prior) {
+  auto& L = M->lr;
+  const int R = L.R, k = M->kc;
+  const int64_t n = M->n;
+  const hipStream_t st = M->stream();
+  double* Y = prior ? L.Ypr : L.Ypo;
+  M->bt->solve(B, L.U, n, Y, n, first, R, R);
+  check(lap_gram(B, R, R, n, L.U, n, 0, Y, n, (int64_t)R * n, L.part, L.gout, st), "gram U Y");
+  std::vector<double> G((size_t)B * R * R), al((size_t)B * k);
+  d2h(G.data(), L.gout, G.size() * sizeof(double), st);
+  d2h(al.data(), L.alpha, al.size() * sizeof(double), st);
+  std::vector<double>& kin = prior ? L.hk_pr : L.hk_po;
+  std::vector<double>& ld = prior ? L.ld_pr : L.ld_po;
+  std::vector<int32_t>& ok = prior ? L.ok_pr : L.ok_po;
+  for (int b = 0; b < B; ++b) {
+    std::vector<double> Kb(G.begin() + (size_t)b * R * R, G.begin() + (size_t)(b + 1) * R * R);
+    double logE = 0.0;
+    for (int t = 0; t < k; ++t) {
+      Kb[t * R + t] += 1.0 / M->kappa;
+      Kb[(k + t) * R + k + t] -= 1.0 / al[(size_t)b * k + t];
+      logE += std::log(M->kappa) + std::log(al[(size_t)b * k + t]);
+    }
+    double lk = 0.0;
+    int sg = 0;
+    const bool good = lu_inverse(R, Kb, kin.data() + (size_t)b * R * R, lk, sg);
+    // det(E) has sign (-1)^k; the augmented matrix is positive definite only if
+    // det(E) det(K) > 0
+    const int se = (k & 1) ? -1 : 1;
+    ok[b] = good && se * sg > 0 && std::isfinite(lk);
+    ld[b] = logE + lk;
+  }
+  h2d(prior ? L.kinv_pr : L.kinv_po, kin.data(), (size_t)B * R * R * sizeof(double), st);
+}
+
+// x_b <- x_b - Y_b Kinv_b U^T x_b: turns Q_g^-1 r into (Q_g + U E U^T)^-1 r.
+void lowrank_correct(gmrf_model* M, int B, bool prior, double* x) {
+  auto& L = M->lr;
+  const int R = L.R;
+  const int64_t n = M->n;
+  const hipStream_t st = M->stream();
+  check(lap_gram(B, R, 1, n, L.U, n, 0, x, n, n, L.part, L.gout, st), "gram U x");
+  std::vector<double> g((size_t)B * R), h((size_t)B * R, 0.0);
+  d2h(g.data(), L.gout, g.size() * sizeof(double), st);
+  const std::vector<double>& kin = prior ? L.hk_pr : L.hk_po;
+  for (int b = 0; b < B; ++b)
+    for (int i = 0; i < R; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < R; ++j) a += kin[((size_t)b * R + i) * R + j] * g[(size_t)b * R + j];
+      h[(size_t)b * R + i] = a;
+    }
+  h2d(L.h, h.data(), h.size() * sizeof(double), st);
+  check(lap_lowrank_axpy(B, R, n, prior ? L.Ypr : L.Ypo, n, (int64_t)R * n, L.h, x, n, st), "Y h");
+}
+
+// tr(Kinv_b Y_b^T Q_k Y_b) for the K components (B x K, host).
+std::vector<double> lowrank_traces(gmrf_model* M, int B, bool prior) {
+  auto& L = M->lr;
+  const int R = L.R, K = M->K;
+  const int64_t n = M->n, nnz = M->nnz;
+  const hipStream_t st = M->stream();
+  const double* Y = prior ? L.Ypr : L.Ypo;
+  const std::vector<double>& kin = prior ? L.hk_pr : L.hk_po;
+  std::vector<double> out((size_t)B * K), G((size_t)B * R * R);
+  for (int c = 0; c < K; ++c) {
+    check(lap_spmv(B * R, n, M->q_ptr, M->q_idx, M->comps + (size_t)c * nnz, 0, Y, n, 1.0, 0.0, L.W, n,
+                   st),
+          "Q_k Y");
+    check(lap_gram(B, R, R, n, Y, n, (int64_t)R * n, L.W, n, (int64_t)R * n, L.part, L.gout, st),
+          "gram Y Q_k Y");
+    d2h(G.data(), L.gout, G.size() * sizeof(double), st);
+    for (int b = 0; b < B; ++b) {
+      double t = 0.0;
+      for (int i = 0; i < R; ++i)
+        for (int j = 0; j < R; ++j)
+          t += kin[((size_t)b * R + i) * R + j] * G[((size_t)b * R + j) * R + i];
+      out[(size_t)b * K + c] = t;
+    }
+  }
+  return out;
 }
 
 }  // namespace
@@ -167,6 +314,12 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     return fail(GMRF_INVALID_ARGUMENT, "unknown likelihood family or link");
   if (!d->a_indptr && d->n_obs != d->n)
     return fail(GMRF_INVALID_ARGUMENT, "Observations length must match prior precision dimension");
+  if (d->order != GMRF_ORDER_CHOOSER && d->order != GMRF_ORDER_AMD)
+    return fail(GMRF_INVALID_ARGUMENT, "model ordering must be the chooser or AMD");
+  if (d->constr_kappa < 0.0 || (d->constr_kappa > 0.0 && (d->n_constr < 1 || d->n_constr > 32)))
+    return fail(GMRF_INVALID_ARGUMENT, "low-rank constraint augmentation needs 1..32 constraints");
+  if (d->constr_kappa > 0.0 && (int64_t)d->capacity * 2 * d->n_constr > 65535)
+    return fail(GMRF_INVALID_ARGUMENT, "capacity x 2 k exceeds 65535 right-hand sides");
   try {
     auto M = std::make_unique<gmrf_model>();
     const int64_t n = d->n;
@@ -177,7 +330,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->cap = d->capacity;
     M->nobs = d->n_obs;
     M->kc = d->n_constr;
-    M->sym = analyze_cached(n, d->indptr, d->indices, GMRF_ORDER_CHOOSER, nullptr);
+    M->sym = analyze_cached(n, d->indptr, d->indices, d->order, nullptr);
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     const Symbolic& S = M->sym->S;
     const int64_t nnz = S.nnz_full;
@@ -302,6 +455,55 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
       M->h_ca.assign(d->constr_a, d->constr_a + (size_t)M->kc * n);
       M->h_ce.assign(d->constr_e, d->constr_e + M->kc);
       M->ca = M->arena.upload(M->h_ca);
+      const int k = M->kc;
+      M->h_cct.assign((size_t)k * k, 0.0);
+      for (int a2 = 0; a2 < k; ++a2)
+        for (int b2 = 0; b2 < k; ++b2) {
+          double s2 = 0.0;
+          for (int64_t c = 0; c < n; ++c) s2 += M->h_ca[a2 * n + c] * M->h_ca[b2 * n + c];
+          M->h_cct[a2 * k + b2] = s2;
+        }
+    }
+    // ---- low-rank constraint augmentation
+    if (d->constr_kappa > 0.0) {
+      auto& L = M->lr;
+      const int k = M->kc, R = 2 * k;
+      M->kappa = d->constr_kappa;
+      L.R = R;
+      // grounded node of constraint row t: the largest |A_c[t][j]| among nodes not
+      // yet taken (first on ties)
+      std::vector<char> used(n, 0);
+      std::vector<int64_t> gi(k);
+      std::vector<double> hu((size_t)R * n, 0.0);
+      for (int t = 0; t < k; ++t) {
+        int64_t best = -1;
+        for (int64_t j = 0; j < n; ++j)
+          if (!used[j] && (best < 0 || std::fabs(M->h_ca[t * n + j]) > std::fabs(M->h_ca[t * n + best])))
+            best = j;
+        used[best] = 1;
+        gi[t] = didx[best];
+        std::copy(M->h_ca.begin() + (size_t)t * n, M->h_ca.begin() + (size_t)(t + 1) * n,
+                  hu.begin() + (size_t)t * n);
+        hu[(size_t)(k + t) * n + best] = 1.0;
+      }
+      const size_t C = (size_t)M->cap;
+      L.gidx = M->arena.upload(gi);
+      L.U = M->arena.upload(hu);
+      L.Ypr = M->arena.alloc<double>(C * R * n);
+      L.Ypo = M->arena.alloc<double>(C * R * n);
+      L.W = M->arena.alloc<double>(C * R * std::max<int64_t>(n, M->nobs));
+      L.part = M->arena.alloc<double>(C * ((n + 2047) / 2048) * R * R);
+      L.gout = M->arena.alloc<double>(C * R * R);
+      L.kinv_pr = M->arena.alloc<double>(C * R * R);
+      L.kinv_po = M->arena.alloc<double>(C * R * R);
+      L.h = M->arena.alloc<double>(C * R);
+      L.alpha = M->arena.alloc<double>(C * k);
+      L.hk_pr.assign(C * R * R, 0.0);
+      L.hk_po.assign(C * R * R, 0.0);
+      L.ld_pr.assign(C, 0.0);
+      L.ld_po.assign(C, 0.0);
+      L.ok_pr.assign(C, 0);
+      L.ok_po.assign(C, 0);
     }
     // ---- workspaces
     const size_t C = (size_t)M->cap;
@@ -365,6 +567,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
                          hipMemcpyDeviceToDevice, st),
           "copy prior");
     if (M->kc == 0) check(lap_add_diag(B, n, nnz, M->diag_idx, kPriorJitter, vprior_f, st), "jitter");
+    const bool lr = M->kappa > 0.0;
+    if (lr) check(lap_ground(B, M->kc, nnz, M->lr.gidx, M->vprior, vprior_f, M->lr.alpha, st), "ground");
     bool prior_pending = true;
     // ---- Newton (inference.rs:842-942)
     check(hipMemsetAsync(M->x, 0, (size_t)B * n * sizeof(double), st), "zero x");
@@ -384,13 +588,21 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       check(lap_add_atwa(B, nnz, m, M->atwa_ptr, M->atwa_obs, M->atwa_coef, M->w, M->vprior,
                          M->vals, st),
             "atwa");
+      if (lr) check(lap_ground(B, M->kc, nnz, M->lr.gidx, M->vprior, M->vals, M->lr.alpha, st), "ground");
       check(lap_spmv(B, n, M->a_cptr, M->a_ridx, M->a_cval, 0, M->g, m, 1.0, 0.0, M->rhs, n, st),
             "A^T g");
       check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, -1.0, 1.0, M->rhs, n, st),
             "- Q x");
+      if (lr) {  // - kappa C^T C x
+        check(lap_constr_dot(B, M->kc, n, M->ca, M->x, M->cres, st), "C x");
+        check(lap_constr_apply(B, M->kc, n, M->ca, M->cres, M->rhs, st, M->kappa), "kappa C^T C x");
+      }
       bt->factorize(prior_pending ? 2 * B : B, M->vals, nnz, 0);
+      if (lr && prior_pending) lowrank_factor(M, B, B, true);
       prior_pending = false;
+      if (lr) lowrank_factor(M, B, 0, false);
       bt->solve(B, M->rhs, n, M->step, n, 0);
+      if (lr) lowrank_correct(M, B, false, M->step);
       check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, M->red, st), "max step");
       d2h(hred.data(), M->red, B * sizeof(double), st);
       d2h(hbad.data(), M->bad, B * sizeof(int32_t), st);
@@ -399,6 +611,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       if (a->max_iter == 0) {
         for (int b = 0; b < B; ++b) status[b] = GMRF_SOLVE_FAILURE, state[b] = 3;
         bt->factorize(B, vprior_f, nnz, B);
+        if (lr) lowrank_factor(M, B, B, true);
         prior_pending = false;
       } else {
         newton_system();
@@ -487,6 +700,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     }
     if (prior_pending) {  // no Newton system was formed
       bt->factorize(B, vprior_f, nnz, B);
+      if (lr) lowrank_factor(M, B, B, true);
       prior_pending = false;
     }
     // ---- log-determinants and factor status
@@ -501,6 +715,12 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       };
       if (code(fp) != GMRF_OK) status[b] = code(fp);
       else if (status[b] == GMRF_OK && code(fq) != GMRF_OK) status[b] = code(fq);
+      if (lr) {
+        ld[b] += M->lr.ld_po[b];
+        ld[B + b] += M->lr.ld_pr[b];
+        if (status[b] == GMRF_OK && (!M->lr.ok_pr[b] || !M->lr.ok_po[b]))
+          status[b] = GMRF_NOT_POSITIVE_DEFINITE;
+      }
     }
     // ---- mlik at x*: sum log p(y | A x*) - x*^T Q x* / 2 + (log|Q_prior| - log|Q_post|) / 2
     check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, st), "eta*");
@@ -511,6 +731,14 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st), "quad");
     check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + B, st), "loglik");
     d2h(hred.data(), M->red, 2 * B * sizeof(double), st);
+    if (lr) {  // x*^T (kappa C^T C) x*
+      const int k = M->kc;
+      check(lap_constr_dot(B, k, n, M->ca, M->x, M->cres, st), "C x*");
+      std::vector<double> cx((size_t)B * k);
+      d2h(cx.data(), M->cres, cx.size() * sizeof(double), st);
+      for (int b = 0; b < B; ++b)
+        for (int t = 0; t < k; ++t) hred[b] += M->kappa * cx[(size_t)b * k + t] * cx[(size_t)b * k + t];
+    }
     std::vector<double> mlik(B);
     for (int b = 0; b < B; ++b) {
       mlik[b] = hred[B + b] - 0.5 * hred[b] + 0.5 * ld[B + b] - 0.5 * ld[b];
@@ -524,7 +752,13 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       bt->selinv(2 * B, 0);
     else if (a->var)
       bt->selinv(B, 0);
-    if (a->var) check(launch_diag_out(M->sym->plan, B, bt->S, a->var, n, st), "var");
+    if (a->var) {
+      check(launch_diag_out(M->sym->plan, B, bt->S, a->var, n, st), "var");
+      if (lr)
+        check(lap_lowrank_diag(B, M->lr.R, n, M->lr.Ypo, n, (int64_t)M->lr.R * n, M->lr.kinv_po, a->var,
+                               n, st),
+              "var correction");
+    }
     // ---- gradient
     if (want_grad) {
       const int mt = a->n_theta;
@@ -538,6 +772,14 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
             "contract prior");
       // v_i = (A Sigma_post A^T)_ii
       check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
+      if (lr) {  // v_i -= (A Y Kinv Y^T A^T)_ii
+        const int R = M->lr.R;
+        check(lap_spmv(B * R, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->lr.Ypo, n, 1.0, 0.0, M->lr.W, m,
+                       st),
+              "A Y");
+        check(lap_lowrank_diag(B, R, m, M->lr.W, m, (int64_t)R * m, M->lr.kinv_po, M->v, m, st),
+              "v correction");
+      }
       std::vector<double> hql((size_t)B * K, 0.0);
       if (!gauss_id) {
         // mode sensitivity: u = v .* dw/deta, lambda = Q_post^-1 A^T u
@@ -545,6 +787,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_spmv(B, n, M->a_cptr, M->a_ridx, M->a_cval, 0, M->u, m, 1.0, 0.0, M->rhs, n, st),
               "A^T u");
         bt->solve(B, M->rhs, n, M->lam, n, 0);
+        if (lr) lowrank_correct(M, B, false, M->lam);
         check(lap_contract(B, K, nnz, M->comps, nullptr, 0, nullptr, M->q_row, M->q_col, M->lam, M->x,
                            n, nullptr, M->ql, M->cpart, st),
               "lambda^T Q_k x");
@@ -558,6 +801,13 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       std::vector<double> htp((size_t)B * K), htq((size_t)B * K), hqx((size_t)B * K);
       d2h(htp.data(), M->tr_post, htp.size() * sizeof(double), st);
       d2h(htq.data(), M->tr_prior, htq.size() * sizeof(double), st);
+      if (lr) {  // tr((Q_g + U E U^T)^-1 Q_k) = tr(Q_g^-1 Q_k) - tr(Kinv Y^T Q_k Y)
+        const std::vector<double> cpo = lowrank_traces(M, B, false), cpr = lowrank_traces(M, B, true);
+        for (size_t e = 0; e < htp.size(); ++e) {
+          htp[e] -= cpo[e];
+          htq[e] -= cpr[e];
+        }
+      }
       d2h(hqx.data(), M->qx, hqx.size() * sizeof(double), st);
       std::vector<double> dc((size_t)B * mt * K), dlp((size_t)B * mt, 0.0);
       d2h(dc.data(), a->dcoefs, dc.size() * sizeof(double), st);

@@ -42,20 +42,21 @@ hipError_t launch_logdet(const DevPlan&, int, const double*, double*, uint32_t*,
                          hipStream_t);
 hipError_t launch_logdet_seg(const DevPlan&, int, const double*, const int64_t*, const int32_t*, int,
                              double*, int32_t*, hipStream_t);
-hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t);
+hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, double*, hipStream_t,
+                              int period);
 hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
-hipError_t launch_diag_scale(const DevPlan&, int, const double*, double*, hipStream_t);
-hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
+hipError_t launch_diag_scale(const DevPlan&, int, int, const double*, double*, hipStream_t);
+hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, int, const double*, double*,
                             double*, hipStream_t);
-hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
+hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, int, const double*, double*,
                             hipStream_t);
-hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, const double*,
+hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, int, const double*,
                                 const double*, double*, const double*, hipStream_t);
-hipError_t launch_solve_fwd_rows(const DevPlan&, const int32_t*, int, int, int, const double*,
+hipError_t launch_solve_fwd_rows(const DevPlan&, const int32_t*, int, int, int, int, const double*,
                                  const double*, double*, hipStream_t);
-hipError_t launch_solve_bwd_rows(const DevPlan&, const int32_t*, int, int, const double*,
+hipError_t launch_solve_bwd_rows(const DevPlan&, const int32_t*, int, int, int, const double*,
                                  const double*, double*, hipStream_t);
-hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, const double*,
+hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, int, const double*,
                                 const double*, double*, const double*, hipStream_t);
 hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                double*, double*, const double*, hipStream_t);
@@ -710,6 +711,10 @@ struct gmrf_batch {
   int32_t* d_status = nullptr;
   double* d_ldpart = nullptr;     // per-matrix chunk partials of log|D|
   uint32_t* d_ldcount = nullptr;  // per-matrix chunk counters (zero between launches)
+  // forward/backward workspaces for solves with several right-hand sides per
+  // matrix (grown on demand; one vector = n + v_size + t_size doubles)
+  double* mrhs = nullptr;
+  int64_t mrhs_vecs = 0;
 
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
@@ -736,6 +741,7 @@ struct gmrf_batch {
       (void)hipStreamSynchronize(own);
       (void)hipStreamDestroy(own);
     }
+    if (mrhs) (void)hipFree(mrhs);
   }
 
   // Matrices [first, first + count) of the batch.
@@ -782,52 +788,73 @@ struct gmrf_batch {
     });
   }
 
-  void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0) {
+  // Matrices [first, first + count), nr right-hand sides each: vector j of
+  // matrix first + b is rhs + (b nr + j) ldr (rhs + ((b nr + j) % period) ldr when
+  // period > 0: one set of vectors for every matrix), its solution x + (b nr + j) ldx.
+  void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0,
+             int nr = 1, int period = 0) {
     const DevPlan& P = sym->plan;
+    const int64_t vsz = std::max<int64_t>(P.v_size, 1);
+    double *Xf, *Vf, *Tf;
+    if (nr == 1) {
+      Xf = X + (int64_t)first * P.n;
+      Vf = V + (int64_t)first * vsz;
+      Tf = T + (int64_t)first * P.t_size;
+    } else {
+      const int64_t vecs = (int64_t)count * nr;
+      if (vecs > mrhs_vecs) {
+        check(hipStreamSynchronize(stream), "sync");
+        if (mrhs) check(hipFree(mrhs), "hipFree");
+        mrhs = nullptr;
+        mrhs_vecs = 0;
+        check(hipMalloc(&mrhs, (size_t)vecs * (P.n + vsz + P.t_size) * sizeof(double)), "hipMalloc");
+        mrhs_vecs = vecs;
+      }
+      Xf = mrhs;
+      Vf = Xf + vecs * P.n;
+      Tf = Vf + vecs * vsz;
+    }
     const double* Lf = L + (int64_t)first * P.l_size;
     const double* Zf = Z + (int64_t)first * P.z_size;
-    double* Xf = X + (int64_t)first * P.n;
-    double* Vf = V + (int64_t)first * std::max<int64_t>(P.v_size, 1);
-    double* Tf = T + (int64_t)first * P.t_size;
     const int32_t* tk = sym->d_tasks;
-    check(launch_gather_perm(P, count, rhs, ldr, Xf, stream), "gather");
+    check(launch_gather_perm(P, count * nr, rhs, ldr, Xf, stream, period), "gather");
     for (const LevelInfo& lv : sym->levels) {
       if (lv.se > lv.sb)
-        prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count, 0.0, stream, [&] {
-          return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
+        prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
+          return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count, nr,
                                   Lf, Xf, Vf, stream);
         });
       if (lv.le > lv.lb) {
-        prof_launch("k_solve_fwd_tri", 0.5 * lv.sol_tri * count, 0.0, stream, [&] {
+        prof_launch("k_solve_fwd_tri", 0.5 * lv.sol_tri * count * nr, 0.0, stream, [&] {
           return launch_solve_fwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
-                                      count, Lf, Zf, Xf, Vf, stream);
+                                      count, nr, Lf, Zf, Xf, Vf, stream);
         });
-        prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count, 0.0, stream, [&] {
+        prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
           return launch_solve_fwd_rows(P, tk + 3 * lv.frow.off, (int)lv.frow.cnt, lv.maxw_large, count,
-                                       Lf, Xf, Vf, stream);
+                                       nr, Lf, Xf, Vf, stream);
         });
       }
     }
-    check(launch_diag_scale(P, count, Lf, Xf, stream), "diag_scale");
+    check(launch_diag_scale(P, count, nr, Lf, Xf, stream), "diag_scale");
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
-        prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count, 0.0, stream, [&] {
-          return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, Lf, Xf, Tf,
+        prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
+          return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, nr, Lf, Xf, Tf,
                                        stream);
         });
-        prof_launch("k_solve_bwd_tri", 0.5 * lv.sol_tri * count, 0.0, stream, [&] {
+        prof_launch("k_solve_bwd_tri", 0.5 * lv.sol_tri * count * nr, 0.0, stream, [&] {
           return launch_solve_bwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
-                                      count, Lf, Zf, Xf, Tf, stream);
+                                      count, nr, Lf, Zf, Xf, Tf, stream);
         });
       }
       if (lv.se > lv.sb)
-        prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count, 0.0, stream, [&] {
-          return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count,
+        prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
+          return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count, nr,
                                   Lf, Xf, stream);
         });
     }
-    check(launch_scatter_perm(P, count, Xf, x, ldx, stream), "scatter");
+    check(launch_scatter_perm(P, count * nr, Xf, x, ldx, stream), "scatter");
   }
 
   void selinv(int count, int first = 0) {

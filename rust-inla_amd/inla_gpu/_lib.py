@@ -73,6 +73,8 @@ class ModelDesc(C.Structure):
         ("constr_a", _pdbl),
         ("constr_e", _pdbl),
         ("capacity", C.c_int32),
+        ("constr_kappa", C.c_double),
+        ("order", C.c_int32),
     ]
 
 
@@ -114,6 +116,7 @@ SIGNATURES = {
     "gmrf_batch_logdet": (_i32, [_vp, _i32, _pdbl, _pi32]),
     "gmrf_batch_logdet_device": (_i32, [_vp, _i32, _vp, _vp]),
     "gmrf_batch_solve": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64]),
+    "gmrf_batch_solve_multi": (_i32, [_vp, _i32, _i32, _vp, _i64, _vp, _i64]),
     "gmrf_batch_selinv": (_i32, [_vp, _i32]),
     "gmrf_batch_selinv_diag": (_i32, [_vp, _i32, _vp, _i64]),
     "gmrf_batch_selinv_pattern": (_i32, [_vp, _i32, _vp, _i64]),

@@ -53,7 +53,13 @@ class LaplaceModel:
     """Latent Gaussian model with affine Q(theta) = sum_k c_k Q_k, resident on the GPU."""
 
     def __init__(self, n, indptr, indices, components, y, family="gaussian", link=None, A=None,
-                 obs_a=None, obs_b=None, constraints=None, capacity=16):
+                 obs_a=None, obs_b=None, constraints=None, capacity=16, constraint_kappa=None,
+                 order="chooser"):
+        """constraints = (A_c (k x n), e): hard constraints by projection. With
+        constraint_kappa, the augmentation Q + kappa A_c^T A_c of the prior and the
+        posterior is applied as an exact low-rank correction (the components then
+        must not contain it); otherwise a kappa A_c^T A_c component carries it, as
+        the reference forms it. order: "chooser" (the reference's choice) or "amd"."""
         link = link or {"gaussian": "identity", "poisson": "log", "binomial": "logit",
                         "nbinomial": "log"}[family]
         self.n = int(n)
@@ -106,6 +112,11 @@ class LaplaceModel:
             d.constr_a = ca.ctypes.data_as(_pd)
             d.constr_e = ce.ctypes.data_as(_pd)
         d.capacity = self.capacity
+        if constraint_kappa is not None:
+            if constraints is None:
+                raise InvalidArgument("constraint_kappa needs constraints")
+            d.constr_kappa = float(constraint_kappa)
+        d.order = {"chooser": L.ORDER_CHOOSER, "amd": L.ORDER_AMD}[order]
         h = C.c_void_p()
         check(L.lib.gmrf_model_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -221,6 +221,20 @@ class Batch:
                                      C.c_void_p(out.data_ptr()), self.n))
         return out
 
+    def solve_multi(self, rhs, out=None):
+        """rhs of shape (count, nrhs, n): nrhs right-hand sides per matrix, one launch sequence."""
+        import torch
+
+        if rhs.dim() != 3 or rhs.shape[2] != self.n:
+            raise ValueError("rhs must have shape (count, nrhs, n)")
+        r = self._dev(rhs.reshape(-1, self.n), self.n, "rhs").reshape(rhs.shape)
+        if out is None:
+            out = torch.empty_like(r)
+        self._stream()
+        check(L.lib.gmrf_batch_solve_multi(self._h, r.shape[0], r.shape[1], C.c_void_p(r.data_ptr()), self.n,
+                                           C.c_void_p(out.data_ptr()), self.n))
+        return out
+
     def selinv(self, count=None):
         count = self.count if count is None else count
         self._stream()

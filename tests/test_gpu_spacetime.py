@@ -92,3 +92,57 @@ def test_spde_rw2_negbin_matches_oracle():
         assert abs(r["mlik"][b].item() - ref.mlik) < 1e-8 * max(1.0, abs(ref.mlik))
         assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
         assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < 1e-9
+
+
+def test_besag_ar1_lowrank_constraints_match_augmented():
+    """The augmentation applied as a low-rank correction of the grounded precision
+    (constraint_kappa) against the explicit kappa A_c^T A_c component and the oracle.
+    The explicit form carries the conditioning of kappa A_c^T A_c (log-determinant
+    errors near 1e-8 at this size); the low-rank form is exact up to rounding (see
+    tests/test_lowrank_identity.py), so the bars here are 1e-7."""
+    from inla_gpu.laplace import LaplaceModel
+
+    wl, model, ca, ce = _besag_ar1_model()
+    g = wl.gmrf
+    lr = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="binomial", obs_a=wl.obs_a,
+                      constraints=(ca, ce), capacity=3, constraint_kappa=KAPPA)
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.3, -0.2], [-0.2, 0.4]])
+    c3, dc3, _, _ = wl.params(thetas)
+    ra = model.eval(np.c_[c3, np.ones(len(thetas))],
+                    dcoefs=np.concatenate([dc3, np.zeros((3, 2, 1))], axis=2))
+    rl = lr.eval(c3, dcoefs=dc3)
+    assert np.all(rl["status"] == 0)
+    for b in range(len(thetas)):
+        ref = orc.find_mode(g.csc(c3[b]), "binomial", "logit", wl.y, wl.obs_a, constraints=(ca, ce),
+                            keep_post=True)
+        m = rl["mlik"][b].item()
+        assert abs(m - ref.mlik) < 1e-7 * max(1.0, abs(ref.mlik))
+        assert abs(m - ra["mlik"][b].item()) < 1e-7 * max(1.0, abs(ref.mlik))
+        x = rl["x"][b].cpu().numpy()
+        assert _rel(x, ref.x) < 1e-7
+        assert np.max(np.abs(ca @ x)) < 1e-8
+        assert _rel(rl["var"][b].cpu().numpy(), ra["var"][b].cpu().numpy()) < 1e-7
+        assert _rel(rl["grad"][b].cpu().numpy(), ra["grad"][b].cpu().numpy()) < 1e-6
+
+
+def test_besag_ar1_lowrank_gradient_matches_central_fd():
+    from inla_gpu.laplace import LaplaceModel
+
+    wl, _, ca, ce = _besag_ar1_model()
+    g = wl.gmrf
+    lr = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="binomial", obs_a=wl.obs_a,
+                      constraints=(ca, ce), capacity=1, constraint_kappa=KAPPA, order="amd")
+    th = wl.theta_true + np.array([0.1, -0.1])
+    h = 1e-4
+    c3, dc3, _, _ = wl.params(th[None])
+    gr = lr.eval(c3, dcoefs=dc3)["grad"][0].cpu().numpy()
+    fd = []
+    for l in range(2):
+        vals = []
+        for s in (h, -h):
+            t = th.copy()
+            t[l] += s
+            c, _, _, _ = wl.params(t[None])
+            vals.append(lr.eval(c)["mlik"][0].item())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    assert _rel(gr, np.array(fd)) < 1e-5, (gr, fd)
