@@ -301,6 +301,91 @@ def c4_solver_line(nr=100, T=20, probes=6):
     return big_solver_line(Q, f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7", probes)
 
 
+def c4_fit_line(order="amd", nr=100, T=20):
+    """Full-size Besag x AR1 binomial(10) (100 x 100 rook lattice x T = 20, n = 200,000) with
+    its 20 sum-to-zero rows applied as the low-rank augmentation (the explicit
+    kappa A_c^T A_c has 2e9 entries): one batched evaluation of the 9 CCD points with
+    the adjoint gradient (theta-points/s, also under the reference chooser's RCM
+    order), and the end-to-end fit (mode, Hessian, CCD nodes, mixture): batched BFGS
+    on the adjoint gradient vs the one-point Nelder-Mead sequence. Priors: the
+    reference's ar1 defaults pc.prec(1, 0.01) and pc.cor1(0.5, 0.75)."""
+    import numpy as np
+    import torch
+
+    from inla_gpu import ccd
+    from inla_gpu import fit as F
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    t0 = time.perf_counter()
+    wl = W.besag_ar1_workload(nr=nr, T=T)
+    ca, ce = wl.constraints
+    ca = ca / math.sqrt(nr * nr)
+    g = wl.gmrf
+    log(f"[c4 fit] workload generated in {time.perf_counter() - t0:.1f} s")
+
+    def make(order_):
+        return LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="binomial", obs_a=wl.obs_a,
+                            constraints=(ca, ce), constraint_kappa=3.059023205018258e6, capacity=9,
+                            order=order_)
+
+    model = make(order)
+    m = len(wl.theta_true)
+    hh = max(1e-4, 0.05 * float(np.max(np.abs(wl.theta_true))))
+    stencil = ccd.hessian_points(wl.theta_true, hh)
+    ob = F.Objective(model, wl.params, wl.log_prior)
+    lpost = ob.values(stencil)["g"]
+    lam, V = ccd.scaling_from_hessian(ccd.hessian_from_values(lpost, m, hh))
+    z, _ = ccd.ccd_design(m, 1.0)
+    thetas = np.array([ccd.theta_from_z(wl.theta_true, lam, V, zz) for zz in z])
+    coefs, dcoefs, _, _ = wl.params(thetas)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    coefs, dcoefs = dev(coefs), dev(dcoefs)
+
+    def timed_eval(mod):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        out = mod.eval(coefs, dcoefs=dcoefs)
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - a
+
+    timed_eval(model)
+    out, dt = timed_eval(model)
+    res = {"workload": f"besag {nr}x{nr} x ar1(T={T}) binomial(10), n={g.n}, 20 sum-to-zero rows (low rank)",
+           "order": order, "theta_points": len(thetas), "eval_s": dt, "value": len(thetas) / dt,
+           "unit": "theta-points/s", "newton_iters": [int(v) for v in out["n_iter"]],
+           "status_ok": bool((out["status"] == 0).all())}
+    log(f"[c4 fit] batched 9-point evaluation: {dt:.2f} s")
+    del model
+    if order != "chooser":
+        mc = make("chooser")
+        oc, dtc = timed_eval(mc)
+        res["chooser_eval_s"] = dtc
+        res["chooser_vs_amd_mlik_maxdiff"] = float((oc["mlik"] - out["mlik"]).abs().max().item())
+        log(f"[c4 fit] the same under the chooser's order: {dtc:.2f} s")
+        del mc
+        model = make(order)
+    class Logged(F.Objective):  # a progress line every 10 calls (long one-point sequences)
+        def values(self, *a, **k):
+            if self.calls % 10 == 0:
+                log(f"[c4 fit] {self.calls} calls, {self.points} theta-points")
+            return super().values(*a, **k)
+
+    theta0 = wl.theta_true + np.array([0.5, -0.3])
+    rb = F.fit_batched(Logged(model, wl.params, wl.log_prior), theta0)
+    log(f"[c4 fit] batched BFGS fit {rb.seconds['total']:.1f} s")
+    rp = F.fit_plain(Logged(model, wl.params, wl.log_prior), theta0)
+    log(f"[c4 fit] Nelder-Mead fit {rp.seconds['total']:.1f} s")
+    res.update({"fit_batched_s": rb.seconds["total"], "fit_plain_s": rp.seconds["total"],
+                "fit_speedup": rp.seconds["total"] / rb.seconds["total"],
+                "batched_calls": rb.n_evals, "batched_points": rb.n_points,
+                "plain_calls": rp.n_evals, "plain_points": rp.n_points,
+                "mode_batched": [float(v) for v in rb.mode], "mode_plain": [float(v) for v in rp.mode],
+                "mlik_batched": float(rb.mlik), "mlik_plain": float(rp.mlik),
+                "theta_true": [float(v) for v in wl.theta_true]})
+    return res
+
+
 def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
     """Full-size SPDE x RW2 (245 x 245 lattice x 24 time steps, n = 1,440,600; RCM chain of
     fronts up to 13k wide, about 100 GB of L): Q_post with the negative binomial (size 5)
@@ -325,6 +410,8 @@ def main():
                     help="skip the Poisson and heterogeneous-batch lines")
     ap.add_argument("--c4", action="store_true",
                     help="add the full-size Besag x AR1 solver line (about 5 GB of HBM)")
+    ap.add_argument("--c4-fit", action="store_true",
+                    help="add the full-size Besag x AR1 batched evaluation and fit comparison")
     ap.add_argument("--c5", action="store_true",
                     help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
     ap.add_argument("--nx", type=int, default=142)
@@ -462,6 +549,10 @@ def main():
         log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
         res["c4"] = c4_solver_line()
         log(json.dumps(res["c4"]))
+    if args.c4_fit:
+        log("== full-size Besag x AR1 binomial: batched CCD evaluation, BFGS vs Nelder-Mead fit")
+        res["c4_fit"] = c4_fit_line()
+        log(json.dumps(res["c4_fit"]))
     if args.c5:
         log("== full-size SPDE x RW2 (n = 1.44M): factorise, solve, selected inversion")
         res["c5"] = c5_solver_line()

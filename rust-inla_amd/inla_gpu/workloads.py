@@ -236,9 +236,8 @@ def sample_gmrf(Q: sp.csc_matrix, rng: np.random.Generator) -> np.ndarray:
 
     Lt = sp.diags(1.0 / d) @ lu.U
     y = spsolve_triangular(sp.csr_matrix(Lt), z, lower=False, unit_diagonal=True)
-    x = np.empty(n)
-    x[lu.perm_c] = y
-    return x
+    # P Q P^T = L D L^T with (P Q P^T)[perm[i], perm[j]] = Q[i, j]: x = P^T y
+    return y[lu.perm_c]
 
 
 # --------------------------------------------------------------------------
@@ -252,6 +251,37 @@ def loggamma_prior(theta, shape=1.0, rate=5e-5):
     """log pi(theta) and d/dtheta for Gamma(shape, rate) on exp(theta) (priors.rs:836-855)."""
     e = math.exp(theta)
     return shape * math.log(rate) - math.lgamma(shape) + shape * theta - rate * e, shape - rate * e
+
+
+def pc_prec_prior(theta, u=1.0, alpha=0.01):
+    """PC prior of a precision on theta = log tau (priors.rs:532-546) and its gradient:
+    log(lambda / 2) - lambda e^(-theta/2) - theta/2 with lambda = -log(alpha) / u."""
+    lam = -math.log(alpha) / u
+    e = math.exp(-0.5 * theta)
+    return math.log(lam) - math.log(2.0) - lam * e - 0.5 * theta, 0.5 * lam * e - 0.5
+
+
+def pc_cor1_lambda(u=0.5, alpha=0.75):
+    """lambda of the PC prior for a correlation with base model rho = 1: the root of
+    (1 - e^(-lambda sqrt(1 - u))) / (1 - e^(-lambda sqrt 2)) = alpha (the equation of
+    priors.rs:611-660, solved here by bracketing instead of a golden-section search)."""
+    from scipy.optimize import brentq
+
+    def f(lam):
+        return (1.0 - math.exp(-lam * math.sqrt(1.0 - u))) / (1.0 - math.exp(-lam * math.sqrt(2.0))) - alpha
+
+    return brentq(f, 1e-9, 1e3, xtol=1e-15, rtol=4 * np.finfo(float).eps, maxiter=500)
+
+
+def pc_cor1_prior(theta, lam):
+    """PC prior of rho = 2 / (1 + e^-theta) - 1 with base model rho = 1, on theta
+    (priors.rs:662-705), and its gradient."""
+    e = math.exp(theta)
+    rho = 2.0 * e / (1.0 + e) - 1.0
+    s = math.sqrt(1.0 - rho)
+    logp = (math.log(lam) - lam * s - math.log(1.0 - math.exp(-math.sqrt(2.0) * lam)) - math.log(2.0 * s)
+            + math.log(2.0) + theta - 2.0 * math.log1p(e))
+    return logp, (lam * s + 1.0) * (1.0 + rho) / 4.0 - rho
 
 
 def pc_spde_prior(log_tau, log_kappa, range0=0.1, alpha_r=0.5, sigma0=1.0, alpha_s=0.01, d=2.0):
@@ -557,6 +587,7 @@ class KronWorkload:
     constraints: object
     theta_true: np.ndarray
     params_fn: object
+    log_prior_fn: object = None
 
     @property
     def n(self):
@@ -564,6 +595,9 @@ class KronWorkload:
 
     def params(self, thetas):
         return self.params_fn(np.atleast_2d(thetas))
+
+    def log_prior(self, theta):
+        return self.log_prior_fn(np.asarray(theta, dtype=float))
 
 
 def besag_ar1_workload(nr=100, T=20, seed=20241206, tau=2.0, rho=0.7, trials=10, scale=None,
@@ -609,8 +643,16 @@ def besag_ar1_workload(nr=100, T=20, seed=20241206, tau=2.0, rho=0.7, trials=10,
             dco[b, 1] = t_ * np.array([0.0, 2.0 * r_, -1.0]) * dr
         return coefs, dco, None, None
 
+    lam_rho = pc_cor1_lambda(0.5, 0.75)
+
+    def log_prior(th):
+        # the reference's ar1 defaults (priors.rs:369-372): pc.prec(1, 0.01), pc.cor1(0.5, 0.75)
+        a, ga = pc_prec_prior(th[0])
+        b, gb = pc_cor1_prior(th[1], lam_rho)
+        return a + b, np.array([ga, gb])
+
     return KronWorkload(g, None, y, np.full(n, float(trials)), None, "binomial", (ca, np.zeros(T)),
-                        np.array([math.log(tau), th_rho]), params)
+                        np.array([math.log(tau), th_rho]), params, log_prior)
 
 
 def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3):
