@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_fl_input(DevPlan P, const int32_t* __re
   const int64_t q0 = P.fa_ptr[s], q1 = P.fa_ptr[s + 1];
   for (int64_t qb = q0; qb < q1; qb += 4 * 256) {
     int slot[4];
-    double v[4];
+    double v[4], old[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t q = min(qb + threadIdx.x + 256 * u, q1 - 1);
@@ -533,8 +533,11 @@ __global__ __launch_bounds__(256) void k_fl_input(DevPlan P, const int32_t* __re
     }
     ISSUE_FENCE();
 #pragma unroll
+    for (int u = 0; u < 4; ++u) old[u] = Lb[slot[u]];
+    ISSUE_FENCE();
+#pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (qb + threadIdx.x + 256 * u < q1) Lb[slot[u]] += v[u];
+      if (qb + threadIdx.x + 256 * u < q1) Lb[slot[u]] = old[u] + v[u];
   }
 }
 
@@ -625,8 +628,9 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
 __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __restrict__ tasks,
                                                    int step, double* __restrict__ L,
                                                    double* __restrict__ U) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double sh[2 * 64 * (kPanel + 1)];  // the tile engine's operands, then the tile
+  double (*As)[kPanel + 1] = reinterpret_cast<double (*)[kPanel + 1]>(sh);
+  double (*Bs)[kPanel + 1] = reinterpret_cast<double (*)[kPanel + 1]>(sh + 64 * (kPanel + 1));
   __shared__ double dpiv[kTile];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1], J = tk[2];
@@ -652,10 +656,34 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
         return j < m ? v : 0.0;
       },
       nb, As, Bs, acc);
-  tile_foreach(acc, [&](int ii, int jj, double v) {
-    const int i = R0 + ii, j = C0 + jj;
-    if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) -= v;
-  });
+  // C -= tile: the tile goes through LDS (the tile engine ends with a barrier) so
+  // that thread (row ii = tid % 64) updates columns tid / 64 + 4u: each wave covers
+  // 64 consecutive rows of a column (coalesced), and all 16 loads are issued (at
+  // clamped valid offsets) before any is used.
+  static_assert(kTile * (kTile + 1) <= 2 * 64 * (kPanel + 1), "tile staging exceeds the operand LDS");
+  double (*Ct)[kTile + 1] = reinterpret_cast<double (*)[kTile + 1]>(sh);
+  tile_foreach(acc, [&](int ii, int jj, double v) { Ct[jj][ii] = v; });
+  __syncthreads();
+  const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
+  const int i = R0 + ii, ic = min(i, m - 1);
+  int off[16];
+  double old[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int j = C0 + jb + 4 * u;
+    const bool ok = i < m && j < m && i >= j;
+    const int jc = ok ? j : min(C0, ic), i2 = ok ? i : ic;
+    // within-front offsets fit in 32 bits (m w and r^2 below 2^31)
+    off[u] = jc >= w ? (jc - w) * r + (i2 - w) : jc * m + i2;
+    old[u] = (jc >= w ? Ub : Lb)[off[u]];
+  }
+  ISSUE_FENCE();
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int j = C0 + jb + 4 * u;
+    const double nv = old[u] - Ct[jb + 4 * u][ii];
+    if (i < m && j < m && i >= j) (j >= w ? Ub : Lb)[off[u]] = nv;
+  }
 }
 
 // log|Q| = sum log|D_j| per matrix, plus the pivot status: an exact zero pivot
@@ -879,20 +907,21 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
   for (int j = tid; j < w; j += blockDim.x) Xb[a0 + j] = t[j];
 }
 
-// Large fronts, forward, in two launches per level. The triangular part (one
-// workgroup per front) solves the supernode's own w x w block, blocked by the
-// factorisation's 64-column panels with the stored Z = L_PP^-1: y_P = Z f_P, then
-// f_B -= L[B,P] y_P for the rows B below the panel inside the supernode. Both
-// products split their K dimension over waves so that each panel costs one or two
-// rounds of loads, reduced through LDS. Children's update entries that land in
-// the supernode's columns are added first; those landing in R_s are gathered by
-// k_solve_fwd_rows.
-template <int NT>
+// Large fronts, forward. The triangular part (one workgroup per front) solves the
+// supernode's own w x w block, blocked by the factorisation's 64-column panels with
+// the stored Z = L_PP^-1: y_P = Z f_P, then f_B -= L[B,P] y_P for the rows B below
+// the panel. Both products split their K dimension over waves so that each panel
+// costs one or two rounds of loads, reduced through LDS. With FULL the workgroup
+// also carries the rows R_s below (f of length m, children's updates added in
+// full, V_s written): levels of short fronts. Otherwise only the entries landing
+// in the supernode's columns are added here, and the rows below are a separate
+// row-parallel launch (k_solve_fwd_rows): tall fronts.
+template <int NT, bool FULL>
 __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                       const double* __restrict__ L,
                                                       const double* __restrict__ Z,
                                                       double* __restrict__ X,
-                                                      const double* __restrict__ V) {
+                                                      double* __restrict__ V) {
   constexpr int NQ = NT / 64, QW = 64 / NQ, NR = NT / 4;  // y: NQ parts of QW columns; rows: NR per round
   extern __shared__ double f[];
   __shared__ double part[NQ][kTile];
@@ -902,10 +931,11 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
   const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, m = w + P.sn_r[s];
+  const int mr = FULL ? m : w;  // rows carried by this workgroup
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  for (int i = tid; i < w; i += NT) f[i] = Xb[a0 + i];
+  for (int i = tid; i < mr; i += NT) f[i] = i < w ? Xb[a0 + i] : 0.0;
   __syncthreads();
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int c = P.child_idx[cq];
@@ -914,7 +944,7 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
     const int32_t* relc = P.rel + P.rows_ptr[c];
     for (int i = tid; i < rc; i += NT) {
       const int pi = relc[i];
-      if (pi < w) f[pi] += Vc[i];
+      if (pi < mr) f[pi] += Vc[i];
     }
     __syncthreads();
   }
@@ -946,8 +976,8 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
       f[c0 + tid] = v;
     }
     __syncthreads();
-    for (int row0 = c1; row0 < w; row0 += NR) {
-      const int row = min(row0 + rr, w - 1);
+    for (int row0 = c1; row0 < mr; row0 += NR) {
+      const int row = min(row0 + rr, mr - 1);
       double lv[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(16 * qp + u, nb - 1)) * m + row];
@@ -960,11 +990,17 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
       }
       rp[qp][rr] = a;
       __syncthreads();
-      if (tid < NR && row0 + tid < w) f[row0 + tid] -= rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
+      if (tid < NR && row0 + tid < mr) f[row0 + tid] -= rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
       __syncthreads();
     }
   }
-  for (int k = tid; k < w; k += NT) Xb[a0 + k] = f[k];
+  double* Vs = V + (int64_t)b * P.v_size + P.v_off[s];
+  for (int k = tid; k < mr; k += NT) {
+    if (k < w)
+      Xb[a0 + k] = f[k];
+    else
+      Vs[k - w] = f[k];
+  }
 }
 
 // Forward, rows below a large supernode: task (s, chunk) forms the update entries
@@ -1058,9 +1094,10 @@ __global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t
 
 // Backward, triangular part of a large front, panels right to left:
 // t_P = x_P - sum_parts (L[R,P]^T x_R) - L[B,P]^T x_B (B = rows below the panel
-// inside the supernode), x_P = Z^T t_P. Wave wv covers panel columns
+// inside the supernode), x_P = Z^T t_P. With FULL, B extends over R_s (x_R
+// gathered into LDS) and there are no partial sums. Wave wv covers panel columns
 // q = (wv & 3) + 4u (u < 16) over rows c1 + 64 (wv >> 2) + lane (+NT / 4 k).
-template <int NT>
+template <int NT, bool FULL>
 __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* __restrict__ list, int nr,
                                                       const double* __restrict__ L,
                                                       const double* __restrict__ Z,
@@ -1079,11 +1116,17 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
   const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
   const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
-  const int nparts = (r + kSolvePart - 1) / kSolvePart;
-  for (int k = tid; k < w; k += NT) {
-    double v = Xb[a0 + k];
-    for (int q = 0; q < nparts; ++q) v -= Tb[(int64_t)q * w + k];
-    xf[k] = v;
+  const int mr = FULL ? m : w;
+  if (FULL) {
+    const int32_t* rows = P.rows + P.rows_ptr[s];
+    for (int k = tid; k < m; k += NT) xf[k] = k < w ? Xb[a0 + k] : Xb[rows[k - w]];
+  } else {
+    const int nparts = (r + kSolvePart - 1) / kSolvePart;
+    for (int k = tid; k < w; k += NT) {
+      double v = Xb[a0 + k];
+      for (int q = 0; q < nparts; ++q) v -= Tb[(int64_t)q * w + k];
+      xf[k] = v;
+    }
   }
   __syncthreads();
   const int np = (w + kTile - 1) / kTile;
@@ -1092,9 +1135,9 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
     double acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    for (int k = c1 + 64 * kg + lane; k < w + 63; k += 64 * NK) {
-      const int kc = min(k, w - 1);
-      const double xk = k < w ? xf[kc] : 0.0;
+    for (int k = c1 + 64 * kg + lane; k < mr + 63; k += 64 * NK) {
+      const int kc = min(k, mr - 1);
+      const double xk = k < mr ? xf[kc] : 0.0;
       double lv[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(qg + 4 * u, nb - 1)) * m + kc];
@@ -1719,25 +1762,29 @@ static hipError_t lds_limit(const void* fn, size_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// The triangular parts hold the supernode's w entries in LDS; few fronts get
-// 1024 threads each for latency, many get 256 for occupancy.
-hipError_t launch_solve_fwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
-                                int nr, const double* L, const double* Z, double* X, const double* V,
-                                hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  const size_t sh = (size_t)maxw * sizeof(double);
-  if ((int64_t)count * B * nr < 256) {
-    hipError_t e = lds_limit((const void*)k_solve_fwd_tri<1024>, sh);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_tri<1024>, dim3(count, B * nr), dim3(1024), sh, st, P, list, nr,
-                       L, Z, X, V);
-  } else {
-    hipError_t e = lds_limit((const void*)k_solve_fwd_tri<256>, sh);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_fwd_tri<256>, dim3(count, B * nr), dim3(256), sh, st, P, list, nr,
-                       L, Z, X, V);
-  }
+// The triangular parts hold the supernode's w entries (m with full) in LDS; few
+// fronts get 1024 threads each for latency, many get 256 for occupancy.
+template <int NT, bool FULL>
+static hipError_t fwd_tri(const DevPlan& P, const int32_t* list, int count, size_t sh, int B, int nr,
+                          const double* L, const double* Z, double* X, double* V, hipStream_t st) {
+  hipError_t e = lds_limit((const void*)k_solve_fwd_tri<NT, FULL>, sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_solve_fwd_tri<NT, FULL>), dim3(count, B * nr), dim3(NT), sh, st, P, list, nr, L,
+                     Z, X, V);
   return hipGetLastError();
+}
+
+hipError_t launch_solve_fwd_tri(const DevPlan& P, const int32_t* list, int count, int maxlen, int B,
+                                int nr, bool full, const double* L, const double* Z, double* X,
+                                double* V, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const size_t sh = (size_t)maxlen * sizeof(double);
+  const bool few = (int64_t)count * B * nr < 256;
+  if (full)
+    return few ? fwd_tri<1024, true>(P, list, count, sh, B, nr, L, Z, X, V, st)
+               : fwd_tri<256, true>(P, list, count, sh, B, nr, L, Z, X, V, st);
+  return few ? fwd_tri<1024, false>(P, list, count, sh, B, nr, L, Z, X, V, st)
+             : fwd_tri<256, false>(P, list, count, sh, B, nr, L, Z, X, V, st);
 }
 
 hipError_t launch_solve_fwd_rows(const DevPlan& P, const int32_t* tasks, int count, int maxw,
@@ -1757,23 +1804,27 @@ hipError_t launch_solve_bwd_rows(const DevPlan& P, const int32_t* tasks, int cou
   return hipGetLastError();
 }
 
-hipError_t launch_solve_bwd_tri(const DevPlan& P, const int32_t* list, int count, int maxw, int B,
-                                int nr, const double* L, const double* Z, double* X, const double* T,
-                                hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  const size_t sh = (size_t)maxw * sizeof(double);
-  if ((int64_t)count * B * nr < 256) {
-    hipError_t e = lds_limit((const void*)k_solve_bwd_tri<1024>, sh);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_tri<1024>, dim3(count, B * nr), dim3(1024), sh, st, P, list, nr,
-                       L, Z, X, T);
-  } else {
-    hipError_t e = lds_limit((const void*)k_solve_bwd_tri<256>, sh);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve_bwd_tri<256>, dim3(count, B * nr), dim3(256), sh, st, P, list, nr,
-                       L, Z, X, T);
-  }
+template <int NT, bool FULL>
+static hipError_t bwd_tri(const DevPlan& P, const int32_t* list, int count, size_t sh, int B, int nr,
+                          const double* L, const double* Z, double* X, const double* T, hipStream_t st) {
+  hipError_t e = lds_limit((const void*)k_solve_bwd_tri<NT, FULL>, sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_solve_bwd_tri<NT, FULL>), dim3(count, B * nr), dim3(NT), sh, st, P, list, nr, L,
+                     Z, X, T);
   return hipGetLastError();
+}
+
+hipError_t launch_solve_bwd_tri(const DevPlan& P, const int32_t* list, int count, int maxlen, int B,
+                                int nr, bool full, const double* L, const double* Z, double* X,
+                                const double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  const size_t sh = (size_t)maxlen * sizeof(double);
+  const bool few = (int64_t)count * B * nr < 256;
+  if (full)
+    return few ? bwd_tri<1024, true>(P, list, count, sh, B, nr, L, Z, X, T, st)
+               : bwd_tri<256, true>(P, list, count, sh, B, nr, L, Z, X, T, st);
+  return few ? bwd_tri<1024, false>(P, list, count, sh, B, nr, L, Z, X, T, st)
+             : bwd_tri<256, false>(P, list, count, sh, B, nr, L, Z, X, T, st);
 }
 
 hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count, int maxm, int B,

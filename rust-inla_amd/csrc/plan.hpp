@@ -49,5 +49,6 @@ constexpr int kSmallFront = 64;  // fronts with m <= 64 run one wavefront each, 
 constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
 constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 constexpr int kSolvePart = 1024;  // rows of R_s per partial sum in the backward solve
+constexpr int kSolveSplitRows = 1024;  // levels whose tallest front has r >= this split the solve
 
 }  // namespace gmrf

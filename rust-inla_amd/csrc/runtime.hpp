@@ -50,14 +50,14 @@ hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, int, 
                             double*, hipStream_t);
 hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, int, const double*, double*,
                             hipStream_t);
-hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, int, const double*,
-                                const double*, double*, const double*, hipStream_t);
+hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, int, bool,
+                                const double*, const double*, double*, double*, hipStream_t);
 hipError_t launch_solve_fwd_rows(const DevPlan&, const int32_t*, int, int, int, int, const double*,
                                  const double*, double*, hipStream_t);
 hipError_t launch_solve_bwd_rows(const DevPlan&, const int32_t*, int, int, int, const double*,
                                  const double*, double*, hipStream_t);
-hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, int, const double*,
-                                const double*, double*, const double*, hipStream_t);
+hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, int, bool,
+                                const double*, const double*, double*, const double*, hipStream_t);
 hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                double*, double*, const double*, hipStream_t);
 hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
@@ -189,6 +189,7 @@ struct LevelInfo {
   };
   Range asm_t, gather;
   Range frow, brow;  // solve: rows below the large supernodes (forward, backward)
+  bool sol_split = false;  // tall fronts: the rows below are separate row-parallel launches
   double sol_tri = 0;  // solve work of the large supernodes' own blocks (per matrix)
   std::vector<Range> fdiag, ftrsm, fupd;   // per factor panel step
   std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
@@ -639,6 +640,13 @@ struct SymData {
           for (int64_t I = J; I < nt; ++I) add(s, I, J);
       }
       L.gather.cnt = (int64_t)tk.size() / 3 - L.gather.off;
+      int64_t rmax = 0;
+      for (int64_t s : fronts) {
+        int64_t w, r, m;
+        mw(s, w, r, m);
+        rmax = std::max(rmax, r);
+      }
+      L.sol_split = rmax >= kSolveSplitRows;
       L.frow.off = (int64_t)tk.size() / 3;
       for (int64_t s : fronts) {
         int64_t w, r, m;
@@ -825,27 +833,33 @@ struct gmrf_batch {
                                   Lf, Xf, Vf, stream);
         });
       if (lv.le > lv.lb) {
-        prof_launch("k_solve_fwd_tri", 0.5 * lv.sol_tri * count * nr, 0.0, stream, [&] {
-          return launch_solve_fwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
-                                      count, nr, Lf, Zf, Xf, Vf, stream);
+        const bool sp = lv.sol_split;
+        prof_launch("k_solve_fwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, 0.0, stream, [&] {
+          return launch_solve_fwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb),
+                                      sp ? lv.maxw_large : lv.maxm_large, count, nr, !sp, Lf, Zf, Xf, Vf,
+                                      stream);
         });
-        prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
-          return launch_solve_fwd_rows(P, tk + 3 * lv.frow.off, (int)lv.frow.cnt, lv.maxw_large, count,
-                                       nr, Lf, Xf, Vf, stream);
-        });
+        if (sp)
+          prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
+            return launch_solve_fwd_rows(P, tk + 3 * lv.frow.off, (int)lv.frow.cnt, lv.maxw_large, count,
+                                         nr, Lf, Xf, Vf, stream);
+          });
       }
     }
     check(launch_diag_scale(P, count, nr, Lf, Xf, stream), "diag_scale");
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
-        prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
-          return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, nr, Lf, Xf, Tf,
-                                       stream);
-        });
-        prof_launch("k_solve_bwd_tri", 0.5 * lv.sol_tri * count * nr, 0.0, stream, [&] {
-          return launch_solve_bwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), lv.maxw_large,
-                                      count, nr, Lf, Zf, Xf, Tf, stream);
+        const bool sp = lv.sol_split;
+        if (sp)
+          prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
+            return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, nr, Lf, Xf, Tf,
+                                         stream);
+          });
+        prof_launch("k_solve_bwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, 0.0, stream, [&] {
+          return launch_solve_bwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb),
+                                      sp ? lv.maxw_large : lv.maxm_large, count, nr, !sp, Lf, Zf, Xf, Tf,
+                                      stream);
         });
       }
       if (lv.se > lv.sb)

@@ -240,3 +240,45 @@ def test_solve_multi_equals_single_solves(case):
         assert np.array_equal(xm[:, j, :], xs)
     ref = orc.Factor(q)
     assert _rel(xm[0, 3], ref.solve(rhs[0, 3].cpu().numpy())) < SOLVE_RTOL
+
+
+def test_tall_fronts_split_solve_and_selinv():
+    """A banded matrix under the natural order: a chain of fronts up to 1417 wide
+    with more than 1024 rows below the supernode, so the solves take the split
+    path (triangle per front, row-parallel GEMV launches for the rows below).
+    Dense numpy references at n = 3000."""
+    import torch
+
+    from inla_gpu import _lib as LL
+    from inla_gpu.solver import Batch, Symbolic
+
+    n, bw = 3000, 1100
+    rng = np.random.default_rng(0)
+    rows, cols = [], []
+    for d in (1, 2, bw):
+        rows += list(range(d, n))
+        cols += list(range(0, n - d))
+    a = sp.csc_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    q = sp.csc_matrix(a + a.T + sp.diags(np.abs(a).sum(0).A1 + np.abs(a).sum(1).A1 + 1.0))
+    q.sort_indices()
+    sym = Symbolic.from_csc(q, order=LL.ORDER_NATURAL)
+    assert sym.stats()["max_front"] > 1024 + 64
+    bt = Batch(sym, 2)
+    bt.factorize(torch.from_numpy(np.stack([q.data, 3.0 * q.data])).cuda())
+    ld, st = bt.logdet()
+    assert (st == 0).all()
+    qd = q.toarray()
+    ref_ld = np.linalg.slogdet(qd)[1]
+    assert abs(ld[0] - ref_ld) < LOGDET_RTOL * abs(ref_ld)
+    assert abs(ld[1] - (ref_ld + n * np.log(3.0))) < LOGDET_RTOL * abs(ref_ld)
+    rhs = rng.standard_normal((2, 3, n))
+    xm = bt.solve_multi(torch.from_numpy(rhs).cuda()).cpu().numpy()
+    for j in range(3):
+        xs = np.linalg.solve(qd, rhs[0, j])
+        assert _rel(xm[0, j], xs) < SOLVE_RTOL
+        assert _rel(xm[1, j], np.linalg.solve(qd, rhs[1, j]) / 3.0) < SOLVE_RTOL
+    bt.selinv()
+    dg = bt.selinv_diag().cpu().numpy()
+    di = np.diag(np.linalg.inv(qd))
+    assert _rel(dg[0], di) < SELINV_RTOL
+    assert _rel(dg[1], di / 3.0) < SELINV_RTOL
