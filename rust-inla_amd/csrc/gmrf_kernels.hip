@@ -248,10 +248,135 @@ __device__ __forceinline__ void ldlt_base16(double* F, int k0, int kn, double* Z
   }
 }
 
+// The same in 4-column steps: per step, the 4 x 4 pivot block, row i's and the
+// lane's columns' entries in the 4 pivot columns and the 4 Z rows travel in one
+// round of shuffles, every lane factors the pivot block redundantly, and the
+// rank-4 update follows. A quarter of the dependent shuffle rounds of the
+// column-by-column form, at about twice its registers (used where the front
+// kernel runs few workgroups: the 64-column diagonal blocks of large fronts).
+template <int LD>
+__device__ __forceinline__ void ldlt_base16_b4(double* F, int k0, int kn, double* Zk) {
+  const int lane = threadIdx.x & 63, i = lane & 15, c = lane >> 4;
+  double d[4], z[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = c + 4 * t;
+    const double fv = F[(k0 + min(i, kn - 1)) * LD + k0 + min(j, kn - 1)];
+    d[t] = (i < kn && j < kn) ? (j <= i ? fv : 0.0) : (i == j ? 1.0 : 0.0);
+    z[t] = (i == j) ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    const int q0 = 4 * kb;
+    if (q0 < kn) {
+      const double v = d[kb];
+      double ci[4], pv[4][4], cj[4][4], zk[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ci[q] = __shfl(v, 16 * q + i, 64);  // D[i][q0 + q]
+#pragma unroll
+        for (int p = 0; p < 4; ++p) pv[p][q] = (q <= p) ? __shfl(v, 16 * q + q0 + p, 64) : 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          cj[t][q] = __shfl(v, 16 * q + c + 4 * t, 64);  // D[c + 4t][q0 + q]
+          zk[t][q] = __shfl(z[t], 16 * c + q0 + q, 64);  // Z[q0 + q][c + 4t]
+        }
+      }
+      // pivot block LDL^T (identity beyond kn by construction)
+      double dp[4], rp[4], lp[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double a = pv[q][q];
+#pragma unroll
+        for (int u = 0; u < q; ++u) a -= lp[q][u] * lp[q][u] * dp[u];
+        dp[q] = a;
+        rp[q] = recip_nr(a);
+#pragma unroll
+        for (int p = q + 1; p < 4; ++p) {
+          double b = pv[p][q];
+#pragma unroll
+          for (int u = 0; u < q; ++u) b -= lp[p][u] * lp[q][u] * dp[u];
+          lp[p][q] = b * rp[q];
+        }
+      }
+      // L entries of row i in the pivot columns
+      double li[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double a = ci[q];
+#pragma unroll
+        for (int u = 0; u < q; ++u) a -= li[u] * dp[u] * lp[q][u];
+        li[q] = a * rp[q];
+      }
+      const bool below = i >= q0 + 4 && i < kn;
+      const bool inblk = i >= q0 && i < q0 + 4;
+      const int pi = i - q0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = c + 4 * t;
+        // L entries of column j and the final Z rows of the pivot block (column j)
+        double lj[4], zb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double a = cj[t][q], b = zk[t][q];
+#pragma unroll
+          for (int u = 0; u < q; ++u) {
+            a -= lj[u] * dp[u] * lp[q][u];
+            b -= lp[q][u] * zb[u];
+          }
+          lj[q] = a * rp[q];
+          zb[q] = b;
+        }
+        double upd = 0.0, zup = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          upd += li[q] * dp[q] * lj[q];
+          zup += li[q] * zb[q];
+        }
+        // j - q0 and pi are lane-dependent: select instead of indexing registers
+        const int qj = j - q0;
+        double lij = li[0], lpp = lp[1][0] , dpi = dp[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+          lij = (qj == q) ? li[q] : lij;
+          dpi = (pi == q) ? dp[q] : dpi;
+        }
+        // lp[pi][qj] for qj < pi
+        lpp = (pi == 2 && qj == 0) ? lp[2][0] : lpp;
+        lpp = (pi == 2 && qj == 1) ? lp[2][1] : lpp;
+        lpp = (pi == 3 && qj == 0) ? lp[3][0] : lpp;
+        lpp = (pi == 3 && qj == 1) ? lp[3][1] : lpp;
+        lpp = (pi == 3 && qj == 2) ? lp[3][2] : lpp;
+        double nd = d[t];
+        if (below && j >= q0 + 4 && j <= i) nd = d[t] - upd;
+        if (below && qj >= 0 && qj < 4) nd = lij;
+        if (inblk && qj >= 0 && j < i) nd = lpp;
+        if (inblk && j == i) nd = dpi;
+        d[t] = nd;
+        double nz = z[t];
+        if (below) nz = z[t] - zup;
+        if (inblk) {
+          double sel = zb[0];
+#pragma unroll
+          for (int q = 1; q < 4; ++q) sel = (pi == q) ? zb[q] : sel;
+          nz = sel;
+        }
+        z[t] = nz;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = c + 4 * t;
+    if (i < kn && j <= i) F[(k0 + i) * LD + k0 + j] = d[t];
+    Zk[i * 17 + j] = z[t];
+  }
+}
+
 // Blocked LDL^T of the first ncols columns of an m x m front in F (m <= MB;
 // the calling workgroup has MB / 16 waves). With WITHZ (ncols == m), Zs (same
 // layout as F) receives Z = L^-1. Zk (16 x 17) and Wb (MB x 17) are scratch.
-template <int MB, bool WITHZ>
+template <int MB, bool WITHZ, bool B4 = false>
 __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zs, double* Zk,
                                            double* Wb) {
   constexpr int LD = MB + 1, NW = MB / 16;
@@ -260,7 +385,12 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
   if (WITHZ) {
     for (int idx = threadIdx.x; idx < MB * LD; idx += 64 * NW) Zs[idx] = 0.0;
   }
-  if (wv == 0 && ncols > 0) ldlt_base16<LD>(F, 0, min(16, ncols), Zk);
+  if (wv == 0 && ncols > 0) {
+    if (B4)
+      ldlt_base16_b4<LD>(F, 0, min(16, ncols), Zk);
+    else
+      ldlt_base16<LD>(F, 0, min(16, ncols), Zk);
+  }
   __syncthreads();
   for (int k0 = 0; k0 < ncols; k0 += 16) {
     const int kn = min(16, ncols - k0), t0 = k0 + kn, kb = k0 >> 4;
@@ -370,7 +500,12 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         }
       }
     }
-    if (wv == 0 && k1 < ncols) ldlt_base16<LD>(F, k1, min(16, ncols - k1), Zk);
+    if (wv == 0 && k1 < ncols) {
+      if (B4)
+        ldlt_base16_b4<LD>(F, k1, min(16, ncols - k1), Zk);
+      else
+        ldlt_base16<LD>(F, k1, min(16, ncols - k1), Zk);
+    }
     __syncthreads();
   }
 }
@@ -574,7 +709,7 @@ __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __res
     }
   }
   __syncthreads();
-  front_ldlt<kTile, true>(F, nb, nb, Zs, Zk, Wb);
+  front_ldlt<kTile, true, true>(F, nb, nb, Zs, Zk, Wb);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int j = c + 4 * t;
