@@ -56,24 +56,38 @@ struct TileAcc {
   d4 c[2][2];  // [bj][bi]
 };
 
+__device__ __forceinline__ void tile_zero(TileAcc& acc) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
 // AK / BK: the operand is contiguous along k in memory, so consecutive lanes
 // load consecutive k (otherwise consecutive rows). The accessors are called with
 // k < K only and must load unconditionally from a valid (clamped) address and
 // select zero afterwards: a branch around a load serialises the memory latency.
 // The next K chunk is fetched into registers while the current one is on MFMA.
-template <bool AK = false, bool BK = false, typename FA, typename FB>
+//
+// NT = 512: eight waves; waves w and w + 4 own the same quadrant and take
+// alternate k-steps of each chunk, so every SIMD interleaves two independent MFMA
+// streams (one wave alone leaves the FP64 MFMA pipe about half idle); the two
+// partial accumulators are summed through LDS at the end and the result is held
+// by waves 0-3 (threads < 256) only.
+template <bool AK = false, bool BK = false, int NT = 256, typename FA, typename FB>
 __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPanel + 1],
                                          double (*Bs)[kPanel + 1], TileAcc& acc) {
-  constexpr int NL = 64 * kPanel / 256;
+  static_assert(NT == 256 || NT == 512, "tile engine runs 4 or 8 waves");
+  constexpr int NL = 64 * kPanel / NT, NH = NT / 256;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  const int qi = wv & 1, qj = wv >> 1;
+  const int half = NH == 2 ? (wv >> 2) : 0, qi = wv & 1, qj = (wv >> 1) & 1;
   const int lr = lane & 15, lk = lane >> 4;
   double ra[NL], rb[NL];
   auto fetch = [&](int k0) {
     const int kc = min(kPanel, K - k0);
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
-      const int idx = tid + 256 * it;
+      const int idx = tid + NT * it;
       {
         const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
         const double v = fa(ii, k0 + min(kk, kc - 1));
@@ -91,7 +105,7 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
   for (int k0 = 0; k0 < K; k0 += kPanel) {
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
-      const int idx = tid + 256 * it;
+      const int idx = tid + NT * it;
       {
         const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
         As[ii][kk] = ra[it];
@@ -105,8 +119,9 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
     const int kc = min(kPanel, K - k0);
     if (k0 + kPanel < K) fetch(k0 + kPanel);
 #pragma unroll
-    for (int kk = 0; kk < kPanel; kk += 4) {
-      if (kk >= kc) break;  // uniform: short last chunk
+    for (int st = 0; st < kPanel / (4 * NH); ++st) {
+      const int kk = 4 * (st * NH + half);
+      if (4 * st * NH >= kc) break;  // uniform: short last chunk (zero padded)
       const double a0 = Bs[qj * 32 + lr][kk + lk];
       const double a1 = Bs[qj * 32 + 16 + lr][kk + lk];
       const double b0 = As[qi * 32 + lr][kk + lk];
@@ -115,6 +130,23 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
       acc.c[0][1] = mfma16(a0, b1, acc.c[0][1]);
       acc.c[1][0] = mfma16(a1, b0, acc.c[1][0]);
       acc.c[1][1] = mfma16(a1, b1, acc.c[1][1]);
+    }
+    __syncthreads();
+  }
+  if (NH == 2) {
+    // quadrant q's partials: waves 4, 5 through the A operand area, 6, 7 through B's
+    static_assert(2 * 16 * 64 <= 64 * (kPanel + 1), "reduction exceeds the operand LDS");
+    const int q = wv & 3;
+    double* red = (q < 2 ? &As[0][0] : &Bs[0][0]) + (q & 1) * 16 * 64;
+    if (half == 1) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) red[e * 64 + lane] = acc.c[e >> 3][(e >> 2) & 1][e & 3];
+      tile_zero(acc);  // so that a following product on the same accumulator adds once
+    }
+    __syncthreads();
+    if (half == 0) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc.c[e >> 3][(e >> 2) & 1][e & 3] += red[e * 64 + lane];
     }
     __syncthreads();
   }
@@ -135,13 +167,6 @@ __device__ __forceinline__ void tile_foreach(const TileAcc& acc, F f) {
         const int j = qj * 32 + bj * 16 + 4 * v + (lane >> 4);
         f(i, j, acc.c[bj][bi][v]);
       }
-}
-
-__device__ __forceinline__ void tile_zero(TileAcc& acc) {
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
 // ---------------------------------------------------------------------------
@@ -1641,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
 // -sum_{k in B} W[i][k] Lh[k][p] (K = m - c1) and stores it into S (lower) and
 // both triangles of W (the transposed copy staged through LDS so that both
 // stores are coalesced).
-__global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, double* __restrict__ S,
                                                  double* __restrict__ W,
                                                  const double* __restrict__ T) {
@@ -1661,7 +1686,7 @@ __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __res
   const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   TileAcc acc;
   tile_zero(acc);
-  mma_tile<false, true>(
+  mma_tile<false, true, 512>(
       [&](int ii, int kk) {
         const int i = R0 + ii;
         const double v = Ws[(int64_t)(c1 + kk) * m + min(i, m - 1)];
@@ -1672,19 +1697,21 @@ __global__ __launch_bounds__(256) void k_sl_gemm(DevPlan P, const int32_t* __res
         return jj < nb ? v : 0.0;
       },
       m - c1, As, Bs, acc);
-  // the tile engine ends with a barrier, so its LDS can hold the tile now
+  // the tile engine ends with a barrier, so its LDS can hold the tile now; waves
+  // 0-3 hold the result
   double (*Ct)[kTile + 1] = (double (*)[kTile + 1])sh;
-  tile_foreach(acc, [&](int ii, int jj, double v) {
-    const int i = R0 + ii;
-    Ct[jj][ii] = -v;
-    if (i < m && jj < nb) {
-      Sb[(int64_t)(c0 + jj) * m + i] = -v;
-      Ws[(int64_t)(c0 + jj) * m + i] = -v;
-    }
-  });
+  if (threadIdx.x < 256)
+    tile_foreach(acc, [&](int ii, int jj, double v) {
+      const int i = R0 + ii;
+      Ct[jj][ii] = -v;
+      if (i < m && jj < nb) {
+        Sb[(int64_t)(c0 + jj) * m + i] = -v;
+        Ws[(int64_t)(c0 + jj) * m + i] = -v;
+      }
+    });
   __syncthreads();
   // W[c0 + jj][i]: consecutive lanes take consecutive jj
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 512) {
     const int jj = idx & 63, ii = idx >> 6;
     const int i = R0 + ii;
     if (i < m && jj < nb) Ws[(int64_t)i * m + c0 + jj] = Ct[jj][ii];
@@ -1994,7 +2021,7 @@ hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int
 hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
                           double* S, double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
+  hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(512), 0, st, P, tasks, step, S, W, T);
   return hipGetLastError();
 }
 
