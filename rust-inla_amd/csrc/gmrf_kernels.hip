@@ -1782,6 +1782,95 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   });
 }
 
+// Sigma[P,P] of tall fronts in two launches: task (s, part) forms the partial
+// product over rows c1 + 2048 part .. of B (part 0 also adds Z^T D^-1 Z) into the
+// front's staging area after L_hat (parts x 64 x 64); k_sl_pp_sum adds the parts in
+// order and stores as k_sl_pp does. A single workgroup over K = m - c1 would walk
+// thousands of rows serially.
+__global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __restrict__ tasks,
+                                                    int step, const double* __restrict__ L,
+                                                    const double* __restrict__ Z,
+                                                    const double* __restrict__ W,
+                                                    double* __restrict__ T) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double dinv[kTile];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], part = tk[1];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int ks = part * kSelSplitK, ke = min(m - c1, ks + kSelSplitK);
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
+  const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  TileAcc acc;
+  tile_zero(acc);
+  if (part == 0) {
+    if (tid < kTile) {
+      const int q = min(tid, nb - 1);
+      dinv[tid] = 1.0 / Lb[(int64_t)(c0 + q) * m + c0 + q];
+    }
+    __syncthreads();
+    mma_tile<true, true>(
+        [&](int pp, int k) {
+          const double x = Zb[pp * kTile + k] * dinv[k];
+          return pp < nb ? x : 0.0;
+        },
+        [&](int qq, int k) {
+          const double x = Zb[qq * kTile + k];
+          return qq < nb ? x : 0.0;
+        },
+        nb, As, Bs, acc);
+  }
+  if (ke > ks)
+    mma_tile<true, true>(
+        [&](int ii, int kk) {
+          const double v = Tb[(int64_t)min(ii, nb - 1) * m + c1 + ks + kk];
+          return ii < nb ? -v : 0.0;
+        },
+        [&](int jj, int kk) {
+          const double v = Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + ks + kk];
+          return jj < nb ? v : 0.0;
+        },
+        ke - ks, As, Bs, acc);
+  double* Pp = Tb + (int64_t)m * kTile + (int64_t)part * kTile * kTile;
+  tile_foreach(acc, [&](int ii, int jj, double v) { Pp[jj * kTile + ii] = v; });
+}
+
+__global__ __launch_bounds__(256) void k_sl_pp_sum(DevPlan P, const int32_t* __restrict__ tasks,
+                                                   int step, double* __restrict__ S,
+                                                   double* __restrict__ W,
+                                                   const double* __restrict__ T) {
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int parts = max(1, (m - c1 + kSelSplitK - 1) / kSelSplitK);
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Pp = T + (int64_t)b * P.t_size + P.t_off[s] + (int64_t)m * kTile;
+  const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int u = 0; u < 16; ++u) {
+    const int jj = jb + 4 * u;
+    double v = 0.0;
+    for (int q = 0; q < parts; ++q) v += Pp[(int64_t)q * kTile * kTile + jj * kTile + ii];
+    if (ii < nb && jj < nb && ii >= jj) {
+      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Extraction
 // ---------------------------------------------------------------------------
@@ -2030,6 +2119,21 @@ hipError_t launch_sl_pp(const DevPlan& P, const int32_t* tasks, int count, int s
                         hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_pp, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, S, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_pp_part(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                             const double* L, const double* Z, const double* W, double* T,
+                             hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_pp_part, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_pp_sum(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                            double* S, double* W, const double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_pp_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
   return hipGetLastError();
 }
 

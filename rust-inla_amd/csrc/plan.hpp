@@ -50,5 +50,6 @@ constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
 constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 constexpr int kSolvePart = 1024;  // rows of R_s per partial sum in the backward solve
 constexpr int kSolveSplitRows = 1024;  // levels whose tallest front has r >= this split the solve
+constexpr int kSelSplitK = 2048;  // rows below a panel per partial product of Sigma[P,P] (tall fronts)
 
 }  // namespace gmrf

@@ -38,6 +38,10 @@ hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*
                           const double*, hipStream_t);
 hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const double*,
                         const double*, double*, double*, const double*, hipStream_t);
+hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
+                             const double*, const double*, double*, hipStream_t);
+hipError_t launch_sl_pp_sum(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+                            const double*, hipStream_t);
 hipError_t launch_logdet(const DevPlan&, int, const double*, double*, uint32_t*, double*, int32_t*,
                          hipStream_t);
 hipError_t launch_logdet_seg(const DevPlan&, int, const double*, const int64_t*, const int32_t*, int,
@@ -193,6 +197,7 @@ struct LevelInfo {
   double sol_tri = 0;  // solve work of the large supernodes' own blocks (per matrix)
   std::vector<Range> fdiag, ftrsm, fupd;   // per factor panel step
   std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
+  std::vector<Range> sppart;  // tall fronts: Sigma[P,P] partial products (empty: single launch)
 };
 
 // First-fit allocator over one workspace, in doubles; freed ranges coalesce.
@@ -300,7 +305,7 @@ inline Pools plan_pools(const Symbolic& S) {
       if (large(s)) {
         P.w_off[s] = wf.alloc(m * m);
         P.t_off[s] = tl;
-        tl += m * kTile;
+        tl += m * kTile + ((m + kSelSplitK - 1) / kSelSplitK) * kTile * kTile;
       } else if (r > 0) {
         P.u_sel[s] = us.alloc(r * r);
       }
@@ -554,6 +559,7 @@ struct SymData {
       L.lhat.resize(maxsteps);
       L.sgemm.resize(maxsteps);
       L.spp.resize(maxsteps);
+      L.sppart.resize(maxsteps);
       // dense work shares of each front per launch kind and step
       for (int64_t s : fronts) {
         int64_t w, r, m;
@@ -689,12 +695,23 @@ struct SymData {
         }
         L.sgemm[k].cnt = (int64_t)tk.size() / 3 - L.sgemm[k].off;
         L.spp[k].off = (int64_t)tk.size() / 3;
+        bool tall = false;
         for (int64_t s : fronts) {
           int64_t nB;
           if (!rows_of(s, nB)) continue;
           add(s, 0, 0);
+          tall |= nB > kSelSplitK;
         }
         L.spp[k].cnt = (int64_t)tk.size() / 3 - L.spp[k].off;
+        L.sppart[k].off = (int64_t)tk.size() / 3;
+        if (tall)
+          for (int64_t s : fronts) {
+            int64_t nB;
+            if (!rows_of(s, nB)) continue;
+            for (int64_t q = 0; q < std::max<int64_t>(1, (nB + kSelSplitK - 1) / kSelSplitK); ++q) add(s, q, 0);
+          }
+        L.sppart[k].cnt = (int64_t)tk.size() / 3 - L.sppart[k].off;
+        L.sppart[k].flops = L.spp[k].flops;
       }
     }
     d_tasks = arena.upload(tk.empty() ? std::vector<int32_t>(3, 0) : tk);
@@ -896,10 +913,21 @@ struct gmrf_batch {
             return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
                                   Sf, Wf, Tf, stream);
           });
-          prof_launch("k_sl_pp", lv.spp[k].flops * count, 0.0, stream, [&] {
-            return launch_sl_pp(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Lf, Zf,
-                                Sf, Wf, Tf, stream);
-          });
+          if (lv.sppart[k].cnt > 0) {
+            prof_launch("k_sl_pp_part", lv.sppart[k].flops * count, 0.0, stream, [&] {
+              return launch_sl_pp_part(P, tk + 3 * lv.sppart[k].off, (int)lv.sppart[k].cnt, (int)k, count,
+                                       Lf, Zf, Wf, Tf, stream);
+            });
+            prof_launch("k_sl_pp_sum", 0.0, 0.0, stream, [&] {
+              return launch_sl_pp_sum(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Sf, Wf,
+                                      Tf, stream);
+            });
+          } else {
+            prof_launch("k_sl_pp", lv.spp[k].flops * count, 0.0, stream, [&] {
+              return launch_sl_pp(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Lf, Zf,
+                                  Sf, Wf, Tf, stream);
+            });
+          }
         }
       }
       for (int c = 0; c < 3; ++c)

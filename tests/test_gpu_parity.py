@@ -242,17 +242,18 @@ def test_solve_multi_equals_single_solves(case):
     assert _rel(xm[0, 3], ref.solve(rhs[0, 3].cpu().numpy())) < SOLVE_RTOL
 
 
-def test_tall_fronts_split_solve_and_selinv():
-    """A banded matrix under the natural order: a chain of fronts up to 1417 wide
-    with more than 1024 rows below the supernode, so the solves take the split
-    path (triangle per front, row-parallel GEMV launches for the rows below).
-    Dense numpy references at n = 3000."""
+@pytest.mark.parametrize("n,bw", [(3000, 1100), (4500, 2200)])
+def test_tall_fronts_split_solve_and_selinv(n, bw):
+    """A banded matrix under the natural order: a chain of fronts with more than
+    1024 rows below the supernode, so the solves take the split path (triangle per
+    front, row-parallel GEMV launches for the rows below); at bandwidth 2200 the
+    rows below a panel exceed 2048 and Sigma[P,P] is formed from partial products.
+    Dense numpy references."""
     import torch
 
     from inla_gpu import _lib as LL
     from inla_gpu.solver import Batch, Symbolic
 
-    n, bw = 3000, 1100
     rng = np.random.default_rng(0)
     rows, cols = [], []
     for d in (1, 2, bw):
@@ -262,7 +263,7 @@ def test_tall_fronts_split_solve_and_selinv():
     q = sp.csc_matrix(a + a.T + sp.diags(np.abs(a).sum(0).A1 + np.abs(a).sum(1).A1 + 1.0))
     q.sort_indices()
     sym = Symbolic.from_csc(q, order=LL.ORDER_NATURAL)
-    assert sym.stats()["max_front"] > 1024 + 64
+    assert sym.stats()["max_front"] > bw
     bt = Batch(sym, 2)
     bt.factorize(torch.from_numpy(np.stack([q.data, 3.0 * q.data])).cuda())
     ld, st = bt.logdet()
