@@ -1782,11 +1782,89 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   });
 }
 
+// Sigma[B,P] split over K for launches with few row blocks (tall fronts of a small
+// batch): task (s, I, part) forms the partial product of rows c1 + 64 I .. over
+// the rows c1 + kGemmSplitK part .. of B into the front's staging area;
+// k_sl_gemm_sum adds the parts in order and stores as k_sl_gemm does.
+__global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* __restrict__ tasks,
+                                                      int step, const double* __restrict__ W,
+                                                      double* __restrict__ T) {
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
+  double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1], part = tk[2];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0, nB = m - c1;
+  const int parts = max(1, (nB + kGemmSplitK - 1) / kGemmSplitK);
+  const int ks = part * kGemmSplitK, ke = min(nB, ks + kGemmSplitK);
+  const int R0 = c1 + I * kTile;
+  const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  TileAcc acc;
+  tile_zero(acc);
+  mma_tile<false, true, 512>(
+      [&](int ii, int kk) {
+        const int i = R0 + ii;
+        const double v = Ws[(int64_t)(c1 + ks + kk) * m + min(i, m - 1)];
+        return i < m ? v : 0.0;
+      },
+      [&](int jj, int kk) {
+        const double v = Tb[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk];
+        return jj < nb ? v : 0.0;
+      },
+      ke - ks, As, Bs, acc);
+  double* Pg = Tb + stage_gemm_off(m) + ((int64_t)I * parts + part) * kTile * kTile;
+  if (threadIdx.x < 256) tile_foreach(acc, [&](int ii, int jj, double v) { Pg[jj * kTile + ii] = v; });
+}
+
+__global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* __restrict__ tasks,
+                                                     int step, double* __restrict__ S,
+                                                     double* __restrict__ W,
+                                                     const double* __restrict__ T) {
+  __shared__ double Ct[kTile][kTile + 1];
+  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int s = tk[0], I = tk[1];
+  const int b = blockIdx.y;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0, nB = m - c1;
+  const int parts = max(1, (nB + kGemmSplitK - 1) / kGemmSplitK);
+  const int R0 = c1 + I * kTile;
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Pg = T + (int64_t)b * P.t_size + P.t_off[s] + stage_gemm_off(m) + (int64_t)I * parts * kTile * kTile;
+  const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
+  const int i = R0 + ii;
+#pragma unroll 4
+  for (int u = 0; u < 16; ++u) {
+    const int jj = jb + 4 * u;
+    double v = 0.0;
+    for (int q = 0; q < parts; ++q) v += Pg[(int64_t)q * kTile * kTile + jj * kTile + ii];
+    Ct[jj][ii] = -v;
+    if (i < m && jj < nb) {
+      Sb[(int64_t)(c0 + jj) * m + i] = -v;
+      Ws[(int64_t)(c0 + jj) * m + i] = -v;
+    }
+  }
+  __syncthreads();
+  // W[c0 + jj][i]: consecutive lanes take consecutive jj
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
+    const int jj = idx & 63, i2 = idx >> 6;
+    const int row = R0 + i2;
+    if (row < m && jj < nb) Ws[(int64_t)row * m + c0 + jj] = Ct[jj][i2];
+  }
+}
+
 // Sigma[P,P] of tall fronts in two launches: task (s, part) forms the partial
-// product over rows c1 + 2048 part .. of B (part 0 also adds Z^T D^-1 Z) into the
-// front's staging area after L_hat (parts x 64 x 64); k_sl_pp_sum adds the parts in
-// order and stores as k_sl_pp does. A single workgroup over K = m - c1 would walk
-// thousands of rows serially.
+// product over rows c1 + kSelSplitK part .. of B (part 0 also adds Z^T D^-1 Z) into
+// the front's staging area after L_hat (parts x 64 x 64); k_sl_pp_sum adds the
+// parts in order and stores as k_sl_pp does. A single workgroup over K = m - c1
+// would walk thousands of rows serially.
 __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __restrict__ tasks,
                                                     int step, const double* __restrict__ L,
                                                     const double* __restrict__ Z,
@@ -1838,7 +1916,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
           return jj < nb ? v : 0.0;
         },
         ke - ks, As, Bs, acc);
-  double* Pp = Tb + (int64_t)m * kTile + (int64_t)part * kTile * kTile;
+  double* Pp = Tb + stage_pp_off(m) + (int64_t)part * kTile * kTile;
   tile_foreach(acc, [&](int ii, int jj, double v) { Pp[jj * kTile + ii] = v; });
 }
 
@@ -1856,7 +1934,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_sum(DevPlan P, const int32_t* __r
   const int parts = max(1, (m - c1 + kSelSplitK - 1) / kSelSplitK);
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
-  const double* Pp = T + (int64_t)b * P.t_size + P.t_off[s] + (int64_t)m * kTile;
+  const double* Pp = T + (int64_t)b * P.t_size + P.t_off[s] + stage_pp_off(m);
   const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
 #pragma unroll 4
   for (int u = 0; u < 16; ++u) {
@@ -2119,6 +2197,20 @@ hipError_t launch_sl_pp(const DevPlan& P, const int32_t* tasks, int count, int s
                         hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_pp, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, S, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_gemm_part(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                               const double* W, double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_gemm_part, dim3(count, B), dim3(512), 0, st, P, tasks, step, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_gemm_sum(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+                              double* S, double* W, const double* T, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_gemm_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
   return hipGetLastError();
 }
 

@@ -50,6 +50,18 @@ constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
 constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 constexpr int kSolvePart = 1024;  // rows of R_s per partial sum in the backward solve
 constexpr int kSolveSplitRows = 1024;  // levels whose tallest front has r >= this split the solve
-constexpr int kSelSplitK = 2048;  // rows below a panel per partial product of Sigma[P,P] (tall fronts)
+constexpr int kSelSplitK = 512;  // rows below a panel per partial product of Sigma[P,P] (tall fronts)
+constexpr int kGemmSplitK = 512;  // K per partial product of Sigma[B,P] when a level has few row blocks
+constexpr int kGemmSplitMaxTasks = 96;  // split Sigma[B,P] when a launch has at most this many row blocks
+
+// Staging area of a large front (T, per level): L_hat of the current panel (m x 64),
+// the Sigma[P,P] partial products, then the Sigma[B,P] partial tiles.
+constexpr int64_t stage_pp_off(int64_t m) { return m * 64; }
+constexpr int64_t stage_gemm_off(int64_t m) {
+  return m * 64 + ((m + kSelSplitK - 1) / kSelSplitK) * 64 * 64;
+}
+constexpr int64_t stage_size(int64_t m) {
+  return stage_gemm_off(m) + ((m + 63) / 64) * ((m + kGemmSplitK - 1) / kGemmSplitK) * 64 * 64;
+}
 
 }  // namespace gmrf

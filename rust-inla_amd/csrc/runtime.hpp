@@ -40,6 +40,10 @@ hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const dou
                         const double*, double*, double*, const double*, hipStream_t);
 hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
                              const double*, const double*, double*, hipStream_t);
+hipError_t launch_sl_gemm_part(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
+                               hipStream_t);
+hipError_t launch_sl_gemm_sum(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+                              const double*, hipStream_t);
 hipError_t launch_sl_pp_sum(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                             const double*, hipStream_t);
 hipError_t launch_logdet(const DevPlan&, int, const double*, double*, uint32_t*, double*, int32_t*,
@@ -198,6 +202,7 @@ struct LevelInfo {
   std::vector<Range> fdiag, ftrsm, fupd;   // per factor panel step
   std::vector<Range> lhat, sgemm, spp;     // per selinv panel step (from the right)
   std::vector<Range> sppart;  // tall fronts: Sigma[P,P] partial products (empty: single launch)
+  std::vector<Range> sgpart;  // Sigma[B,P] split over K (used when a launch has few row blocks)
 };
 
 // First-fit allocator over one workspace, in doubles; freed ranges coalesce.
@@ -305,7 +310,7 @@ inline Pools plan_pools(const Symbolic& S) {
       if (large(s)) {
         P.w_off[s] = wf.alloc(m * m);
         P.t_off[s] = tl;
-        tl += m * kTile + ((m + kSelSplitK - 1) / kSelSplitK) * kTile * kTile;
+        tl += stage_size(m);
       } else if (r > 0) {
         P.u_sel[s] = us.alloc(r * r);
       }
@@ -560,6 +565,7 @@ struct SymData {
       L.sgemm.resize(maxsteps);
       L.spp.resize(maxsteps);
       L.sppart.resize(maxsteps);
+      L.sgpart.resize(maxsteps);
       // dense work shares of each front per launch kind and step
       for (int64_t s : fronts) {
         int64_t w, r, m;
@@ -694,6 +700,22 @@ struct SymData {
           for (int64_t rb = 0; rb < ceil64(nB); ++rb) add(s, rb, 0);
         }
         L.sgemm[k].cnt = (int64_t)tk.size() / 3 - L.sgemm[k].off;
+        L.sgpart[k].off = (int64_t)tk.size() / 3;
+        bool longk = false;
+        for (int64_t s : fronts) {
+          int64_t nB;
+          if (rows_of(s, nB)) longk |= nB > kGemmSplitK;
+        }
+        if (longk)
+          for (int64_t s : fronts) {
+            int64_t nB;
+            if (!rows_of(s, nB)) continue;
+            const int64_t parts = std::max<int64_t>(1, (nB + kGemmSplitK - 1) / kGemmSplitK);
+            for (int64_t rb = 0; rb < ceil64(nB); ++rb)
+              for (int64_t q = 0; q < parts; ++q) add(s, rb, q);
+          }
+        L.sgpart[k].cnt = (int64_t)tk.size() / 3 - L.sgpart[k].off;
+        L.sgpart[k].flops = L.sgemm[k].flops;
         L.spp[k].off = (int64_t)tk.size() / 3;
         bool tall = false;
         for (int64_t s : fronts) {
@@ -909,10 +931,23 @@ struct gmrf_batch {
             return launch_sl_lhat(P, tk + 3 * lv.lhat[k].off, (int)lv.lhat[k].cnt, (int)k, count, Lf,
                                   Zf, Tf, stream);
           });
-          prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, 0.0, stream, [&] {
-            return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
-                                  Sf, Wf, Tf, stream);
-          });
+          // few row blocks with a long K (the matrices of tall fronts, a small batch):
+          // split K so that the product spreads over the device
+          if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= kGemmSplitMaxTasks) {
+            prof_launch("k_sl_gemm_part", lv.sgpart[k].flops * count, 0.0, stream, [&] {
+              return launch_sl_gemm_part(P, tk + 3 * lv.sgpart[k].off, (int)lv.sgpart[k].cnt, (int)k, count,
+                                         Wf, Tf, stream);
+            });
+            prof_launch("k_sl_gemm_sum", 0.0, 0.0, stream, [&] {
+              return launch_sl_gemm_sum(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
+                                        Sf, Wf, Tf, stream);
+            });
+          } else {
+            prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, 0.0, stream, [&] {
+              return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
+                                    Sf, Wf, Tf, stream);
+            });
+          }
           if (lv.sppart[k].cnt > 0) {
             prof_launch("k_sl_pp_part", lv.sppart[k].flops * count, 0.0, stream, [&] {
               return launch_sl_pp_part(P, tk + 3 * lv.sppart[k].off, (int)lv.sppart[k].cnt, (int)k, count,
