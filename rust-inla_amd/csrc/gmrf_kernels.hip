@@ -1360,10 +1360,11 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
                                                          const double* __restrict__ L,
                                                          double* __restrict__ S,
                                                          double* __restrict__ U,
-                                                         const double* __restrict__ W) {
+                                                         const double* __restrict__ W,
+                                                         int ldl) {
   constexpr int NT = 4 * MB, NJ = MB / 4, LD = MB + 1, NW = MB / 16;
   __shared__ double Sg[MB * LD];
-  __shared__ double Ls[MB * LD];
+  extern __shared__ double Ls[];  // MB x ldl: the L block, ldl > widest supernode of the launch
   __shared__ double Zk[16 * 17];
   __shared__ double Lh[MB * 17];
   __shared__ double dinv[16];
@@ -1387,7 +1388,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
     for (int u = 0; u < 8; ++u) {
       const int e = e0 + tid + NT * u;
       const int k = e / m, i = e - k * m;
-      if (e < nl) Ls[i * LD + k] = v[u];
+      if (e < nl) Ls[i * ldl + k] = v[u];
     }
   }
   // Sigma_{R R} from the parent front (its S block and Sigma_RR workspace, or
@@ -1444,7 +1445,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
 #pragma unroll
       for (int t = 0; t < 4; ++t) z[t] = (i == c + 4 * t) ? 1.0 : 0.0;
       for (int k = 0; k < kn; ++k) {
-        const double lik = Ls[(c0 + min(i, kn - 1)) * LD + c0 + k];
+        const double lik = Ls[(c0 + min(i, kn - 1)) * ldl + c0 + k];
         double zk[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) zk[t] = __shfl(z[t], 16 * c + k, 64);
@@ -1459,7 +1460,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       for (int t = 0; t < 4; ++t) Zk[i * 17 + c + 4 * t] = z[t];
       if (lane < 16) {
         const int q = c0 + min(lane, kn - 1);
-        dinv[lane] = 1.0 / Ls[q * LD + q];
+        dinv[lane] = 1.0 / Ls[q * ldl + q];
       }
     }
     __syncthreads();
@@ -1469,7 +1470,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       d4 acc = {0.0, 0.0, 0.0, 0.0};
       acc = wgemm16(
           [&](int ii, int q) {
-            const double x = Ls[(c1 + min(r0 + ii, nB - 1)) * LD + c0 + min(q, kn - 1)];
+            const double x = Ls[(c1 + min(r0 + ii, nB - 1)) * ldl + c0 + min(q, kn - 1)];
             return (r0 + ii < nB && q < kn) ? x : 0.0;
           },
           [&](int q, int pp) {
@@ -2156,16 +2157,23 @@ hipError_t launch_solve_bwd_tri(const DevPlan& P, const int32_t* list, int count
              : bwd_tri<256, false>(P, list, count, sh, B, nr, L, Z, X, T, st);
 }
 
-hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
-                               const double* L, double* S, double* U, const double* W,
+// The L block lives in dynamic LDS sized by the widest supernode of the launch
+// (row stride odd for the banks): narrow supernodes leave room for more
+// workgroups per CU.
+hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count, int maxm, int maxw,
+                               int B, const double* L, double* S, double* U, const double* W,
                                hipStream_t st) {
   if (count <= 0) return hipSuccess;
+  const int ldl = std::max(maxw, 1) | 1;
   if (maxm <= 16)
-    hipLaunchKernelGGL(k_selinv_small<16>, dim3(count, B), dim3(64), 0, st, P, list, L, S, U, W);
+    hipLaunchKernelGGL(k_selinv_small<16>, dim3(count, B), dim3(64), (size_t)16 * ldl * sizeof(double), st,
+                       P, list, L, S, U, W, ldl);
   else if (maxm <= 32)
-    hipLaunchKernelGGL(k_selinv_small<32>, dim3(count, B), dim3(128), 0, st, P, list, L, S, U, W);
+    hipLaunchKernelGGL(k_selinv_small<32>, dim3(count, B), dim3(128), (size_t)32 * ldl * sizeof(double),
+                       st, P, list, L, S, U, W, ldl);
   else if (maxm <= 64)
-    hipLaunchKernelGGL(k_selinv_small<64>, dim3(count, B), dim3(256), 0, st, P, list, L, S, U, W);
+    hipLaunchKernelGGL(k_selinv_small<64>, dim3(count, B), dim3(256), (size_t)64 * ldl * sizeof(double),
+                       st, P, list, L, S, U, W, ldl);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -66,7 +66,7 @@ hipError_t launch_solve_bwd_rows(const DevPlan&, const int32_t*, int, int, int, 
                                  const double*, double*, hipStream_t);
 hipError_t launch_solve_bwd_tri(const DevPlan&, const int32_t*, int, int, int, int, bool,
                                 const double*, const double*, double*, const double*, hipStream_t);
-hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, const double*,
+hipError_t launch_selinv_small(const DevPlan&, const int32_t*, int, int, int, int, const double*,
                                double*, double*, const double*, hipStream_t);
 hipError_t launch_diag_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
 hipError_t launch_pattern_out(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
@@ -184,6 +184,7 @@ struct LevelInfo {
   int64_t sb, se;     // small fronts [sb, se) in small list, ordered by size class
   int64_t cls_b[4];   // class q (m <= 16 << q) covers [cls_b[q], cls_b[q + 1])
   double fl_cls[3] = {0, 0, 0}, sel_cls[3] = {0, 0, 0}, by_cls[3] = {0, 0, 0};
+  int32_t maxw_cls[3] = {0, 0, 0};  // widest supernode of each small class
   int64_t lb, le;     // large fronts [lb, le) in large list
   int64_t ab, ae;     // all fronts [ab, ae) in all list
   int32_t maxm_small, maxm_large, maxm_all, maxw_large;
@@ -457,6 +458,7 @@ struct SymData {
         if (m <= kSmallFront) {
           const int c = m <= 16 ? 0 : (m <= 32 ? 1 : 2);
           cls[c].push_back((int32_t)s);
+          L.maxw_cls[c] = std::max(L.maxw_cls[c], (int32_t)w);
           L.sol_small += sol;
           L.maxm_small = std::max(L.maxm_small, m);
           L.fl_small += fl;
@@ -969,7 +971,7 @@ struct gmrf_batch {
         if (lv.cls_b[c + 1] > lv.cls_b[c])
           prof_launch("k_selinv_small", lv.sel_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
             return launch_selinv_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
-                                       16 << c, count, Lf, Sf, Uf, Wf, stream);
+                                       16 << c, lv.maxw_cls[c], count, Lf, Sf, Uf, Wf, stream);
           });
     }
   }
