@@ -1991,6 +1991,8 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
     hipLaunchKernelGGL(k_factor_small<16>, dim3(count, B), dim3(64), 0, st, P, list, vals, ldv, L, U);
   else if (maxm <= 32)
     hipLaunchKernelGGL(k_factor_small<32>, dim3(count, B), dim3(128), 0, st, P, list, vals, ldv, L, U);
+  else if (maxm <= 48)
+    hipLaunchKernelGGL(k_factor_small<48>, dim3(count, B), dim3(192), 0, st, P, list, vals, ldv, L, U);
   else if (maxm <= 64)
     hipLaunchKernelGGL(k_factor_small<64>, dim3(count, B), dim3(256), 0, st, P, list, vals, ldv, L, U);
   else
@@ -2170,6 +2172,9 @@ hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count,
                        P, list, L, S, U, W, ldl);
   else if (maxm <= 32)
     hipLaunchKernelGGL(k_selinv_small<32>, dim3(count, B), dim3(128), (size_t)32 * ldl * sizeof(double),
+                       st, P, list, L, S, U, W, ldl);
+  else if (maxm <= 48)
+    hipLaunchKernelGGL(k_selinv_small<48>, dim3(count, B), dim3(192), (size_t)48 * ldl * sizeof(double),
                        st, P, list, L, S, U, W, ldl);
   else if (maxm <= 64)
     hipLaunchKernelGGL(k_selinv_small<64>, dim3(count, B), dim3(256), (size_t)64 * ldl * sizeof(double),

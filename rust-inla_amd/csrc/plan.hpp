@@ -45,7 +45,10 @@ struct DevPlan {
   const int32_t* cinv;        // parent front position -> child row index, -1 if absent
 };
 
-constexpr int kSmallFront = 64;  // fronts with m <= 64 run one wavefront each, in LDS
+constexpr int kSmallFront = 64;  // fronts with m <= 64 run one workgroup each, in LDS
+// small-front classes by front size: one kernel instance per class, 4 m / 16 waves
+constexpr int kSmallClasses = 4;
+constexpr int kClassM[kSmallClasses] = {16, 32, 48, 64};
 constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
 constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 constexpr int kSolvePart = 1024;  // rows of R_s per partial sum in the backward solve

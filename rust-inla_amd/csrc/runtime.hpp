@@ -182,9 +182,9 @@ struct DevArena {
 
 struct LevelInfo {
   int64_t sb, se;     // small fronts [sb, se) in small list, ordered by size class
-  int64_t cls_b[4];   // class q (m <= 16 << q) covers [cls_b[q], cls_b[q + 1])
-  double fl_cls[3] = {0, 0, 0}, sel_cls[3] = {0, 0, 0}, by_cls[3] = {0, 0, 0};
-  int32_t maxw_cls[3] = {0, 0, 0};  // widest supernode of each small class
+  int64_t cls_b[kSmallClasses + 1];  // class q (m <= kClassM[q]) covers [cls_b[q], cls_b[q + 1])
+  double fl_cls[kSmallClasses] = {}, sel_cls[kSmallClasses] = {}, by_cls[kSmallClasses] = {};
+  int32_t maxw_cls[kSmallClasses] = {};  // widest supernode of each small class
   int64_t lb, le;     // large fronts [lb, le) in large list
   int64_t ab, ae;     // all fronts [ab, ae) in all list
   int32_t maxm_small, maxm_large, maxm_all, maxw_large;
@@ -440,7 +440,7 @@ struct SymData {
       L.maxm_large = 0;
       L.maxw_large = 0;
       L.maxm_all = 0;
-      std::vector<int32_t> cls[3];
+      std::vector<int32_t> cls[kSmallClasses];
       for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
         int64_t s = S.level_sn[q];
         int32_t m = (int32_t)((S.sn_start[s + 1] - S.sn_start[s]) + sn_r[s]);
@@ -456,7 +456,8 @@ struct SymData {
         const int64_t w = S.sn_start[s + 1] - S.sn_start[s];
         const double by = 8.0 * (double)m * (double)w + 12.0 * (double)(fa_ptr[s + 1] - fa_ptr[s]);
         if (m <= kSmallFront) {
-          const int c = m <= 16 ? 0 : (m <= 32 ? 1 : 2);
+          int c = 0;
+          while (m > kClassM[c]) ++c;
           cls[c].push_back((int32_t)s);
           L.maxw_cls[c] = std::max(L.maxw_cls[c], (int32_t)w);
           L.sol_small += sol;
@@ -478,11 +479,11 @@ struct SymData {
           L.by_large += by;
         }
       }
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < kSmallClasses; ++c) {
         L.cls_b[c] = (int64_t)small.size();
         small.insert(small.end(), cls[c].begin(), cls[c].end());
       }
-      L.cls_b[3] = (int64_t)small.size();
+      L.cls_b[kSmallClasses] = (int64_t)small.size();
       L.se = (int64_t)small.size();
       L.le = (int64_t)large.size();
       L.ae = (int64_t)all.size();
@@ -800,11 +801,11 @@ struct gmrf_batch {
     double* Uf = U + (int64_t)first * P.u_size;
     double* Zf = Z + (int64_t)first * P.z_size;
     for (const LevelInfo& lv : sym->levels) {
-      for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < kSmallClasses; ++c)
         if (lv.cls_b[c + 1] > lv.cls_b[c])
           prof_launch("k_factor_small", lv.fl_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
             return launch_factor_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
-                                       16 << c, count, vals, ldv, Lf, Uf, stream);
+                                       kClassM[c], count, vals, ldv, Lf, Uf, stream);
           });
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
@@ -967,11 +968,11 @@ struct gmrf_batch {
           }
         }
       }
-      for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < kSmallClasses; ++c)
         if (lv.cls_b[c + 1] > lv.cls_b[c])
           prof_launch("k_selinv_small", lv.sel_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
             return launch_selinv_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
-                                       16 << c, lv.maxw_cls[c], count, Lf, Sf, Uf, Wf, stream);
+                                       kClassM[c], lv.maxw_cls[c], count, Lf, Sf, Uf, Wf, stream);
           });
     }
   }
