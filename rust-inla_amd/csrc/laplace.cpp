@@ -52,8 +52,31 @@ constexpr double kPriorJitter = 1e-12;  // inference.rs:831-834
 enum { RED_MAXABS = 0, RED_SUM = 1, RED_DOT = 2, RED_MAXDIFF = 3 };
 }  // namespace
 
+// Pinned host staging for the small per-evaluation read-backs: copies into
+// pageable memory are staged one at a time with a host round trip each.
+struct PinnedBuf {
+  char* p = nullptr;
+  size_t n = 0;
+  // callers ensure no copy into the buffer is in flight
+  char* get(size_t bytes) {
+    if (bytes > n) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+      check(hipHostMalloc((void**)&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+      n = bytes;
+    }
+    return p;
+  }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct gmrf_model {
   std::shared_ptr<SymData> sym;
+  PinnedBuf pin;      // read-backs (reused group after group)
+  PinnedBuf pin_out;  // results copied to the caller's device buffers (until the final sync)
   int64_t n = 0, nnz = 0, nobs = 0;
   int K = 0, fam = 0, link = 0, cap = 0, kc = 0;
   DevArena arena;
@@ -165,6 +188,15 @@ int64_t csc_find(const std::vector<int64_t>& ip, const std::vector<int64_t>& ix,
 void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
   check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "D2H");
   check(hipStreamSynchronize(st), "sync");
+}
+
+// Queued copies; the caller synchronises once for a group (each sync idles the
+// device for a host round trip) and keeps the host buffers alive until then.
+void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "D2H");
+}
+void h2d_async(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "H2D");
 }
 
 // Host sources are short-lived vectors: wait until the copy has consumed them.
@@ -604,8 +636,14 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       bt->solve(B, M->rhs, n, M->step, n, 0);
       if (lr) lowrank_correct(M, B, false, M->step);
       check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, M->red, st), "max step");
-      d2h(hred.data(), M->red, B * sizeof(double), st);
-      d2h(hbad.data(), M->bad, B * sizeof(int32_t), st);
+      {
+        char* pb = M->pin.get((size_t)B * (sizeof(double) + sizeof(int32_t)));
+        d2h_async(pb, M->red, B * sizeof(double), st);
+        d2h_async(pb + B * sizeof(double), M->bad, B * sizeof(int32_t), st);
+        check(hipStreamSynchronize(st), "sync");
+        std::memcpy(hred.data(), pb, B * sizeof(double));
+        std::memcpy(hbad.data(), pb + B * sizeof(double), B * sizeof(int32_t));
+      }
     };
     if (gauss_id) {
       if (a->max_iter == 0) {
@@ -703,11 +741,32 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       if (lr) lowrank_factor(M, B, B, true);
       prior_pending = false;
     }
-    // ---- log-determinants and factor status
-    std::vector<double> ld(2 * B);
+    // ---- mlik at x*: sum log p(y | A x*) - x*^T Q x* / 2 + (log|Q_prior| - log|Q_post|) / 2
+    check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, st), "eta*");
+    check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
+                  M->dw, M->dl, nullptr, st),
+          "lik*");
+    check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, st), "Q x*");
+    check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st), "quad");
+    check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + B, st), "loglik");
+    if (lr) check(lap_constr_dot(B, M->kc, n, M->ca, M->x, M->cres, st), "C x*");
+    // ---- log-determinants, factor status and the mlik terms in one host round trip
+    std::vector<double> ld(2 * B), cx(lr ? (size_t)B * M->kc : 0);
     std::vector<int32_t> fst(2 * B);
-    d2h(ld.data(), bt->d_logdet, 2 * B * sizeof(double), st);
-    d2h(fst.data(), bt->d_status, 2 * B * sizeof(int32_t), st);
+    {
+      const size_t o1 = 2 * B * sizeof(double), o2 = o1 + 2 * B * sizeof(double),
+                   o3 = o2 + 2 * B * sizeof(int32_t);
+      char* pb = M->pin.get(o3 + cx.size() * sizeof(double));
+      d2h_async(pb, bt->d_logdet, 2 * B * sizeof(double), st);
+      d2h_async(pb + o1, M->red, 2 * B * sizeof(double), st);
+      d2h_async(pb + o2, bt->d_status, 2 * B * sizeof(int32_t), st);
+      if (lr) d2h_async(pb + o3, M->cres, cx.size() * sizeof(double), st);
+      check(hipStreamSynchronize(st), "sync");
+      std::memcpy(ld.data(), pb, 2 * B * sizeof(double));
+      std::memcpy(hred.data(), pb + o1, 2 * B * sizeof(double));
+      std::memcpy(fst.data(), pb + o2, 2 * B * sizeof(int32_t));
+      if (lr) std::memcpy(cx.data(), pb + o3, cx.size() * sizeof(double));
+    }
     for (int b = 0; b < B; ++b) {
       const int32_t fp = fst[B + b], fq = fst[b];
       auto code = [](int32_t f) {
@@ -722,20 +781,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
           status[b] = GMRF_NOT_POSITIVE_DEFINITE;
       }
     }
-    // ---- mlik at x*: sum log p(y | A x*) - x*^T Q x* / 2 + (log|Q_prior| - log|Q_post|) / 2
-    check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, st), "eta*");
-    check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
-                  M->dw, M->dl, nullptr, st),
-          "lik*");
-    check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, st), "Q x*");
-    check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st), "quad");
-    check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + B, st), "loglik");
-    d2h(hred.data(), M->red, 2 * B * sizeof(double), st);
     if (lr) {  // x*^T (kappa C^T C) x*
       const int k = M->kc;
-      check(lap_constr_dot(B, k, n, M->ca, M->x, M->cres, st), "C x*");
-      std::vector<double> cx((size_t)B * k);
-      d2h(cx.data(), M->cres, cx.size() * sizeof(double), st);
       for (int b = 0; b < B; ++b)
         for (int t = 0; t < k; ++t) hred[b] += M->kappa * cx[(size_t)b * k + t] * cx[(size_t)b * k + t];
     }
@@ -744,7 +791,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       mlik[b] = hred[B + b] - 0.5 * hred[b] + 0.5 * ld[B + b] - 0.5 * ld[b];
       if (status[b] != GMRF_OK) mlik[b] = -INFINITY;
     }
-    h2d(a->mlik, mlik.data(), B * sizeof(double), st);
+    double* pout = reinterpret_cast<double*>(
+        M->pin_out.get((size_t)B * (1 + std::max(a->n_theta, 0)) * sizeof(double)));
+    std::memcpy(pout, mlik.data(), B * sizeof(double));
+    h2d_async(a->mlik, pout, B * sizeof(double), st);
     if (a->x) check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
     // ---- selected inversion, variances
     // posterior (and, for the gradient, prior) selected inverses in one call
@@ -760,6 +810,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
               "var correction");
     }
     // ---- gradient
+    std::vector<double> grad;
     if (want_grad) {
       const int mt = a->n_theta;
       const DevPlan& P = M->sym->plan;
@@ -791,16 +842,42 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_contract(B, K, nnz, M->comps, nullptr, 0, nullptr, M->q_row, M->q_col, M->lam, M->x,
                            n, nullptr, M->ql, M->cpart, st),
               "lambda^T Q_k x");
-        d2h(hql.data(), M->ql, hql.size() * sizeof(double), st);
       }
       // Gaussian precision hyperparameter at fixed eta: sum dlogp/dlog(prec), sum v w
       check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, M->red, st), "sum dl");
       check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, M->red + B, st), "sum v w");
       std::vector<double> hs(2 * B);
-      d2h(hs.data(), M->red, 2 * B * sizeof(double), st);
       std::vector<double> htp((size_t)B * K), htq((size_t)B * K), hqx((size_t)B * K);
-      d2h(htp.data(), M->tr_post, htp.size() * sizeof(double), st);
-      d2h(htq.data(), M->tr_prior, htq.size() * sizeof(double), st);
+      std::vector<double> dc((size_t)B * mt * K), dlp((size_t)B * mt, 0.0);
+      {
+        // one pinned area, one host round trip
+        std::vector<std::pair<double*, size_t>> parts = {{hs.data(), hs.size()},   {htp.data(), htp.size()},
+                                                         {htq.data(), htq.size()}, {hqx.data(), hqx.size()},
+                                                         {dc.data(), dc.size()}};
+        std::vector<const double*> srcs = {M->red, M->tr_post, M->tr_prior, M->qx, a->dcoefs};
+        if (a->dlog_lpar) {
+          parts.push_back({dlp.data(), dlp.size()});
+          srcs.push_back(a->dlog_lpar);
+        }
+        if (!gauss_id) {
+          parts.push_back({hql.data(), hql.size()});
+          srcs.push_back(M->ql);
+        }
+        size_t tot = 0;
+        for (auto& q : parts) tot += q.second;
+        double* pb = reinterpret_cast<double*>(M->pin.get(tot * sizeof(double)));
+        size_t off = 0;
+        for (size_t i = 0; i < parts.size(); ++i) {
+          d2h_async(pb + off, srcs[i], parts[i].second * sizeof(double), st);
+          off += parts[i].second;
+        }
+        check(hipStreamSynchronize(st), "sync");
+        off = 0;
+        for (auto& q : parts) {
+          std::memcpy(q.first, pb + off, q.second * sizeof(double));
+          off += q.second;
+        }
+      }
       if (lr) {  // tr((Q_g + U E U^T)^-1 Q_k) = tr(Q_g^-1 Q_k) - tr(Kinv Y^T Q_k Y)
         const std::vector<double> cpo = lowrank_traces(M, B, false), cpr = lowrank_traces(M, B, true);
         for (size_t e = 0; e < htp.size(); ++e) {
@@ -808,11 +885,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
           htq[e] -= cpr[e];
         }
       }
-      d2h(hqx.data(), M->qx, hqx.size() * sizeof(double), st);
-      std::vector<double> dc((size_t)B * mt * K), dlp((size_t)B * mt, 0.0);
-      d2h(dc.data(), a->dcoefs, dc.size() * sizeof(double), st);
-      if (a->dlog_lpar) d2h(dlp.data(), a->dlog_lpar, dlp.size() * sizeof(double), st);
-      std::vector<double> grad((size_t)B * mt);
+      grad.assign((size_t)B * mt, 0.0);
       for (int b = 0; b < B; ++b)
         for (int l = 0; l < mt; ++l) {
           double gq = 0.0;
@@ -824,7 +897,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
           if (M->fam == 0) gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[B + b]);
           grad[(size_t)b * mt + l] = status[b] == GMRF_OK ? gq : NAN;
         }
-      h2d(a->grad, grad.data(), grad.size() * sizeof(double), st);
+      std::memcpy(pout + B, grad.data(), grad.size() * sizeof(double));
+      h2d_async(a->grad, pout + B, grad.size() * sizeof(double), st);
     }
     check(hipStreamSynchronize(st), "sync");
     for (int b = 0; b < B; ++b) {
