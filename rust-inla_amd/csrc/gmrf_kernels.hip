@@ -1626,16 +1626,13 @@ __device__ __forceinline__ void sel_panel(int w, int step, int& c0, int& c1, int
 }
 
 // Lh rows of panel P, as a product with the stored Z = L_PP^-1: task (s, I)
-// computes rows c1 + 64 I .. of Lh[i][p] = sum_q L[i][q] Z[q][p] into T.
-__global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __restrict__ tasks,
-                                                 int step, const double* __restrict__ L,
-                                                 const double* __restrict__ Z,
-                                                 double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+// computes rows c1 + 64 I .. of Lh[i][p] = sum_q L[i][q] Z[q][p] into the panel's
+// L_hat buffer of the staging area.
+__device__ __forceinline__ void sl_lhat_task(const DevPlan& P, const int32_t* tk, int step, int b,
+                                             const double* __restrict__ L,
+                                             const double* __restrict__ Z, double* __restrict__ T,
+                                             double (*As)[kPanel + 1], double (*Bs)[kPanel + 1]) {
   const int s = tk[0], I = tk[1];
-  const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   int c0, c1, pk;
   sel_panel(w, step, c0, c1, pk);
@@ -1643,7 +1640,7 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
   const int R0 = c1 + I * kTile;
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
-  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
   mma_tile<false, true>(
@@ -1659,8 +1656,17 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
       nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int pp, double v) {
     const int i = R0 + ii;
-    if (i < m && pp < nb) Tb[(int64_t)pp * m + i] = v;
+    if (i < m && pp < nb) Lh[(int64_t)pp * m + i] = v;
   });
+}
+
+__global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __restrict__ tasks,
+                                                 int step, const double* __restrict__ L,
+                                                 const double* __restrict__ Z,
+                                                 double* __restrict__ T) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  sl_lhat_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, T, As, Bs);
 }
 
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
@@ -1684,7 +1690,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
   const int R0 = c1 + I * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
-  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
   mma_tile<false, true, 512>(
@@ -1694,7 +1700,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
         return i < m ? v : 0.0;
       },
       [&](int jj, int kk) {
-        const double v = Tb[(int64_t)min(jj, nb - 1) * m + c1 + kk];
+        const double v = Lh[(int64_t)min(jj, nb - 1) * m + c1 + kk];
         return jj < nb ? v : 0.0;
       },
       m - c1, As, Bs, acc);
@@ -1723,17 +1729,12 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
 // (K = nb, then K = m - c1); lower part into S, both triangles into W. The
 // strict upper triangle of S's diagonal blocks is not part of the layout's
 // contract and is left as is.
-__global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restrict__ tasks,
-                                               int step, const double* __restrict__ L,
-                                               const double* __restrict__ Z,
-                                               double* __restrict__ S, double* __restrict__ W,
-                                               const double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
-  __shared__ double dinv[kTile];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+__device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, int step, int b,
+                                           const double* __restrict__ L, const double* __restrict__ Z,
+                                           double* __restrict__ S, double* __restrict__ W,
+                                           const double* __restrict__ T, double (*As)[kPanel + 1],
+                                           double (*Bs)[kPanel + 1], double* dinv) {
   const int s = tk[0];
-  const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   int c0, c1, pk;
@@ -1743,7 +1744,7 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
-  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   if (tid < kTile) {
     const int q = min(tid, nb - 1);
     dinv[tid] = 1.0 / Lb[(int64_t)(c0 + q) * m + c0 + q];
@@ -1765,7 +1766,7 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
   if (m > c1)
     mma_tile<true, true>(
         [&](int ii, int kk) {
-          const double v = Tb[(int64_t)min(ii, nb - 1) * m + c1 + kk];
+          const double v = Lh[(int64_t)min(ii, nb - 1) * m + c1 + kk];
           return ii < nb ? -v : 0.0;
         },
         [&](int jj, int kk) {
@@ -1781,6 +1782,35 @@ __global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restr
       Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
     }
   });
+}
+
+__global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restrict__ tasks,
+                                               int step, const double* __restrict__ L,
+                                               const double* __restrict__ Z,
+                                               double* __restrict__ S, double* __restrict__ W,
+                                               const double* __restrict__ T) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double dinv[kTile];
+  sl_pp_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, As, Bs, dinv);
+}
+
+// Sigma[P,P] of panel step `step` and L_hat of step + 1 in one launch: the first
+// npp workgroups take the Sigma[P,P] tasks, the rest the L_hat tasks (they read
+// only L and Z and write the other L_hat buffer, so the two sets are independent).
+__global__ __launch_bounds__(256) void k_sl_pp_lhat(DevPlan P, const int32_t* __restrict__ pp_tasks,
+                                                    int npp, const int32_t* __restrict__ lhat_tasks,
+                                                    int step, const double* __restrict__ L,
+                                                    const double* __restrict__ Z,
+                                                    double* __restrict__ S, double* __restrict__ W,
+                                                    double* __restrict__ T) {
+  __shared__ double As[64][kPanel + 1];
+  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double dinv[kTile];
+  if ((int)blockIdx.x < npp)
+    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, As, Bs, dinv);
+  else
+    sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp), step + 1, blockIdx.y, L, Z, T, As, Bs);
 }
 
 // Sigma[B,P] split over K for launches with few row blocks (tall fronts of a small
@@ -1805,6 +1835,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* 
   const int R0 = c1 + I * kTile;
   const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  const double* Lh = Tb + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
   mma_tile<false, true, 512>(
@@ -1814,7 +1845,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* 
         return i < m ? v : 0.0;
       },
       [&](int jj, int kk) {
-        const double v = Tb[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk];
+        const double v = Lh[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk];
         return jj < nb ? v : 0.0;
       },
       ke - ks, As, Bs, acc);
@@ -1909,7 +1940,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
   if (ke > ks)
     mma_tile<true, true>(
         [&](int ii, int kk) {
-          const double v = Tb[(int64_t)min(ii, nb - 1) * m + c1 + ks + kk];
+          const double v = Tb[stage_lhat_off(m, step) + (int64_t)min(ii, nb - 1) * m + c1 + ks + kk];
           return ii < nb ? -v : 0.0;
         },
         [&](int jj, int kk) {
@@ -2224,6 +2255,15 @@ hipError_t launch_sl_gemm_sum(const DevPlan& P, const int32_t* tasks, int count,
                               double* S, double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_gemm_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sl_pp_lhat(const DevPlan& P, const int32_t* pp_tasks, int npp,
+                             const int32_t* lhat_tasks, int nlhat, int step, int B, const double* L,
+                             const double* Z, double* S, double* W, double* T, hipStream_t st) {
+  if (npp + nlhat <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_pp_lhat, dim3(npp + nlhat, B), dim3(256), 0, st, P, pp_tasks, npp, lhat_tasks,
+                     step, L, Z, S, W, T);
   return hipGetLastError();
 }
 

@@ -57,11 +57,14 @@ constexpr int kSelSplitK = 512;  // rows below a panel per partial product of Si
 constexpr int kGemmSplitK = 512;  // K per partial product of Sigma[B,P] when a level has few row blocks
 constexpr int kGemmSplitMaxTasks = 96;  // split Sigma[B,P] when a launch has at most this many row blocks
 
-// Staging area of a large front (T, per level): L_hat of the current panel (m x 64),
-// the Sigma[P,P] partial products, then the Sigma[B,P] partial tiles.
-constexpr int64_t stage_pp_off(int64_t m) { return m * 64; }
+// Staging area of a large front (T, per level): L_hat of two panels (m x 64 each,
+// panel step k in buffer k % 2, so that the next panel's L_hat can be formed while
+// the current one is still read), the Sigma[P,P] partial products, then the
+// Sigma[B,P] partial tiles.
+constexpr int64_t stage_lhat_off(int64_t m, int step) { return (step & 1) * m * 64; }
+constexpr int64_t stage_pp_off(int64_t m) { return 2 * m * 64; }
 constexpr int64_t stage_gemm_off(int64_t m) {
-  return m * 64 + ((m + kSelSplitK - 1) / kSelSplitK) * 64 * 64;
+  return stage_pp_off(m) + ((m + kSelSplitK - 1) / kSelSplitK) * 64 * 64;
 }
 constexpr int64_t stage_size(int64_t m) {
   return stage_gemm_off(m) + ((m + 63) / 64) * ((m + kGemmSplitK - 1) / kGemmSplitK) * 64 * 64;

@@ -38,6 +38,8 @@ hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*
                           const double*, hipStream_t);
 hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const double*,
                         const double*, double*, double*, const double*, hipStream_t);
+hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int,
+                             const double*, const double*, double*, double*, double*, hipStream_t);
 hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
                              const double*, const double*, double*, hipStream_t);
 hipError_t launch_sl_gemm_part(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
@@ -929,11 +931,15 @@ struct gmrf_batch {
           return launch_sl_gather(P, tk + 3 * lv.gather.off, (int)lv.gather.cnt, count, Sf, Uf, Wf,
                                   stream);
         });
-        for (size_t k = 0; k < lv.lhat.size(); ++k) {
-          prof_launch("k_sl_lhat", lv.lhat[k].flops * count, 0.0, stream, [&] {
-            return launch_sl_lhat(P, tk + 3 * lv.lhat[k].off, (int)lv.lhat[k].cnt, (int)k, count, Lf,
-                                  Zf, Tf, stream);
+        // L_hat of panel step k sits in buffer k % 2 of the staging area; the L_hat of
+        // step k + 1 is formed by the launch that closes step k
+        if (!lv.lhat.empty())
+          prof_launch("k_sl_lhat", lv.lhat[0].flops * count, 0.0, stream, [&] {
+            return launch_sl_lhat(P, tk + 3 * lv.lhat[0].off, (int)lv.lhat[0].cnt, 0, count, Lf, Zf, Tf,
+                                  stream);
           });
+        for (size_t k = 0; k < lv.lhat.size(); ++k) {
+          const bool next = k + 1 < lv.lhat.size() && lv.lhat[k + 1].cnt > 0;
           // few row blocks with a long K (the matrices of tall fronts, a small batch):
           // split K so that the product spreads over the device
           if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= kGemmSplitMaxTasks) {
@@ -959,6 +965,17 @@ struct gmrf_batch {
             prof_launch("k_sl_pp_sum", 0.0, 0.0, stream, [&] {
               return launch_sl_pp_sum(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Sf, Wf,
                                       Tf, stream);
+            });
+            if (next)
+              prof_launch("k_sl_lhat", lv.lhat[k + 1].flops * count, 0.0, stream, [&] {
+                return launch_sl_lhat(P, tk + 3 * lv.lhat[k + 1].off, (int)lv.lhat[k + 1].cnt, (int)k + 1,
+                                      count, Lf, Zf, Tf, stream);
+              });
+          } else if (next) {
+            prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + lv.lhat[k + 1].flops) * count, 0.0, stream, [&] {
+              return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt,
+                                       tk + 3 * lv.lhat[k + 1].off, (int)lv.lhat[k + 1].cnt, (int)k, count,
+                                       Lf, Zf, Sf, Wf, Tf, stream);
             });
           } else {
             prof_launch("k_sl_pp", lv.spp[k].flops * count, 0.0, stream, [&] {
