@@ -52,6 +52,25 @@ __device__ __forceinline__ double* front_at(double* Lb, double* Ub, int m, int w
 // i), which makes the 16 lanes of a register row consecutive tile rows i —
 // contiguous addresses in column-major storage.
 // ---------------------------------------------------------------------------
+// An operand element as returned by a tile-engine accessor: the value loaded from
+// a valid (clamped) address, whether the element lies inside the operand and, for
+// LVs, a factor (from LDS or registers) applied with the mask. Engines that prefetch
+// resolve these only when the chunk goes to LDS, so the next chunk's loads stay in
+// flight while the current one is on MFMA.
+struct LV {
+  double v;
+  bool ok;
+  __device__ double get() const { return ok ? v : 0.0; }
+};
+struct LVs {
+  double v, s;
+  bool ok;
+  __device__ double get() const { return ok ? v * s : 0.0; }
+};
+__device__ __forceinline__ double lv_get(double x) { return x; }
+__device__ __forceinline__ double lv_get(const LV& x) { return x.get(); }
+__device__ __forceinline__ double lv_get(const LVs& x) { return x.get(); }
+
 struct TileAcc {
   d4 c[2][2];  // [bj][bi]
 };
@@ -63,11 +82,20 @@ __device__ __forceinline__ void tile_zero(TileAcc& acc) {
     for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
+__device__ __forceinline__ void tile_negate(TileAcc& acc) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc.c[a][b] = -acc.c[a][b];
+}
+
 // AK / BK: the operand is contiguous along k in memory, so consecutive lanes
 // load consecutive k (otherwise consecutive rows). The accessors are called with
 // k < K only and must load unconditionally from a valid (clamped) address and
-// select zero afterwards: a branch around a load serialises the memory latency.
-// The next K chunk is fetched into registers while the current one is on MFMA.
+// report validity beside the value (LV): a branch around a load serialises the
+// memory latency. The next K chunk is fetched into registers while the current
+// one is on MFMA; masks and factors are applied when a chunk goes to LDS, so no
+// wait for the fetched values precedes the MFMA steps.
 //
 // NT = 512: eight waves; waves w and w + 4 own the same quadrant and take
 // alternate k-steps of each chunk, so every SIMD interleaves two independent MFMA
@@ -82,7 +110,8 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int half = NH == 2 ? (wv >> 2) : 0, qi = wv & 1, qj = (wv >> 1) & 1;
   const int lr = lane & 15, lk = lane >> 4;
-  double ra[NL], rb[NL];
+  decltype(fa(0, 0)) ra[NL];
+  decltype(fb(0, 0)) rb[NL];
   auto fetch = [&](int k0) {
     const int kc = min(kPanel, K - k0);
 #pragma unroll
@@ -90,33 +119,31 @@ __device__ __forceinline__ void mma_tile(FA fa, FB fb, int K, double (*As)[kPane
       const int idx = tid + NT * it;
       {
         const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
-        const double v = fa(ii, k0 + min(kk, kc - 1));
-        ra[it] = kk < kc ? v : 0.0;
+        ra[it] = fa(ii, k0 + min(kk, kc - 1));
       }
       {
         const int ii = BK ? idx / kPanel : (idx & 63), kk = BK ? idx % kPanel : (idx >> 6);
-        const double v = fb(ii, k0 + min(kk, kc - 1));
-        rb[it] = kk < kc ? v : 0.0;
+        rb[it] = fb(ii, k0 + min(kk, kc - 1));
       }
     }
     ISSUE_FENCE();
   };
   if (K > 0) fetch(0);
   for (int k0 = 0; k0 < K; k0 += kPanel) {
+    const int kc = min(kPanel, K - k0);
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
       const int idx = tid + NT * it;
       {
         const int ii = AK ? idx / kPanel : (idx & 63), kk = AK ? idx % kPanel : (idx >> 6);
-        As[ii][kk] = ra[it];
+        As[ii][kk] = kk < kc ? lv_get(ra[it]) : 0.0;
       }
       {
         const int ii = BK ? idx / kPanel : (idx & 63), kk = BK ? idx % kPanel : (idx >> 6);
-        Bs[ii][kk] = rb[it];
+        Bs[ii][kk] = kk < kc ? lv_get(rb[it]) : 0.0;
       }
     }
     __syncthreads();
-    const int kc = min(kPanel, K - k0);
     if (k0 + kPanel < K) fetch(k0 + kPanel);
 #pragma unroll
     for (int st = 0; st < kPanel / (4 * NH); ++st) {
@@ -169,6 +196,12 @@ __device__ __forceinline__ void tile_foreach(const TileAcc& acc, F f) {
       }
 }
 
+// The engine on one LDS block sh of 2 x 64 x (kPanel + 1) doubles (A, then B).
+template <bool AK, bool BK, typename FA, typename FB>
+__device__ __forceinline__ void tile_product(FA fa, FB fb, int K, double* sh, TileAcc& acc) {
+  mma_tile<AK, BK>(fa, fb, K, (double (*)[kPanel + 1])sh, (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1)), acc);
+}
+
 // ---------------------------------------------------------------------------
 // Probe: T = A * B^T for one 64x64 tile (A, B 64 x K column-major, K <= 64).
 // Used by the tests to pin the MFMA lane layout against a host product.
@@ -179,8 +212,8 @@ __global__ __launch_bounds__(256) void k_mma_probe(const double* A, const double
   __shared__ double Bs[64][kPanel + 1];
   TileAcc acc;
   tile_zero(acc);
-  mma_tile([&](int i, int k) { return A[(int64_t)k * 64 + i]; },
-           [&](int j, int k) { return B[(int64_t)k * 64 + j]; }, K, As, Bs, acc);  // k < K
+  mma_tile([&](int i, int k) { return LV{A[(int64_t)k * 64 + i], true}; },
+           [&](int j, int k) { return LV{B[(int64_t)k * 64 + j], true}; }, K, As, Bs, acc);  // k < K
   tile_foreach(acc, [&](int i, int j, double v) { T[(int64_t)j * 64 + i] = v; });
 }
 
@@ -205,8 +238,8 @@ __device__ __forceinline__ d4 wgemm16(FA a, FB b, d4 acc) {
   double av[4], bv[4];
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) {
-    av[s4] = a(r, 4 * s4 + q);
-    bv[s4] = b(4 * s4 + q, r);
+    av[s4] = lv_get(a(r, 4 * s4 + q));
+    bv[s4] = lv_get(b(4 * s4 + q, r));
   }
   ISSUE_FENCE();
 #pragma unroll
@@ -428,12 +461,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         const int r0 = t0 + 16 * task;
         acc = wgemm16(
             [&](int ii, int q) {
-              const double x = F[min(r0 + ii, m - 1) * LD + k0 + min(q, kn - 1)];
-              return (r0 + ii < m && q < kn) ? x : 0.0;
+              return LV{F[min(r0 + ii, m - 1) * LD + k0 + min(q, kn - 1)], (r0 + ii < m && q < kn)};
             },
             [&](int q, int pp) {
-              const double x = Zk[pp * 17 + q];
-              return q < kn ? x : 0.0;
+              return LV{Zk[pp * 17 + q], q < kn};
             },
             acc);
 #pragma unroll
@@ -448,12 +479,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         const int jb = task - nrb;
         acc = wgemm16(
             [&](int ii, int q) {
-              const double x = Zk[ii * 17 + q];
-              return q < kn ? x : 0.0;
+              return LV{Zk[ii * 17 + q], q < kn};
             },
             [&](int q, int jj) {
-              const double x = Zs[(k0 + min(q, kn - 1)) * LD + 16 * jb + jj];
-              return q < kn ? x : 0.0;
+              return LV{Zs[(k0 + min(q, kn - 1)) * LD + 16 * jb + jj], q < kn};
             },
             acc);
 #pragma unroll
@@ -492,12 +521,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         const int R = t0 + 16 * ib, C = t0 + 16 * jb;
         acc = wgemm16(
             [&](int ii, int pp) {
-              const double x = Wb[(16 * ib + ii) * 17 + min(pp, kn - 1)];
-              return pp < kn ? x : 0.0;
+              return LV{Wb[(16 * ib + ii) * 17 + min(pp, kn - 1)], pp < kn};
             },
             [&](int pp, int jj) {
-              const double x = F[min(C + jj, m - 1) * LD + k0 + min(pp, kn - 1)];
-              return (pp < kn && C + jj < m) ? x : 0.0;
+              return LV{F[min(C + jj, m - 1) * LD + k0 + min(pp, kn - 1)], (pp < kn && C + jj < m)};
             },
             acc);
 #pragma unroll
@@ -510,12 +537,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         const int R = t0 + 16 * ib;
         acc = wgemm16(
             [&](int ii, int pp) {
-              const double x = F[min(R + ii, m - 1) * LD + k0 + min(pp, kn - 1)];
-              return pp < kn ? x : 0.0;
+              return LV{F[min(R + ii, m - 1) * LD + k0 + min(pp, kn - 1)], pp < kn};
             },
             [&](int pp, int jj) {
-              const double x = Zs[(k0 + min(pp, kn - 1)) * LD + 16 * jb + jj];
-              return pp < kn ? x : 0.0;
+              return LV{Zs[(k0 + min(pp, kn - 1)) * LD + 16 * jb + jj], pp < kn};
             },
             acc);
 #pragma unroll
@@ -768,12 +793,10 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
   mma_tile(
       [&](int ii, int q) {
         const int i = R0 + ii;
-        const double v = Lb[(int64_t)(c0 + q) * m + min(i, m - 1)];
-        return i < m ? v : 0.0;
+        return LV{Lb[(int64_t)(c0 + q) * m + min(i, m - 1)], i < m};
       },
       [&](int pp, int q) {
-        const double v = Zb[q * kTile + pp] * dinv[pp];
-        return pp < nb ? v : 0.0;
+        return LVs{Zb[q * kTile + pp], dinv[pp], pp < nb};
       },
       nb, As, Bs, acc);
   tile_foreach(acc, [&](int ii, int pp, double v) {
@@ -807,13 +830,11 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
   mma_tile(
       [&](int ii, int p) {
         const int i = R0 + ii;
-        const double v = Lb[(int64_t)(c0 + p) * m + min(i, m - 1)] * dpiv[p];
-        return i < m ? v : 0.0;
+        return LVs{Lb[(int64_t)(c0 + p) * m + min(i, m - 1)], dpiv[p], i < m};
       },
       [&](int jj, int p) {
         const int j = C0 + jj;
-        const double v = Lb[(int64_t)(c0 + p) * m + min(j, m - 1)];
-        return j < m ? v : 0.0;
+        return LV{Lb[(int64_t)(c0 + p) * m + min(j, m - 1)], j < m};
       },
       nb, As, Bs, acc);
   // C -= tile: the tile goes through LDS (the tile engine ends with a barrier) so
@@ -1470,12 +1491,10 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       d4 acc = {0.0, 0.0, 0.0, 0.0};
       acc = wgemm16(
           [&](int ii, int q) {
-            const double x = Ls[(c1 + min(r0 + ii, nB - 1)) * ldl + c0 + min(q, kn - 1)];
-            return (r0 + ii < nB && q < kn) ? x : 0.0;
+            return LV{Ls[(c1 + min(r0 + ii, nB - 1)) * ldl + c0 + min(q, kn - 1)], (r0 + ii < nB && q < kn)};
           },
           [&](int q, int pp) {
-            const double x = Zk[q * 17 + pp];
-            return q < kn ? x : 0.0;
+            return LV{Zk[q * 17 + pp], q < kn};
           },
           acc);
 #pragma unroll
@@ -1489,12 +1508,10 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       for (int k0 = 0; k0 < nB; k0 += 16)
         acc = wgemm16(
             [&](int ii, int kk) {
-              const double x = Sg[(c1 + min(r0 + ii, nB - 1)) * LD + c1 + min(k0 + kk, nB - 1)];
-              return (r0 + ii < nB && k0 + kk < nB) ? x : 0.0;
+              return LV{Sg[(c1 + min(r0 + ii, nB - 1)) * LD + c1 + min(k0 + kk, nB - 1)], (r0 + ii < nB && k0 + kk < nB)};
             },
             [&](int kk, int pp) {
-              const double x = Lh[min(k0 + kk, nB - 1) * 17 + pp];
-              return k0 + kk < nB ? x : 0.0;
+              return LV{Lh[min(k0 + kk, nB - 1) * 17 + pp], k0 + kk < nB};
             },
             acc);
 #pragma unroll
@@ -1512,12 +1529,10 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       d4 acc = {0.0, 0.0, 0.0, 0.0};
       acc = wgemm16(
           [&](int pp, int k) {
-            const double x = Zk[k * 17 + pp] * dinv[k];
-            return k < kn ? x : 0.0;
+            return LVs{Zk[k * 17 + pp], dinv[k], k < kn};
           },
           [&](int k, int qq) {
-            const double x = Zk[k * 17 + qq];
-            return k < kn ? x : 0.0;
+            return LV{Zk[k * 17 + qq], k < kn};
           },
           acc);
       for (int k0 = 0; k0 < nB; k0 += 16)
@@ -1527,8 +1542,7 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
               return k0 + ii < nB ? -x : 0.0;
             },
             [&](int ii, int qq) {
-              const double x = Sg[(c1 + min(k0 + ii, nB - 1)) * LD + c0 + qq];
-              return k0 + ii < nB ? x : 0.0;
+              return LV{Sg[(c1 + min(k0 + ii, nB - 1)) * LD + c0 + qq], k0 + ii < nB};
             },
             acc);
 #pragma unroll
@@ -1631,7 +1645,7 @@ __device__ __forceinline__ void sel_panel(int w, int step, int& c0, int& c1, int
 __device__ __forceinline__ void sl_lhat_task(const DevPlan& P, const int32_t* tk, int step, int b,
                                              const double* __restrict__ L,
                                              const double* __restrict__ Z, double* __restrict__ T,
-                                             double (*As)[kPanel + 1], double (*Bs)[kPanel + 1]) {
+                                             double* sh) {
   const int s = tk[0], I = tk[1];
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   int c0, c1, pk;
@@ -1643,30 +1657,28 @@ __device__ __forceinline__ void sl_lhat_task(const DevPlan& P, const int32_t* tk
   double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
-  mma_tile<false, true>(
+  tile_product<false, true>(
       [&](int ii, int q) {
         const int i = R0 + ii;
-        const double v = Lb[(int64_t)(c0 + q) * m + min(i, m - 1)];
-        return i < m ? v : 0.0;
+        return LV{Lb[(int64_t)(c0 + q) * m + min(i, m - 1)], i < m};
       },
       [&](int pp, int q) {
-        const double v = Zb[pp * kTile + q];
-        return pp < nb ? v : 0.0;
+        return LV{Zb[pp * kTile + q], pp < nb};
       },
-      nb, As, Bs, acc);
-  tile_foreach(acc, [&](int ii, int pp, double v) {
-    const int i = R0 + ii;
-    if (i < m && pp < nb) Lh[(int64_t)pp * m + i] = v;
-  });
+      nb, sh, acc);
+  if (threadIdx.x < 256)
+    tile_foreach(acc, [&](int ii, int pp, double v) {
+      const int i = R0 + ii;
+      if (i < m && pp < nb) Lh[(int64_t)pp * m + i] = v;
+    });
 }
 
 __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, const double* __restrict__ L,
                                                  const double* __restrict__ Z,
                                                  double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
-  sl_lhat_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, T, As, Bs);
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  sl_lhat_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, T, sh);
 }
 
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
@@ -1677,6 +1689,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
                                                  int step, double* __restrict__ S,
                                                  double* __restrict__ W,
                                                  const double* __restrict__ T) {
+  constexpr int NT = 512;
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
   double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
@@ -1693,17 +1706,14 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
   const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
-  mma_tile<false, true, 512>(
-      [&](int ii, int kk) {
-        const int i = R0 + ii;
-        const double v = Ws[(int64_t)(c1 + kk) * m + min(i, m - 1)];
-        return i < m ? v : 0.0;
-      },
-      [&](int jj, int kk) {
-        const double v = Lh[(int64_t)min(jj, nb - 1) * m + c1 + kk];
-        return jj < nb ? v : 0.0;
-      },
-      m - c1, As, Bs, acc);
+  auto fa = [&](int ii, int kk) {
+    const int i = R0 + ii;
+    return LV{Ws[(int64_t)(c1 + kk) * m + min(i, m - 1)], i < m};
+  };
+  auto fb = [&](int jj, int kk) {
+    return LV{Lh[(int64_t)min(jj, nb - 1) * m + c1 + kk], jj < nb};
+  };
+  mma_tile<false, true, NT>(fa, fb, m - c1, As, Bs, acc);
   // the tile engine ends with a barrier, so its LDS can hold the tile now; waves
   // 0-3 hold the result
   double (*Ct)[kTile + 1] = (double (*)[kTile + 1])sh;
@@ -1718,7 +1728,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
     });
   __syncthreads();
   // W[c0 + jj][i]: consecutive lanes take consecutive jj
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 512) {
+  for (int idx = threadIdx.x; idx < kTile * kTile; idx += NT) {
     const int jj = idx & 63, ii = idx >> 6;
     const int i = R0 + ii;
     if (i < m && jj < nb) Ws[(int64_t)i * m + c0 + jj] = Ct[jj][ii];
@@ -1732,8 +1742,7 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
 __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, int step, int b,
                                            const double* __restrict__ L, const double* __restrict__ Z,
                                            double* __restrict__ S, double* __restrict__ W,
-                                           const double* __restrict__ T, double (*As)[kPanel + 1],
-                                           double (*Bs)[kPanel + 1], double* dinv) {
+                                           const double* __restrict__ T, double* sh, double* dinv) {
   const int s = tk[0];
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
@@ -1753,46 +1762,36 @@ __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, 
   TileAcc acc;
   tile_zero(acc);
   // (Z^T D^-1 Z)[p][q] = sum_k Z[k][p] Z[k][q] / d_k, Zb[q * 64 + k] = Z[k][q]
-  mma_tile<true, true>(
+  tile_product<true, true>(
       [&](int pp, int k) {
-        const double x = Zb[pp * kTile + k] * dinv[k];
-        return pp < nb ? x : 0.0;
+        return LVs{Zb[pp * kTile + k], dinv[k], pp < nb};
       },
       [&](int qq, int k) {
-        const double x = Zb[qq * kTile + k];
-        return qq < nb ? x : 0.0;
+        return LV{Zb[qq * kTile + k], qq < nb};
       },
-      nb, As, Bs, acc);
-  if (m > c1)
-    mma_tile<true, true>(
+      nb, sh, acc);
+  if (m > c1) {
+    // acc - Lh^T Sigma[B,P] as -(-acc + Lh^T Sigma[B,P]) (negation is exact)
+    tile_negate(acc);
+    tile_product<true, true>(
         [&](int ii, int kk) {
-          const double v = Lh[(int64_t)min(ii, nb - 1) * m + c1 + kk];
-          return ii < nb ? -v : 0.0;
+          return LV{Lh[(int64_t)min(ii, nb - 1) * m + c1 + kk], ii < nb};
         },
         [&](int jj, int kk) {
-          const double v = Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk];
-          return jj < nb ? v : 0.0;
+          return LV{Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk], jj < nb};
         },
-        m - c1, As, Bs, acc);
-  tile_foreach(acc, [&](int ii, int jj, double v) {
-    // tile row ii = p, tile column jj = q; lower part p >= q
-    if (ii < nb && jj < nb && ii >= jj) {
-      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
-    }
-  });
-}
-
-__global__ __launch_bounds__(256) void k_sl_pp(DevPlan P, const int32_t* __restrict__ tasks,
-                                               int step, const double* __restrict__ L,
-                                               const double* __restrict__ Z,
-                                               double* __restrict__ S, double* __restrict__ W,
-                                               const double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
-  __shared__ double dinv[kTile];
-  sl_pp_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, As, Bs, dinv);
+        m - c1, sh, acc);
+    tile_negate(acc);
+  }
+  if (threadIdx.x < 256)
+    tile_foreach(acc, [&](int ii, int jj, double v) {
+      // tile row ii = p, tile column jj = q; lower part p >= q
+      if (ii < nb && jj < nb && ii >= jj) {
+        Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+        Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
+        Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
+      }
+    });
 }
 
 // Sigma[P,P] of panel step `step` and L_hat of step + 1 in one launch: the first
@@ -1804,13 +1803,12 @@ __global__ __launch_bounds__(256) void k_sl_pp_lhat(DevPlan P, const int32_t* __
                                                     const double* __restrict__ Z,
                                                     double* __restrict__ S, double* __restrict__ W,
                                                     double* __restrict__ T) {
-  __shared__ double As[64][kPanel + 1];
-  __shared__ double Bs[64][kPanel + 1];
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
   __shared__ double dinv[kTile];
   if ((int)blockIdx.x < npp)
-    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, As, Bs, dinv);
+    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, sh, dinv);
   else
-    sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp), step + 1, blockIdx.y, L, Z, T, As, Bs);
+    sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp), step + 1, blockIdx.y, L, Z, T, sh);
 }
 
 // Sigma[B,P] split over K for launches with few row blocks (tall fronts of a small
@@ -1841,12 +1839,10 @@ __global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* 
   mma_tile<false, true, 512>(
       [&](int ii, int kk) {
         const int i = R0 + ii;
-        const double v = Ws[(int64_t)(c1 + ks + kk) * m + min(i, m - 1)];
-        return i < m ? v : 0.0;
+        return LV{Ws[(int64_t)(c1 + ks + kk) * m + min(i, m - 1)], i < m};
       },
       [&](int jj, int kk) {
-        const double v = Lh[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk];
-        return jj < nb ? v : 0.0;
+        return LV{Lh[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk], jj < nb};
       },
       ke - ks, As, Bs, acc);
   double* Pg = Tb + stage_gemm_off(m) + ((int64_t)I * parts + part) * kTile * kTile;
@@ -1928,26 +1924,26 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
     __syncthreads();
     mma_tile<true, true>(
         [&](int pp, int k) {
-          const double x = Zb[pp * kTile + k] * dinv[k];
-          return pp < nb ? x : 0.0;
+          return LVs{Zb[pp * kTile + k], dinv[k], pp < nb};
         },
         [&](int qq, int k) {
-          const double x = Zb[qq * kTile + k];
-          return qq < nb ? x : 0.0;
+          return LV{Zb[qq * kTile + k], qq < nb};
         },
         nb, As, Bs, acc);
   }
-  if (ke > ks)
+  if (ke > ks) {
+    // acc - Lh^T Sigma[B,P] over this part, as -(-acc + Lh^T Sigma[B,P])
+    tile_negate(acc);
     mma_tile<true, true>(
         [&](int ii, int kk) {
-          const double v = Tb[stage_lhat_off(m, step) + (int64_t)min(ii, nb - 1) * m + c1 + ks + kk];
-          return ii < nb ? -v : 0.0;
+          return LV{Tb[stage_lhat_off(m, step) + (int64_t)min(ii, nb - 1) * m + c1 + ks + kk], ii < nb};
         },
         [&](int jj, int kk) {
-          const double v = Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + ks + kk];
-          return jj < nb ? v : 0.0;
+          return LV{Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + ks + kk], jj < nb};
         },
         ke - ks, As, Bs, acc);
+    tile_negate(acc);
+  }
   double* Pp = Tb + stage_pp_off(m) + (int64_t)part * kTile * kTile;
   tile_foreach(acc, [&](int ii, int jj, double v) { Pp[jj * kTile + ii] = v; });
 }
@@ -2233,14 +2229,6 @@ hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int
                           double* S, double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(512), 0, st, P, tasks, step, S, W, T);
-  return hipGetLastError();
-}
-
-hipError_t launch_sl_pp(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                        const double* L, const double* Z, double* S, double* W, const double* T,
-                        hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_pp, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, S, W, T);
   return hipGetLastError();
 }
 

@@ -36,8 +36,6 @@ hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const d
                           const double*, double*, hipStream_t);
 hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                           const double*, hipStream_t);
-hipError_t launch_sl_pp(const DevPlan&, const int32_t*, int, int, int, const double*,
-                        const double*, double*, double*, const double*, hipStream_t);
 hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int,
                              const double*, const double*, double*, double*, double*, hipStream_t);
 hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
@@ -971,17 +969,14 @@ struct gmrf_batch {
                 return launch_sl_lhat(P, tk + 3 * lv.lhat[k + 1].off, (int)lv.lhat[k + 1].cnt, (int)k + 1,
                                       count, Lf, Zf, Tf, stream);
               });
-          } else if (next) {
-            prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + lv.lhat[k + 1].flops) * count, 0.0, stream, [&] {
-              return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt,
-                                       tk + 3 * lv.lhat[k + 1].off, (int)lv.lhat[k + 1].cnt, (int)k, count,
-                                       Lf, Zf, Sf, Wf, Tf, stream);
-            });
           } else {
-            prof_launch("k_sl_pp", lv.spp[k].flops * count, 0.0, stream, [&] {
-              return launch_sl_pp(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Lf, Zf,
-                                  Sf, Wf, Tf, stream);
-            });
+            const int nl = next ? (int)lv.lhat[k + 1].cnt : 0;
+            const int32_t* tl = tk + 3 * (next ? lv.lhat[k + 1].off : 0);
+            prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + (next ? lv.lhat[k + 1].flops : 0.0)) * count, 0.0,
+                        stream, [&] {
+                          return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, tl, nl, (int)k,
+                                                   count, Lf, Zf, Sf, Wf, Tf, stream);
+                        });
           }
         }
       }
