@@ -652,12 +652,15 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
 
 // Assembly of one 64x64 tile (tile row I, tile column J, I >= J) of a front:
 // the sum of every child's update through the child's inverse row map (children
-// in fixed order: deterministic); the input entries are added afterwards by
-// k_fl_input. Thread (row i = tid % 64) owns tile columns tid / 64 + 4u, u < 16;
-// each round of loads covers all 16 elements before any is used.
+// in fixed order: deterministic), then the input entries that fall into the tile
+// (every L slot receives at most one). Thread (row i = tid % 64) owns tile
+// columns tid / 64 + 4u, u < 16; each round of loads covers all 16 elements
+// before any is used. The tile passes through LDS for the input entries.
 __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
-                                                     double* __restrict__ L,
+                                                     int ain_first, const double* __restrict__ vals,
+                                                     int64_t ldv, double* __restrict__ L,
                                                      double* __restrict__ U) {
+  __shared__ double Ft[kTile * (kTile + 1)];  // Ft[65 jj + ii]
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1], J = tk[2];
   const int b = blockIdx.y;
@@ -665,7 +668,8 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
   const double* Um = U + (int64_t)b * P.u_size;
-  const int i = I * kTile + (threadIdx.x & 63), jb = J * kTile + (threadIdx.x >> 6);
+  const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
+  const int i = I * kTile + ii, j0 = J * kTile + jb;
   const int ic = min(i, m - 1);
   double v[16];
 #pragma unroll
@@ -678,7 +682,7 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
     const int ci = inv[ic];
     int cj[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) cj[u] = inv[min(jb + 4 * u, m - 1)];
+    for (int u = 0; u < 16; ++u) cj[u] = inv[min(j0 + 4 * u, m - 1)];
     double x[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -689,40 +693,21 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
 #pragma unroll
     for (int u = 0; u < 16; ++u) v[u] += (ci >= 0 && cj[u] >= 0) ? x[u] : 0.0;
   }
+  const int e0 = P.ain_ptr[ain_first + blockIdx.x], e1 = P.ain_ptr[ain_first + blockIdx.x + 1];
+  if (e1 > e0) {
+    const double* vb = vals + (int64_t)b * ldv;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ft[(jb + 4 * u) * (kTile + 1) + ii] = v[u];
+    __syncthreads();
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) Ft[P.ain_pos[e]] += vb[P.ain_csc[e]];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = Ft[(jb + 4 * u) * (kTile + 1) + ii];
+  }
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int j = jb + 4 * u;
+    const int j = j0 + 4 * u;
     if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) = v[u];
-  }
-}
-
-// Input entries of the large fronts of a level, added after the children's
-// updates: one workgroup per (front, matrix); every L slot receives at most one
-// entry (the lower-triangle map of the input pattern), so the adds do not race.
-__global__ __launch_bounds__(256) void k_fl_input(DevPlan P, const int32_t* __restrict__ list,
-                                                  const double* __restrict__ vals, int64_t ldv,
-                                                  double* __restrict__ L) {
-  const int s = list[blockIdx.x];
-  const int b = blockIdx.y;
-  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
-  const double* vb = vals + (int64_t)b * ldv;
-  const int64_t q0 = P.fa_ptr[s], q1 = P.fa_ptr[s + 1];
-  for (int64_t qb = q0; qb < q1; qb += 4 * 256) {
-    int slot[4];
-    double v[4], old[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t q = min(qb + threadIdx.x + 256 * u, q1 - 1);
-      slot[u] = P.fa_slot[q];
-      v[u] = vb[P.fa_csc[q]];
-    }
-    ISSUE_FENCE();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) old[u] = Lb[slot[u]];
-    ISSUE_FENCE();
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (qb + threadIdx.x + 256 * u < q1) Lb[slot[u]] = old[u] + v[u];
   }
 }
 
@@ -1572,14 +1557,11 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
 
 // Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into W (both triangles; the
 // transposed copy is staged through LDS so both stores are coalesced).
-__global__ __launch_bounds__(256) void k_sl_gather(DevPlan P, const int32_t* __restrict__ tasks,
-                                                   const double* __restrict__ S,
-                                                   const double* __restrict__ U,
-                                                   double* __restrict__ W) {
-  __shared__ double Ct[kTile][kTile + 1];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+__device__ __forceinline__ void sl_gather_task(const DevPlan& P, const int32_t* tk, int b,
+                                               const double* __restrict__ S,
+                                               const double* __restrict__ U, double* __restrict__ W,
+                                               double (*Ct)[kTile + 1]) {
   const int s = tk[0], I = tk[1], J = tk[2];
-  const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const int p = P.sn_parent[s];
   const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
@@ -1679,6 +1661,26 @@ __global__ __launch_bounds__(256) void k_sl_lhat(DevPlan P, const int32_t* __res
                                                  double* __restrict__ T) {
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   sl_lhat_task(P, tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, T, sh);
+}
+
+// The gather of a level and the L_hat of its first panel step in one launch: the
+// first ng workgroups gather Sigma_{R R} tiles (they write the R x R part of W),
+// the rest form L_hat (from L and Z into the staging area); the two sets touch
+// disjoint memory.
+__global__ __launch_bounds__(256) void k_sl_gather_lhat(DevPlan P, const int32_t* __restrict__ g_tasks,
+                                                        int ng, const int32_t* __restrict__ l_tasks,
+                                                        const double* __restrict__ S,
+                                                        const double* __restrict__ U,
+                                                        double* __restrict__ W,
+                                                        const double* __restrict__ L,
+                                                        const double* __restrict__ Z,
+                                                        double* __restrict__ T) {
+  static_assert(kTile * (kTile + 1) <= 2 * 64 * (kPanel + 1), "gather tile exceeds the LDS block");
+  __shared__ double sh[2 * 64 * (kPanel + 1)];
+  if ((int)blockIdx.x < ng)
+    sl_gather_task(P, g_tasks + 3 * blockIdx.x, blockIdx.y, S, U, W, (double (*)[kTile + 1])sh);
+  else
+    sl_lhat_task(P, l_tasks + 3 * (blockIdx.x - ng), 0, blockIdx.y, L, Z, T, sh);
 }
 
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
@@ -2027,17 +2029,10 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
   return hipGetLastError();
 }
 
-hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int B, double* L,
-                              double* U, hipStream_t st) {
+hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int ain_first, int B,
+                              const double* vals, int64_t ldv, double* L, double* U, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, L, U);
-  return hipGetLastError();
-}
-
-hipError_t launch_fl_input(const DevPlan& P, const int32_t* list, int count, int B, const double* vals,
-                           int64_t ldv, double* L, hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_input, dim3(count, B), dim3(256), 0, st, P, list, vals, ldv, L);
+  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, ain_first, vals, ldv, L, U);
   return hipGetLastError();
 }
 
@@ -2211,10 +2206,12 @@ hipError_t launch_selinv_small(const DevPlan& P, const int32_t* list, int count,
   return hipGetLastError();
 }
 
-hipError_t launch_sl_gather(const DevPlan& P, const int32_t* tasks, int count, int B,
-                            const double* S, const double* U, double* W, hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_gather, dim3(count, B), dim3(256), 0, st, P, tasks, S, U, W);
+hipError_t launch_sl_gather_lhat(const DevPlan& P, const int32_t* g_tasks, int ng, const int32_t* l_tasks,
+                                int nl, int B, const double* S, const double* U, double* W, const double* L,
+                                const double* Z, double* T, hipStream_t st) {
+  if (ng + nl <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sl_gather_lhat, dim3(ng + nl, B), dim3(256), 0, st, P, g_tasks, ng, l_tasks, S, U, W,
+                     L, Z, T);
   return hipGetLastError();
 }
 

@@ -29,6 +29,12 @@ struct DevPlan {
   const int64_t* fa_ptr;     // [nsuper+1] input entries assembled into supernode s
   const int32_t* fa_csc;     // CSC entry index
   const int32_t* fa_slot;    // slot j*m+i inside the L block of s
+  // input entries of the large fronts grouped by assembly tile: tile t of the
+  // assembly task list holds entries ain_ptr[t] .. ain_ptr[t + 1], each at LDS
+  // position ain_pos = 65 (j - 64 J) + (i - 64 I) of the tile, value vals[ain_csc]
+  const int32_t* ain_ptr;
+  const int32_t* ain_pos;
+  const int32_t* ain_csc;
   const int32_t* perm;       // [n] original index of factor position k
   const int64_t* diag_slot;  // [n] L-layout slot of D_k
   const int64_t* slot_full;  // [nnz] L-layout slot of the (symmetric) CSC entry

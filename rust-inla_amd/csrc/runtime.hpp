@@ -20,18 +20,17 @@
 namespace gmrf {
 hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                int64_t, double*, double*, hipStream_t);
-hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, double*, double*,
+hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, int, const double*, int64_t, double*, double*,
                               hipStream_t);
-hipError_t launch_fl_input(const DevPlan&, const int32_t*, int, int, const double*, int64_t, double*,
-                           hipStream_t);
 hipError_t launch_fl_diag(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                           hipStream_t);
 hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
                           hipStream_t);
 hipError_t launch_fl_update(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                             hipStream_t);
-hipError_t launch_sl_gather(const DevPlan&, const int32_t*, int, int, const double*, const double*,
-                            double*, hipStream_t);
+hipError_t launch_sl_gather_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int,
+                                const double*, const double*, double*, const double*, const double*, double*,
+                                hipStream_t);
 hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const double*,
                           const double*, double*, hipStream_t);
 hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, double*,
@@ -197,6 +196,7 @@ struct LevelInfo {
     double flops = 0, bytes = 0;
   };
   Range asm_t, gather;
+  int64_t ain_first = 0;  // the level's first assembly tile in the input-entry grouping
   Range frow, brow;  // solve: rows below the large supernodes (forward, backward)
   bool sol_split = false;  // tall fronts: the rows below are separate row-parallel launches
   double sol_tri = 0;  // solve work of the large supernodes' own blocks (per matrix)
@@ -491,11 +491,12 @@ struct SymData {
     d_small = arena.upload(small);
     d_large = arena.upload(large);
     d_all = arena.upload(all);
-    build_large_plan(large, fa_ptr);
+    build_large_plan(large, fa_ptr, fa_csc, fa_slot);
   }
 
   // Task lists, inverse child maps and Z offsets of the large fronts.
-  void build_large_plan(const std::vector<int32_t>& large, const std::vector<int64_t>& fa_ptr) {
+  void build_large_plan(const std::vector<int32_t>& large, const std::vector<int64_t>& fa_ptr,
+                        const std::vector<int32_t>& fa_csc, const std::vector<int32_t>& fa_slot) {
     const int64_t ns = S.nsuper;
     auto mw = [&](int64_t s, int64_t& w, int64_t& r, int64_t& m) {
       w = S.sn_start[s + 1] - S.sn_start[s];
@@ -551,6 +552,7 @@ struct SymData {
       tk.push_back((int32_t)b);
     };
     auto ceil64 = [](int64_t x) { return (x + kTile - 1) / kTile; };
+    std::vector<int32_t> ain_ptr(1, 0), ain_pos, ain_csc;
     for (LevelInfo& L : levels) {
       std::vector<int64_t> fronts;
       for (int64_t q = L.lb; q < L.le; ++q) fronts.push_back(large[q]);
@@ -609,12 +611,29 @@ struct SymData {
         L.asm_t.bytes += 8.0 * (double)(m * w) + 12.0 * (double)(fa_ptr[s + 1] - fa_ptr[s]);
       }
       L.asm_t.off = (int64_t)tk.size() / 3;
+      L.ain_first = (int64_t)ain_ptr.size() - 1;
       for (int64_t s : fronts) {
         int64_t w, r, m;
         mw(s, w, r, m);
         const int64_t nt = ceil64(m);
+        // the front's input entries by tile, tiles in task order (J, then I >= J)
+        auto tile_of = [&](int64_t I, int64_t J) { return J * nt - J * (J - 1) / 2 + (I - J); };
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> by_tile((size_t)(nt * (nt + 1) / 2));
+        for (int64_t q = fa_ptr[s]; q < fa_ptr[s + 1]; ++q) {
+          const int64_t slot = fa_slot[q], j = slot / m, i = slot % m;
+          by_tile[(size_t)tile_of(i / kTile, j / kTile)].push_back(
+              {(int32_t)((j % kTile) * (kTile + 1) + i % kTile), fa_csc[q]});
+        }
         for (int64_t J = 0; J < nt; ++J)
-          for (int64_t I = J; I < nt; ++I) add(s, I, J);
+          for (int64_t I = J; I < nt; ++I) {
+            add(s, I, J);
+            for (const auto& e : by_tile[(size_t)tile_of(I, J)]) {
+              ain_pos.push_back(e.first);
+              ain_csc.push_back(e.second);
+            }
+            if (ain_pos.size() >= ((size_t)1 << 31)) throw std::runtime_error("too many input entries");
+            ain_ptr.push_back((int32_t)ain_pos.size());
+          }
       }
       L.asm_t.cnt = (int64_t)tk.size() / 3 - L.asm_t.off;
       for (int64_t k = 0; k < maxsteps; ++k) {
@@ -740,6 +759,9 @@ struct SymData {
       }
     }
     d_tasks = arena.upload(tk.empty() ? std::vector<int32_t>(3, 0) : tk);
+    plan.ain_ptr = arena.upload(ain_ptr);
+    plan.ain_pos = arena.upload(ain_pos.empty() ? std::vector<int32_t>(1, 0) : ain_pos);
+    plan.ain_csc = arena.upload(ain_csc.empty() ? std::vector<int32_t>(1, 0) : ain_csc);
   }
 };
 
@@ -810,11 +832,8 @@ struct gmrf_batch {
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
         prof_launch("k_fl_assemble", 0.0, lv.asm_t.bytes * count, stream, [&] {
-          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, count, Lf, Uf, stream);
-        });
-        prof_launch("k_fl_input", 0.0, 0.0, stream, [&] {
-          return launch_fl_input(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb), count, vals, ldv, Lf,
-                                 stream);
+          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, (int)lv.ain_first, count, vals,
+                                    ldv, Lf, Uf, stream);
         });
         for (size_t k = 0; k < lv.fdiag.size(); ++k) {
           prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
@@ -925,17 +944,14 @@ struct gmrf_batch {
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
-        prof_launch("k_sl_gather", 0.0, 0.0, stream, [&] {
-          return launch_sl_gather(P, tk + 3 * lv.gather.off, (int)lv.gather.cnt, count, Sf, Uf, Wf,
-                                  stream);
+        // L_hat of panel step k sits in buffer k % 2 of the staging area; the gather
+        // launch forms the L_hat of step 0, the launch that closes step k that of k + 1
+        const bool l0 = !lv.lhat.empty();
+        prof_launch("k_sl_gather_lhat", l0 ? lv.lhat[0].flops * count : 0.0, 0.0, stream, [&] {
+          return launch_sl_gather_lhat(P, tk + 3 * lv.gather.off, (int)lv.gather.cnt,
+                                       tk + 3 * (l0 ? lv.lhat[0].off : 0), l0 ? (int)lv.lhat[0].cnt : 0, count,
+                                       Sf, Uf, Wf, Lf, Zf, Tf, stream);
         });
-        // L_hat of panel step k sits in buffer k % 2 of the staging area; the L_hat of
-        // step k + 1 is formed by the launch that closes step k
-        if (!lv.lhat.empty())
-          prof_launch("k_sl_lhat", lv.lhat[0].flops * count, 0.0, stream, [&] {
-            return launch_sl_lhat(P, tk + 3 * lv.lhat[0].off, (int)lv.lhat[0].cnt, 0, count, Lf, Zf, Tf,
-                                  stream);
-          });
         for (size_t k = 0; k < lv.lhat.size(); ++k) {
           const bool next = k + 1 < lv.lhat.size() && lv.lhat[k + 1].cnt > 0;
           // few row blocks with a long K (the matrices of tall fronts, a small batch):
