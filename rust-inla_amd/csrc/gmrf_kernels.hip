@@ -650,17 +650,64 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
 // (supernode, a, b) whose meaning depends on the kernel.
 // ---------------------------------------------------------------------------
 
+// LDS of one diagonal-block factorisation: F and Zs (64 x 65 each), Zk (16 x 17),
+// Wb (64 x 17).
+constexpr int kDiagLds = 2 * kTile * (kTile + 1) + 16 * 17 + kTile * 17;
+
+__device__ __forceinline__ void fl_diag_block(const DevPlan& P, int s, int step, int b,
+                                              double* __restrict__ L, double* __restrict__ Z,
+                                              double* lds) {
+  constexpr int LD = kTile + 1;
+  double* F = lds;
+  double* Zs = F + kTile * LD;
+  double* Zk = Zs + kTile * LD;
+  double* Wb = Zk + 16 * 17;
+  const int tid = threadIdx.x;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int c0 = step * kTile, nb = min(kTile, w - c0);
+  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
+  const int i = tid & 63, c = tid >> 6;
+  {
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) v[t] = Lb[(int64_t)(c0 + min(c + 4 * t, nb - 1)) * m + c0 + min(i, nb - 1)];
+    ISSUE_FENCE();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int j = c + 4 * t;
+      F[i * LD + j] = (i < nb && j <= i) ? v[t] : 0.0;
+    }
+  }
+  __syncthreads();
+  front_ldlt<kTile, true, true>(F, nb, nb, Zs, Zk, Wb);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int j = c + 4 * t;
+    if (i < nb && j < nb) {
+      Lb[(int64_t)(c0 + j) * m + c0 + i] = (j <= i) ? F[i * LD + j] : 0.0;
+      Zb[j * kTile + i] = (j <= i) ? Zs[i * LD + j] : 0.0;
+    }
+  }
+}
+
 // Assembly of one 64x64 tile (tile row I, tile column J, I >= J) of a front:
 // the sum of every child's update through the child's inverse row map (children
 // in fixed order: deterministic), then the input entries that fall into the tile
 // (every L slot receives at most one). Thread (row i = tid % 64) owns tile
 // columns tid / 64 + 4u, u < 16; each round of loads covers all 16 elements
 // before any is used. The tile passes through LDS for the input entries.
+//
+// With DIAG (launches of few tiles), the workgroup of tile (0, 0) of a front goes
+// on to factor the front's first diagonal block (k_fl_diag, step 0), which lies
+// inside that tile: one launch less per level.
+template <bool DIAG>
 __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
                                                      int ain_first, const double* __restrict__ vals,
                                                      int64_t ldv, double* __restrict__ L,
-                                                     double* __restrict__ U) {
-  __shared__ double Ft[kTile * (kTile + 1)];  // Ft[65 jj + ii]
+                                                     double* __restrict__ U, double* __restrict__ Z) {
+  __shared__ double lds[DIAG ? kDiagLds : kTile * (kTile + 1)];
+  double* Ft = lds;  // Ft[65 jj + ii]
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1], J = tk[2];
   const int b = blockIdx.y;
@@ -709,6 +756,10 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
     const int j = j0 + 4 * u;
     if (i < m && j < m && i >= j) *front_at(Lb, Ub, m, w, r, i, j) = v[u];
   }
+  if (DIAG && I == 0 && J == 0) {
+    __syncthreads();  // the tile's stores, before the block is read back
+    fl_diag_block(P, s, 0, b, L, Z, lds);
+  }
 }
 
 // Diagonal block of panel step `step` (columns [64 step, 64 step + nb)), one
@@ -718,41 +769,8 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
 __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, double* __restrict__ L,
                                                  double* __restrict__ Z) {
-  constexpr int LD = kTile + 1;
-  __shared__ double F[kTile * LD];
-  __shared__ double Zs[kTile * LD];
-  __shared__ double Zk[16 * 17];
-  __shared__ double Wb[kTile * 17];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
-  const int s = tk[0];
-  const int b = blockIdx.y;
-  const int tid = threadIdx.x;
-  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
-  const int c0 = step * kTile, nb = min(kTile, w - c0);
-  double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
-  double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
-  const int i = tid & 63, c = tid >> 6;
-  {
-    double v[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) v[t] = Lb[(int64_t)(c0 + min(c + 4 * t, nb - 1)) * m + c0 + min(i, nb - 1)];
-    ISSUE_FENCE();
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int j = c + 4 * t;
-      F[i * LD + j] = (i < nb && j <= i) ? v[t] : 0.0;
-    }
-  }
-  __syncthreads();
-  front_ldlt<kTile, true, true>(F, nb, nb, Zs, Zk, Wb);
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const int j = c + 4 * t;
-    if (i < nb && j < nb) {
-      Lb[(int64_t)(c0 + j) * m + c0 + i] = (j <= i) ? F[i * LD + j] : 0.0;
-      Zb[j * kTile + i] = (j <= i) ? Zs[i * LD + j] : 0.0;
-    }
-  }
+  __shared__ double lds[kDiagLds];
+  fl_diag_block(P, tasks[3 * blockIdx.x], step, blockIdx.y, L, Z, lds);
 }
 
 // Panel solve as a product: task (s, I) computes 64 rows below the diagonal block,
@@ -793,10 +811,16 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
 // Trailing update of panel step `step`: task (s, I, J), I >= J, is the 64x64
 // tile at rows t0 + 64 I, columns t0 + 64 J of the trailing lower triangle,
 // t0 = 64 step + nb:  C -= L_I D L_J^T on FP64 MFMA.
+//
+// With DIAG (launches of few tiles), the workgroup of tile (0, 0) goes on to
+// factor the front's next diagonal block (k_fl_diag, step + 1), which is that
+// tile, when the supernode has a next panel.
+template <bool DIAG>
 __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __restrict__ tasks,
                                                    int step, double* __restrict__ L,
-                                                   double* __restrict__ U) {
-  __shared__ double sh[2 * 64 * (kPanel + 1)];  // the tile engine's operands, then the tile
+                                                   double* __restrict__ U, double* __restrict__ Z) {
+  static_assert(2 * 64 * (kPanel + 1) <= kDiagLds, "diagonal block LDS below the engine's");
+  __shared__ double sh[DIAG ? kDiagLds : 2 * 64 * (kPanel + 1)];  // the tile engine's operands, then the tile
   double (*As)[kPanel + 1] = reinterpret_cast<double (*)[kPanel + 1]>(sh);
   double (*Bs)[kPanel + 1] = reinterpret_cast<double (*)[kPanel + 1]>(sh + 64 * (kPanel + 1));
   __shared__ double dpiv[kTile];
@@ -849,6 +873,10 @@ __global__ __launch_bounds__(256) void k_fl_update(DevPlan P, const int32_t* __r
     const int j = C0 + jb + 4 * u;
     const double nv = old[u] - Ct[jb + 4 * u][ii];
     if (i < m && j < m && i >= j) (j >= w ? Ub : Lb)[off[u]] = nv;
+  }
+  if (DIAG && I == 0 && J == 0 && t0 < w) {
+    __syncthreads();  // the tile's stores, before the block is read back
+    fl_diag_block(P, s, step + 1, b, L, Z, sh);
   }
 }
 
@@ -2030,9 +2058,15 @@ hipError_t launch_factor_small(const DevPlan& P, const int32_t* list, int count,
 }
 
 hipError_t launch_fl_assemble(const DevPlan& P, const int32_t* tasks, int count, int ain_first, int B,
-                              const double* vals, int64_t ldv, double* L, double* U, hipStream_t st) {
+                              bool diag, const double* vals, int64_t ldv, double* L, double* U, double* Z,
+                              hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_assemble, dim3(count, B), dim3(256), 0, st, P, tasks, ain_first, vals, ldv, L, U);
+  if (diag)
+    hipLaunchKernelGGL(k_fl_assemble<true>, dim3(count, B), dim3(256), 0, st, P, tasks, ain_first, vals, ldv, L,
+                       U, Z);
+  else
+    hipLaunchKernelGGL(k_fl_assemble<false>, dim3(count, B), dim3(256), 0, st, P, tasks, ain_first, vals, ldv,
+                       L, U, Z);
   return hipGetLastError();
 }
 
@@ -2050,10 +2084,13 @@ hipError_t launch_fl_trsm(const DevPlan& P, const int32_t* tasks, int count, int
   return hipGetLastError();
 }
 
-hipError_t launch_fl_update(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                            double* L, double* U, hipStream_t st) {
+hipError_t launch_fl_update(const DevPlan& P, const int32_t* tasks, int count, int step, int B, bool diag,
+                            double* L, double* U, double* Z, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_update, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, U);
+  if (diag)
+    hipLaunchKernelGGL(k_fl_update<true>, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, U, Z);
+  else
+    hipLaunchKernelGGL(k_fl_update<false>, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, U, Z);
   return hipGetLastError();
 }
 

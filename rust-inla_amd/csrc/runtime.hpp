@@ -20,13 +20,14 @@
 namespace gmrf {
 hipError_t launch_factor_small(const DevPlan&, const int32_t*, int, int, int, const double*,
                                int64_t, double*, double*, hipStream_t);
-hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, int, const double*, int64_t, double*, double*,
+hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, int, bool, const double*, int64_t, double*,
+                              double*, double*,
                               hipStream_t);
 hipError_t launch_fl_diag(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                           hipStream_t);
 hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
                           hipStream_t);
-hipError_t launch_fl_update(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+hipError_t launch_fl_update(const DevPlan&, const int32_t*, int, int, int, bool, double*, double*, double*,
                             hipStream_t);
 hipError_t launch_sl_gather_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int,
                                 const double*, const double*, double*, const double*, const double*, double*,
@@ -831,23 +832,32 @@ struct gmrf_batch {
           });
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
-        prof_launch("k_fl_assemble", 0.0, lv.asm_t.bytes * count, stream, [&] {
-          return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, (int)lv.ain_first, count, vals,
-                                    ldv, Lf, Uf, stream);
-        });
+        // launches of few tiles also factor the next diagonal blocks (one launch
+        // less per panel step); large launches keep their occupancy
+        auto fuse = [&](int64_t tiles) { return tiles * count < kFuseDiagBelow; };
+        bool diag_done = fuse(lv.asm_t.cnt);
+        prof_launch("k_fl_assemble", diag_done && !lv.fdiag.empty() ? lv.fdiag[0].flops * count : 0.0,
+                    lv.asm_t.bytes * count, stream, [&] {
+                      return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, (int)lv.ain_first,
+                                                count, diag_done, vals, ldv, Lf, Uf, Zf, stream);
+                    });
         for (size_t k = 0; k < lv.fdiag.size(); ++k) {
-          prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
-            return launch_fl_diag(P, tk + 3 * lv.fdiag[k].off, (int)lv.fdiag[k].cnt, (int)k, count,
-                                  Lf, Zf, stream);
-          });
+          if (!diag_done)
+            prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
+              return launch_fl_diag(P, tk + 3 * lv.fdiag[k].off, (int)lv.fdiag[k].cnt, (int)k, count,
+                                    Lf, Zf, stream);
+            });
           prof_launch("k_fl_trsm", lv.ftrsm[k].flops * count, 0.0, stream, [&] {
             return launch_fl_trsm(P, tk + 3 * lv.ftrsm[k].off, (int)lv.ftrsm[k].cnt, (int)k, count,
                                   Lf, Zf, stream);
           });
-          prof_launch("k_fl_update", lv.fupd[k].flops * count, 0.0, stream, [&] {
-            return launch_fl_update(P, tk + 3 * lv.fupd[k].off, (int)lv.fupd[k].cnt, (int)k, count,
-                                    Lf, Uf, stream);
-          });
+          const bool next = k + 1 < lv.fdiag.size();
+          diag_done = next && fuse(lv.fupd[k].cnt);
+          prof_launch("k_fl_update", (lv.fupd[k].flops + (diag_done ? lv.fdiag[k + 1].flops : 0.0)) * count, 0.0,
+                      stream, [&] {
+                        return launch_fl_update(P, tk + 3 * lv.fupd[k].off, (int)lv.fupd[k].cnt, (int)k, count,
+                                                diag_done, Lf, Uf, Zf, stream);
+                      });
         }
       }
     }
