@@ -574,24 +574,35 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
                                                          const double* __restrict__ vals, int64_t ldv,
                                                          double* __restrict__ L,
                                                          double* __restrict__ U) {
-  constexpr int NT = 4 * MB, LD = MB + 1;
+  constexpr int NT = 4 * MB, LD = MB + 1, NCM = 32;
   __shared__ double F[MB * LD];
   __shared__ double Zk[16 * 17];
   __shared__ double Wb[MB * 17];
+  __shared__ int64_t ch_uoff[NCM], ch_rp[NCM];  // the first NCM children's U offset and row list
+  __shared__ int ch_rc[NCM];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int cp0 = P.child_ptr[s], cp1 = P.child_ptr[s + 1];
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
   const double* Um = U + (int64_t)b * P.u_size;
   const double* vb = vals + (int64_t)b * ldv;
-  for (int idx = tid; idx < m * LD; idx += NT) F[idx] = 0.0;
-  __syncthreads();
   const int64_t q0 = P.fa_ptr[s], q1 = P.fa_ptr[s + 1];
-  for (int64_t qb = q0; qb < q1; qb += 4 * NT) {
-    int slot[4];
-    double v[4];
+  // the children's metadata (for the child loop below) and the first round of input
+  // entries are loaded together, before the front is cleared
+  int cm_rc = 0;
+  int64_t cm_uoff = 0, cm_rp = 0;
+  if (tid < min(cp1 - cp0, NCM)) {
+    const int c = P.child_idx[cp0 + tid];
+    cm_rc = P.sn_r[c];
+    cm_uoff = P.u_off[c];
+    cm_rp = P.rows_ptr[c];
+  }
+  int slot[4];
+  double v[4];
+  auto load_inputs = [&](int64_t qb) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t q = min(qb + tid + NT * u, q1 - 1);
@@ -599,6 +610,17 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
       v[u] = vb[P.fa_csc[q]];
     }
     ISSUE_FENCE();
+  };
+  if (q1 > q0) load_inputs(q0);
+  for (int idx = tid; idx < m * LD; idx += NT) F[idx] = 0.0;
+  if (tid < min(cp1 - cp0, NCM)) {
+    ch_rc[tid] = cm_rc;
+    ch_uoff[tid] = cm_uoff;
+    ch_rp[tid] = cm_rp;
+  }
+  __syncthreads();
+  for (int64_t qb = q0; qb < q1; qb += 4 * NT) {
+    if (qb > q0) load_inputs(qb);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (qb + tid + NT * u < q1) {
@@ -607,11 +629,22 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
       }
   }
   __syncthreads();
-  for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
-    const int c = P.child_idx[cq];
-    const int rc = P.sn_r[c];
-    const double* Uc = Um + P.u_off[c];
-    const int32_t* relc = P.rel + P.rows_ptr[c];
+  for (int cq = cp0; cq < cp1; ++cq) {
+    const int k = cq - cp0;
+    int rc;
+    int64_t uo, rp;
+    if (k < NCM) {
+      rc = ch_rc[k];
+      uo = ch_uoff[k];
+      rp = ch_rp[k];
+    } else {
+      const int c = P.child_idx[cq];
+      rc = P.sn_r[c];
+      uo = P.u_off[c];
+      rp = P.rows_ptr[c];
+    }
+    const double* Uc = Um + uo;
+    const int32_t* relc = P.rel + rp;
     const int ne = rc * rc;
     for (int e0 = 0; e0 < ne; e0 += 4 * NT) {
       double v[4];
@@ -706,26 +739,48 @@ __global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* _
                                                      int ain_first, const double* __restrict__ vals,
                                                      int64_t ldv, double* __restrict__ L,
                                                      double* __restrict__ U, double* __restrict__ Z) {
+  constexpr int NCM = 16;
   __shared__ double lds[DIAG ? kDiagLds : kTile * (kTile + 1)];
+  __shared__ int64_t ch_inv[NCM], ch_uoff[NCM];  // the first NCM children's inverse map and U offset
+  __shared__ int ch_rc[NCM];
   double* Ft = lds;  // Ft[65 jj + ii]
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1], J = tk[2];
   const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int cp0 = P.child_ptr[s], cp1 = P.child_ptr[s + 1];
   double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   double* Ub = U + (int64_t)b * P.u_size + P.u_off[s];
   const double* Um = U + (int64_t)b * P.u_size;
   const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
   const int i = I * kTile + ii, j0 = J * kTile + jb;
   const int ic = min(i, m - 1);
+  if ((int)threadIdx.x < min(cp1 - cp0, NCM)) {
+    const int cq = cp0 + threadIdx.x, c = P.child_idx[cq];
+    ch_inv[threadIdx.x] = P.cinv_off[cq];
+    ch_uoff[threadIdx.x] = P.u_off[c];
+    ch_rc[threadIdx.x] = P.sn_r[c];
+  }
+  __syncthreads();
   double v[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) v[u] = 0.0;
-  for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
-    const int32_t* inv = P.cinv + P.cinv_off[cq];
-    const int c = P.child_idx[cq];
-    const int rc = P.sn_r[c];
-    const double* Uc = Um + P.u_off[c];
+  for (int cq = cp0; cq < cp1; ++cq) {
+    const int k = cq - cp0;
+    int64_t io, uo;
+    int rc;
+    if (k < NCM) {
+      io = ch_inv[k];
+      uo = ch_uoff[k];
+      rc = ch_rc[k];
+    } else {
+      const int c = P.child_idx[cq];
+      io = P.cinv_off[cq];
+      uo = P.u_off[c];
+      rc = P.sn_r[c];
+    }
+    const int32_t* inv = P.cinv + io;
+    const double* Uc = Um + uo;
     const int ci = inv[ic];
     int cj[16];
 #pragma unroll
@@ -1411,6 +1466,20 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
   double* Sm = S + (int64_t)b * P.l_size;
   double* Um = U + (int64_t)b * P.u_size;
   const double* Wm = W + (int64_t)b * P.w_size;
+  // the parent's metadata and this front's row map are loaded first, so that their
+  // latency overlaps the L block's loads
+  const int i = tid % MB, c = tid / MB;
+  const int p = max(P.sn_parent[s], 0);
+  const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
+  const bool plarge = P.is_large[p] != 0;
+  const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off_sel[p];
+  const int32_t* rl = P.rel + P.rows_ptr[s];
+  int ri = 0, rj[NJ];
+  if (r > 0) {
+    ri = rl[min(max(i - w, 0), r - 1)];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) rj[t] = rl[min(max(c + 4 * t - w, 0), r - 1)];
+  }
   // L block into LDS, row-major
   const int nl = m * w;
   for (int e0 = 0; e0 < nl; e0 += 8 * NT) {
@@ -1429,16 +1498,6 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
   // its dense Sigma front when the parent is large); the lower part is also
   // kept in this front's U workspace for the children.
   if (r > 0) {
-    const int i = tid % MB, c = tid / MB;
-    const int p = P.sn_parent[s];
-    const int wp = P.sn_start[p + 1] - P.sn_start[p], rp = P.sn_r[p], mp = wp + rp;
-    const bool plarge = P.is_large[p] != 0;
-    const int64_t woff = P.w_off[p], loff = P.l_off[p], uoff = P.u_off_sel[p];
-    const int32_t* rl = P.rel + P.rows_ptr[s];
-    const int ri = rl[min(max(i - w, 0), r - 1)];
-    int rj[NJ];
-#pragma unroll
-    for (int t = 0; t < NJ; ++t) rj[t] = rl[min(max(c + 4 * t - w, 0), r - 1)];
     double v[NJ];
 #pragma unroll
     for (int t = 0; t < NJ; ++t) {
