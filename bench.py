@@ -490,7 +490,8 @@ def main():
     avg_s = d["ms"] * 1e-3 / d["launches"]
     fl = d["flops"] / d["launches"]
     achieved = fl / avg_s
-    bound = "mfma" if name in ("k_fl_diag", "k_fl_trsm", "k_fl_update", "k_sl_gemm", "k_sl_pp", "k_sl_lhat") else "hbm"
+    bound = "mfma" if name in ("k_fl_diag", "k_fl_trsm", "k_fl_update", "k_sl_gemm", "k_sl_pp_lhat", "k_sl_lhat",
+                               "k_sl_gather_lhat", "k_sl_gemm_part", "k_sl_pp_part") else "hbm"
     traffic = traffic_per_launch(name)
     if bound == "mfma":
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",

@@ -1980,7 +1980,8 @@ __global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* _
 // Sigma[P,P] of tall fronts in two launches: task (s, part) forms the partial
 // product over rows c1 + kSelSplitK part .. of B (part 0 also adds Z^T D^-1 Z) into
 // the front's staging area after L_hat (parts x 64 x 64); k_sl_pp_sum adds the
-// parts in order and stores as k_sl_pp does. A single workgroup over K = m - c1
+// parts in order and stores as the Sigma[P,P] tasks of k_sl_pp_lhat do. A single
+// workgroup over K = m - c1
 // would walk thousands of rows serially.
 __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __restrict__ tasks,
                                                     int step, const double* __restrict__ L,
