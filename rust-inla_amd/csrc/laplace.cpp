@@ -609,7 +609,12 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     std::vector<double> hred(4 * B), halpha(B);
     std::vector<int32_t> hbad(B);
     const bool gauss_id = M->fam == 0 && M->link == 0;
-    auto newton_system = [&]() {
+    const int sel_count = want_grad ? 2 * B : (a->var ? B : 0);
+    bool sel_started = false;
+    // final_factor: every member's posterior factor of this system is its last (the
+    // prior's, from the first system, is never refactorised), so their selected
+    // inversion can run on the batch's second stream beside the rest of the evaluation
+    auto newton_system = [&](bool final_factor) {
       // eta = A x; likelihood; Q_post = Q + A^T diag(w) A; rhs = A^T g - Q x
       check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, st),
             "A x");
@@ -630,6 +635,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_constr_apply(B, M->kc, n, M->ca, M->cres, M->rhs, st, M->kappa), "kappa C^T C x");
       }
       bt->factorize(prior_pending ? 2 * B : B, M->vals, nnz, 0);
+      if (final_factor && sel_count > 0) {
+        bt->selinv_async(sel_count, 0);
+        sel_started = true;
+      }
       if (lr && prior_pending) lowrank_factor(M, B, B, true);
       prior_pending = false;
       if (lr) lowrank_factor(M, B, 0, false);
@@ -652,7 +661,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         if (lr) lowrank_factor(M, B, B, true);
         prior_pending = false;
       } else {
-        newton_system();
+        newton_system(true);
         for (int b = 0; b < B; ++b) {
           niter[b] = 1;
           if (hbad[b] || !std::isfinite(hred[b])) {
@@ -672,7 +681,9 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         bool any = false;
         for (int b = 0; b < B; ++b) any |= state[b] < 2;
         if (!any) break;
-        newton_system();
+        bool fin = true;  // no member still iterating: this system confirms every mode
+        for (int b = 0; b < B; ++b) fin &= state[b] != 0;
+        newton_system(fin);
         bool any_active = false;
         for (int b = 0; b < B; ++b) {
           if (state[b] == 1) {
@@ -798,10 +809,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     if (a->x) check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
     // ---- selected inversion, variances
     // posterior (and, for the gradient, prior) selected inverses in one call
-    if (want_grad)
-      bt->selinv(2 * B, 0);
-    else if (a->var)
-      bt->selinv(B, 0);
+    if (sel_started)
+      bt->selinv_join();
+    else if (sel_count > 0)
+      bt->selinv(sel_count, 0);
     if (a->var) {
       check(launch_diag_out(M->sym->plan, B, bt->S, a->var, n, st), "var");
       if (lr)

@@ -788,6 +788,13 @@ struct gmrf_batch {
   // matrix (grown on demand; one vector = n + v_size + t_size doubles)
   double* mrhs = nullptr;
   int64_t mrhs_vecs = 0;
+  // selected inversion on a second stream beside the main stream's solves of the
+  // same factors (selinv_async / selinv_join); while it runs, single-vector solves
+  // keep their partial sums in TS instead of the shared staging area T
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  double* TS = nullptr;
+  bool sel_inflight = false;
 
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
@@ -810,6 +817,12 @@ struct gmrf_batch {
     check(hipMemset(d_ldcount, 0, (size_t)cap * sizeof(uint32_t)), "hipMemset");
   }
   ~gmrf_batch() {
+    if (aux) {
+      (void)hipStreamSynchronize(aux);
+      (void)hipStreamDestroy(aux);
+      (void)hipEventDestroy(ev_fork);
+      (void)hipEventDestroy(ev_join);
+    }
     if (own) {
       (void)hipStreamSynchronize(own);
       (void)hipStreamDestroy(own);
@@ -817,8 +830,41 @@ struct gmrf_batch {
     if (mrhs) (void)hipFree(mrhs);
   }
 
+  // Selected inversion of matrices [first, first + count) on the second stream,
+  // after everything enqueued so far on the main stream; selinv_join makes the main
+  // stream wait for it. Until then the main stream may solve with the same factors
+  // (L and Z are only read) but must not refactorise.
+  void selinv_async(int count, int first = 0) {
+    selinv_join();
+    if (!aux) {
+      check(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking), "hipStreamCreate");
+      check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
+      TS = arena.alloc<double>((size_t)cap * sym->plan.t_size);
+    }
+    check(hipEventRecord(ev_fork, stream), "hipEventRecord");
+    check(hipStreamWaitEvent(aux, ev_fork, 0), "hipStreamWaitEvent");
+    hipStream_t main = stream;
+    stream = aux;
+    try {
+      selinv(count, first);
+    } catch (...) {
+      stream = main;
+      throw;
+    }
+    stream = main;
+    check(hipEventRecord(ev_join, aux), "hipEventRecord");
+    sel_inflight = true;
+  }
+  void selinv_join() {
+    if (!sel_inflight) return;
+    check(hipStreamWaitEvent(stream, ev_join, 0), "hipStreamWaitEvent");
+    sel_inflight = false;
+  }
+
   // Matrices [first, first + count) of the batch.
   void factorize(int count, const double* vals, int64_t ldv, int first = 0) {
+    selinv_join();
     const DevPlan& P = sym->plan;
     double* Lf = L + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
@@ -878,7 +924,7 @@ struct gmrf_batch {
     if (nr == 1) {
       Xf = X + (int64_t)first * P.n;
       Vf = V + (int64_t)first * vsz;
-      Tf = T + (int64_t)first * P.t_size;
+      Tf = (sel_inflight ? TS : T) + (int64_t)first * P.t_size;
     } else {
       const int64_t vecs = (int64_t)count * nr;
       if (vecs > mrhs_vecs) {
@@ -943,6 +989,7 @@ struct gmrf_batch {
   }
 
   void selinv(int count, int first = 0) {
+    if (stream != aux) selinv_join();
     const DevPlan& P = sym->plan;
     const double* Lf = L + (int64_t)first * P.l_size;
     double* Sf = S + (int64_t)first * P.l_size;
