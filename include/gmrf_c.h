@@ -257,6 +257,11 @@ typedef struct gmrf_eval_args {
 gmrf_status gmrf_model_create(const gmrf_model_desc* desc, gmrf_model** out);
 void gmrf_model_destroy(gmrf_model* model);
 gmrf_status gmrf_model_eval(gmrf_model* model, gmrf_eval_args* args);
+/* The model's stream (or the caller's, after set_stream). gmrf_model_eval orders its
+ * work after what is already enqueued there and leaves its device outputs ordered
+ * before what follows; inside, the selected inversion of the final factors runs on a
+ * second stream of the model (default priority) that forks from and joins back into
+ * this one with events. */
 void* gmrf_model_stream(gmrf_model* model);
 gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_given);
 
