@@ -1331,7 +1331,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] += lv[u] * xs;
   }
-  double* Tb = T + (int64_t)b * P.t_size + P.t_off[s] + (int64_t)part * w;
+  double* Tb = T + (int64_t)b * P.ts_size + P.ts_off[s] + (int64_t)part * w;
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     double a = acc[u];
@@ -1364,7 +1364,7 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  const double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
+  const double* Tb = T + (int64_t)b * P.ts_size + P.ts_off[s];
   const int mr = FULL ? m : w;
   if (FULL) {
     const int32_t* rows = P.rows + P.rows_ptr[s];

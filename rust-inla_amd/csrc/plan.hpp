@@ -43,8 +43,10 @@ struct DevPlan {
   const int32_t* is_large;    // [nsuper]
   const int64_t* w_off;       // [nsuper] dense m x m Sigma front (large fronts, pooled), -1 otherwise
   int64_t w_size;             // doubles of the Sigma-front pool per matrix
-  const int64_t* t_off;       // [nsuper] m x 64 panel staging L_hat (large fronts, pooled)
-  int64_t t_size;             // doubles of the staging pool per matrix
+  const int64_t* t_off;       // [nsuper] selected-inversion staging (large fronts, per level)
+  int64_t t_size;             // doubles of the selected-inversion staging per matrix
+  const int64_t* ts_off;      // [nsuper] backward-solve partial sums (large fronts, per level)
+  int64_t ts_size;            // doubles of the solve staging per right-hand side
   const int64_t* z_off;       // [nsuper] per-panel 64 x 64 blocks Z = L11^-1 (large fronts)
   int64_t z_size;             // doubles of the Z workspace per matrix
   const int64_t* cinv_off;    // [n children] per (front, child) slot: offset into cinv
@@ -67,7 +69,8 @@ constexpr int kGemmSplitMaxTasks = 96;  // split Sigma[B,P] when a launch has at
 // Staging area of a large front (T, per level): L_hat of two panels (m x 64 each,
 // panel step k in buffer k % 2, so that the next panel's L_hat can be formed while
 // the current one is still read), the Sigma[P,P] partial products, then the
-// Sigma[B,P] partial tiles.
+// Sigma[B,P] partial tiles. The backward solves stage elsewhere (ts_off: ceil(r /
+// kSolvePart) x w per large front and right-hand side).
 constexpr int64_t stage_lhat_off(int64_t m, int step) { return (step & 1) * m * 64; }
 constexpr int64_t stage_pp_off(int64_t m) { return 2 * m * 64; }
 constexpr int64_t stage_gemm_off(int64_t m) {

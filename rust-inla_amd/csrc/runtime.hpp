@@ -265,8 +265,8 @@ class FirstFit {
 //  - dense Sigma front of a large front s (m x m): lifetime as Sigma_RR;
 //  - L_hat panel staging of a large front (m x 64): its own level only.
 struct Pools {
-  std::vector<int64_t> u_fac, u_sel, w_off, t_off;
-  int64_t u_size = 1, w_size = 1, t_size = 1;
+  std::vector<int64_t> u_fac, u_sel, w_off, t_off, ts_off;
+  int64_t u_size = 1, w_size = 1, t_size = 1, ts_size = 1;
 };
 
 inline Pools plan_pools(const Symbolic& S) {
@@ -276,6 +276,7 @@ inline Pools plan_pools(const Symbolic& S) {
   P.u_sel.assign(ns, 0);
   P.w_off.assign(ns, -1);
   P.t_off.assign(ns, -1);
+  P.ts_off.assign(ns, -1);
   auto r_of = [&](int64_t s) { return S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s]; };
   auto m_of = [&](int64_t s) { return S.sn_start[s + 1] - S.sn_start[s] + r_of(s); };
   auto large = [&](int64_t s) { return m_of(s) > kSmallFront; };
@@ -304,20 +305,23 @@ inline Pools plan_pools(const Symbolic& S) {
     free_after[fl].push_back(s);
   }
   FirstFit us, wf;
-  int64_t tmax = 0;
+  int64_t tmax = 0, tsmax = 0;
   for (int l = S.nlevels - 1; l >= 0; --l) {
-    int64_t tl = 0;
+    int64_t tl = 0, tsl = 0;
     for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
       const int64_t s = S.level_sn[q], r = r_of(s), m = m_of(s);
       if (large(s)) {
         P.w_off[s] = wf.alloc(m * m);
         P.t_off[s] = tl;
         tl += stage_size(m);
+        P.ts_off[s] = tsl;
+        tsl += (r + kSolvePart - 1) / kSolvePart * (m - r);
       } else if (r > 0) {
         P.u_sel[s] = us.alloc(r * r);
       }
     }
     tmax = std::max(tmax, tl);
+    tsmax = std::max(tsmax, tsl);
     for (int64_t s : free_after[l]) {
       const int64_t r = r_of(s), m = m_of(s);
       if (large(s))
@@ -329,6 +333,7 @@ inline Pools plan_pools(const Symbolic& S) {
   P.u_size = std::max<int64_t>({uf.size(), us.size(), 1});
   P.w_size = std::max<int64_t>(wf.size(), 1);
   P.t_size = std::max<int64_t>(tmax, 1);
+  P.ts_size = std::max<int64_t>(tsmax, 1);
   return P;
 }
 
@@ -533,6 +538,8 @@ struct SymData {
     plan.w_size = pools.w_size;
     plan.t_off = arena.upload(pools.t_off);
     plan.t_size = pools.t_size;
+    plan.ts_off = arena.upload(pools.ts_off);
+    plan.ts_size = pools.ts_size;
     plan.z_off = arena.upload(z_off);
     plan.z_size = std::max<int64_t>(zsz, 1);
     plan.cinv_off = arena.upload(cinv_off);
@@ -694,7 +701,6 @@ struct SymData {
         int64_t w, r, m;
         mw(s, w, r, m);
         const int64_t parts = (r + kSolvePart - 1) / kSolvePart;
-        if (parts * w > m * kTile) throw std::runtime_error("front too tall for the solve staging");
         for (int64_t p = 0; p < parts; ++p)
           for (int64_t c = 0; c < ceil64(w); ++c) add(s, c, p);
       }
@@ -785,12 +791,12 @@ struct gmrf_batch {
   double* d_ldpart = nullptr;     // per-matrix chunk partials of log|D|
   uint32_t* d_ldcount = nullptr;  // per-matrix chunk counters (zero between launches)
   // forward/backward workspaces for solves with several right-hand sides per
-  // matrix (grown on demand; one vector = n + v_size + t_size doubles)
+  // matrix (grown on demand; one vector = n + v_size + ts_size doubles)
   double* mrhs = nullptr;
   int64_t mrhs_vecs = 0;
   // selected inversion on a second stream beside the main stream's solves of the
-  // same factors (selinv_async / selinv_join); while it runs, single-vector solves
-  // keep their partial sums in TS instead of the shared staging area T
+  // same factors (selinv_async / selinv_join); the solves stage their partial sums
+  // in TS (ts_size per vector), the selected inversion in T
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* TS = nullptr;
@@ -806,6 +812,7 @@ struct gmrf_batch {
     U = arena.alloc<double>((size_t)cap * sym->plan.u_size);
     S = arena.alloc<double>((size_t)cap * Sy.l_size);
     T = arena.alloc<double>((size_t)cap * sym->plan.t_size);
+    TS = arena.alloc<double>((size_t)cap * sym->plan.ts_size);
     X = arena.alloc<double>((size_t)cap * Sy.n);
     V = arena.alloc<double>((size_t)cap * std::max<int64_t>(Sy.v_size, 1));
     W = arena.alloc<double>((size_t)cap * sym->plan.w_size);
@@ -840,7 +847,6 @@ struct gmrf_batch {
       check(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking), "hipStreamCreate");
       check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
       check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
-      TS = arena.alloc<double>((size_t)cap * sym->plan.t_size);
     }
     check(hipEventRecord(ev_fork, stream), "hipEventRecord");
     check(hipStreamWaitEvent(aux, ev_fork, 0), "hipStreamWaitEvent");
@@ -924,7 +930,7 @@ struct gmrf_batch {
     if (nr == 1) {
       Xf = X + (int64_t)first * P.n;
       Vf = V + (int64_t)first * vsz;
-      Tf = (sel_inflight ? TS : T) + (int64_t)first * P.t_size;
+      Tf = TS + (int64_t)first * P.ts_size;
     } else {
       const int64_t vecs = (int64_t)count * nr;
       if (vecs > mrhs_vecs) {
@@ -932,7 +938,7 @@ struct gmrf_batch {
         if (mrhs) check(hipFree(mrhs), "hipFree");
         mrhs = nullptr;
         mrhs_vecs = 0;
-        check(hipMalloc(&mrhs, (size_t)vecs * (P.n + vsz + P.t_size) * sizeof(double)), "hipMalloc");
+        check(hipMalloc(&mrhs, (size_t)vecs * (P.n + vsz + P.ts_size) * sizeof(double)), "hipMalloc");
         mrhs_vecs = vecs;
       }
       Xf = mrhs;
