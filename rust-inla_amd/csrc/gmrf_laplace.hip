@@ -6,11 +6,52 @@
 
 #include <cstdint>
 
+#include "plan.hpp"
+
 namespace gmrf {
 
 // Compiler-only fence after a round of independent loads: keeps them issued
 // ahead of the conditional code that consumes them (no instruction emitted).
 #define LAP_ISSUE_FENCE() asm volatile("" ::: "memory")
+
+// Gather lists longer than kLongGather (a dense row such as an intercept's column
+// of A, summing every observation) go to a workgroup each (k_*_long) instead of one
+// thread.
+// The same sum by a 256-thread workgroup: thread t takes t0 + t, t0 + t + 256, ...
+// (coalesced index and value loads, 8 per round), then a fixed-order tree over the
+// threads. Every thread of the block must call it; the block total is returned to
+// all threads.
+template <typename IT>
+__device__ __forceinline__ double gather_dot_block(int64_t t0, int64_t t1, const IT* __restrict__ idx,
+                                                   const double* __restrict__ c,
+                                                   const double* __restrict__ x, double* red) {
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  const int64_t tl = t1 > t0 ? t1 - 1 : 0;
+  for (int64_t tb = t0; tb < t1; tb += 8 * 256) {
+    IT ix[8];
+    double cv[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t t = min(tb + tid + 256 * u, tl);
+      ix[u] = idx[t];
+      cv[u] = c[t];
+    }
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = x[ix[u]];
+    LAP_ISSUE_FENCE();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (tb + tid + 256 * u < t1) ? cv[u] * xv[u] : 0.0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return tot;
+}
 
 // sum_{t in [t0, t1)} c[t] * x[idx[t]] with the index and value loads issued in
 // rounds of 8 (clamped addresses, masked terms). idx and c are never empty
@@ -220,8 +261,26 @@ __global__ void k_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* __re
     const int b = (int)(e / nnz);
     const int64_t q = e - (int64_t)b * nnz;
     const double* wb = w + (int64_t)b * nobs;
-    out[e] = base[e] + gather_dot(ptr[q], ptr[q + 1], obs, coef, wb);
+    const int64_t p0 = ptr[q], p1 = ptr[q + 1];
+    if (p1 - p0 > kLongGather) continue;  // k_add_atwa_long
+    out[e] = base[e] + gather_dot(p0, p1, obs, coef, wb);
   }
+}
+
+// The entries with long observation lists: one workgroup per (entry, matrix).
+__global__ __launch_bounds__(256) void k_add_atwa_long(int64_t nnz, int64_t nobs, const int64_t* __restrict__ longs,
+                                                       const int64_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ obs,
+                                                       const double* __restrict__ coef,
+                                                       const double* __restrict__ w,
+                                                       const double* __restrict__ base,
+                                                       double* __restrict__ out) {
+  __shared__ double red[4];
+  const int64_t q = longs[blockIdx.x];
+  const int b = blockIdx.y;
+  const double s = gather_dot_block(ptr[q], ptr[q + 1], obs, coef, w + (int64_t)b * nobs, red);
+  const int64_t e = (int64_t)b * nnz + q;
+  if (threadIdx.x == 0) out[e] = base[e] + s;
 }
 
 // y[b][o] = alpha * sum_{p in [ptr[o], ptr[o+1])} vals[b*vld + p] x[b][idx[p]] + beta * y[b][o]
@@ -236,7 +295,27 @@ __global__ void k_gather_spmv(int B, int64_t nout, const int64_t* __restrict__ p
     const int64_t o = e - (int64_t)b * nout;
     const double* vb = vals + (int64_t)b * vld;
     const double* xb = x + (int64_t)b * ldx;
-    const double s = gather_dot(ptr[o], ptr[o + 1], idx, vb, xb);
+    const int64_t p0 = ptr[o], p1 = ptr[o + 1];
+    if (p1 - p0 > kLongGather) continue;  // k_gather_spmv_long
+    const double s = gather_dot(p0, p1, idx, vb, xb);
+    double* yo = y + (int64_t)b * ldy + o;
+    *yo = alpha * s + (beta == 0.0 ? 0.0 : beta * *yo);
+  }
+}
+
+// The outputs with long lists: one workgroup per (output, matrix).
+__global__ __launch_bounds__(256) void k_gather_spmv_long(const int64_t* __restrict__ longs,
+                                                          const int64_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ vals, int64_t vld,
+                                                          const double* __restrict__ x, int64_t ldx,
+                                                          double alpha, double beta, double* __restrict__ y,
+                                                          int64_t ldy) {
+  __shared__ double red[4];
+  const int64_t o = longs[blockIdx.x];
+  const int b = blockIdx.y;
+  const double s = gather_dot_block(ptr[o], ptr[o + 1], idx, vals + (int64_t)b * vld, x + (int64_t)b * ldx, red);
+  if (threadIdx.x == 0) {
     double* yo = y + (int64_t)b * ldy + o;
     *yo = alpha * s + (beta == 0.0 ? 0.0 : beta * *yo);
   }
@@ -607,19 +686,29 @@ hipError_t lap_add_diag(int B, int64_t n, int64_t nnz, const int64_t* diag_idx, 
   return hipGetLastError();
 }
 
+// `longs` lists the entries / outputs whose gather list exceeds kLongGather
+// (nlong of them, built with the map); the per-thread kernel skips them.
 hipError_t lap_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* ptr, const int32_t* obs,
                         const double* coef, const double* w, const double* base, double* out,
-                        hipStream_t st) {
+                        const int64_t* longs, int64_t nlong, hipStream_t st) {
   hipLaunchKernelGGL(k_add_atwa, dim3(grid_for((int64_t)B * nnz)), dim3(256), 0, st, B, nnz, nobs,
                      ptr, obs, coef, w, base, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nlong <= 0) return e;
+  hipLaunchKernelGGL(k_add_atwa_long, dim3((unsigned)nlong, B), dim3(256), 0, st, nnz, nobs, longs, ptr, obs,
+                     coef, w, base, out);
   return hipGetLastError();
 }
 
 hipError_t lap_spmv(int B, int64_t nout, const int64_t* ptr, const int32_t* idx, const double* vals,
                     int64_t vld, const double* x, int64_t ldx, double alpha, double beta, double* y,
-                    int64_t ldy, hipStream_t st) {
+                    int64_t ldy, const int64_t* longs, int64_t nlong, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_spmv, dim3(grid_for((int64_t)B * nout)), dim3(256), 0, st, B, nout,
                      ptr, idx, vals, vld, x, ldx, alpha, beta, y, ldy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nlong <= 0) return e;
+  hipLaunchKernelGGL(k_gather_spmv_long, dim3((unsigned)nlong, B), dim3(256), 0, st, longs, ptr, idx, vals,
+                     vld, x, ldx, alpha, beta, y, ldy);
   return hipGetLastError();
 }
 

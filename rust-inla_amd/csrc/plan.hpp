@@ -61,6 +61,7 @@ constexpr int kPanel = 32;       // K chunk of the MFMA tile engine
 constexpr int kTile = 64;        // panel width / tile edge of the large-front kernels
 constexpr int kSolvePart = 1024;  // rows of R_s per partial sum in the backward solve
 constexpr int kSolveSplitRows = 1024;  // levels whose tallest front has r >= this split the solve
+constexpr int64_t kLongGather = 512;  // Laplace gather lists longer than this get a workgroup each
 constexpr int kFuseDiagBelow = 2048;  // factor launches of fewer workgroups also factor the next diagonal block
 constexpr int kSelSplitK = 512;  // rows below a panel per partial product of Sigma[P,P] (tall fronts)
 constexpr int kGemmSplitK = 512;  // K per partial product of Sigma[B,P] when a level has few row blocks

@@ -212,3 +212,42 @@ def test_spde_poisson_with_intercept_matches_oracle():
         assert abs(r["mlik"][b].item() - ref.mlik) < MLIK_ATOL * max(1.0, abs(ref.mlik))
         assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
         assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < VAR_RTOL
+
+
+def test_spde_poisson_long_gather_rows_match_oracle():
+    """More observations than the per-thread gather limit (512): the intercept's
+    column of A (A^T g, A^T u) and its diagonal entry of A^T W A (every
+    observation) are summed by one workgroup each. Mode, mlik and variances against
+    the oracle; the adjoint gradient against central differences of the oracle's
+    mlik (1e-5 relative, as the other gradient tests)."""
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.spde_poisson_workload(nx=12, n_obs=2000, seed=21)
+    g = wl.gmrf
+    assert np.diff(wl.A.indptr).max() > 512  # the intercept column
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.15, -0.1]])
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="poisson", A=wl.A,
+                         capacity=len(thetas))
+    c, dc, _, _ = wl.params(thetas)
+    r = model.eval(c, dcoefs=dc, tol=1e-12)
+    assert np.all(r["status"] == 0)
+    ones = np.ones(len(wl.y))
+    for b in range(len(thetas)):
+        ref = orc.find_mode(g.csc(c[b]), "poisson", "log", wl.y, ones, A=wl.A, tol=1e-12, keep_post=True)
+        assert abs(r["mlik"][b].item() - ref.mlik) < MLIK_ATOL * max(1.0, abs(ref.mlik))
+        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
+        assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < VAR_RTOL
+    th = thetas[1]
+    h = 1e-4
+    fd = np.zeros(2)
+    for l in range(2):
+        vals = []
+        for sgn in (1.0, -1.0):
+            t = th.copy()
+            t[l] += sgn * h
+            cc, _, _, _ = wl.params(t[None])
+            vals.append(orc.find_mode(g.csc(cc[0]), "poisson", "log", wl.y, ones, A=wl.A, tol=1e-12).mlik)
+        fd[l] = (vals[0] - vals[1]) / (2 * h)
+    grad = r["grad"][1].cpu().numpy()
+    assert np.max(np.abs(grad - fd) / np.maximum(np.abs(fd), 1.0)) < 1e-5, (grad, fd)
