@@ -1352,14 +1352,16 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
                                                       const double* __restrict__ Z,
                                                       double* __restrict__ X,
                                                       const double* __restrict__ T) {
-  constexpr int NW = NT / 64, NK = NW / 4;
+  // wave wv owns the panel columns q = wv + NW u (u < CPW) over all rows below the
+  // panel (lane = row, RB row blocks of 64 per round of loads), so a column's sum
+  // is one wave's lane reduction
+  constexpr int NW = NT / 64, CPW = 64 / NW, RB = 16 / CPW;
   extern __shared__ double xf[];
-  __shared__ double red[NK][kTile];
+  __shared__ double red[kTile];
   __shared__ double tq[kTile];
   const int s = list[blockIdx.x];
   const int b = blockIdx.y, bl = b / nr;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int qg = wv & 3, kg = wv >> 2;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)bl * P.z_size + P.z_off[s];
@@ -1381,33 +1383,33 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
   const int np = (w + kTile - 1) / kTile;
   for (int pk = np - 1; pk >= 0; --pk) {
     const int c0 = pk * kTile, nb = min(kTile, w - c0), c1 = c0 + nb;
-    double acc[16];
+    double acc[CPW];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    for (int k = c1 + 64 * kg + lane; k < mr + 63; k += 64 * NK) {
-      const int kc = min(k, mr - 1);
-      const double xk = k < mr ? xf[kc] : 0.0;
-      double lv[16];
+    for (int u = 0; u < CPW; ++u) acc[u] = 0.0;
+    for (int k0 = c1 + lane; k0 < mr + 63; k0 += 64 * RB) {
+      double lv[RB][CPW], xk[RB];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = Lb[(int64_t)(c0 + min(qg + 4 * u, nb - 1)) * m + kc];
+      for (int j = 0; j < RB; ++j) {
+        const int k = k0 + 64 * j, kc = min(k, mr - 1);
+        xk[j] = k < mr ? xf[kc] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) lv[j][u] = Lb[(int64_t)(c0 + min(wv + NW * u, nb - 1)) * m + kc];
+      }
       ISSUE_FENCE();
 #pragma unroll
-      for (int u = 0; u < 16; ++u) acc[u] += lv[u] * xk;
+      for (int j = 0; j < RB; ++j)
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) acc[u] += lv[j][u] * xk[j];
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < CPW; ++u) {
       double a = acc[u];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      if (lane == 0) red[kg][qg + 4 * u] = a;
+      if (lane == 0) red[wv + NW * u] = a;
     }
     __syncthreads();
-    if (tid < nb) {
-      double a = 0.0;
-#pragma unroll
-      for (int q = 0; q < NK; ++q) a += red[q][tid];
-      tq[tid] = xf[c0 + tid] - a;
-    }
+    if (tid < nb) tq[tid] = xf[c0 + tid] - red[tid];
     __syncthreads();
     const double* Zp = Zb + (int64_t)pk * kTile * kTile;
     double zv[64 / NW];
