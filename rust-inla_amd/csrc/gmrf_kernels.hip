@@ -1055,7 +1055,9 @@ __global__ __launch_bounds__(256) void k_logdet_seg(DevPlan P, const double* __r
 }
 
 // ---------------------------------------------------------------------------
-// Solves. A launch covers B * nr right-hand sides: vector v uses the factor of
+// Solves. A launch covers B * nr right-hand sides (grid x = front or task x nr,
+// the nr vectors of one matrix side by side so that they share the front's L in
+// the caches; grid y = matrix): vector v uses the factor of
 // matrix v / nr and its own slices of X, V and the staging area.
 // ---------------------------------------------------------------------------
 __global__ void k_gather_perm(DevPlan P, int B, const double* __restrict__ rhs, int64_t ldr,
@@ -1095,8 +1097,9 @@ __global__ __launch_bounds__(128) void k_solve_fwd(DevPlan P, const int32_t* __r
                                                    double* __restrict__ X,
                                                    double* __restrict__ V) {
   extern __shared__ double f[];
-  const int s = list[blockIdx.x];
-  const int b = blockIdx.y, bl = b / nr;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+  const int s = list[fi];
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
@@ -1130,8 +1133,9 @@ __global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __r
                                                    const double* __restrict__ L,
                                                    double* __restrict__ X) {
   extern __shared__ double sh[];
-  const int s = list[blockIdx.x];
-  const int b = blockIdx.y, bl = b / nr;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+  const int s = list[fi];
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
@@ -1176,8 +1180,9 @@ __global__ __launch_bounds__(NT) void k_solve_fwd_tri(DevPlan P, const int32_t* 
   __shared__ double part[NQ][kTile];
   __shared__ double rp[4][NR];
   __shared__ double y[kTile];
-  const int s = list[blockIdx.x];
-  const int b = blockIdx.y, bl = b / nr;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+  const int s = list[fi];
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, m = w + P.sn_r[s];
   const int mr = FULL ? m : w;  // rows carried by this workgroup
@@ -1262,9 +1267,10 @@ __global__ __launch_bounds__(256) void k_solve_fwd_rows(DevPlan P, const int32_t
                                                         double* __restrict__ V) {
   extern __shared__ double yv[];
   __shared__ double red[4][kTile];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a task and matrix: consecutive workgroups
+  const int32_t* tk = tasks + 3 * fi;
   const int s = tk[0], ch = tk[1];
-  const int b = blockIdx.y, bl = b / nr;
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
@@ -1307,9 +1313,10 @@ __global__ __launch_bounds__(256) void k_solve_bwd_rows(DevPlan P, const int32_t
                                                         const double* __restrict__ L,
                                                         const double* __restrict__ X,
                                                         double* __restrict__ T) {
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a task and matrix: consecutive workgroups
+  const int32_t* tk = tasks + 3 * fi;
   const int s = tk[0], cc = tk[1], part = tk[2];
-  const int b = blockIdx.y, bl = b / nr;
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
@@ -1359,8 +1366,9 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
   extern __shared__ double xf[];
   __shared__ double red[kTile];
   __shared__ double tq[kTile];
-  const int s = list[blockIdx.x];
-  const int b = blockIdx.y, bl = b / nr;
+  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+  const int s = list[fi];
+  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
@@ -2195,7 +2203,7 @@ hipError_t launch_diag_scale(const DevPlan& P, int B, int nr, const double* L, d
 hipError_t launch_solve_fwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
                             int nr, const double* L, double* X, double* V, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_fwd, dim3(count, B * nr), dim3(128), (size_t)maxm * sizeof(double), st, P,
+  hipLaunchKernelGGL(k_solve_fwd, dim3(count * nr, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
                      list, nr, L, X, V);
   return hipGetLastError();
 }
@@ -2203,7 +2211,7 @@ hipError_t launch_solve_fwd(const DevPlan& P, const int32_t* list, int count, in
 hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
                             int nr, const double* L, double* X, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_bwd, dim3(count, B * nr), dim3(128), (size_t)maxm * sizeof(double), st, P,
+  hipLaunchKernelGGL(k_solve_bwd, dim3(count * nr, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
                      list, nr, L, X);
   return hipGetLastError();
 }
@@ -2222,7 +2230,7 @@ static hipError_t fwd_tri(const DevPlan& P, const int32_t* list, int count, size
                           const double* L, const double* Z, double* X, double* V, hipStream_t st) {
   hipError_t e = lds_limit((const void*)k_solve_fwd_tri<NT, FULL>, sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_solve_fwd_tri<NT, FULL>), dim3(count, B * nr), dim3(NT), sh, st, P, list, nr, L,
+  hipLaunchKernelGGL((k_solve_fwd_tri<NT, FULL>), dim3(count * nr, B), dim3(NT), sh, st, P, list, nr, L,
                      Z, X, V);
   return hipGetLastError();
 }
@@ -2246,14 +2254,14 @@ hipError_t launch_solve_fwd_rows(const DevPlan& P, const int32_t* tasks, int cou
   const size_t sh = (size_t)maxw * sizeof(double);
   hipError_t e = lds_limit((const void*)k_solve_fwd_rows, sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_solve_fwd_rows, dim3(count, B * nr), dim3(256), sh, st, P, tasks, nr, L, X, V);
+  hipLaunchKernelGGL(k_solve_fwd_rows, dim3(count * nr, B), dim3(256), sh, st, P, tasks, nr, L, X, V);
   return hipGetLastError();
 }
 
 hipError_t launch_solve_bwd_rows(const DevPlan& P, const int32_t* tasks, int count, int B, int nr,
                                  const double* L, const double* X, double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_bwd_rows, dim3(count, B * nr), dim3(256), 0, st, P, tasks, nr, L, X, T);
+  hipLaunchKernelGGL(k_solve_bwd_rows, dim3(count * nr, B), dim3(256), 0, st, P, tasks, nr, L, X, T);
   return hipGetLastError();
 }
 
@@ -2262,7 +2270,7 @@ static hipError_t bwd_tri(const DevPlan& P, const int32_t* list, int count, size
                           const double* L, const double* Z, double* X, const double* T, hipStream_t st) {
   hipError_t e = lds_limit((const void*)k_solve_bwd_tri<NT, FULL>, sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_solve_bwd_tri<NT, FULL>), dim3(count, B * nr), dim3(NT), sh, st, P, list, nr, L,
+  hipLaunchKernelGGL((k_solve_bwd_tri<NT, FULL>), dim3(count * nr, B), dim3(NT), sh, st, P, list, nr, L,
                      Z, X, T);
   return hipGetLastError();
 }
