@@ -1091,73 +1091,94 @@ __global__ void k_diag_scale(DevPlan P, int B, int nr, const double* __restrict_
   }
 }
 
+// Small fronts (m <= 64): a workgroup takes NV right-hand sides of one front and
+// matrix, one wave each (lane = front row or column); the front's L block is staged
+// once in LDS (dynamic, sized by the launch's largest block) for all of them, and
+// the column sweeps run in registers with the pivot value broadcast from its lane
+// (no barrier per column). Grid x = front x ceil(nr / NV), y = matrix.
+//
 // Forward: f = [x_J; 0] + sum_c extend(v_c); y_J = L11^-1 f_J; v_s = f_R - L21 y_J.
-__global__ __launch_bounds__(128) void k_solve_fwd(DevPlan P, const int32_t* __restrict__ list, int nr,
-                                                   const double* __restrict__ L,
-                                                   double* __restrict__ X,
-                                                   double* __restrict__ V) {
-  extern __shared__ double f[];
-  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+template <int NV>
+__global__ __launch_bounds__(64 * NV) void k_solve_fwd(DevPlan P, const int32_t* __restrict__ list, int nr,
+                                                       const double* __restrict__ L,
+                                                       double* __restrict__ X,
+                                                       double* __restrict__ V) {
+  extern __shared__ double Ls[];  // Ls[k * m + i] = L[i][k], i < m, k < w
+  __shared__ double f[NV][kSmallFront];
+  const int nvb = (nr + NV - 1) / NV;
+  const int fi = blockIdx.x / nvb, vb = blockIdx.x - fi * nvb;
   const int s = list[fi];
-  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int jv = vb * NV + wv;  // this wave's vector of the matrix (jv >= nr: idle, writes nothing)
+  const int bl = blockIdx.y, b = bl * nr + min(jv, nr - 1);
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  for (int i = tid; i < m; i += blockDim.x) f[i] = i < w ? Xb[a0 + i] : 0.0;
+  const int nl = m * w;
+  for (int e = tid; e < nl; e += 64 * NV) Ls[e] = Lb[e];
+  {
+    const double xv = Xb[a0 + min(lane, w - 1)];
+    f[wv][lane] = lane < w ? xv : 0.0;
+  }
   __syncthreads();
   for (int cq = P.child_ptr[s]; cq < P.child_ptr[s + 1]; ++cq) {
     const int c = P.child_idx[cq];
-    const int rc = P.sn_r[c];
+    const int rc = P.sn_r[c];  // rc <= m: the child's rows map into distinct rows of this front
     const double* Vc = V + (int64_t)b * P.v_size + P.v_off[c];
     const int32_t* relc = P.rel + P.rows_ptr[c];
-    for (int i = tid; i < rc; i += blockDim.x) f[relc[i]] += Vc[i];
+    if (lane < rc) f[wv][relc[lane]] += Vc[lane];
     __syncthreads();
   }
+  double fv = f[wv][lane];
   for (int k = 0; k < w; ++k) {
-    const double yk = f[k];
-    for (int i = k + 1 + tid; i < m; i += blockDim.x) f[i] -= Lb[(int64_t)k * m + i] * yk;
-    __syncthreads();
+    const double yk = __shfl(fv, k, 64);
+    if (lane > k && lane < m) fv -= Ls[k * m + lane] * yk;
   }
-  double* Vs = V + (int64_t)b * P.v_size + P.v_off[s];
-  for (int i = tid; i < m; i += blockDim.x) {
-    if (i < w)
-      Xb[a0 + i] = f[i];
+  if (jv < nr && lane < m) {
+    if (lane < w)
+      Xb[a0 + lane] = fv;
     else
-      Vs[i - w] = f[i];
+      V[(int64_t)b * P.v_size + P.v_off[s] + lane - w] = fv;
   }
 }
 
-// Backward: x_J = L11^-T (z_J - L21^T x_R) with x_R gathered from final values.
-__global__ __launch_bounds__(128) void k_solve_bwd(DevPlan P, const int32_t* __restrict__ list, int nr,
-                                                   const double* __restrict__ L,
-                                                   double* __restrict__ X) {
-  extern __shared__ double sh[];
-  const int fi = blockIdx.x / nr;  // the nr vectors of a front and matrix: consecutive workgroups
+// Backward: x_J = L11^-T (z_J - L21^T x_R) with x_R gathered from final values
+// (lane = column; L staged with an odd leading dimension so that the column-wise
+// reads of a wave do not collide in the LDS banks).
+template <int NV>
+__global__ __launch_bounds__(64 * NV) void k_solve_bwd(DevPlan P, const int32_t* __restrict__ list, int nr,
+                                                       const double* __restrict__ L,
+                                                       double* __restrict__ X) {
+  extern __shared__ double Ls[];  // Ls[j * ld + i] = L[i][j], ld = m | 1
+  __shared__ double g[NV][kSmallFront];
+  const int nvb = (nr + NV - 1) / NV;
+  const int fi = blockIdx.x / nvb, vb = blockIdx.x - fi * nvb;
   const int s = list[fi];
-  const int bl = blockIdx.y, b = bl * nr + (blockIdx.x - fi * nr);
-  const int tid = threadIdx.x;
-  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int jv = vb * NV + wv;
+  const int bl = blockIdx.y, b = bl * nr + min(jv, nr - 1);
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r, ld = m | 1;
   const double* Lb = L + (int64_t)bl * P.l_size + P.l_off[s];
   double* Xb = X + (int64_t)b * P.n;
-  double* g = sh;      // r
-  double* t = sh + r;  // w
+  const int nl = m * w;
+  for (int e = tid; e < nl; e += 64 * NV) {
+    const int j = e / m, i = e - j * m;
+    Ls[j * ld + i] = Lb[e];
+  }
   const int32_t* rows = P.rows + P.rows_ptr[s];
-  for (int i = tid; i < r; i += blockDim.x) g[i] = Xb[rows[i]];
+  g[wv][lane] = lane < r ? Xb[rows[lane]] : 0.0;
   __syncthreads();
-  for (int j = tid; j < w; j += blockDim.x) {
-    double acc = Xb[a0 + j];
-    const double* col = Lb + (int64_t)j * m + w;
-    for (int i = 0; i < r; ++i) acc -= col[i] * g[i];
-    t[j] = acc;
+  double t = 0.0;
+  if (lane < w) {
+    double acc = Xb[a0 + lane];
+    for (int i = 0; i < r; ++i) acc -= Ls[lane * ld + w + i] * g[wv][i];
+    t = acc;
   }
-  __syncthreads();
   for (int k = w - 1; k >= 0; --k) {
-    const double xk = t[k];
-    for (int j = tid; j < k; j += blockDim.x) t[j] -= Lb[(int64_t)j * m + k] * xk;
-    __syncthreads();
+    const double xk = __shfl(t, k, 64);
+    if (lane < k) t -= Ls[lane * ld + k] * xk;
   }
-  for (int j = tid; j < w; j += blockDim.x) Xb[a0 + j] = t[j];
+  if (jv < nr && lane < w) Xb[a0 + lane] = t;
 }
 
 // Large fronts, forward. The triangular part (one workgroup per front) solves the
@@ -2200,19 +2221,29 @@ hipError_t launch_diag_scale(const DevPlan& P, int B, int nr, const double* L, d
   return hipGetLastError();
 }
 
-hipError_t launch_solve_fwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
+// Small fronts: maxm / maxw bound the launch's fronts (m <= 64); four vectors per
+// workgroup when a matrix has several right-hand sides.
+hipError_t launch_solve_fwd(const DevPlan& P, const int32_t* list, int count, int maxm, int maxw, int B,
                             int nr, const double* L, double* X, double* V, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_fwd, dim3(count * nr, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
-                     list, nr, L, X, V);
+  if (maxm > kSmallFront || maxw > maxm) return hipErrorInvalidValue;
+  const size_t sh = (size_t)maxm * maxw * sizeof(double);
+  if (nr >= 4)
+    hipLaunchKernelGGL(k_solve_fwd<4>, dim3(count * ((nr + 3) / 4), B), dim3(256), sh, st, P, list, nr, L, X, V);
+  else
+    hipLaunchKernelGGL(k_solve_fwd<1>, dim3(count * nr, B), dim3(64), sh, st, P, list, nr, L, X, V);
   return hipGetLastError();
 }
 
-hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, int maxm, int B,
+hipError_t launch_solve_bwd(const DevPlan& P, const int32_t* list, int count, int maxm, int maxw, int B,
                             int nr, const double* L, double* X, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_solve_bwd, dim3(count * nr, B), dim3(128), (size_t)maxm * sizeof(double), st, P,
-                     list, nr, L, X);
+  if (maxm > kSmallFront || maxw > maxm) return hipErrorInvalidValue;
+  const size_t sh = (size_t)(maxm | 1) * maxw * sizeof(double);
+  if (nr >= 4)
+    hipLaunchKernelGGL(k_solve_bwd<4>, dim3(count * ((nr + 3) / 4), B), dim3(256), sh, st, P, list, nr, L, X);
+  else
+    hipLaunchKernelGGL(k_solve_bwd<1>, dim3(count * nr, B), dim3(64), sh, st, P, list, nr, L, X);
   return hipGetLastError();
 }
 

@@ -54,9 +54,9 @@ hipError_t launch_gather_perm(const DevPlan&, int, const double*, int64_t, doubl
                               int period);
 hipError_t launch_scatter_perm(const DevPlan&, int, const double*, double*, int64_t, hipStream_t);
 hipError_t launch_diag_scale(const DevPlan&, int, int, const double*, double*, hipStream_t);
-hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, int, const double*, double*,
+hipError_t launch_solve_fwd(const DevPlan&, const int32_t*, int, int, int, int, int, const double*, double*,
                             double*, hipStream_t);
-hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, int, const double*, double*,
+hipError_t launch_solve_bwd(const DevPlan&, const int32_t*, int, int, int, int, int, const double*, double*,
                             hipStream_t);
 hipError_t launch_solve_fwd_tri(const DevPlan&, const int32_t*, int, int, int, int, bool,
                                 const double*, const double*, double*, double*, hipStream_t);
@@ -188,6 +188,7 @@ struct LevelInfo {
   int64_t lb, le;     // large fronts [lb, le) in large list
   int64_t ab, ae;     // all fronts [ab, ae) in all list
   int32_t maxm_small, maxm_large, maxm_all, maxw_large;
+  int32_t maxw_small = 0;  // widest small supernode of the level
   // algorithmic work of the level's launches (per matrix)
   double fl_small = 0, fl_large = 0, sel_small = 0, sel_large = 0, sol_small = 0, sol_large = 0;
   double by_small = 0, by_large = 0;
@@ -466,6 +467,7 @@ struct SymData {
           while (m > kClassM[c]) ++c;
           cls[c].push_back((int32_t)s);
           L.maxw_cls[c] = std::max(L.maxw_cls[c], (int32_t)w);
+          L.maxw_small = std::max(L.maxw_small, (int32_t)w);
           L.sol_small += sol;
           L.maxm_small = std::max(L.maxm_small, m);
           L.fl_small += fl;
@@ -952,7 +954,7 @@ struct gmrf_batch {
     for (const LevelInfo& lv : sym->levels) {
       if (lv.se > lv.sb)
         prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
-          return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count, nr,
+          return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
                                   Lf, Xf, Vf, stream);
         });
       if (lv.le > lv.lb) {
@@ -987,7 +989,7 @@ struct gmrf_batch {
       }
       if (lv.se > lv.sb)
         prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
-          return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, count, nr,
+          return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
                                   Lf, Xf, stream);
         });
     }
