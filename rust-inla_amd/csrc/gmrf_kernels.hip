@@ -734,8 +734,10 @@ __device__ __forceinline__ void fl_diag_block(const DevPlan& P, int s, int step,
 // With DIAG (launches of few tiles), the workgroup of tile (0, 0) of a front goes
 // on to factor the front's first diagonal block (k_fl_diag, step 0), which lies
 // inside that tile: one launch less per level.
+// Without DIAG the register budget is held to four waves per SIMD (the kernel is
+// bound by its load latency, so occupancy pays).
 template <bool DIAG>
-__global__ __launch_bounds__(256) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIAG ? 1 : 4))) void k_fl_assemble(DevPlan P, const int32_t* __restrict__ tasks,
                                                      int ain_first, const double* __restrict__ vals,
                                                      int64_t ldv, double* __restrict__ L,
                                                      double* __restrict__ U, double* __restrict__ Z) {
