@@ -384,8 +384,11 @@ __global__ __launch_bounds__(1024) void k_rowreduce(int64_t len, int op, const d
 // Adjoint contractions over the CSC pattern, two passes (deterministic order):
 //   tr[b][k]   = sum_e comps[k][e] * Sigma_b[slot_full[e]]     (= tr(Sigma_b Q_k))
 //   quad[b][k] = sum_e comps[k][e] * x_b[row_e] * z_b[col_e]   (= x_b^T Q_k z_b)
-// Pass 1: block (chunk, k, b) reduces one chunk of the entries into part[].
+// Pass 1: block (chunk, g, b) reduces one chunk of the entries into part[] for the
+// components k = kContractGroup g .. (at most kContractGroup at once): the entry's
+// index and Sigma / x / z gathers are shared by the group's components.
 constexpr int kContractChunks = 64;
+constexpr int kContractGroup = 8;
 
 __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
                                                        const double* __restrict__ comps,
@@ -396,56 +399,79 @@ __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
                                                        const double* __restrict__ x,
                                                        const double* __restrict__ z, int64_t n,
                                                        double* __restrict__ part) {
-  __shared__ double r1[256], r2[256];
-  const int ch = blockIdx.x, k = blockIdx.y, b = blockIdx.z;
+  constexpr int G = kContractGroup;
+  __shared__ double red[4][2 * G];
+  const int ch = blockIdx.x, k0 = blockIdx.y * G, b = blockIdx.z;
+  const int kn = min(G, K - k0);
   const int64_t len = (nnz + kContractChunks - 1) / kContractChunks;
   const int64_t e0 = ch * len, e1 = min(nnz, e0 + len);
-  const double* ck = comps + (int64_t)k * nnz;
   const double* Sb = S ? S + (int64_t)b * l_size : nullptr;
   const double* xb = x ? x + (int64_t)b * n : nullptr;
   const double* zb = z ? z + (int64_t)b * n : nullptr;
-  double a1 = 0.0, a2 = 0.0;
-  for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += 4 * 256) {
-    double cv[4], sv[4], xv[4], zv[4];
-    int64_t sl[4];
-    int32_t qr[4], qc[4];
+  double a1[G], a2[G];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+  for (int g = 0; g < G; ++g) a1[g] = a2[g] = 0.0;
+  for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += 2 * 256) {
+    double sv[2], xz[2];
+    int64_t sl[2];
+    int32_t qr[2], qc[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
       const int64_t e = min(eb + 256 * u, e1 - 1);
-      cv[u] = ck[e];
       sl[u] = Sb ? slot_full[e] : 0;
       qr[u] = xb ? qrow[e] : 0;
       qc[u] = xb ? qcol[e] : 0;
     }
     LAP_ISSUE_FENCE();
+    double sg[2], xg[2], zg[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      sv[u] = Sb ? Sb[max(sl[u], (int64_t)0)] : 0.0;
-      xv[u] = xb ? xb[qr[u]] : 0.0;
-      zv[u] = xb ? zb[qc[u]] : 0.0;
+    for (int u = 0; u < 2; ++u) {
+      sg[u] = Sb ? Sb[max(sl[u], (int64_t)0)] : 0.0;
+      xg[u] = xb ? xb[qr[u]] : 0.0;
+      zg[u] = xb ? zb[qc[u]] : 0.0;
     }
     LAP_ISSUE_FENCE();
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (eb + 256 * u >= e1) continue;
-      if (sl[u] >= 0) a1 += cv[u] * sv[u];
-      a2 += cv[u] * xv[u] * zv[u];
+    for (int u = 0; u < 2; ++u) {
+      const bool ok = eb + 256 * u < e1;
+      sv[u] = (ok && sl[u] >= 0) ? sg[u] : 0.0;
+      xz[u] = ok ? xg[u] * zg[u] : 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (g < kn) {
+        const double* ck = comps + (int64_t)(k0 + g) * nnz;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const double cv = ck[min(eb + 256 * u, e1 - 1)];
+          a1[g] += cv * sv[u];
+          a2[g] += cv * xz[u];
+        }
+      }
     }
   }
-  r1[threadIdx.x] = a1;
-  r2[threadIdx.x] = a2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a1[g] += __shfl_xor(a1[g], o, 64);
+      a2[g] += __shfl_xor(a2[g], o, 64);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      red[wv][2 * g] = a1[g];
+      red[wv][2 * g + 1] = a2[g];
+    }
+  }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      r1[threadIdx.x] += r1[threadIdx.x + o];
-      r2[threadIdx.x] += r2[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const int64_t q = ((int64_t)b * K + k) * kContractChunks + ch;
-    part[2 * q] = r1[0];
-    part[2 * q + 1] = r2[0];
+  if ((int)threadIdx.x < 2 * kn) {
+    const int t = threadIdx.x;
+    const double v = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    const int64_t q = ((int64_t)b * K + k0 + (t >> 1)) * kContractChunks + ch;
+    part[2 * q + (t & 1)] = v;
   }
 }
 
@@ -729,7 +755,8 @@ hipError_t lap_contract(int B, int K, int64_t nnz, const double* comps, const do
                         int64_t l_size, const int64_t* slot_full, const int32_t* qrow,
                         const int32_t* qcol, const double* x, const double* z, int64_t n, double* tr,
                         double* quad, double* part, hipStream_t st) {
-  hipLaunchKernelGGL(k_contract_part, dim3(kContractChunks, K, B), dim3(256), 0, st, K, nnz, comps, S,
+  hipLaunchKernelGGL(k_contract_part, dim3(kContractChunks, (K + kContractGroup - 1) / kContractGroup, B),
+                     dim3(256), 0, st, K, nnz, comps, S,
                      l_size, slot_full, qrow, qcol, x, z, n, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

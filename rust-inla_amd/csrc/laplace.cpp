@@ -660,10 +660,15 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_constr_dot(B, M->kc, n, M->ca, M->x, M->cres, st), "C x");
         check(lap_constr_apply(B, M->kc, n, M->ca, M->cres, M->rhs, st, M->kappa), "kappa C^T C x");
       }
-      bt->factorize(prior_pending ? 2 * B : B, M->vals, nnz, 0);
+      const int nf = prior_pending ? 2 * B : B;
       if (final_factor && sel_count > 0) {
+        // the inversion needs the factors only: it starts before the log determinants
+        bt->factorize_levels(nf, M->vals, nnz, 0);
         bt->selinv_async(sel_count, 0);
         sel_started = true;
+        bt->logdet(nf, 0);
+      } else {
+        bt->factorize(nf, M->vals, nnz, 0);
       }
       if (lr && prior_pending) lowrank_factor(M, B, B, true);
       prior_pending = false;

@@ -870,8 +870,15 @@ struct gmrf_batch {
     sel_inflight = false;
   }
 
-  // Matrices [first, first + count) of the batch.
+  // Matrices [first, first + count) of the batch: the level loop, then the log
+  // determinants and statuses.
   void factorize(int count, const double* vals, int64_t ldv, int first = 0) {
+    factorize_levels(count, vals, ldv, first);
+    logdet(count, first);
+  }
+  // The factors only (L, Z); logdet() completes the factorisation. A selected
+  // inversion may be started in between (it reads L and Z only).
+  void factorize_levels(int count, const double* vals, int64_t ldv, int first = 0) {
     selinv_join();
     const DevPlan& P = sym->plan;
     double* Lf = L + (int64_t)first * P.l_size;
@@ -915,6 +922,10 @@ struct gmrf_batch {
         }
       }
     }
+  }
+  void logdet(int count, int first = 0) {
+    const DevPlan& P = sym->plan;
+    const double* Lf = L + (int64_t)first * P.l_size;
     prof_launch("k_logdet", 0.0, 16.0 * P.n * count, stream, [&] {
       return launch_logdet(P, count, Lf, d_ldpart + (size_t)first * 2 * 16, d_ldcount + first,
                            d_logdet + first, d_status + first, stream);
