@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void k_mma_probe(const double* A, const double
 //
 // The front sits row-major in LDS (F[i * LD + j], lower triangle). Columns are
 // eliminated in blocks of 16: one wave factors the 16 x 16 diagonal block in
-// registers (shuffles only, no barrier), then all waves form the panel
+// registers (lane exchanges and, in the MFMA form, 16x16x4 products; no barrier),
+// then all waves form the panel
 // W = A21 Z11^T (Z11 = L11^-1), L21 = W D^-1 and the trailing update
 // A22 -= W L21^T as 16 x 16 x 16 FP64 MFMA products. Optionally the inverse
 // Z = L^-1 of the whole block is built alongside (block Gauss transforms):
@@ -306,135 +307,119 @@ __device__ __forceinline__ void ldlt_base16(double* F, int k0, int kn, double* Z
   }
 }
 
-// The same in 4-column steps: per step, the 4 x 4 pivot block, row i's and the
-// lane's columns' entries in the 4 pivot columns and the 4 Z rows travel in one
-// round of shuffles, every lane factors the pivot block redundantly, and the
-// rank-4 update follows. A quarter of the dependent shuffle rounds of the
-// column-by-column form, at about twice its registers (used where the front
-// kernel runs few workgroups: the 64-column diagonal blocks of large fronts).
+// Lane l's double of a uniform source lane (two 32-bit reads into scalar registers).
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long v = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The same contract on the MFMA pipe. The symmetric block M is held in the MFMA
+// accumulator layout (lane l = 16 g + c, register v: M[4 v + g][c]), Z = L^-1 as
+// well. Per 4-column step p (pivot rows q0 = 4 p ..): the 4 x 4 pivot block is read
+// from lanes into scalar registers and factored redundantly (Lp Dp Lp^T, Y =
+// Lp^-1); lane (c, g) gathers row c's entries x_a = M[c][q0 + a] (the symmetric
+// copy in register p of lanes 16 a + c) and forms L[c][q0 + g] = (x Y^T)_g / d_g;
+// the trailing update M -= L Dp L^T is one 16x16x4 MFMA whose operands are the
+// lane's own values (A[c][g] = L[c][q0 + g], B[g][c] = d_g L[c][q0 + g]); Z's pivot
+// rows become Y Z_p and the rows below take -L Z_p, another MFMA. Rows and columns
+// beyond kn are the identity throughout.
 template <int LD>
-__device__ __forceinline__ void ldlt_base16_b4(double* F, int k0, int kn, double* Zk) {
-  const int lane = threadIdx.x & 63, i = lane & 15, c = lane >> 4;
-  double d[4], z[4];
+__device__ __forceinline__ void ldlt_base16_mfma(double* F, int k0, int kn, double* Zk) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  d4 M, Zv;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int j = c + 4 * t;
-    const double fv = F[(k0 + min(i, kn - 1)) * LD + k0 + min(j, kn - 1)];
-    d[t] = (i < kn && j < kn) ? (j <= i ? fv : 0.0) : (i == j ? 1.0 : 0.0);
-    z[t] = (i == j) ? 1.0 : 0.0;
+  for (int v = 0; v < 4; ++v) {
+    const int i = 4 * v + g;
+    const int hi = max(i, c), lo = min(i, c);
+    const double fv = F[(k0 + min(hi, kn - 1)) * LD + k0 + min(lo, kn - 1)];
+    M[v] = (i < kn && c < kn) ? fv : (i == c ? 1.0 : 0.0);
+    Zv[v] = i == c ? 1.0 : 0.0;
   }
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    const int q0 = 4 * kb;
-    if (q0 < kn) {
-      const double v = d[kb];
-      double ci[4], pv[4][4], cj[4][4], zk[4][4];
+  for (int p = 0; p < 4; ++p) {
+    const int q0 = 4 * p;
+    if (q0 >= kn) break;
+    // pivot block (lower), P[a][b] = M[q0 + a][q0 + b] at lane 16 a + q0 + b
+    const double p00 = readlane_d(M[p], q0), p10 = readlane_d(M[p], 16 + q0),
+                 p11 = readlane_d(M[p], 16 + q0 + 1), p20 = readlane_d(M[p], 32 + q0),
+                 p21 = readlane_d(M[p], 32 + q0 + 1), p22 = readlane_d(M[p], 32 + q0 + 2),
+                 p30 = readlane_d(M[p], 48 + q0), p31 = readlane_d(M[p], 48 + q0 + 1),
+                 p32 = readlane_d(M[p], 48 + q0 + 2), p33 = readlane_d(M[p], 48 + q0 + 3);
+    // row c's entries in the pivot columns (symmetric copy: M[q0 + a][c])
+    double x[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        ci[q] = __shfl(v, 16 * q + i, 64);  // D[i][q0 + q]
+    for (int a = 0; a < 4; ++a) x[a] = __shfl(M[p], 16 * a + c, 64);
+    double z[4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) pv[p][q] = (q <= p) ? __shfl(v, 16 * q + q0 + p, 64) : 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          cj[t][q] = __shfl(v, 16 * q + c + 4 * t, 64);  // D[c + 4t][q0 + q]
-          zk[t][q] = __shfl(z[t], 16 * c + q0 + q, 64);  // Z[q0 + q][c + 4t]
-        }
-      }
-      // pivot block LDL^T (identity beyond kn by construction)
-      double dp[4], rp[4], lp[4][4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        double a = pv[q][q];
-#pragma unroll
-        for (int u = 0; u < q; ++u) a -= lp[q][u] * lp[q][u] * dp[u];
-        dp[q] = a;
-        rp[q] = recip_nr(a);
-#pragma unroll
-        for (int p = q + 1; p < 4; ++p) {
-          double b = pv[p][q];
-#pragma unroll
-          for (int u = 0; u < q; ++u) b -= lp[p][u] * lp[q][u] * dp[u];
-          lp[p][q] = b * rp[q];
-        }
-      }
-      // L entries of row i in the pivot columns
-      double li[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        double a = ci[q];
-#pragma unroll
-        for (int u = 0; u < q; ++u) a -= li[u] * dp[u] * lp[q][u];
-        li[q] = a * rp[q];
-      }
-      const bool below = i >= q0 + 4 && i < kn;
-      const bool inblk = i >= q0 && i < q0 + 4;
-      const int pi = i - q0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int j = c + 4 * t;
-        // L entries of column j and the final Z rows of the pivot block (column j)
-        double lj[4], zb[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          double a = cj[t][q], b = zk[t][q];
-#pragma unroll
-          for (int u = 0; u < q; ++u) {
-            a -= lj[u] * dp[u] * lp[q][u];
-            b -= lp[q][u] * zb[u];
-          }
-          lj[q] = a * rp[q];
-          zb[q] = b;
-        }
-        double upd = 0.0, zup = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          upd += li[q] * dp[q] * lj[q];
-          zup += li[q] * zb[q];
-        }
-        // j - q0 and pi are lane-dependent: select instead of indexing registers
-        const int qj = j - q0;
-        double lij = li[0], lpp = lp[1][0] , dpi = dp[0];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) {
-          lij = (qj == q) ? li[q] : lij;
-          dpi = (pi == q) ? dp[q] : dpi;
-        }
-        // lp[pi][qj] for qj < pi
-        lpp = (pi == 2 && qj == 0) ? lp[2][0] : lpp;
-        lpp = (pi == 2 && qj == 1) ? lp[2][1] : lpp;
-        lpp = (pi == 3 && qj == 0) ? lp[3][0] : lpp;
-        lpp = (pi == 3 && qj == 1) ? lp[3][1] : lpp;
-        lpp = (pi == 3 && qj == 2) ? lp[3][2] : lpp;
-        double nd = d[t];
-        if (below && j >= q0 + 4 && j <= i) nd = d[t] - upd;
-        if (below && qj >= 0 && qj < 4) nd = lij;
-        if (inblk && qj >= 0 && j < i) nd = lpp;
-        if (inblk && j == i) nd = dpi;
-        d[t] = nd;
-        double nz = z[t];
-        if (below) nz = z[t] - zup;
-        if (inblk) {
-          double sel = zb[0];
-#pragma unroll
-          for (int q = 1; q < 4; ++q) sel = (pi == q) ? zb[q] : sel;
-          nz = sel;
-        }
-        z[t] = nz;
-      }
+    for (int a = 0; a < 4; ++a) z[a] = __shfl(Zv[p], 16 * a + c, 64);
+    // Lp Dp Lp^T of the pivot block, column by column (as the register form)
+    const double d0 = p00, r0 = recip_nr(d0);
+    const double l10 = p10 * r0, l20 = p20 * r0, l30 = p30 * r0;
+    const double s11 = p11 - l10 * p10, s21 = p21 - l20 * p10, s31 = p31 - l30 * p10;
+    const double s22 = p22 - l20 * p20, s32 = p32 - l30 * p20, s33 = p33 - l30 * p30;
+    const double d1 = s11, r1 = recip_nr(d1);
+    const double l21 = s21 * r1, l31 = s31 * r1;
+    const double t22 = s22 - l21 * s21, t32 = s32 - l31 * s21, t33 = s33 - l31 * s31;
+    const double d2 = t22, r2 = recip_nr(d2);
+    const double l32 = t32 * r2;
+    const double d3 = t33 - l32 * t32, r3 = recip_nr(d3);
+    // Y = Lp^-1 (unit lower)
+    const double y10 = -l10, y21 = -l21, y32 = -l32;
+    const double y20 = -(l20 + l21 * y10), y31 = -(l31 + l32 * y21);
+    const double y30 = -(l30 + l31 * y10 + l32 * y20);
+    // this lane's row g of Y, d_g, 1 / d_g
+    const double ya0 = g == 1 ? y10 : (g == 2 ? y20 : y30);
+    const double ya1 = g == 2 ? y21 : y31;
+    const double dg = g == 0 ? d0 : (g == 1 ? d1 : (g == 2 ? d2 : d3));
+    const double rg = g == 0 ? r0 : (g == 1 ? r1 : (g == 2 ? r2 : r3));
+    // L[c][q0 + g] = (sum_{a <= g} Y[g][a] x_a) / d_g
+    double u = x[g == 0 ? 0 : (g == 1 ? 1 : (g == 2 ? 2 : 3))];
+    u = g >= 1 ? u + ya0 * x[0] : u;
+    u = g >= 2 ? u + ya1 * x[1] : u;
+    u = g >= 3 ? u + y32 * x[2] : u;
+    double lcg = u * rg;
+    // pivot rows: the block's own factor
+    const int a = c - q0;
+    if (a >= 0 && a < 4) {
+      const double la0 = a == 1 ? l10 : (a == 2 ? l20 : l30), la1 = a == 2 ? l21 : l31;
+      lcg = a == g ? 1.0 : (g == 0 ? la0 : (g == 1 ? la1 : (g == 2 ? l32 : 0.0)));
+      lcg = a < g ? 0.0 : lcg;
     }
+    const int col = q0 + g;
+    if (c < kn && col < kn && c >= col) F[(k0 + c) * LD + k0 + col] = c == col ? dg : lcg;
+    // Z: pivot rows Y Z_p, rows below minus L Z_p
+    double zn = z[g == 0 ? 0 : (g == 1 ? 1 : (g == 2 ? 2 : 3))];
+    zn = g >= 1 ? zn + ya0 * z[0] : zn;
+    zn = g >= 2 ? zn + ya1 * z[1] : zn;
+    zn = g >= 3 ? zn + y32 * z[2] : zn;
+    Zv[p] = zn;
+    const bool below = c >= q0 + 4;
+    Zv = mfma16(below ? -lcg : 0.0, zn, Zv);
+    // trailing update of the rows and columns below the pivot block
+    M = mfma16(below ? lcg : 0.0, -dg * lcg, M);
   }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int j = c + 4 * t;
-    if (i < kn && j <= i) F[(k0 + i) * LD + k0 + j] = d[t];
-    Zk[i * 17 + j] = z[t];
-  }
+  for (int v = 0; v < 4; ++v) Zk[(4 * v + g) * 17 + c] = Zv[v];
+}
+
+// The 16 x 16 base case of a front class: the MFMA form for fronts of 48 rows or
+// more and the 64-column diagonal blocks (fewer dependent steps), the register form
+// for the 16- and 32-row classes (fewer registers: those launches run many
+// workgroups per CU).
+template <int MB, int LD>
+__device__ __forceinline__ void base16(double* F, int k0, int kn, double* Zk) {
+  if constexpr (MB >= 48)
+    ldlt_base16_mfma<LD>(F, k0, kn, Zk);
+  else
+    ldlt_base16<LD>(F, k0, kn, Zk);
 }
 
 // Blocked LDL^T of the first ncols columns of an m x m front in F (m <= MB;
 // the calling workgroup has MB / 16 waves). With WITHZ (ncols == m), Zs (same
 // layout as F) receives Z = L^-1. Zk (16 x 17) and Wb (MB x 17) are scratch.
-template <int MB, bool WITHZ, bool B4 = false>
+template <int MB, bool WITHZ>
 __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zs, double* Zk,
                                            double* Wb) {
   constexpr int LD = MB + 1, NW = MB / 16;
@@ -443,12 +428,7 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
   if (WITHZ) {
     for (int idx = threadIdx.x; idx < MB * LD; idx += 64 * NW) Zs[idx] = 0.0;
   }
-  if (wv == 0 && ncols > 0) {
-    if (B4)
-      ldlt_base16_b4<LD>(F, 0, min(16, ncols), Zk);
-    else
-      ldlt_base16<LD>(F, 0, min(16, ncols), Zk);
-  }
+  if (wv == 0 && ncols > 0) base16<MB, LD>(F, 0, min(16, ncols), Zk);
   __syncthreads();
   for (int k0 = 0; k0 < ncols; k0 += 16) {
     const int kn = min(16, ncols - k0), t0 = k0 + kn, kb = k0 >> 4;
@@ -550,12 +530,7 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         }
       }
     }
-    if (wv == 0 && k1 < ncols) {
-      if (B4)
-        ldlt_base16_b4<LD>(F, k1, min(16, ncols - k1), Zk);
-      else
-        ldlt_base16<LD>(F, k1, min(16, ncols - k1), Zk);
-    }
+    if (wv == 0 && k1 < ncols) base16<MB, LD>(F, k1, min(16, ncols - k1), Zk);
     __syncthreads();
   }
 }
@@ -713,7 +688,7 @@ __device__ __forceinline__ void fl_diag_block(const DevPlan& P, int s, int step,
     }
   }
   __syncthreads();
-  front_ldlt<kTile, true, true>(F, nb, nb, Zs, Zk, Wb);
+  front_ldlt<kTile, true>(F, nb, nb, Zs, Zk, Wb);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int j = c + 4 * t;
