@@ -1446,6 +1446,47 @@ __global__ __launch_bounds__(NT) void k_solve_bwd_tri(DevPlan P, const int32_t* 
 // evaluated densely inside the front).
 // ---------------------------------------------------------------------------
 
+// One wave: Zk = L_PP^-1 of the unit lower kn x kn block at (c0, c0) of the
+// row-major Ls (stride ldl), identity beyond kn (16 x 17), on the MFMA pipe: Z in
+// the accumulator layout (lane 16 g + c, register v: Z[4 v + g][c]); per 4-row step
+// p the pivot rows become Y Z_p (Y = inverse of the 4 x 4 diagonal block, from
+// uniform reads) and the rows below take -L[below, p] Z_p as one 16x16x4 MFMA.
+__device__ __forceinline__ void unit_lower_inv16_mfma(const double* Ls, int ldl, int c0, int kn,
+                                                      double* Zk) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  auto lat = [&](int i, int j) {  // L[i][j] of the block, 0 beyond kn
+    return (i < kn && j < kn) ? Ls[(c0 + min(i, kn - 1)) * ldl + c0 + min(j, kn - 1)] : 0.0;
+  };
+  d4 Zv;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Zv[v] = 4 * v + g == c ? 1.0 : 0.0;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int q0 = 4 * p;
+    if (q0 >= kn) break;
+    const double l10 = lat(q0 + 1, q0), l20 = lat(q0 + 2, q0), l21 = lat(q0 + 2, q0 + 1);
+    const double l30 = lat(q0 + 3, q0), l31 = lat(q0 + 3, q0 + 1), l32 = lat(q0 + 3, q0 + 2);
+    const bool below = c >= q0 + 4;
+    const double lcg = below ? lat(c, q0 + g) : 0.0;
+    double z[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) z[a] = __shfl(Zv[p], 16 * a + c, 64);
+    const double y10 = -l10, y21 = -l21, y32 = -l32;
+    const double y20 = -(l20 + l21 * y10), y31 = -(l31 + l32 * y21);
+    const double y30 = -(l30 + l31 * y10 + l32 * y20);
+    const double ya0 = g == 1 ? y10 : (g == 2 ? y20 : y30);
+    const double ya1 = g == 2 ? y21 : y31;
+    double zn = z[g == 0 ? 0 : (g == 1 ? 1 : (g == 2 ? 2 : 3))];
+    zn = g >= 1 ? zn + ya0 * z[0] : zn;
+    zn = g >= 2 ? zn + ya1 * z[1] : zn;
+    zn = g >= 3 ? zn + y32 * z[2] : zn;
+    Zv[p] = zn;
+    Zv = mfma16(-lcg, zn, Zv);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Zk[(4 * v + g) * 17 + c] = Zv[v];
+}
+
 // Small fronts (m <= MB), 4 MB threads, blocked by 16 columns from the right.
 // The whole Sigma front (both triangles) and the L block sit in LDS; Sigma_{R R}
 // is gathered from the parent first. For block P = [c0, c1) and B = [c1, m):
@@ -1539,7 +1580,13 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
   for (int c0 = ((w - 1) >> 4) << 4; c0 >= 0; c0 -= 16) {
     const int c1 = min(w, c0 + 16), kn = c1 - c0, nB = m - c1;
     const int nrb = (nB + 15) >> 4;
-    if (wv == 0) {
+    if (wv == 0 && MB >= 48) {
+      unit_lower_inv16_mfma(Ls, ldl, c0, kn, Zk);
+      if (lane < 16) {
+        const int q = c0 + min(lane, kn - 1);
+        dinv[lane] = 1.0 / Ls[q * ldl + q];
+      }
+    } else if (wv == 0) {
       // Zk = L_PP^-1 by Gauss transforms; lane (i, c) holds Zk[i][c + 4t]
       const int i = lane & 15, c = lane >> 4;
       double z[4];
