@@ -417,14 +417,31 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     std::vector<int64_t> acp(n + 1, 0), arp(m + 1, 0);
     std::vector<int32_t> ari, aci;
     std::vector<double> acv, arv;
+    // Observations are kept internally in the order of their smallest latent index
+    // (stable): the gathers of w and g over an entry's or a column's observations
+    // and of x over an observation's columns then touch nearby memory. Only sums
+    // over observations see the order (their rounding); nothing per observation
+    // leaves the model.
+    std::vector<int64_t> operm;  // internal observation -> caller's (empty: identity)
     if (d->a_indptr) {
       acp.assign(d->a_indptr, d->a_indptr + n + 1);
       const int64_t annz = acp[n];
       ari.resize(annz);
       acv.assign(d->a_values, d->a_values + annz);
+      std::vector<int64_t> key(m, n);
+      for (int64_t c = 0; c < n; ++c)
+        for (int64_t q = acp[c]; q < acp[c + 1]; ++q) {
+          const int64_t r = d->a_indices[q];
+          if (r < 0 || r >= m) throw std::runtime_error("A row index out of bounds");
+          key[r] = std::min(key[r], c);
+        }
+      operm.resize(m);
+      for (int64_t r = 0; r < m; ++r) operm[r] = r;
+      std::stable_sort(operm.begin(), operm.end(), [&](int64_t u, int64_t v) { return key[u] < key[v]; });
+      std::vector<int64_t> oinv(m);
+      for (int64_t k = 0; k < m; ++k) oinv[operm[k]] = k;
       for (int64_t q = 0; q < annz; ++q) {
-        const int64_t r = d->a_indices[q];
-        if (r < 0 || r >= m) throw std::runtime_error("A row index out of bounds");
+        const int64_t r = oinv[d->a_indices[q]];
         ari[q] = (int32_t)r;
         arp[r + 1]++;
       }
@@ -499,14 +516,17 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->v_slot = M->arena.upload(vs);
     M->v_coef = M->arena.upload(vc);
     // ---- observations
-    std::vector<double> hy(d->y, d->y + m), hpa(m, 1.0), hpb;
-    if (d->obs_a) hpa.assign(d->obs_a, d->obs_a + m);
+    auto obs_order = [&](const double* src) {
+      std::vector<double> h(src, src + m);
+      if (!operm.empty())
+        for (int64_t k = 0; k < m; ++k) h[k] = src[operm[k]];
+      return h;
+    };
+    std::vector<double> hy = obs_order(d->y), hpa(m, 1.0);
+    if (d->obs_a) hpa = obs_order(d->obs_a);
     M->y = M->arena.upload(hy);
     M->pa = M->arena.upload(hpa);
-    if (d->obs_b) {
-      hpb.assign(d->obs_b, d->obs_b + m);
-      M->pb = M->arena.upload(hpb);
-    }
+    if (d->obs_b) M->pb = M->arena.upload(obs_order(d->obs_b));
     // ---- constraints
     if (M->kc > 0) {
       if (!d->constr_a || !d->constr_e) throw std::runtime_error("constraints need A and e");
