@@ -56,27 +56,27 @@ __device__ __forceinline__ double gather_dot_block(int64_t t0, int64_t t1, const
 // sum_{t in [t0, t1)} c[t] * x[idx[t]] with the index and value loads issued in
 // rounds of 8 (clamped addresses, masked terms). idx and c are never empty
 // arrays (allocations hold at least one element).
-template <typename IT>
+template <typename IT, int R = 8>
 __device__ __forceinline__ double gather_dot(int64_t t0, int64_t t1, const IT* __restrict__ idx,
                                              const double* __restrict__ c,
                                              const double* __restrict__ x) {
   double s = 0.0;
   const int64_t tl = t1 > t0 ? t1 - 1 : 0;
-  for (int64_t tb = t0; tb < t1; tb += 8) {
-    IT ix[8];
-    double cv[8], xv[8];
+  for (int64_t tb = t0; tb < t1; tb += R) {
+    IT ix[R];
+    double cv[R], xv[R];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < R; ++u) {
       const int64_t t = min(tb + u, tl);
       ix[u] = idx[t];
       cv[u] = c[t];
     }
     LAP_ISSUE_FENCE();
 #pragma unroll
-    for (int u = 0; u < 8; ++u) xv[u] = x[ix[u]];
+    for (int u = 0; u < R; ++u) xv[u] = x[ix[u]];
     LAP_ISSUE_FENCE();
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += (tb + u < t1) ? cv[u] * xv[u] : 0.0;
+    for (int u = 0; u < R; ++u) s += (tb + u < t1) ? cv[u] * xv[u] : 0.0;
   }
   return s;
 }
@@ -263,7 +263,8 @@ __global__ void k_add_atwa(int B, int64_t nnz, int64_t nobs, const int64_t* __re
     const double* wb = w + (int64_t)b * nobs;
     const int64_t p0 = ptr[q], p1 = ptr[q + 1];
     if (p1 - p0 > kLongGather) continue;  // k_add_atwa_long
-    out[e] = base[e] + gather_dot(p0, p1, obs, coef, wb);
+    // rounds of 4 terms: most lists hold 1-4 observation pairs
+    out[e] = base[e] + gather_dot<int32_t, 4>(p0, p1, obs, coef, wb);
   }
 }
 
