@@ -1727,8 +1727,10 @@ __device__ __forceinline__ void sl_gather_task(const DevPlan& P, const int32_t* 
     const double* base;
     int64_t off;
     if (plarge) {
+      // the parent's W holds both triangles: column rj, row ri (consecutive lanes,
+      // consecutive rows)
       base = Wm;
-      off = woff + (int64_t)pj * mp + pi;
+      off = woff + (int64_t)rj[u] * mp + ri;
     } else if (pj < wp) {
       base = Sm;
       off = loff + (int64_t)pj * mp + pi;
