@@ -374,7 +374,8 @@ __device__ __forceinline__ void ldlt_base16_mfma(double* F, int k0, int kn, doub
     const double ya1 = g == 2 ? y21 : y31;
     const double dg = g == 0 ? d0 : (g == 1 ? d1 : (g == 2 ? d2 : d3));
     const double rg = g == 0 ? r0 : (g == 1 ? r1 : (g == 2 ? r2 : r3));
-    // L[c][q0 + g] = (sum_{a <= g} Y[g][a] x_a) / d_g
+    // L[c][q0 + g] = (sum_{a <= g} Y[g][a] x_a) / d_g; x[g] with the index as a select
+    // chain (measured faster here than selecting among the four values)
     double u = x[g == 0 ? 0 : (g == 1 ? 1 : (g == 2 ? 2 : 3))];
     u = g >= 1 ? u + ya0 * x[0] : u;
     u = g >= 2 ? u + ya1 * x[1] : u;
