@@ -181,6 +181,13 @@ gmrf_status gmrf_solver_create(gmrf_solver** out);
 void gmrf_solver_destroy(gmrf_solver* s);
 gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indptr,
                                   const int64_t* indices, const double* data);
+/* Dense input (LdltBackend::factorize_dense, crates/inla_math/src/backend.rs:14-20,59-69;
+ * dense_faer.rs:25-75): a is n x n row-major, of which the lower triangle is read; it
+ * is factorised as one dense front (the full pattern), after which solve / diag_inv /
+ * log_abs_det apply as for gmrf_solver_factorize. A failed factorisation returns
+ * GMRF_SINGULAR (the reference maps every dense failure to MathError::Singular) and
+ * keeps the previous factor; n <= 0 is GMRF_INVALID_ARGUMENT. */
+gmrf_status gmrf_solver_factorize_dense(gmrf_solver* s, int64_t n, const double* a);
 gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x);
 gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out);
 gmrf_status gmrf_solver_log_abs_det(const gmrf_solver* s, double* out);

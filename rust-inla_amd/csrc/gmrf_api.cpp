@@ -354,6 +354,31 @@ gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indp
   }
 }
 
+gmrf_status gmrf_solver_factorize_dense(gmrf_solver* s, int64_t n, const double* a) {
+  if (!s || !a) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (n <= 0) return fail(GMRF_INVALID_ARGUMENT, "empty or invalid matrix: n must be > 0");
+  std::vector<int64_t> ip, ix;
+  std::vector<double> v;
+  try {
+    // full symmetric pattern, column j = rows 0..n-1; entry (i, j) from the lower
+    // triangle a[max(i, j) n + min(i, j)]
+    ip.resize(n + 1);
+    ix.resize((size_t)n * n);
+    v.resize((size_t)n * n);
+    for (int64_t j = 0; j <= n; ++j) ip[j] = j * n;
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < n; ++i) {
+        ix[(size_t)j * n + i] = i;
+        v[(size_t)j * n + i] = a[(size_t)std::max(i, j) * n + std::min(i, j)];
+      }
+  } catch (...) {
+    return from_exception();
+  }
+  const gmrf_status rc = gmrf_solver_factorize(s, n, ip.data(), ix.data(), v.data());
+  if (rc == GMRF_NOT_POSITIVE_DEFINITE) return fail(GMRF_SINGULAR, "Matrix is singular or structurally rank-deficient");
+  return rc;
+}
+
 gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x) {
   if (!s) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (s->cur < 0) return fail(GMRF_SOLVE_FAILURE, "Linear system solve failed: Factorization not computed yet");

@@ -133,6 +133,7 @@ SIGNATURES = {
     "gmrf_solver_create": (_i32, [C.POINTER(_vp)]),
     "gmrf_solver_destroy": (None, [_vp]),
     "gmrf_solver_factorize": (_i32, [_vp, _i64, _pi64, _pi64, _pdbl]),
+    "gmrf_solver_factorize_dense": (_i32, [_vp, _i64, _pdbl]),
     "gmrf_solver_solve": (_i32, [_vp, _pdbl, _pdbl]),
     "gmrf_solver_diag_inv": (_i32, [_vp, _pdbl]),
     "gmrf_solver_log_abs_det": (_i32, [_vp, _pdbl]),

@@ -346,6 +346,13 @@ class HipSolver:
         x = np.ascontiguousarray(q.data, dtype=np.float64)
         check(L.lib.gmrf_solver_factorize(self._h, n, _i64p(ip), _i64p(ix), _dp(x)))
 
+    def factorize_dense(self, a) -> None:
+        """LdltBackend::factorize_dense (backend.rs:59-69): a is n x n, lower triangle read."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.ndim != 2 or a.shape[0] != a.shape[1]:
+            raise InvalidArgument(f"dimension mismatch: dense LDLᵀ matrix must be square (got {a.shape})")
+        check(L.lib.gmrf_solver_factorize_dense(self._h, a.shape[0], _dp(a)))
+
     def n(self) -> int:
         return int(L.lib.gmrf_solver_n(self._h))
 

@@ -283,3 +283,36 @@ def test_tall_fronts_split_solve_and_selinv(n, bw):
     di = np.diag(np.linalg.inv(qd))
     assert _rel(dg[0], di) < SELINV_RTOL
     assert _rel(dg[1], di / 3.0) < SELINV_RTOL
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 150])
+def test_factorize_dense_matches_numpy(n):
+    # LdltBackend::factorize_dense (backend.rs:59-69, dense_faer.rs:25-75): the lower
+    # triangle of a row-major n x n matrix; one dense front (n = 7 and 64: small-front
+    # kernels, 150: the tile-parallel path). log|A|, a solve and diag(A^-1) against
+    # numpy on the same matrix; the upper triangle holds garbage that must not be read.
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    full = G @ G.T / n + np.eye(n)
+    a_in = full.copy()
+    a_in[np.triu_indices(n, 1)] = 1e300
+    s = _solver()
+    s.factorize_dense(a_in)
+    _, ld = np.linalg.slogdet(full)
+    assert abs(s.log_abs_det() - ld) <= LOGDET_RTOL * max(abs(ld), 1.0)
+    b = rng.standard_normal(n)
+    assert _rel(s.solve(b), np.linalg.solve(full, b)) < SOLVE_RTOL
+    assert _rel(s.diag_inv(), np.diag(np.linalg.inv(full))) < SELINV_RTOL
+
+
+def test_factorize_dense_failure_is_singular_and_keeps_factor():
+    # dense_faer.rs:54-67: every dense failure is MathError::Singular; the solver keeps
+    # the previous factor (solver.rs:70-74)
+    from inla_gpu.solver import Singular
+
+    s = _solver()
+    s.factorize_dense(np.eye(3) * 2.0)
+    with pytest.raises(Singular):
+        s.factorize_dense(np.zeros((3, 3)))
+    assert abs(s.log_abs_det() - 3 * np.log(2.0)) < 1e-12
+    assert np.allclose(s.solve(np.array([2.0, 4.0, 6.0])), [1.0, 2.0, 3.0])
