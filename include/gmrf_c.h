@@ -270,6 +270,16 @@ gmrf_status gmrf_model_eval(gmrf_model* model, gmrf_eval_args* args);
  * second stream of the model (default priority) that forks from and joins back into
  * this one with events. */
 void* gmrf_model_stream(gmrf_model* model);
+
+/* Per-observation log-likelihood log pi(y_i | eta) for count predictor vectors
+ * (eta[b * n_obs + i], device; y, obs_a, obs_b per observation and lpar per vector,
+ * device; lpar NULL = 1, obs_b only for the negative binomial): eval_likelihood(..).logp
+ * of crates/inla_stats/src/inference.rs:170-311 as summed by deviance_at_config /
+ * compute_waic (crates/inla_stats/src/model_selection.rs:82-91,152-199). Enqueued on
+ * `stream` (NULL: the null stream); logp[b * n_obs + i] (device). */
+gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
+                            const double* eta, const double* y, const double* obs_a,
+                            const double* obs_b, const double* lpar, double* logp, void* stream);
 gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_given);
 
 /* ---- diagnostics -------------------------------------------------------------

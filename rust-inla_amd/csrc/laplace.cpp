@@ -974,4 +974,23 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   }
 }
 
+gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
+                            const double* eta, const double* y, const double* obs_a,
+                            const double* obs_b, const double* lpar, double* logp, void* stream) {
+  if (count < 0 || n_obs < 0) return fail(GMRF_INVALID_ARGUMENT, "negative size");
+  if (count == 0 || n_obs == 0) return GMRF_OK;
+  if (!eta || !y || !obs_a || !logp) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (family < 0 || family > 3 || link < 0 || link > 2) return fail(GMRF_INVALID_ARGUMENT, "unknown family or link");
+  if (family == 3 && !obs_b) return fail(GMRF_INVALID_ARGUMENT, "negative binomial needs obs_b (size)");
+  try {
+    check(hipSetDevice(g_device), "hipSetDevice");
+    check(lap_lik(count, n_obs, family, link, eta, y, obs_a, obs_b, lpar, logp, nullptr, nullptr, nullptr,
+                  nullptr, nullptr, (hipStream_t)stream),
+          "lik");
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 }  // extern "C"

@@ -143,6 +143,7 @@ SIGNATURES = {
     "gmrf_model_destroy": (None, [_vp]),
     "gmrf_model_eval": (_i32, [_vp, C.POINTER(EvalArgs)]),
     "gmrf_model_stream": (_vp, [_vp]),
+    "gmrf_obs_loglik": (_i32, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gmrf_model_set_stream": (_i32, [_vp, _vp, _i32]),
     "gmrf_profile_enable": (_i32, [_i32]),
     "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _pi32]),
