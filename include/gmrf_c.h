@@ -186,7 +186,10 @@ gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indp
  * is factorised as one dense front (the full pattern), after which solve / diag_inv /
  * log_abs_det apply as for gmrf_solver_factorize. A failed factorisation returns
  * GMRF_SINGULAR (the reference maps every dense failure to MathError::Singular) and
- * keeps the previous factor; n <= 0 is GMRF_INVALID_ARGUMENT. */
+ * keeps the previous factor. n == 0 stores the empty factor (dense_faer.rs:33-39):
+ * solve and diag_inv then write nothing and log_abs_det is 0; n < 0 is
+ * GMRF_INVALID_ARGUMENT. The full pattern's analysis stays with the solver (reused
+ * while n is unchanged), outside the process-wide pattern cache. */
 gmrf_status gmrf_solver_factorize_dense(gmrf_solver* s, int64_t n, const double* a);
 gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x);
 gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out);
@@ -275,8 +278,12 @@ void* gmrf_model_stream(gmrf_model* model);
  * (eta[b * n_obs + i], device; y, obs_a, obs_b per observation and lpar per vector,
  * device; lpar NULL = 1, obs_b only for the negative binomial): eval_likelihood(..).logp
  * of crates/inla_stats/src/inference.rs:170-311 as summed by deviance_at_config /
- * compute_waic (crates/inla_stats/src/model_selection.rs:82-91,152-199). Enqueued on
- * `stream` (NULL: the null stream); logp[b * n_obs + i] (device). */
+ * compute_waic (crates/inla_stats/src/model_selection.rs:82-91,152-199), on `stream`
+ * (NULL: the null stream); logp[b * n_obs + i] (device). The call synchronises the
+ * stream and returns GMRF_INVALID_ARGUMENT when an input is one on which
+ * eval_likelihood returns an error (binomial y > n or y < 0, a negative count or
+ * exposure, a Gaussian precision <= 0, a negative-binomial size <= 0 or mean <= 0,
+ * a non-finite eta); a zero Poisson rate with y > 0 is logp = -inf, not an error. */
 gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
                             const double* eta, const double* y, const double* obs_a,
                             const double* obs_b, const double* lpar, double* logp, void* stream);

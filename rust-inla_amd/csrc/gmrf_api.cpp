@@ -14,7 +14,7 @@
 
 namespace gmrf_rt {
 thread_local std::string g_err;
-int g_device = 0;
+thread_local int g_device = 0;
 Profiler g_prof;
 }  // namespace gmrf_rt
 
@@ -26,6 +26,23 @@ std::vector<std::shared_ptr<SymData>> g_cache;  // most recent last
 constexpr size_t kCacheMax = 16;
 
 }  // namespace
+
+std::shared_ptr<SymData> analyze_uncached(int64_t n, const int64_t* indptr, const int64_t* indices,
+                                          int order, const int64_t* given) {
+  std::string err = validate_pattern(n, indptr, indices);
+  if (!err.empty()) throw std::runtime_error(err);
+  auto d = std::make_shared<SymData>();
+  OrderRequest req = order == GMRF_ORDER_AMD       ? OrderRequest::AmdOnly
+                     : order == GMRF_ORDER_NATURAL ? OrderRequest::Natural
+                     : order == GMRF_ORDER_GIVEN   ? OrderRequest::Given
+                                                   : OrderRequest::Chooser;
+  d->S = analyze(n, indptr, indices, req, given);
+  d->pools = plan_pools(d->S);
+  d->device = g_device;
+  d->order_req = order;
+  if (order == GMRF_ORDER_GIVEN) d->given.assign(given, given + n);
+  return d;
+}
 
 std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                         int order, const int64_t* given) {
@@ -47,16 +64,7 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
       return hit;
     }
   }
-  auto d = std::make_shared<SymData>();
-  OrderRequest req = order == GMRF_ORDER_AMD       ? OrderRequest::AmdOnly
-                     : order == GMRF_ORDER_NATURAL ? OrderRequest::Natural
-                     : order == GMRF_ORDER_GIVEN   ? OrderRequest::Given
-                                                   : OrderRequest::Chooser;
-  d->S = analyze(n, indptr, indices, req, given);
-  d->pools = plan_pools(d->S);
-  d->device = g_device;
-  d->order_req = order;
-  if (order == GMRF_ORDER_GIVEN) d->given.assign(given, given + n);
+  auto d = analyze_uncached(n, indptr, indices, order, given);
   std::lock_guard<std::mutex> lk(g_cache_mu);
   g_cache.push_back(d);
   if (g_cache.size() > kCacheMax) g_cache.erase(g_cache.begin());
@@ -314,9 +322,46 @@ gmrf_status gmrf_solver_create(gmrf_solver** out) {
 void gmrf_solver_destroy(gmrf_solver* s) { delete s; }
 
 int64_t gmrf_solver_n(const gmrf_solver* s) {
-  if (!s || s->cur < 0) return 0;
+  if (!s || s->empty || s->cur < 0) return 0;
   return s->slot[s->cur].bt->sym->S.n;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Numeric factorisation of `data` (values on sym's pattern) into the solver's spare
+// slot; the slot becomes current only on success (solver.rs:70-74 keeps the old factor).
+gmrf_status solver_factorize_sym(gmrf_solver* s, const std::shared_ptr<SymData>& sym,
+                                 const double* data) {
+  const int target = s->cur < 0 ? 0 : 1 - s->cur;
+  auto& sl = s->slot[target];
+  if (!sl.bt || sl.bt->sym != sym) {
+    sl.bt.reset();
+    sl.bt.reset(new gmrf_batch(sym, 1));
+  }
+  const int64_t nnz = sym->S.nnz_full;
+  s->ensure(s->d_vals, s->d_vals_cap, (size_t)std::max<int64_t>(nnz, 1));
+  check(hipMemcpyAsync(s->d_vals, data, nnz * sizeof(double), hipMemcpyHostToDevice, sl.bt->stream),
+        "upload values");
+  sl.bt->factorize(1, s->d_vals, nnz);
+  double ld;
+  int32_t st;
+  gmrf_status rc = gmrf_batch_logdet(sl.bt.get(), 1, &ld, &st);
+  if (rc != GMRF_OK) return rc;
+  if (st == GMRF_NOT_POSITIVE_DEFINITE)
+    return fail(GMRF_NOT_POSITIVE_DEFINITE, "Matrix is not positive-definite during factorization");
+  if (st == GMRF_SINGULAR) return fail(GMRF_SINGULAR, "Matrix is singular or structurally rank-deficient");
+  sl.logdet = ld;
+  sl.have_selinv = false;
+  s->cur = target;
+  s->empty = false;
+  return GMRF_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indptr,
                                   const int64_t* indices, const double* data) {
@@ -325,62 +370,52 @@ gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indp
   try {
     check(hipSetDevice(g_device), "hipSetDevice");
     auto sym = analyze_cached(n, indptr, indices, GMRF_ORDER_CHOOSER, nullptr);
-    const int target = s->cur < 0 ? 0 : 1 - s->cur;
-    auto& sl = s->slot[target];
-    if (!sl.bt || sl.bt->sym != sym) {
-      sl.bt.reset();
-      sl.bt.reset(new gmrf_batch(sym, 1));
-    }
-    const int64_t nnz = indptr[n];
-    s->ensure(s->d_vals, s->d_vals_cap, (size_t)std::max<int64_t>(nnz, 1));
-    check(hipMemcpyAsync(s->d_vals, data, nnz * sizeof(double), hipMemcpyHostToDevice,
-                         sl.bt->stream),
-          "upload values");
-    sl.bt->factorize(1, s->d_vals, nnz);
-    double ld;
-    int32_t st;
-    gmrf_status rc = gmrf_batch_logdet(sl.bt.get(), 1, &ld, &st);
-    if (rc != GMRF_OK) return rc;
-    if (st == GMRF_NOT_POSITIVE_DEFINITE)
-      return fail(GMRF_NOT_POSITIVE_DEFINITE, "Matrix is not positive-definite during factorization");
-    if (st == GMRF_SINGULAR)
-      return fail(GMRF_SINGULAR, "Matrix is singular or structurally rank-deficient");
-    sl.logdet = ld;
-    sl.have_selinv = false;
-    s->cur = target;
-    return GMRF_OK;
+    return solver_factorize_sym(s, sym, data);
   } catch (...) {
     return from_exception();
   }
 }
 
 gmrf_status gmrf_solver_factorize_dense(gmrf_solver* s, int64_t n, const double* a) {
-  if (!s || !a) return fail(GMRF_INVALID_ARGUMENT, "null argument");
-  if (n <= 0) return fail(GMRF_INVALID_ARGUMENT, "empty or invalid matrix: n must be > 0");
-  std::vector<int64_t> ip, ix;
-  std::vector<double> v;
+  if (!s) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (n < 0) return fail(GMRF_INVALID_ARGUMENT, "dense LDL^T matrix length: n must be >= 0");
+  if (n == 0) {  // dense_faer.rs:33-39: an empty factor, not an error
+    s->empty = true;
+    return GMRF_OK;
+  }
+  if (!a) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   try {
-    // full symmetric pattern, column j = rows 0..n-1; entry (i, j) from the lower
-    // triangle a[max(i, j) n + min(i, j)]
-    ip.resize(n + 1);
-    ix.resize((size_t)n * n);
-    v.resize((size_t)n * n);
-    for (int64_t j = 0; j <= n; ++j) ip[j] = j * n;
+    check(hipSetDevice(g_device), "hipSetDevice");
+    // the full pattern's analysis is held by the solver's own slots (reused while n
+    // stays the same), not by the process-wide pattern cache
+    std::shared_ptr<SymData> sym;
+    for (auto& sl : s->slot)
+      if (sl.bt && sl.bt->sym->dense_n == n && sl.bt->sym->device == g_device) sym = sl.bt->sym;
+    if (!sym) {
+      std::vector<int64_t> ip(n + 1), ix((size_t)n * n);
+      for (int64_t j = 0; j <= n; ++j) ip[j] = j * n;
+      for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) ix[(size_t)j * n + i] = i;
+      sym = analyze_uncached(n, ip.data(), ix.data(), GMRF_ORDER_CHOOSER, nullptr);
+      sym->dense_n = n;
+    }
+    // column j of the full pattern holds rows 0..n-1; entry (i, j) comes from the
+    // lower triangle a[max(i, j) n + min(i, j)]
+    std::vector<double> v((size_t)n * n);
     for (int64_t j = 0; j < n; ++j)
-      for (int64_t i = 0; i < n; ++i) {
-        ix[(size_t)j * n + i] = i;
-        v[(size_t)j * n + i] = a[(size_t)std::max(i, j) * n + std::min(i, j)];
-      }
+      for (int64_t i = 0; i < n; ++i) v[(size_t)j * n + i] = a[(size_t)std::max(i, j) * n + std::min(i, j)];
+    const gmrf_status rc = solver_factorize_sym(s, sym, v.data());
+    if (rc == GMRF_NOT_POSITIVE_DEFINITE)
+      return fail(GMRF_SINGULAR, "Matrix is singular or structurally rank-deficient");
+    return rc;
   } catch (...) {
     return from_exception();
   }
-  const gmrf_status rc = gmrf_solver_factorize(s, n, ip.data(), ix.data(), v.data());
-  if (rc == GMRF_NOT_POSITIVE_DEFINITE) return fail(GMRF_SINGULAR, "Matrix is singular or structurally rank-deficient");
-  return rc;
 }
 
 gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x) {
   if (!s) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (s->empty) return GMRF_OK;  // 0 x 0 factor: nothing to write
   if (s->cur < 0) return fail(GMRF_SOLVE_FAILURE, "Linear system solve failed: Factorization not computed yet");
   if (!rhs || !x) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   try {
@@ -402,6 +437,7 @@ gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x) {
 
 gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out) {
   if (!s) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (s->empty) return GMRF_OK;  // 0 x 0 factor: nothing to write
   if (s->cur < 0) return fail(GMRF_SOLVE_FAILURE, "Linear system solve failed: Factorization not computed yet");
   if (!out) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   try {
@@ -426,6 +462,11 @@ gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out) {
 
 gmrf_status gmrf_solver_log_abs_det(const gmrf_solver* s, double* out) {
   if (!s) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (s->empty) {
+    if (!out) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+    *out = 0.0;  // empty sum of log|D|
+    return GMRF_OK;
+  }
   if (s->cur < 0) return fail(GMRF_SOLVE_FAILURE, "Linear system solve failed: Factorization not computed yet");
   if (!out) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   *out = s->slot[s->cur].logdet;

@@ -131,16 +131,19 @@ __device__ __forceinline__ void link_fwd(int link, double eta, double& mu, doubl
 // Per observation: logp, grad = dlogp/deta, w = -hess (the Newton weight the
 // reference uses: Fisher for Poisson / binomial, observed otherwise), dw/deta,
 // and for the Gaussian precision hyperparameter d(logp)/dlog(prec) (dl_par).
-// ok = 0 when the reference would return an error or a non-finite logp.
+// ok = 0 when the reference would return an error or a non-finite logp; inv = 1
+// only for the inputs on which the reference returns an error (Ok(-inf) for a zero
+// Poisson rate with y > 0 is not one).
 struct LikOut {
   double logp, grad, w, dw, dl_par;
-  int ok;
+  int ok, inv;
 };
 
 __device__ LikOut eval_obs(int fam, int link, double eta, double y, double a, double b, double lpar) {
-  LikOut o{0, 0, 0, 0, 0, 1};
+  LikOut o{0, 0, 0, 0, 0, 1, 0};
   if (!isfinite(eta)) {
     o.ok = 0;
+    o.inv = 1;
     return o;
   }
   double mu, d1, d2, d3;
@@ -153,7 +156,7 @@ __device__ LikOut eval_obs(int fam, int link, double eta, double y, double a, do
     o.w = prec * (d1 * d1 + r * d2);
     o.dw = prec * (3.0 * d1 * d2 + r * d3);
     o.dl_par = 0.5 - 0.5 * prec * r * r;
-    if (!(prec > 0.0) || !isfinite(prec) || !isfinite(y)) o.ok = 0;
+    if (!(prec > 0.0) || !isfinite(prec) || !isfinite(y)) o.ok = 0, o.inv = 1;
   } else if (fam == FAM_POISSON) {  // 188-223; a = exposure
     const double m = a * mu, dm = a * d1;
     if (m > 0.0) {
@@ -168,7 +171,7 @@ __device__ LikOut eval_obs(int fam, int link, double eta, double y, double a, do
       o.w = fabs(a * d2);
       o.dw = 0.0;
     }
-    if (a < 0.0 || !isfinite(a) || y < 0.0 || !isfinite(y)) o.ok = 0;
+    if (a < 0.0 || !isfinite(a) || y < 0.0 || !isfinite(y) || mu < 0.0) o.ok = 0, o.inv = 1;
   } else if (fam == FAM_BINOMIAL) {  // 225-267; a = trials
     const double n = a, p = mu;
     const double lc = lanczos_lgamma(n + 1.0) - lanczos_lgamma(y + 1.0) - lanczos_lgamma(n - y + 1.0);
@@ -182,11 +185,12 @@ __device__ LikOut eval_obs(int fam, int link, double eta, double y, double a, do
     o.grad = g * d1;
     o.w = n * d1;
     o.dw = n * d2;
-    if (n < 0.0 || y < 0.0 || y > n || !isfinite(n) || !isfinite(y) || p < 0.0 || p > 1.0) o.ok = 0;
+    if (n < 0.0 || y < 0.0 || y > n || !isfinite(n) || !isfinite(y) || p < 0.0 || p > 1.0) o.ok = 0, o.inv = 1;
   } else {  // 269-311; a = exposure, b = size
     const double m = a * mu, dm = a * d1, d2m = a * d2, d3m = a * d3, r = b, mr = m + r;
-    if (!(m > 0.0) || !(r > 0.0) || y < 0.0) {
+    if (!(m > 0.0) || !(r > 0.0) || !isfinite(r) || y < 0.0 || !isfinite(y) || a < 0.0 || !isfinite(a)) {
       o.ok = 0;
+      o.inv = 1;
       return o;
     }
     o.logp = lanczos_lgamma(y + r) - lanczos_lgamma(r) - lanczos_lgamma(y + 1.0) + r * log(r / mr) +
@@ -207,7 +211,8 @@ __global__ void k_lik(int B, int64_t nobs, int fam, int link, const double* __re
                       const double* __restrict__ y, const double* __restrict__ pa,
                       const double* __restrict__ pb, const double* __restrict__ lpar,
                       double* __restrict__ logp, double* __restrict__ grad, double* __restrict__ w,
-                      double* __restrict__ dw, double* __restrict__ dlpar, int32_t* __restrict__ bad) {
+                      double* __restrict__ dw, double* __restrict__ dlpar, int32_t* __restrict__ bad,
+                      int32_t* __restrict__ invalid) {
   const int64_t tot = (int64_t)B * nobs;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -220,6 +225,7 @@ __global__ void k_lik(int B, int64_t nobs, int fam, int link, const double* __re
     if (dw) dw[e] = o.dw;
     if (dlpar) dlpar[e] = o.dl_par;
     if (!o.ok && bad) bad[b] = 1;  // benign race: every writer stores 1
+    if (o.inv && invalid) invalid[b] = 1;
   }
 }
 
@@ -693,9 +699,9 @@ static inline int grid_for(int64_t tot) {
 
 hipError_t lap_lik(int B, int64_t nobs, int fam, int link, const double* eta, const double* y,
                    const double* pa, const double* pb, const double* lpar, double* logp, double* grad,
-                   double* w, double* dw, double* dlpar, int32_t* bad, hipStream_t st) {
+                   double* w, double* dw, double* dlpar, int32_t* bad, int32_t* invalid, hipStream_t st) {
   hipLaunchKernelGGL(k_lik, dim3(grid_for((int64_t)B * nobs)), dim3(256), 0, st, B, nobs, fam, link,
-                     eta, y, pa, pb, lpar, logp, grad, w, dw, dlpar, bad);
+                     eta, y, pa, pb, lpar, logp, grad, w, dw, dlpar, bad, invalid);
   return hipGetLastError();
 }
 

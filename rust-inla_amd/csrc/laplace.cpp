@@ -17,7 +17,7 @@
 namespace gmrf {
 hipError_t lap_lik(int, int64_t, int, int, const double*, const double*, const double*,
                    const double*, const double*, double*, double*, double*, double*, double*,
-                   int32_t*, hipStream_t);
+                   int32_t*, int32_t*, hipStream_t);
 hipError_t lap_affine(int, int, int64_t, const double*, const double*, double*, hipStream_t);
 hipError_t lap_add_diag(int, int64_t, int64_t, const int64_t*, double, double*, hipStream_t);
 hipError_t lap_add_atwa(int, int64_t, int64_t, const int64_t*, const int32_t*, const double*,
@@ -666,7 +666,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
             "A x");
       check(hipMemsetAsync(M->bad, 0, B * sizeof(int32_t), st), "zero bad");
       check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
-                    nullptr, nullptr, M->bad, st),
+                    nullptr, nullptr, M->bad, nullptr, st),
             "lik");
       check(lap_add_atwa(B, nnz, m, M->atwa_ptr, M->atwa_obs, M->atwa_coef, M->w, M->vprior,
                          M->vals, M->atwa_long.d, M->atwa_long.n, st),
@@ -766,7 +766,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
                 "A x trial");
           check(hipMemsetAsync(M->bad, 0, B * sizeof(int32_t), st), "zero bad");
           check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, nullptr,
-                        nullptr, nullptr, nullptr, M->bad, st),
+                        nullptr, nullptr, nullptr, M->bad, nullptr, st),
                 "lik trial");
           d2h(hbad.data(), M->bad, B * sizeof(int32_t), st);
           bool retry = false;
@@ -806,7 +806,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     // ---- mlik at x*: sum log p(y | A x*) - x*^T Q x* / 2 + (log|Q_prior| - log|Q_post|) / 2
     check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d, M->ar_long.n, st), "eta*");
     check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
-                  M->dw, M->dl, nullptr, st),
+                  M->dw, M->dl, nullptr, nullptr, st),
           "lik*");
     check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d, M->q_long.n, st), "Q x*");
     check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st), "quad");
@@ -909,7 +909,17 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       // Gaussian precision hyperparameter at fixed eta: sum dlogp/dlog(prec), sum v w
       check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, M->red, st), "sum dl");
       check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, M->red + B, st), "sum v w");
-      std::vector<double> hs(2 * B);
+      // and, where the weights depend on eta (a non-identity link), the mode's
+      // sensitivity to the precision: g scales with it, so dx*/dlog(prec) =
+      // Q_post^-1 A^T g(x*), contributing -1/2 lambda^T A^T g(x*)
+      const bool gauss_link = M->fam == 0 && !gauss_id;
+      if (gauss_link) {
+        check(lap_spmv(B, n, M->a_cptr, M->a_ridx, M->a_cval, 0, M->g, m, 1.0, 0.0, M->tmpn, n, M->ac_long.d,
+                       M->ac_long.n, st),
+              "A^T g*");
+        check(lap_rowreduce(B, n, RED_DOT, M->lam, M->tmpn, M->red + 2 * B, st), "lambda^T A^T g");
+      }
+      std::vector<double> hs((gauss_link ? 3 : 2) * B);
       std::vector<double> htp((size_t)B * K), htq((size_t)B * K), hqx((size_t)B * K);
       std::vector<double> dc((size_t)B * mt * K), dlp((size_t)B * mt, 0.0);
       {
@@ -957,7 +967,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
             gq += c * (-0.5 * hqx[b * K + k] + 0.5 * htq[b * K + k] - 0.5 * htp[b * K + k] +
                        0.5 * hql[b * K + k]);
           }
-          if (M->fam == 0) gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[B + b]);
+          if (M->fam == 0)
+            gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[B + b] - (gauss_link ? 0.5 * hs[2 * B + b] : 0.0));
           grad[(size_t)b * mt + l] = status[b] == GMRF_OK ? gq : NAN;
         }
       std::memcpy(pout + B, grad.data(), grad.size() * sizeof(double));
@@ -982,13 +993,27 @@ gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t
   if (!eta || !y || !obs_a || !logp) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (family < 0 || family > 3 || link < 0 || link > 2) return fail(GMRF_INVALID_ARGUMENT, "unknown family or link");
   if (family == 3 && !obs_b) return fail(GMRF_INVALID_ARGUMENT, "negative binomial needs obs_b (size)");
+  int32_t* flag = nullptr;
   try {
     check(hipSetDevice(g_device), "hipSetDevice");
+    const hipStream_t st = (hipStream_t)stream;
+    check(hipMalloc(&flag, count * sizeof(int32_t)), "hipMalloc");
+    check(hipMemsetAsync(flag, 0, count * sizeof(int32_t), st), "zero flag");
     check(lap_lik(count, n_obs, family, link, eta, y, obs_a, obs_b, lpar, logp, nullptr, nullptr, nullptr,
-                  nullptr, nullptr, (hipStream_t)stream),
+                  nullptr, nullptr, flag, st),
           "lik");
+    std::vector<int32_t> inv(count);
+    check(hipMemcpyAsync(inv.data(), flag, count * sizeof(int32_t), hipMemcpyDeviceToHost, st), "D2H");
+    check(hipStreamSynchronize(st), "sync");
+    (void)hipFree(flag);
+    flag = nullptr;
+    for (int32_t b = 0; b < count; ++b)
+      if (inv[b])  // eval_likelihood returns Err for these inputs (inference.rs:170-311)
+        return fail(GMRF_INVALID_ARGUMENT, "invalid observation for the likelihood family (vector " +
+                                               std::to_string(b) + ")");
     return GMRF_OK;
   } catch (...) {
+    if (flag) (void)hipFree(flag);
     return from_exception();
   }
 }

@@ -78,7 +78,7 @@ using namespace gmrf;
 namespace gmrf_rt {
 
 extern thread_local std::string g_err;
-extern int g_device;
+extern thread_local int g_device;
 
 inline gmrf_status fail(gmrf_status st, const std::string& msg) {
   g_err = msg;
@@ -355,6 +355,7 @@ struct SymData {
   int32_t* d_all = nullptr;
   int32_t* d_tasks = nullptr;  // large-front task triples
   int order_req = 0;
+  int64_t dense_n = -1;        // >= 0: the full n x n pattern of a dense input (not cached)
   std::vector<int64_t> given;  // given permutation (cache key)
   std::mutex dev_mu;
   bool dev_built = false;
@@ -1092,6 +1093,7 @@ struct gmrf_solver {
   };
   Slot slot[2];
   int cur = -1;
+  bool empty = false;  // the 0 x 0 dense factor (dense_faer.rs:33-39): valid, nothing stored
   double* d_vals = nullptr;
   size_t d_vals_cap = 0;
   double* d_vec = nullptr;
@@ -1113,3 +1115,7 @@ struct gmrf_solver {
 
 std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                         int order, const int64_t* given);
+// The same analysis kept out of the process-wide cache (dense inputs: the solver
+// holds the full pattern's analysis itself, keyed by n).
+std::shared_ptr<SymData> analyze_uncached(int64_t n, const int64_t* indptr, const int64_t* indices,
+                                          int order, const int64_t* given);

@@ -122,11 +122,13 @@ class LaplaceModel:
         self._h = h
         del keep
 
-    def __del__(self):
+    # the free function is bound when the class is defined: at interpreter shutdown
+    # the module globals may already be gone when the last handles are collected
+    def __del__(self, _free=L.lib.gmrf_model_destroy):
         h = getattr(self, "_h", None)
         if h:
-            L.lib.gmrf_model_destroy(h)
             self._h = None
+            _free(h)
 
     def eval(self, coefs, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
              max_iter=200, tol=1e-5):

@@ -99,11 +99,13 @@ class Symbolic:
         q = as_csc(q)
         return cls(q.shape[0], q.indptr, q.indices, **kw)
 
-    def __del__(self):
+    # the free function is bound when the class is defined: at interpreter shutdown
+    # the module globals may already be gone when the last handles are collected
+    def __del__(self, _free=L.lib.gmrf_symbolic_free):
         h = getattr(self, "_h", None)
         if h:
-            L.lib.gmrf_symbolic_free(h)
             self._h = None
+            _free(h)
 
     @property
     def handle(self):
@@ -164,11 +166,13 @@ class Batch:
         check(L.lib.gmrf_batch_create(sym.handle, self.capacity, C.byref(h)))
         self._h = h
 
-    def __del__(self):
+    # the free function is bound when the class is defined: at interpreter shutdown
+    # the module globals may already be gone when the last handles are collected
+    def __del__(self, _free=L.lib.gmrf_batch_destroy):
         h = getattr(self, "_h", None)
         if h:
-            L.lib.gmrf_batch_destroy(h)
             self._h = None
+            _free(h)
 
     def _stream(self):
         import torch
@@ -292,12 +296,12 @@ class MultiBatch(Batch):
         self.sym = None
         self._h = C.c_void_p(L.lib.gmrf_multi_batch(h))  # owned by the multi handle
 
-    def __del__(self):
+    def __del__(self, _free=L.lib.gmrf_multi_free):
         m = getattr(self, "_m", None)
         if m:
             self._h = None
-            L.lib.gmrf_multi_free(m)
             self._m = None
+            _free(m)
 
     def factorize(self, values):
         v = self._dev(values, self.nnz, "values")
@@ -330,11 +334,13 @@ class HipSolver:
         check(L.lib.gmrf_solver_create(C.byref(h)))
         self._h = h
 
-    def __del__(self):
+    # the free function is bound when the class is defined: at interpreter shutdown
+    # the module globals may already be gone when the last handles are collected
+    def __del__(self, _free=L.lib.gmrf_solver_destroy):
         h = getattr(self, "_h", None)
         if h:
-            L.lib.gmrf_solver_destroy(h)
             self._h = None
+            _free(h)
 
     def factorize(self, q) -> None:
         q = as_csc(q)

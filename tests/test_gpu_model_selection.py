@@ -94,3 +94,35 @@ def test_argument_errors():
         compute_dic("gaussian", "identity", y, eta, [0.5, 0.5], 2)
     with pytest.raises(InvalidArgument):
         compute_waic("gaussian", "identity", y, eta, [1.0])
+
+
+INVALID = [
+    ("binomial", "logit", [3.0], [2.0], None),      # y > n
+    ("binomial", "logit", [-1.0], [2.0], None),     # y < 0
+    ("poisson", "log", [-1.0], [1.0], None),        # negative count
+    ("poisson", "log", [1.0], [-1.0], None),        # negative exposure
+    ("gaussian", "identity", [0.5], [0.0], None),   # precision <= 0
+    ("nbinomial", "log", [1.0], [1.0], [0.0]),      # size <= 0
+    ("nbinomial", "log", [1.0], [0.0], [2.0]),      # mean <= 0 (zero exposure)
+]
+
+
+@pytest.mark.parametrize("case", INVALID, ids=[f"{c[0]}-{i}" for i, c in enumerate(INVALID)])
+def test_obs_loglik_invalid_inputs_are_errors(case):
+    # eval_likelihood returns Err for these (inference.rs:170-311); the device path
+    # reports GMRF_INVALID_ARGUMENT instead of a number
+    from inla_gpu.model_selection import obs_loglik
+    from inla_gpu.solver import InvalidArgument
+
+    fam, link, y, a, b = case
+    eta = np.array([[0.3, 0.3]])
+    with pytest.raises(InvalidArgument):
+        obs_loglik(fam, link, eta, y * 2, a * 2, None if b is None else b * 2)
+
+
+def test_obs_loglik_zero_poisson_rate_is_minus_infinity():
+    # a zero rate with y > 0 is Ok(-inf) in the reference (inference.rs:204-210), not an error
+    from inla_gpu.model_selection import obs_loglik
+
+    got = obs_loglik("poisson", "log", np.array([[0.1, 0.1]]), [2.0, 0.0], [0.0, 0.0]).cpu().numpy()
+    assert got[0, 0] == -np.inf and got[0, 1] == 0.0

@@ -316,3 +316,16 @@ def test_factorize_dense_failure_is_singular_and_keeps_factor():
         s.factorize_dense(np.zeros((3, 3)))
     assert abs(s.log_abs_det() - 3 * np.log(2.0)) < 1e-12
     assert np.allclose(s.solve(np.array([2.0, 4.0, 6.0])), [1.0, 2.0, 3.0])
+
+
+def test_factorize_dense_empty_matrix():
+    # dense_faer.rs:33-39: n = 0 is an empty factor, not an error; the sum of log|D| is 0
+    s = _solver()
+    s.factorize_dense(np.zeros((0, 0)))
+    assert s.n() == 0
+    assert s.log_abs_det() == 0.0
+    assert s.solve(np.zeros(0)).shape == (0,)
+    assert s.diag_inv().shape == (0,)
+    # a later factorisation replaces it as usual
+    s.factorize_dense(np.eye(2) * 4.0)
+    assert abs(s.log_abs_det() - 2 * np.log(4.0)) < 1e-12

@@ -2,8 +2,8 @@
 Besag x AR1 group model, binomial(10), one sum-to-zero row per time point
 (kappa A_c^T A_c augmentation, constraints.rs:275-313); SPDE x RW2, negative
 binomial (size 5) through the space-time projector. Bars as the other Laplace
-tests: mlik 1e-8 absolute (scaled by max(1, |mlik|)), mode 1e-9 relative,
-variances 1e-9 relative."""
+tests: mlik 1e-8 absolute, mode 1e-10 relative, variances 1e-9 relative; each
+achieved error is recorded (the record_error fixture)."""
 
 import math
 
@@ -16,6 +16,9 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 KAPPA = 3.059023205018258e6  # the oracle's (and reference's) augmentation constant
+MLIK_ATOL = 1e-8
+MODE_RTOL = 1e-10
+VAR_RTOL = 1e-9
 
 
 def _rel(a, b):
@@ -39,7 +42,7 @@ def _besag_ar1_model(nr=6, T=4):
     return wl, model, ca, ce
 
 
-def test_besag_ar1_binomial_constrained_matches_oracle():
+def test_besag_ar1_binomial_constrained_matches_oracle(record_error):
     wl, model, ca, ce = _besag_ar1_model()
     thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.3, -0.2], [-0.2, 0.4]])
     c3, _, _, _ = wl.params(thetas)
@@ -48,13 +51,15 @@ def test_besag_ar1_binomial_constrained_matches_oracle():
     for b in range(len(thetas)):
         ref = orc.find_mode(wl.gmrf.csc(c3[b]), "binomial", "logit", wl.y, wl.obs_a, constraints=(ca, ce),
                             keep_post=True)
-        assert abs(r["mlik"][b].item() - ref.mlik) < 1e-8 * max(1.0, abs(ref.mlik))
-        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
+        e_ml = record_error("mlik", abs(r["mlik"][b].item() - ref.mlik), MLIK_ATOL, "abs")
+        e_x = record_error("mode", _rel(r["x"][b].cpu().numpy(), ref.x), MODE_RTOL)
+        assert e_ml < MLIK_ATOL
+        assert e_x < MODE_RTOL
         # every time slice sums to zero at the mode
-        assert np.max(np.abs(ca @ r["x"][b].cpu().numpy())) < 1e-8
+        assert np.max(np.abs(ca @ r["x"][b].cpu().numpy())) < 1e-12
 
 
-def test_besag_ar1_gradient_matches_central_fd():
+def test_besag_ar1_gradient_matches_central_fd(record_error):
     wl, model, ca, ce = _besag_ar1_model()
     th = wl.theta_true + np.array([0.1, -0.1])
     c3, dc3, _, _ = wl.params(th[None])
@@ -71,10 +76,11 @@ def test_besag_ar1_gradient_matches_central_fd():
                                       constraints=(ca, ce)).mlik)
         fd.append((vals[0] - vals[1]) / (2 * h))
     g = r["grad"][0].cpu().numpy()
-    assert _rel(g, np.array(fd)) < 1e-5, (g, fd)
+    err = record_error("gradient vs FD", _rel(g, np.array(fd)), 1e-5)
+    assert err < 1e-5, (g, fd)
 
 
-def test_spde_rw2_negbin_matches_oracle():
+def test_spde_rw2_negbin_matches_oracle(record_error):
     from inla_gpu import workloads as W
     from inla_gpu.laplace import LaplaceModel
 
@@ -89,17 +95,20 @@ def test_spde_rw2_negbin_matches_oracle():
     for b in range(len(thetas)):
         ref = orc.find_mode(g.csc(c[b]), "nbinomial", "log", wl.y, wl.obs_a, pb=wl.obs_b, A=wl.A,
                             keep_post=True)
-        assert abs(r["mlik"][b].item() - ref.mlik) < 1e-8 * max(1.0, abs(ref.mlik))
-        assert _rel(r["x"][b].cpu().numpy(), ref.x) < 1e-9
-        assert _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()) < 1e-9
+        e_ml = record_error("mlik", abs(r["mlik"][b].item() - ref.mlik), MLIK_ATOL, "abs")
+        e_x = record_error("mode", _rel(r["x"][b].cpu().numpy(), ref.x), MODE_RTOL)
+        e_v = record_error("variances", _rel(r["var"][b].cpu().numpy(), ref.post_factor().takahashi_diag()),
+                           VAR_RTOL)
+        assert e_ml < MLIK_ATOL
+        assert e_x < MODE_RTOL
+        assert e_v < VAR_RTOL
 
 
-def test_besag_ar1_lowrank_constraints_match_augmented():
+def test_besag_ar1_lowrank_constraints_match_augmented(record_error):
     """The augmentation applied as a low-rank correction of the grounded precision
-    (constraint_kappa) against the explicit kappa A_c^T A_c component and the oracle.
-    The explicit form carries the conditioning of kappa A_c^T A_c (log-determinant
-    errors near 1e-8 at this size); the low-rank form is exact up to rounding (see
-    tests/test_lowrank_identity.py), so the bars here are 1e-7."""
+    (constraint_kappa) against the explicit kappa A_c^T A_c component and the oracle,
+    at the standard bars (the low-rank identity itself is pinned in
+    tests/test_lowrank_identity.py)."""
     from inla_gpu.laplace import LaplaceModel
 
     wl, model, ca, ce = _besag_ar1_model()
@@ -116,16 +125,19 @@ def test_besag_ar1_lowrank_constraints_match_augmented():
         ref = orc.find_mode(g.csc(c3[b]), "binomial", "logit", wl.y, wl.obs_a, constraints=(ca, ce),
                             keep_post=True)
         m = rl["mlik"][b].item()
-        assert abs(m - ref.mlik) < 1e-7 * max(1.0, abs(ref.mlik))
-        assert abs(m - ra["mlik"][b].item()) < 1e-7 * max(1.0, abs(ref.mlik))
+        e1 = record_error("mlik low-rank vs oracle", abs(m - ref.mlik), MLIK_ATOL, "abs")
+        e2 = record_error("mlik low-rank vs component", abs(m - ra["mlik"][b].item()), MLIK_ATOL, "abs")
         x = rl["x"][b].cpu().numpy()
-        assert _rel(x, ref.x) < 1e-7
-        assert np.max(np.abs(ca @ x)) < 1e-8
-        assert _rel(rl["var"][b].cpu().numpy(), ra["var"][b].cpu().numpy()) < 1e-7
-        assert _rel(rl["grad"][b].cpu().numpy(), ra["grad"][b].cpu().numpy()) < 1e-6
+        e3 = record_error("mode low-rank vs oracle", _rel(x, ref.x), MODE_RTOL)
+        e4 = record_error("variances low-rank vs comp.", _rel(rl["var"][b].cpu().numpy(),
+                                                              ra["var"][b].cpu().numpy()), VAR_RTOL)
+        e5 = record_error("gradient low-rank vs comp.", _rel(rl["grad"][b].cpu().numpy(),
+                                                             ra["grad"][b].cpu().numpy()), 1e-8)
+        assert e1 < MLIK_ATOL and e2 < MLIK_ATOL and e3 < MODE_RTOL and e4 < VAR_RTOL and e5 < 1e-8
+        assert np.max(np.abs(ca @ x)) < 1e-12
 
 
-def test_besag_ar1_lowrank_gradient_matches_central_fd():
+def test_besag_ar1_lowrank_gradient_matches_central_fd(record_error):
     from inla_gpu.laplace import LaplaceModel
 
     wl, _, ca, ce = _besag_ar1_model()
@@ -145,4 +157,5 @@ def test_besag_ar1_lowrank_gradient_matches_central_fd():
             c, _, _, _ = wl.params(t[None])
             vals.append(lr.eval(c)["mlik"][0].item())
         fd.append((vals[0] - vals[1]) / (2 * h))
-    assert _rel(gr, np.array(fd)) < 1e-5, (gr, fd)
+    err = record_error("gradient vs FD (low-rank)", _rel(gr, np.array(fd)), 1e-5)
+    assert err < 1e-5, (gr, fd)
