@@ -128,6 +128,18 @@ gmrf_status gmrf_batch_solve(gmrf_batch* bt, int32_t count, const double* rhs, i
  * (sparse_ldlt.rs:243-265); several columns serve low-rank corrections. */
 gmrf_status gmrf_batch_solve_multi(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
                                    int64_t ld_rhs, double* x, int64_t ld_x);
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * gmrf_batch_solve_multi with the same arguments and the same results, whose
+ * workspace (count * nrhs vectors of n + update + staging doubles; the batch's own
+ * multi-vector area is grown with hipMalloc on demand) is instead a fixed 32 MiB
+ * array inside the library, one per GPU, held until the process ends. Requests
+ * that do not fit return GMRF_INVALID_ARGUMENT (the message gives the limit for the
+ * pattern) before any work is enqueued; nothing is allocated. Every call on one GPU
+ * uses that GPU's array: calls that overlap on one GPU (several streams or threads)
+ * race; calls ordered on one stream do not. */
+gmrf_status gmrf_batch_solve_noalloc(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
+                                     int64_t ld_rhs, double* x, int64_t ld_x);
 /* Selected inversion on the supernodal pattern of L (Takahashi recurrence). */
 gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count);
 /* diag(Q_b^-1) in original order after gmrf_batch_selinv (sparse_ldlt.rs:421-431). */

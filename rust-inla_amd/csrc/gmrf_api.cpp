@@ -20,6 +20,31 @@ Profiler g_prof;
 
 namespace {
 
+// Workspace of gmrf_batch_solve_noalloc: a fixed array in device memory (one copy
+// per GPU, held until the process ends) for callers that cannot allocate. In host
+// code `scratch` only names the symbol; its device address on each GPU comes from
+// hipGetSymbolAddress (scratch_on), and only that address is passed to kernels.
+// Calls that overlap on one GPU share the array and race.
+constexpr int64_t kScratchDoubles = (int64_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The array's address on `device` (looked up once, with that device current).
+double* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    check(hipSetDevice(device), "hipSetDevice");
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[device] = (double*)p;
+  }
+  return g_scratch_ptr[device];
+}
+
 // Process-wide pattern cache (the reference keeps one per solver; scratch.rs:92-118).
 std::mutex g_cache_mu;
 std::vector<std::shared_ptr<SymData>> g_cache;  // most recent last
@@ -251,6 +276,32 @@ gmrf_status gmrf_batch_solve_multi(gmrf_batch* bt, int32_t count, int32_t nrhs, 
   try {
     check(hipSetDevice(bt->sym->device), "hipSetDevice");
     bt->solve(count, rhs, ld_rhs, x, ld_x, 0, nrhs);
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+gmrf_status gmrf_batch_solve_noalloc(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
+                                     int64_t ld_rhs, double* x, int64_t ld_x) {
+  if (!bt || !rhs || !x) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (count < 1 || count > bt->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds batch capacity");
+  if (nrhs < 1 || (int64_t)count * nrhs > 65535)
+    return fail(GMRF_INVALID_ARGUMENT, "nrhs must be >= 1 and count * nrhs <= 65535");
+  if (ld_rhs < bt->sym->S.n || ld_x < bt->sym->S.n)
+    return fail(GMRF_INVALID_ARGUMENT, "leading dimension < n");
+  const int64_t per = bt->solve_ws_per_vector();
+  const int64_t need = (int64_t)count * nrhs * per;
+  if (need > kScratchDoubles)
+    return fail(GMRF_INVALID_ARGUMENT,
+                "allocation-free solve workspace exceeded: count * nrhs = " + std::to_string((int64_t)count * nrhs) +
+                    " vectors need " + std::to_string(need * 8) + " bytes, the fixed area holds " +
+                    std::to_string(kScratchDoubles * 8) + " (at most " + std::to_string(kScratchDoubles / per) +
+                    " vectors of this pattern)");
+  try {
+    double* ws = scratch_on(bt->sym->device);
+    check(hipSetDevice(bt->sym->device), "hipSetDevice");
+    bt->solve(count, rhs, ld_rhs, x, ld_x, 0, nrhs, 0, ws);
     return GMRF_OK;
   } catch (...) {
     return from_exception();

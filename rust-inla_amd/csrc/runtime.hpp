@@ -933,15 +933,29 @@ struct gmrf_batch {
     });
   }
 
+  // Doubles of solve workspace per right-hand side (permuted vector, update
+  // entries, split-K partial sums).
+  int64_t solve_ws_per_vector() const {
+    const DevPlan& P = sym->plan;
+    return P.n + std::max<int64_t>(P.v_size, 1) + P.ts_size;
+  }
+
   // Matrices [first, first + count), nr right-hand sides each: vector j of
   // matrix first + b is rhs + (b nr + j) ldr (rhs + ((b nr + j) % period) ldr when
   // period > 0: one set of vectors for every matrix), its solution x + (b nr + j) ldx.
+  // ws (count nr solve_ws_per_vector() doubles), when given, holds the workspace
+  // instead of the batch's own areas; nothing is allocated then.
   void solve(int count, const double* rhs, int64_t ldr, double* x, int64_t ldx, int first = 0,
-             int nr = 1, int period = 0) {
+             int nr = 1, int period = 0, double* ws = nullptr) {
     const DevPlan& P = sym->plan;
     const int64_t vsz = std::max<int64_t>(P.v_size, 1);
     double *Xf, *Vf, *Tf;
-    if (nr == 1) {
+    if (ws) {
+      const int64_t vecs = (int64_t)count * nr;
+      Xf = ws;
+      Vf = Xf + vecs * P.n;
+      Tf = Vf + vecs * vsz;
+    } else if (nr == 1) {
       Xf = X + (int64_t)first * P.n;
       Vf = V + (int64_t)first * vsz;
       Tf = TS + (int64_t)first * P.ts_size;

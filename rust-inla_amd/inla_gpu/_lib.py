@@ -117,6 +117,7 @@ SIGNATURES = {
     "gmrf_batch_logdet_device": (_i32, [_vp, _i32, _vp, _vp]),
     "gmrf_batch_solve": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64]),
     "gmrf_batch_solve_multi": (_i32, [_vp, _i32, _i32, _vp, _i64, _vp, _i64]),
+    "gmrf_batch_solve_noalloc": (_i32, [_vp, _i32, _i32, _vp, _i64, _vp, _i64]),
     "gmrf_batch_selinv": (_i32, [_vp, _i32]),
     "gmrf_batch_selinv_diag": (_i32, [_vp, _i32, _vp, _i64]),
     "gmrf_batch_selinv_pattern": (_i32, [_vp, _i32, _vp, _i64]),
