@@ -20,7 +20,17 @@
 //    including the binary-search lookup; pinned against dense inverses and the
 //    reference's own known-answer tests (sparse_ldlt.rs:613-665).
 //  * Laplace Newton loop, mlik: restated from inference.rs:776-975 with the
-//    likelihoods of inference.rs:170-311; pinned by the Gaussian closed form.
+//    likelihoods of inference.rs:170-311; pinned by the Gaussian closed form,
+//    stationarity of every family's mode, and a dense brute force of the
+//    constrained (kappa-augmented, projected) Gaussian case.
+//  * ccd_design / grid_design / invert_symmetric_matrix / jacobi_eigen: restated
+//    from crates/inla_math/src/integration.rs; pinned by what the mathematics fixes
+//    (the CCD mixture of a standard normal has identity covariance, the grid of a
+//    quadratic log-density is a closed-form lattice set, A A^-1 = I, V L V^T = A).
+//
+// orc_plant(code) switches on ONE deliberate mistake (the Plant list below) so
+// that tests/test_oracle_pins.py can show every pinning check fails when the
+// function it pins is wrong. It is 0 (no mistake) everywhere else.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -33,19 +43,56 @@ typedef int64_t i64;
 
 namespace {
 
+enum Plant {
+  P_NONE = 0,
+  P_LOWER_BW,        // off-by-one bandwidth
+  P_BW_FWD,          // column index not relabelled
+  P_TIME_MAJOR,      // identity relabelling instead of time-major
+  P_RCM,             // Cuthill-McKee order not reversed
+  P_CHOOSE,          // RCM accepted without the halving test
+  P_MIN_DEGREE,      // natural order instead of minimum degree
+  P_LDLT,            // pivot update term dropped
+  P_LOGDET,          // log D instead of log |D|
+  P_SOLVE,           // diagonal scaling skipped
+  P_TAKAHASHI,       // sign of the off-diagonal recurrence flipped
+  P_SELINV_LOOKUP,   // entries looked up with the wrong permutation
+  P_DENSE_LDLT,      // D dropped from the off-diagonal update
+  P_LGAMMA,          // one Lanczos coefficient perturbed
+  P_LIK_GAUSS,       // Gaussian normalising term dropped
+  P_LIK_POISSON,     // exposure missing from the Poisson weight
+  P_LIK_BINOMIAL,    // (1 - p) missing from the binomial weight
+  P_LIK_NBINOMIAL,   // second-derivative term of the link dropped
+  P_NEWTON_RHS,      // - Q x missing from the Newton right-hand side
+  P_PROJECT,         // projection correction added instead of subtracted
+  P_MLIK,            // log|Q_prior| without the factor 1/2
+  P_KAPPA,           // augmentation constant halved
+  P_CCD5,            // half-fraction sign of the fifth factor flipped
+  P_CCD_WEIGHT,      // f0 instead of f0^2 in the weight
+  P_GRID,            // first point beyond the threshold kept
+  P_JACOBI,          // rotation applied with the wrong sign to the eigenvectors
+  P_INVERT,          // pivot row scaled only in its left half
+  P_NNZ_L,           // diagonal not counted
+  P_PERM,            // inverse permutation returned
+  P_D,               // D returned in original instead of factor order
+};
+int g_plant = P_NONE;
+
 // ----------------------------------------------------------------- ordering.rs
 i64 lower_bandwidth(i64 n, const i64* ip, const i64* ix) {  // ordering.rs:8-18
   i64 bw = 0;
   for (i64 c = 0; c < n; ++c)
     for (i64 p = ip[c]; p < ip[c + 1]; ++p)
-      if (ix[p] >= c) bw = std::max(bw, ix[p] - c);
+      if (ix[p] >= c) bw = std::max(bw, ix[p] - c + (g_plant == P_LOWER_BW ? 1 : 0));
   return bw;
 }
 
 i64 bandwidth_with_fwd(i64 n, const i64* ip, const i64* ix, const std::vector<i64>& fwd) {
   i64 bw = 0;  // ordering.rs:21-30
   for (i64 c = 0; c < n; ++c)
-    for (i64 p = ip[c]; p < ip[c + 1]; ++p) { i64 dd = fwd[ix[p]] - fwd[c]; bw = std::max(bw, dd < 0 ? -dd : dd); }
+    for (i64 p = ip[c]; p < ip[c + 1]; ++p) {
+      i64 dd = fwd[ix[p]] - (g_plant == P_BW_FWD ? c : fwd[c]);
+      bw = std::max(bw, dd < 0 ? -dd : dd);
+    }
   return bw;
 }
 
@@ -62,7 +109,7 @@ bool time_major(i64 n, const i64* ip, const i64* ix, std::vector<i64>& fwd) {  /
     if (nt < 8) continue;
     std::vector<i64> f(n);
     for (i64 c = 0; c < nc; ++c)
-      for (i64 t = 0; t < nt; ++t) f[c * nt + t] = t * nc + c;  // ordering.rs:36-49
+      for (i64 t = 0; t < nt; ++t) f[c * nt + t] = g_plant == P_TIME_MAJOR ? c * nt + t : t * nc + c;  // ordering.rs:36-49
     i64 b = bandwidth_with_fwd(n, ip, ix, f);
     if (b < bw && !(have && best <= b)) {
       have = true;
@@ -115,7 +162,7 @@ void rcm(i64 n, const i64* ip, const i64* ix, std::vector<i64>& fwd) {  // order
         }
     }
   }
-  std::reverse(order.begin(), order.end());
+  if (g_plant != P_RCM) std::reverse(order.begin(), order.end());
   fwd.assign(n, 0);
   for (i64 w = 0; w < n; ++w) fwd[order[w]] = w;
 }
@@ -125,7 +172,7 @@ int choose(i64 n, const i64* ip, const i64* ix, std::vector<i64>& fwd) {  // ord
   if (time_major(n, ip, ix, fwd)) return 1;
   if (bw > 32) {
     rcm(n, ip, ix, fwd);
-    if (bandwidth_with_fwd(n, ip, ix, fwd) * 2 < bw) return 2;
+    if (g_plant == P_CHOOSE || bandwidth_with_fwd(n, ip, ix, fwd) * 2 < bw) return 2;
   }
   fwd.clear();
   return 0;
@@ -143,6 +190,11 @@ void min_degree(i64 n, const i64* ip, const i64* ix, std::vector<i64>& order) {
   for (auto& a : adj) {
     std::sort(a.begin(), a.end());
     a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  if (g_plant == P_MIN_DEGREE) {
+    order.resize(n);
+    for (i64 i = 0; i < n; ++i) order[i] = i;
+    return;
   }
   std::vector<char> done(n, 0);
   std::vector<std::vector<i64>> bucket(n + 1);
@@ -252,7 +304,7 @@ Factor factor(i64 n, const i64* ip, const i64* ix, const double* x, const std::v
       y[i] = 0.0;
       for (i64 p = F.lp[i]; p < fill[i]; ++p) y[F.li[p]] -= F.lx[p] * yi;
       double l = yi / F.d[i];
-      F.d[k] -= l * yi;
+      if (g_plant != P_LDLT) F.d[k] -= l * yi;
       F.li[fill[i]] = k;
       F.lx[fill[i]] = l;
       fill[i]++;
@@ -324,7 +376,7 @@ int takahashi(const Filled& l, i64 n, std::vector<double>& s) {
         if (!lookup_s(l, s, k, i, sv)) return 2;
         acc += l.v[pk] * sv;
       }
-      s[p] = -acc;
+      s[p] = g_plant == P_TAKAHASHI ? acc : -acc;
     }
     double sjj = 1.0 / dj;
     for (i64 pk = start; pk < end; ++pk) {
@@ -353,7 +405,7 @@ double lanczos_log_gamma(double z) {
   if (z < 0.5) return std::log(PI) - std::log(std::sin(PI * z)) - lanczos_log_gamma(1.0 - z);
   double z1 = z - 1.0;
   double x = C[0];
-  for (int i = 0; i < 8; ++i) x += C[i + 1] / (z1 + (double)i + 1.0);
+  for (int i = 0; i < 8; ++i) x += (C[i + 1] + (g_plant == P_LGAMMA && i == 0 ? 1e-6 : 0.0)) / (z1 + (double)i + 1.0);
   double t = z1 + 7.5;
   return 0.5 * std::log(2.0 * PI) + (z1 + 0.5) * std::log(t) - t + std::log(x);
 }
@@ -382,7 +434,7 @@ Eval eval_lik(int fam, int link, double eta, double y, double a, double b) {
   link_fwd(eta, link, mu, d1, d2);
   if (fam == 0) {  // 170-186
     double r = mu - y;
-    e.logp = -0.9189385332046728 + 0.5 * std::log(a) - 0.5 * a * r * r;
+    e.logp = -0.9189385332046728 + (g_plant == P_LIK_GAUSS ? 0.0 : 0.5 * std::log(a)) - 0.5 * a * r * r;
     e.grad = -a * r * d1;
     e.hess = -a * (d1 * d1 + r * d2);
   } else if (fam == 1) {  // 188-223
@@ -390,7 +442,7 @@ Eval eval_lik(int fam, int link, double eta, double y, double a, double b) {
     if (m > 0) e.logp = y * std::log(m) - m - lanczos_log_gamma(y + 1.0);
     else e.logp = (y == 0) ? 0.0 : -INFINITY;
     e.grad = m > 0 ? (y / m - 1.0) * dm : -dm;
-    e.hess = m > 0 ? -(dm * dm) / m : -std::fabs(a * d2);
+    e.hess = m > 0 ? (g_plant == P_LIK_POISSON ? -(d1 * d1) / m : -(dm * dm) / m) : -std::fabs(a * d2);
   } else if (fam == 2) {  // 225-267
     double n = a, p = mu;
     double lc = lanczos_log_gamma(n + 1) - lanczos_log_gamma(y + 1) - lanczos_log_gamma(n - y + 1);
@@ -399,7 +451,7 @@ Eval eval_lik(int fam, int link, double eta, double y, double a, double b) {
     else e.logp = lc + y * std::log(p) + (n - y) * std::log(1 - p);
     double g = (p > 0 && p < 1) ? y / p - (n - y) / (1 - p) : 0.0;
     e.grad = g * d1;
-    e.hess = -n * d1;
+    e.hess = g_plant == P_LIK_BINOMIAL ? -n * p : -n * d1;
   } else {  // 269-311
     double m = a * mu, dm = a * d1, d2m = a * d2, r = b, mr = m + r;
     if (!(m > 0)) { e.ok = false; return e; }
@@ -408,7 +460,7 @@ Eval eval_lik(int fam, int link, double eta, double y, double a, double b) {
     double dl = y / m - (y + r) / mr;
     double d2l = -y / (m * m) + (y + r) / (mr * mr);
     e.grad = dl * dm;
-    e.hess = d2l * dm * dm + dl * d2m;
+    e.hess = d2l * dm * dm + (g_plant == P_LIK_NBINOMIAL ? 0.0 : dl * d2m);
   }
   return e;
 }
@@ -476,16 +528,21 @@ void orc_free(void* h) { delete (Handle*)h; }
 
 i64 orc_nnz_l(void* hp) {  // including the diagonal
   Handle* h = (Handle*)hp;
-  return h->F.lp[h->F.n] + h->F.n;
+  return h->F.lp[h->F.n] + (g_plant == P_NNZ_L ? 0 : h->F.n);
 }
 
 void orc_perm(void* hp, i64* out) {
   Handle* h = (Handle*)hp;
-  std::copy(h->F.perm.begin(), h->F.perm.end(), out);
+  const std::vector<i64>& p = g_plant == P_PERM ? h->F.iperm : h->F.perm;
+  std::copy(p.begin(), p.end(), out);
 }
 
 void orc_d(void* hp, double* out) {  // factor order
   Handle* h = (Handle*)hp;
+  if (g_plant == P_D) {
+    for (i64 k = 0; k < h->F.n; ++k) out[h->F.perm[k]] = h->F.d[k];
+    return;
+  }
   std::copy(h->F.d.begin(), h->F.d.end(), out);
 }
 
@@ -493,7 +550,7 @@ void orc_d(void* hp, double* out) {  // factor order
 double orc_log_abs_det(void* hp) {
   Handle* h = (Handle*)hp;
   double s = 0.0;
-  for (double v : h->F.d) s += std::log(std::fabs(v));
+  for (double v : h->F.d) s += std::log(g_plant == P_LOGDET ? v : std::fabs(v));
   return s;
 }
 
@@ -506,7 +563,8 @@ void orc_solve(void* hp, const double* rhs, double* out) {
   for (i64 k = 0; k < n; ++k) y[k] = rhs[F.perm[k]];
   for (i64 j = 0; j < n; ++j)
     for (i64 p = F.lp[j]; p < F.lp[j + 1]; ++p) y[F.li[p]] -= F.lx[p] * y[j];
-  for (i64 j = 0; j < n; ++j) y[j] /= F.d[j];
+  if (g_plant != P_SOLVE)
+    for (i64 j = 0; j < n; ++j) y[j] /= F.d[j];
   for (i64 j = n - 1; j >= 0; --j) {
     double s = y[j];
     for (i64 p = F.lp[j]; p < F.lp[j + 1]; ++p) s -= F.lx[p] * y[F.li[p]];
@@ -538,7 +596,8 @@ int orc_selinv_entries(void* hp, i64 nq, const i64* rows, const i64* cols, doubl
     if (st) return st;
   }
   for (i64 q = 0; q < nq; ++q) {
-    i64 a = h->F.iperm[rows[q]], b = h->F.iperm[cols[q]];
+    const std::vector<i64>& ip = g_plant == P_SELINV_LOOKUP ? h->F.perm : h->F.iperm;
+    i64 a = ip[rows[q]], b = ip[cols[q]];
     if (!lookup_s(h->L, h->S, a, b, out[q])) return 2;
   }
   return 0;
@@ -556,7 +615,7 @@ int orc_dense_ldlt(i64 n, const double* a, double* logdet, double* diag_inv) {
     d[j] = dj;
     for (i64 i = j + 1; i < n; ++i) {
       double lij = a[i * n + j];
-      for (i64 k = 0; k < j; ++k) lij -= l[i * n + k] * l[j * n + k] * d[k];
+      for (i64 k = 0; k < j; ++k) lij -= l[i * n + k] * l[j * n + k] * (g_plant == P_DENSE_LDLT ? 1.0 : d[k]);
       l[i * n + j] = lij / d[j];
     }
   }
@@ -718,7 +777,7 @@ void newton_system(const Model& M, const Csc& qw, const std::vector<Eval>& ev,
     }
     qpost = to_csc(n, n, t);
     rhs.resize(n);
-    for (i64 i = 0; i < n; ++i) rhs[i] = ev[i].grad - qx[i];
+    for (i64 i = 0; i < n; ++i) rhs[i] = ev[i].grad - (g_plant == P_NEWTON_RHS ? 0.0 : qx[i]);
   } else {
     std::vector<double> nh(ev.size()), g(ev.size());
     for (size_t i = 0; i < ev.size(); ++i) {
@@ -732,7 +791,8 @@ void newton_system(const Model& M, const Csc& qw, const std::vector<Eval>& ev,
     push_all(like, t, false);
     qpost = to_csc(n, n, t);
     rhs = matvec_t(M.a, g);
-    for (i64 i = 0; i < n; ++i) rhs[i] -= qx[i];
+    if (g_plant != P_NEWTON_RHS)
+      for (i64 i = 0; i < n; ++i) rhs[i] -= qx[i];
   }
 }
 
@@ -773,7 +833,7 @@ bool project(std::vector<double>& x, const Model& M) {  // constraints.rs:316-35
   for (i64 c = 0; c < n; ++c) {
     double corr = 0;
     for (i64 a = 0; a < k; ++a) corr += M.ca[a * n + c] * r[a];
-    x[c] -= corr;
+    x[c] += g_plant == P_PROJECT ? corr : -corr;
   }
   return true;
 }
@@ -826,7 +886,7 @@ int orc_find_mode(i64 n, const i64* qp, const i64* qi, const double* qx,
   if (kc) {
     std::vector<Trip> t;
     push_all(M.q, t, true);
-    const double kappa = 3.059023205018258e6;
+    const double kappa = 3.059023205018258e6 * (g_plant == P_KAPPA ? 0.5 : 1.0);
     for (i64 i = 0; i < n; ++i)
       for (i64 j = 0; j < n; ++j) {
         double s = 0;
@@ -944,7 +1004,7 @@ int orc_find_mode(i64 n, const i64* qp, const i64* qi, const double* qx,
   evals_at(M, x, ev);
   double ll = 0;
   for (auto& e : ev) ll += e.logp;
-  *mlik_out = ll - 0.5 * quad + 0.5 * ld_prior - 0.5 * ld_post;
+  *mlik_out = ll - 0.5 * quad + (g_plant == P_MLIK ? 1.0 : 0.5) * ld_prior - 0.5 * ld_post;
   std::copy(x.begin(), x.end(), x_out);
   if (info) {
     info[0] = ld_prior;
@@ -987,18 +1047,142 @@ int orc_ccd_design(int m, double f0, double* points, double* weights) {
         p[i] = s * v;
         sp *= s;
       }
-      p[4] = sp * v;
+      p[4] = (g_plant == P_CCD5 ? -sp : sp) * v;
       pts.push_back(p);
     }
   }
   int ne = (int)pts.size();
-  double w = ne > 1 ? 1.0 / ((ne - 1) * (1.0 + std::exp(-0.5 * f0 * f0) * (f0 * f0 / m - 1.0))) : 1.0;
+  const double f2 = g_plant == P_CCD_WEIGHT ? f0 : f0 * f0;
+  double w = ne > 1 ? 1.0 / ((ne - 1) * (1.0 + std::exp(-0.5 * f0 * f0) * (f2 / m - 1.0))) : 1.0;
   for (int k = 0; k < ne; ++k) {
     for (int i = 0; i < m; ++i) points[k * m + i] = pts[k][i];
     weights[k] = w;
   }
   weights[0] = 1.0 - (ne - 1) * w;
   return ne;
+}
+
+// Test-only: switch on one deliberate mistake (see the Plant list); 0 switches it off.
+void orc_plant(int code) { g_plant = code; }
+
+// grid_design (crates/inla_math/src/integration.rs:64-156): the evaluator is the
+// caller's callback g(z) (the log posterior at z). m = 1 or 2 (m > 2 is the CCD
+// design with f0 = 2.5, handled by the caller). points receives up to cap rows of
+// m; returns the number of points, -1 for m outside 1..2, -2 if cap is too small.
+typedef double (*orc_eval_fn)(const double* z, void* ctx);
+int orc_grid_design(int m, double step, double threshold, orc_eval_fn eval, void* ctx, double* points,
+                    int cap) {
+  if (m < 1 || m > 2) return -1;
+  std::vector<std::vector<double>> pts;
+  auto g = [&](std::vector<double> z) { return eval(z.data(), ctx); };
+  const bool keep_beyond = g_plant == P_GRID;
+  const double g0 = g(std::vector<double>(m, 0.0));
+  if (m == 1) {
+    pts.push_back({0.0});
+    for (int dir = 0; dir < 2; ++dir) {
+      for (int k = 1;; ++k) {
+        std::vector<double> z{(dir == 0 ? 1.0 : -1.0) * (k * step)};
+        const bool beyond = g0 - g(z) > threshold;
+        if (beyond && !keep_beyond) break;
+        pts.push_back(z);
+        if (beyond) break;
+      }
+    }
+  } else {
+    auto extent = [&](double sx, double sy) {
+      int e = 0;
+      for (;;) {
+        std::vector<double> z{sx * (e + 1) * step, sy * (e + 1) * step};
+        if (g0 - g(z) > threshold) break;
+        ++e;
+      }
+      return e;
+    };
+    const int max_i = extent(1, 0), min_i = extent(-1, 0), max_j = extent(0, 1), min_j = extent(0, -1);
+    for (int i = -min_i; i <= max_i; ++i)
+      for (int j = -min_j; j <= max_j; ++j) {
+        std::vector<double> z{i * step, j * step};
+        if (keep_beyond || g0 - g(z) <= threshold) pts.push_back(z);
+      }
+  }
+  if ((int)pts.size() > cap) return -2;
+  for (size_t k = 0; k < pts.size(); ++k)
+    for (int i = 0; i < m; ++i) points[k * m + i] = pts[k][i];
+  return (int)pts.size();
+}
+
+// invert_symmetric_matrix (integration.rs:158-226): Gauss-Jordan with partial
+// pivoting on [A | I]; returns 1 for a pivot below 1e-14 ("Singular Hessian").
+int orc_invert_symmetric(int m, const double* h, double* inv) {
+  const int cols = 2 * m;
+  std::vector<double> a((size_t)m * cols, 0.0);
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) a[i * cols + j] = h[i * m + j];
+    a[i * cols + m + i] = 1.0;
+  }
+  for (int i = 0; i < m; ++i) {
+    int mr = i;
+    for (int r = i + 1; r < m; ++r)
+      if (std::fabs(a[r * cols + i]) > std::fabs(a[mr * cols + i])) mr = r;
+    if (std::fabs(a[mr * cols + i]) < 1e-14) return 1;
+    if (mr != i)
+      for (int c = 0; c < cols; ++c) std::swap(a[i * cols + c], a[mr * cols + c]);
+    const double piv = a[i * cols + i];
+    for (int c = 0; c < (g_plant == P_INVERT ? m : cols); ++c) a[i * cols + c] /= piv;
+    for (int r = 0; r < m; ++r) {
+      if (r == i) continue;
+      const double f = a[r * cols + i];
+      if (f == 0.0) continue;
+      for (int c = 0; c < cols; ++c) a[r * cols + c] -= f * a[i * cols + c];
+    }
+  }
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) inv[i * m + j] = a[i * cols + m + j];
+  return 0;
+}
+
+// jacobi_eigen_scalar (integration.rs:249-318): cyclic Jacobi rotations on a
+// row-major symmetric m x m matrix; eigenvalues in lam[m], eigenvectors as the
+// columns of v (row-major m x m). Returns 1 when max_iter sweeps do not converge.
+int orc_jacobi_eigen(int m, const double* mat, int max_iter, double* lam, double* v) {
+  std::vector<double> a(mat, mat + (size_t)m * m);
+  for (int i = 0; i < m * m; ++i) v[i] = 0.0;
+  for (int i = 0; i < m; ++i) v[i * m + i] = 1.0;
+  for (int sweep = 0; sweep < max_iter; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = i + 1; j < m; ++j) off += std::fabs(a[i * m + j]);
+    if (off < 1e-12) {
+      for (int i = 0; i < m; ++i) lam[i] = a[i * m + i];
+      return 0;
+    }
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = a[p * m + q];
+        if (std::fabs(apq) < 1e-15) continue;
+        const double app = a[p * m + p], aqq = a[q * m + q];
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double t = tau >= 0.0 ? 1.0 / (tau + std::sqrt(1.0 + tau * tau))
+                                    : -1.0 / (-tau + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        a[p * m + p] = app - t * apq;
+        a[q * m + q] = aqq + t * apq;
+        a[p * m + q] = a[q * m + p] = 0.0;
+        for (int r = 0; r < m; ++r) {
+          if (r == p || r == q) continue;
+          const double arp = a[r * m + p], arq = a[r * m + q];
+          a[r * m + p] = a[p * m + r] = c * arp - sn * arq;
+          a[r * m + q] = a[q * m + r] = sn * arp + c * arq;
+        }
+        const double sv = g_plant == P_JACOBI ? -sn : sn;
+        for (int r = 0; r < m; ++r) {
+          const double vrp = v[r * m + p], vrq = v[r * m + q];
+          v[r * m + p] = c * vrp - sv * vrq;
+          v[r * m + q] = sv * vrp + c * vrq;
+        }
+      }
+  }
+  return 1;
 }
 
 }  // extern "C"

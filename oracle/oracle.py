@@ -22,6 +22,9 @@ def build() -> None:
     subprocess.run(["make", "-C", HERE], check=True, capture_output=True)
 
 
+EVAL_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         build()
@@ -50,6 +53,10 @@ def _load():
         "orc_find_mode": (i32, [i64, p64, p64, pd, i64, p64, p64, pd, i32, i32, pd, pd, pd,
                                 i64, pd, pd, i32, dbl, pd, pd, pd, C.POINTER(vp)]),
         "orc_ccd_design": (i32, [i32, dbl, pd, pd]),
+        "orc_plant": (None, [i32]),
+        "orc_grid_design": (i32, [i32, dbl, dbl, EVAL_FN, vp, pd, i32]),
+        "orc_invert_symmetric": (i32, [i32, pd, pd]),
+        "orc_jacobi_eigen": (i32, [i32, pd, i32, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -279,3 +286,62 @@ def ccd_design(m: int, f0: float = 1.0):
     if ne < 0:
         raise OracleError(f"CCD strategy for m={m} is not supported")
     return pts[: ne * m].reshape(ne, m), w[:ne]
+
+
+# Deliberate mistakes the oracle can switch on (the Plant list of gmrf_oracle.cpp),
+# used only by tests/test_oracle_pins.py to show that each pinning check fails when
+# the function it pins is wrong.
+PLANTS = ["none", "lower_bw", "bw_fwd", "time_major", "rcm", "choose", "min_degree", "ldlt", "logdet",
+          "solve", "takahashi", "selinv_lookup", "dense_ldlt", "lgamma", "lik_gauss", "lik_poisson",
+          "lik_binomial", "lik_nbinomial", "newton_rhs", "project", "mlik", "kappa", "ccd5", "ccd_weight",
+          "grid", "jacobi", "invert", "nnz_l", "perm", "d"]
+
+
+class planted:
+    """Context manager: the oracle runs with one planted mistake inside the block."""
+
+    def __init__(self, name):
+        self.code = PLANTS.index(name)
+
+    def __enter__(self):
+        lib.orc_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib.orc_plant(0)
+        return False
+
+
+def grid_design(m: int, step: float, threshold: float, evaluator):
+    """grid_design of integration.rs:64-156 with evaluator(z) -> g(z); (points, weights)."""
+    cap = 1 << 16
+    pts = np.empty(cap * m)
+
+    def cb(zp, _ctx):
+        return float(evaluator(np.array([zp[i] for i in range(m)])))
+
+    fn = EVAL_FN(cb)
+    ne = lib.orc_grid_design(m, step, threshold, fn, None, _pd(pts), cap)
+    if ne < 0:
+        raise OracleError(f"grid design failed ({ne})")
+    return pts[: ne * m].reshape(ne, m), np.ones(ne)
+
+
+def invert_symmetric(h):
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    m = h.shape[0]
+    inv = np.empty((m, m))
+    if lib.orc_invert_symmetric(m, _pd(h), _pd(inv)):
+        raise OracleError("Singular Hessian matrix")
+    return inv
+
+
+def jacobi_eigen(a, max_iter=100):
+    """(eigenvalues, V with eigenvectors as columns) by the Jacobi sweeps of integration.rs:249-318."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    m = a.shape[0]
+    lam = np.empty(m)
+    v = np.empty((m, m))
+    if lib.orc_jacobi_eigen(m, _pd(a), int(max_iter), _pd(lam), _pd(v)):
+        raise OracleError("Jacobi eigenvalue algorithm did not converge")
+    return lam, v

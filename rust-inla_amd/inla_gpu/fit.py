@@ -11,7 +11,11 @@ Two drivers of the same pipeline, both on the batched Laplace evaluator:
   Hessian stencil point by point (hyper_opt.rs:197-259, step from
   inference.rs:1264) and the CCD nodes one by one (inference.rs:1296-1397).
 
-Both end in the mixture of inference.rs:1419-1468.
+Both end in the mixture of inference.rs:1419-1468. ``strategy`` picks the
+integration design as inference.rs:1272-1294 does: "ccd" (ccd_design with f0) or
+"grid" (grid_design with step f0 and threshold 4.0, whose evaluator is the log
+posterior at the z-points: one batched call per line of the grid in
+``fit_batched``, one call per point in ``fit_plain``).
 """
 
 from __future__ import annotations
@@ -198,11 +202,24 @@ def _hessian_step(mode):
     return max(1e-4, 0.05 * float(np.max(np.abs(mode))))
 
 
-def _nodes(mode, H, f0=1.0):
+GRID_THRESHOLD = 4.0  # inference.rs:1291
+
+
+def _nodes(mode, H, strategy="ccd", f0=1.0, evaluate=None):
+    """theta-points and design weights around the mode; ``evaluate(thetas) -> g`` is
+    the log posterior the grid design walks."""
     lam, V = ccd.scaling_from_hessian(H)
-    z, w = ccd.ccd_design(len(mode), f0)
-    thetas = np.array([ccd.theta_from_z(mode, lam, V, zz) for zz in z])
-    return thetas, w
+
+    def to_theta(Z):
+        return np.array([ccd.theta_from_z(mode, lam, V, zz) for zz in np.atleast_2d(Z)])
+
+    if strategy == "grid":
+        z, w = ccd.grid_design(len(mode), f0, GRID_THRESHOLD, lambda Z: evaluate(to_theta(Z)))
+    elif strategy == "ccd":
+        z, w = ccd.ccd_design(len(mode), f0)
+    else:
+        raise ValueError(f"unknown integration strategy {strategy!r}")
+    return to_theta(z), w
 
 
 def _finish(mode, H, thetas, w, r, n_evals, n_points, secs):
@@ -210,8 +227,8 @@ def _finish(mode, H, thetas, w, r, n_evals, n_points, secs):
     return FitResult(mode, H, thetas, wts, r["g"], mean, var, mlik, n_evals, n_points, secs)
 
 
-def fit_batched(obj: Objective, theta0, **bfgs_kw) -> FitResult:
-    """Mode by batched BFGS, Hessian stencil and CCD nodes as single batches."""
+def fit_batched(obj: Objective, theta0, strategy="ccd", f0=1.0, **bfgs_kw) -> FitResult:
+    """Mode by batched BFGS, Hessian stencil and the design's nodes as single batches."""
     secs = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -221,7 +238,7 @@ def fit_batched(obj: Objective, theta0, **bfgs_kw) -> FitResult:
     pts = ccd.hessian_points(mode, h)
     H = ccd.hessian_from_values(obj.values(pts)["g"], len(mode), h)
     t2 = time.perf_counter()
-    thetas, w = _nodes(mode, H)
+    thetas, w = _nodes(mode, H, strategy, f0, lambda th: obj.values(th)["g"])
     r = obj.values(thetas, want_x=True, want_var=True)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
@@ -229,7 +246,7 @@ def fit_batched(obj: Objective, theta0, **bfgs_kw) -> FitResult:
     return _finish(mode, H, thetas, w, r, obj.calls, obj.points, secs)
 
 
-def fit_plain(obj: Objective, theta0) -> FitResult:
+def fit_plain(obj: Objective, theta0, strategy="ccd", f0=1.0) -> FitResult:
     """The reference's sequence, one theta-point per evaluation."""
     secs = {}
     torch.cuda.synchronize()
@@ -240,7 +257,8 @@ def fit_plain(obj: Objective, theta0) -> FitResult:
     pts = ccd.hessian_points(mode, h)
     H = ccd.hessian_from_values(np.concatenate([obj.values(p[None])["g"] for p in pts]), len(mode), h)
     t2 = time.perf_counter()
-    thetas, w = _nodes(mode, H)
+    thetas, w = _nodes(mode, H, strategy, f0,
+                       lambda th: np.concatenate([obj.values(t[None])["g"] for t in th]))
     parts = [obj.values(t[None], want_x=True, want_var=True) for t in thetas]
     r = {k: np.concatenate([p[k] for p in parts]) for k in ("g", "x", "var")}
     torch.cuda.synchronize()

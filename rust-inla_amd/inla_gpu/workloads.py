@@ -722,3 +722,112 @@ def spde_rw2_posterior_precision(nx=245, T=24, n_obs=500_000, seed=20241207, siz
     Q.sum_duplicates()
     Q.sort_indices()
     return Q
+
+
+def scale_factor_dense(R) -> float:
+    """scale_model factor of crates/inla_math/src/design.rs:151-195 through a dense
+    eigen-decomposition (the reference's own route; small structures only):
+    exp(mean log diag(R^+)), eigenvalues below sqrt(eps) max(lambda_max, 1) dropped."""
+    lam, V = np.linalg.eigh(sp.csc_matrix(R).toarray())
+    tol = math.sqrt(np.finfo(float).eps) * max(abs(max(lam.max(), 0.0)), 1.0)
+    keep = lam > tol
+    d = (V[:, keep] ** 2) @ (1.0 / lam[keep])
+    d = d[(d > 0) & np.isfinite(d)]
+    return float(math.exp(np.mean(np.log(d))))
+
+
+KAPPA_HARD = 3.059023205018258e6  # HARD_CONSTRAINT_KAPPA, constraints.rs:268
+
+
+@dataclass
+class BesagPoissonWorkload:
+    """Areal Poisson model: latent [intercept, covariate coefficient, u (regions)],
+    u a scaled Besag ICAR with a sum-to-zero constraint, theta = log tau.
+
+    ``gmrf`` components on the full pattern: 0 the fixed effects' precision
+    (1e-4 on the two coefficients), 1 the scaled Besag structure Q* on u,
+    2 the augmentation kappa a a^T of the constraint row a (the reference's explicit
+    form, constraints.rs:275-313). ``gmrf_lowrank`` holds components 0 and 1 only, on
+    their own sparse pattern, for the low-rank form of the augmentation."""
+
+    gmrf: AffineGmrf
+    gmrf_lowrank: AffineGmrf
+    A: sp.csc_matrix
+    y: np.ndarray
+    x_cov: np.ndarray
+    constraints: tuple
+    scale: float
+    theta_true: np.ndarray
+
+    @property
+    def n(self):
+        return self.gmrf.n
+
+    def params(self, thetas, lowrank=False):
+        thetas = np.atleast_2d(np.asarray(thetas, dtype=float))
+        B = len(thetas)
+        K = 2 if lowrank else 3
+        coefs = np.zeros((B, K))
+        dco = np.zeros((B, 1, K))
+        for b, th in enumerate(thetas):
+            t = math.exp(th[0])
+            coefs[b, 0] = 1.0
+            coefs[b, 1] = t
+            if not lowrank:
+                coefs[b, 2] = 1.0
+            dco[b, 0, 1] = t
+        return coefs, dco, None, None
+
+    def params_lowrank(self, thetas):
+        return self.params(thetas, lowrank=True)
+
+    def log_prior(self, theta):
+        # the reference's default precision prior loggamma(1, 5e-5) (api.py:1357)
+        v, g = loggamma_prior(float(np.asarray(theta).ravel()[0]), 1.0, 5e-5)
+        return v, np.array([g])
+
+    def prior_without_kappa(self, theta):
+        """Q(theta) without the augmentation (the oracle adds kappa a a^T itself)."""
+        c = self.params(np.atleast_1d(theta))[0][0]
+        c[2] = 0.0
+        return self.gmrf.csc(c)
+
+
+def besag_poisson_workload(nr=30, seed=20241208, intercept=0.5, beta=0.3, tau=1.0,
+                           fixed_prec=1e-4):
+    """Besag ICAR + Poisson on an nr x nr rook lattice (node = r nr + c), scale_model,
+    sum-to-zero on u, intercept 0.5 and a N(0, 1) covariate with coefficient 0.3,
+    u drawn from the scaled ICAR with tau = 1, y ~ Poisson(exp(0.5 + 0.3 x + u)).
+    Latent layout as the reference's (fixed effects first, api.py:1508-1537)."""
+    adj = rook_adjacency(nr, nr)
+    R = besag_structure(adj)
+    ns = nr * nr
+    n = ns + 2
+    fac = scale_factor_dense(R)
+    Qs = sp.csc_matrix(R * fac)
+    rng = np.random.default_rng(seed)
+    x_cov = rng.standard_normal(ns)
+    # u ~ N(0, (tau Q*)^+): the eigenvectors of the non-zero eigenvalues
+    lam, V = np.linalg.eigh(Qs.toarray())
+    keep = lam > 1e-9 * lam.max()
+    u = V[:, keep] @ (rng.standard_normal(int(keep.sum())) / np.sqrt(tau * lam[keep]))
+    y = rng.poisson(np.exp(intercept + beta * x_cov + u)).astype(float)
+    # A = [1, x, I]: row i is region i
+    rows = np.r_[np.arange(ns), np.arange(ns), np.arange(ns)]
+    cols = np.r_[np.zeros(ns, dtype=int), np.ones(ns, dtype=int), 2 + np.arange(ns)]
+    vals = np.r_[np.ones(ns), x_cov, np.ones(ns)]
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(ns, n))
+    A.sort_indices()
+    Kfix = sp.csc_matrix(sp.diags(np.r_[fixed_prec, fixed_prec, np.zeros(ns)]))
+    Kb = sp.csc_matrix(sp.block_diag([sp.csc_matrix((2, 2)), Qs]))
+    a = np.zeros((1, n))
+    a[0, 2:] = 1.0 / math.sqrt(ns)  # sum_to_zero_constraint (constraints.rs:113-127)
+    Kk = sp.csc_matrix(KAPPA_HARD * (a.T @ a))
+    ata = sp.csc_matrix(A.T @ A)
+    indptr, indices = union_pattern([Kfix, Kb, Kk, ata])
+    comps = np.stack([values_on_pattern(m, indptr, indices) for m in (Kfix, Kb, Kk)])
+    g = AffineGmrf(n, indptr, indices, comps, ["fixed", "Q* (u)", "kappa a a^T"])
+    ip2, ix2 = union_pattern([Kfix, Kb, ata])
+    g2 = AffineGmrf(n, ip2, ix2, np.stack([values_on_pattern(m, ip2, ix2) for m in (Kfix, Kb)]),
+                    ["fixed", "Q* (u)"])
+    return BesagPoissonWorkload(g, g2, A, y, x_cov, (a, np.zeros(1)), fac, np.array([math.log(tau)]))

@@ -72,3 +72,72 @@ def test_mixture_of_a_single_node():
                                        np.array([[0.5, 0.25]]))
     assert np.allclose(w, [1.0]) and np.allclose(mean, [1.0, 2.0]) and np.allclose(var, [0.5, 0.25])
     assert mlik == pytest.approx(-3.0)
+
+
+# ------------------------------------------------------------------ integration designs
+def test_ccd_design_matches_oracle():
+    from inla_gpu import ccd
+    from oracle import oracle as orc
+
+    for m in range(1, 6):
+        for f0 in (1.0, 2.5):
+            z, w = ccd.ccd_design(m, f0)
+            zo, wo = orc.ccd_design(m, f0)
+            assert np.array_equal(z, zo) and np.array_equal(w, wo)
+
+
+def test_m5_nodes_match_oracle_eigen_step():
+    # CCD nodes of a 5-parameter model: (-H)^-1 by Gauss-Jordan, Jacobi eigenvectors
+    # (order and signs fix which half fraction of the design the nodes land on)
+    from inla_gpu import ccd
+    from oracle import oracle as orc
+
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((5, 5))
+    H = -(G @ G.T + 5 * np.eye(5))
+    lam, V = ccd.scaling_from_hessian(H)
+    sig = orc.invert_symmetric(-H)
+    lo, Vo = orc.jacobi_eigen(sig)
+    assert np.max(np.abs(lam - lo)) < 1e-14 * np.max(np.abs(lo))
+    assert np.max(np.abs(V - Vo)) < 1e-13
+    mode = rng.standard_normal(5)
+    z, _ = ccd.ccd_design(5, 1.0)
+    t = np.array([ccd.theta_from_z(mode, lam, V, zz) for zz in z])
+    to = np.array([mode + (Vo * np.sqrt(np.abs(lo))) @ zz for zz in z])
+    assert np.max(np.abs(t - to)) < 1e-13
+    # and the decomposition is one of Sigma
+    assert np.max(np.abs(V @ np.diag(lam) @ V.T - np.linalg.inv(-H))) < 1e-13
+
+
+@pytest.mark.parametrize("m,step", [(1, 1.0), (1, 0.6), (2, 1.0), (2, 0.7), (3, 1.0)])
+def test_grid_design_matches_oracle(m, step):
+    # a skewed log-density (not quadratic): the walks stop at different distances on
+    # the two sides and the 2-D box holds points beyond the threshold
+    from inla_gpu import ccd
+    from oracle import oracle as orc
+
+    def g(z):
+        z = np.atleast_1d(z)
+        return -0.5 * z @ z - 0.05 * z[0] ** 4 + 0.3 * z[0] + (0.3 * z[0] * z[1] if len(z) > 1 else 0.0)
+
+    pts, w = ccd.grid_design(m, step, 4.0, lambda Z: np.array([g(zz) for zz in np.atleast_2d(Z)]))
+    if m > 2:
+        zo, wo = orc.ccd_design(m, 2.5)
+    else:
+        zo, wo = orc.grid_design(m, step, 4.0, g)
+    assert np.array_equal(pts, zo) and np.array_equal(w, wo)
+
+
+def test_fit_grid_strategy_on_a_quadratic():
+    # the grid design through the fit driver: for g = -1/2 (t - c)^T A (t - c) + 7 the
+    # log posterior at z is -|z|^2 / 2 + 7, so the 1-D grid is -2..2 and the
+    # normalised weights are proportional to exp(-z^2 / 2)
+    from inla_gpu import fit as F
+
+    a = np.array([[2.0]])
+    c = np.array([0.4])
+    ob = QuadObjective(a, c)
+    H = -a
+    thetas, w = F._nodes(c, H, "grid", 1.0, lambda th: ob.values(th)["g"])
+    z = (thetas[:, 0] - c[0]) * np.sqrt(2.0)
+    assert np.allclose(z, [0, 1, 2, -1, -2], atol=1e-12) and np.all(w == 1.0)

@@ -55,3 +55,16 @@ def test_planted_sign_error_is_caught():
     R, kind = structure(case)
     fac, _ = factor_from_grounded(R, kind, planted_sign=-1.0)
     assert not abs(fac - case["value"]) < 1e-3  # NaN (negative variances) also fails the bar
+
+
+@pytest.mark.parametrize("i", range(len(GOLDEN["cases"])))
+def test_dense_scale_factor_matches_reference_values(i):
+    # the dense eigen-decomposition route (the reference's, design.rs:151-195) used by
+    # the small areal workload
+    from inla_gpu import workloads as W
+
+    case = GOLDEN["cases"][i]
+    R, kind = structure(case)
+    fac = W.scale_factor_dense(R)
+    got = fac * R[0, 0] if case["expect"] == "scaled diagonal" else fac
+    assert got == pytest.approx(case["value"], rel=1e-6)
