@@ -1554,12 +1554,12 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
       const int pi = max(ri, rj[t]), pj = min(ri, rj[t]);
       const double* base;
       int64_t off;
-      if (plarge) {
-        base = Wm;
-        off = woff + (int64_t)pj * mp + pi;
-      } else if (pj < wp) {
+      if (pj < wp) {  // the parent's own columns: its S block
         base = Sm;
         off = loff + (int64_t)pj * mp + pi;
+      } else if (plarge) {  // its R x R block: the lower triangle of its dense Sigma front
+        base = Wm;
+        off = woff + (int64_t)pj * mp + pi;
       } else {
         base = Um;
         off = uoff + (int64_t)(pj - wp) * rp + (pi - wp);
@@ -1698,12 +1698,12 @@ __global__ __launch_bounds__(4 * MB) void k_selinv_small(DevPlan P, const int32_
 // (the block form of the Takahashi recurrence).
 // ---------------------------------------------------------------------------
 
-// Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into W (both triangles; the
-// transposed copy is staged through LDS so both stores are coalesced).
+// Gather tile (I, J), I >= J, of Sigma_{R_s R_s} into the lower triangle of W.
+// Sigma of a large front lives in two places, both lower triangles: its own w
+// columns in S, the R x R block in W (W's first w columns are never written).
 __device__ __forceinline__ void sl_gather_task(const DevPlan& P, const int32_t* tk, int b,
                                                const double* __restrict__ S,
-                                               const double* __restrict__ U, double* __restrict__ W,
-                                               double (*Ct)[kTile + 1]) {
+                                               const double* __restrict__ U, double* __restrict__ W) {
   const int s = tk[0], I = tk[1], J = tk[2];
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   const int p = P.sn_parent[s];
@@ -1727,14 +1727,12 @@ __device__ __forceinline__ void sl_gather_task(const DevPlan& P, const int32_t* 
     const int pi = max(ri, rj[u]), pj = min(ri, rj[u]);
     const double* base;
     int64_t off;
-    if (plarge) {
-      // the parent's W holds both triangles: column rj, row ri (consecutive lanes,
-      // consecutive rows)
-      base = Wm;
-      off = woff + (int64_t)rj[u] * mp + ri;
-    } else if (pj < wp) {
+    if (pj < wp) {  // the parent's own columns: its S block
       base = Sm;
       off = loff + (int64_t)pj * mp + pi;
+    } else if (plarge) {  // its R x R block: the lower triangle of its W
+      base = Wm;
+      off = woff + (int64_t)pj * mp + pi;
     } else {
       base = Um;
       off = uoff + (int64_t)(pj - wp) * rp + (pi - wp);
@@ -1744,16 +1742,8 @@ __device__ __forceinline__ void sl_gather_task(const DevPlan& P, const int32_t* 
   ISSUE_FENCE();
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int jj = jb + 4 * u, j = J * kTile + jj;
-    Ct[jj][ii] = v[u];
+    const int j = J * kTile + jb + 4 * u;
     if (i < r && j < r && i >= j) Ws[(int64_t)(w + j) * m + (w + i)] = v[u];
-  }
-  __syncthreads();
-  // W[w + j][w + i] for the same (i >= j): consecutive lanes take consecutive j
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
-    const int jj = idx & 63, i2 = idx >> 6;
-    const int iq = I * kTile + i2, jq = J * kTile + jj;
-    if (iq < r && jq < r && iq >= jq) Ws[(int64_t)(w + iq) * m + (w + jq)] = Ct[jj][i2];
   }
 }
 
@@ -1820,21 +1810,54 @@ __global__ __launch_bounds__(256) void k_sl_gather_lhat(DevPlan P, const int32_t
                                                         const double* __restrict__ L,
                                                         const double* __restrict__ Z,
                                                         double* __restrict__ T) {
-  static_assert(kTile * (kTile + 1) <= 2 * 64 * (kPanel + 1), "gather tile exceeds the LDS block");
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   if ((int)blockIdx.x < ng)
-    sl_gather_task(P, g_tasks + 3 * blockIdx.x, blockIdx.y, S, U, W, (double (*)[kTile + 1])sh);
+    sl_gather_task(P, g_tasks + 3 * blockIdx.x, blockIdx.y, S, U, W);
   else
     sl_lhat_task(P, l_tasks + 3 * (blockIdx.x - ng), 0, blockIdx.y, L, Z, T, sh);
 }
 
+// acc += Sigma[R0 .. R0 + 63][k_lo .. k_hi) Lh[k_lo .. k_hi)[0 .. nb) from the lower
+// triangle of Sigma only (the front's own columns in S, the R x R block in W, the
+// same column stride m). Up to the end of the tile's diagonal block the entries are
+// read down their columns (row max(i, k) of column min(i, k): lanes along rows);
+// beyond it every entry lies above the tile's rows and is read as row k of column i
+// (lanes along k). Both ranges go through the tile engine on one accumulator.
+template <int NT>
+__device__ __forceinline__ void sigma_strip(const double* __restrict__ Sb, const double* __restrict__ Ws,
+                                            const double* __restrict__ Lh, int w, int m, int nb, int R0,
+                                            int k_lo, int k_hi, double (*As)[kPanel + 1],
+                                            double (*Bs)[kPanel + 1], TileAcc& acc) {
+  const int kd = min(max(R0 + kTile, k_lo), k_hi);
+  if (kd > k_lo)
+    mma_tile<false, true, NT>(
+        [&](int ii, int kk) {
+          const int i = min(R0 + ii, m - 1), k = k_lo + kk;
+          const int hi = max(i, k), lo = min(i, k);
+          return LV{(lo < w ? Sb : Ws)[(int64_t)lo * m + hi], R0 + ii < m};
+        },
+        [&](int jj, int kk) {
+          return LV{Lh[(int64_t)min(jj, nb - 1) * m + k_lo + kk], jj < nb};
+        },
+        kd - k_lo, As, Bs, acc);
+  if (k_hi > kd)
+    mma_tile<true, true, NT>(
+        [&](int ii, int kk) {
+          const int i = min(R0 + ii, m - 1);
+          return LV{(i < w ? Sb : Ws)[(int64_t)i * m + kd + kk], R0 + ii < m};
+        },
+        [&](int jj, int kk) {
+          return LV{Lh[(int64_t)min(jj, nb - 1) * m + kd + kk], jj < nb};
+        },
+        k_hi - kd, As, Bs, acc);
+}
+
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
-// -sum_{k in B} W[i][k] Lh[k][p] (K = m - c1) and stores it into S (lower) and
-// both triangles of W (the transposed copy staged through LDS so that both
-// stores are coalesced).
+// -sum_{k in B} Sigma[i][k] Lh[k][p] (K = m - c1, lower triangle only) and stores it
+// into S (column c0 + p, the layout of L).
 __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, double* __restrict__ S,
-                                                 double* __restrict__ W,
+                                                 const double* __restrict__ W,
                                                  const double* __restrict__ T) {
   constexpr int NT = 512;
   __shared__ double sh[2 * 64 * (kPanel + 1)];
@@ -1849,47 +1872,27 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
   const int nb = c1 - c0;
   const int R0 = c1 + I * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
-  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
-  auto fa = [&](int ii, int kk) {
-    const int i = R0 + ii;
-    return LV{Ws[(int64_t)(c1 + kk) * m + min(i, m - 1)], i < m};
-  };
-  auto fb = [&](int jj, int kk) {
-    return LV{Lh[(int64_t)min(jj, nb - 1) * m + c1 + kk], jj < nb};
-  };
-  mma_tile<false, true, NT>(fa, fb, m - c1, As, Bs, acc);
-  // the tile engine ends with a barrier, so its LDS can hold the tile now; waves
-  // 0-3 hold the result
-  double (*Ct)[kTile + 1] = (double (*)[kTile + 1])sh;
+  sigma_strip<NT>(Sb, Ws, Lh, w, m, nb, R0, c1, m, As, Bs, acc);
+  // waves 0-3 hold the result
   if (threadIdx.x < 256)
     tile_foreach(acc, [&](int ii, int jj, double v) {
       const int i = R0 + ii;
-      Ct[jj][ii] = -v;
-      if (i < m && jj < nb) {
-        Sb[(int64_t)(c0 + jj) * m + i] = -v;
-        Ws[(int64_t)(c0 + jj) * m + i] = -v;
-      }
+      if (i < m && jj < nb) Sb[(int64_t)(c0 + jj) * m + i] = -v;
     });
-  __syncthreads();
-  // W[c0 + jj][i]: consecutive lanes take consecutive jj
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += NT) {
-    const int jj = idx & 63, ii = idx >> 6;
-    const int i = R0 + ii;
-    if (i < m && jj < nb) Ws[(int64_t)i * m + c0 + jj] = Ct[jj][ii];
-  }
 }
 
 // Sigma[P,P]: task (s) = Z^T D^-1 Z - Lh^T Sigma[B,P], both products on MFMA
-// (K = nb, then K = m - c1); lower part into S, both triangles into W. The
+// (K = nb, then K = m - c1; Sigma[B,P] read back from S); lower part into S. The
 // strict upper triangle of S's diagonal blocks is not part of the layout's
 // contract and is left as is.
 __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, int step, int b,
                                            const double* __restrict__ L, const double* __restrict__ Z,
-                                           double* __restrict__ S, double* __restrict__ W,
-                                           const double* __restrict__ T, double* sh, double* dinv) {
+                                           double* __restrict__ S, const double* __restrict__ T, double* sh,
+                                           double* dinv) {
   const int s = tk[0];
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
@@ -1899,7 +1902,6 @@ __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, 
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
-  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
   if (tid < kTile) {
     const int q = min(tid, nb - 1);
@@ -1925,7 +1927,7 @@ __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, 
           return LV{Lh[(int64_t)min(ii, nb - 1) * m + c1 + kk], ii < nb};
         },
         [&](int jj, int kk) {
-          return LV{Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk], jj < nb};
+          return LV{Sb[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + kk], jj < nb};
         },
         m - c1, sh, acc);
     tile_negate(acc);
@@ -1933,11 +1935,7 @@ __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, 
   if (threadIdx.x < 256)
     tile_foreach(acc, [&](int ii, int jj, double v) {
       // tile row ii = p, tile column jj = q; lower part p >= q
-      if (ii < nb && jj < nb && ii >= jj) {
-        Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-        Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-        Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
-      }
+      if (ii < nb && jj < nb && ii >= jj) Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
     });
 }
 
@@ -1948,12 +1946,11 @@ __global__ __launch_bounds__(256) void k_sl_pp_lhat(DevPlan P, const int32_t* __
                                                     int npp, const int32_t* __restrict__ lhat_tasks,
                                                     int step, const double* __restrict__ L,
                                                     const double* __restrict__ Z,
-                                                    double* __restrict__ S, double* __restrict__ W,
-                                                    double* __restrict__ T) {
+                                                    double* __restrict__ S, double* __restrict__ T) {
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   __shared__ double dinv[kTile];
   if ((int)blockIdx.x < npp)
-    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, W, T, sh, dinv);
+    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, T, sh, dinv);
   else
     sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp), step + 1, blockIdx.y, L, Z, T, sh);
 }
@@ -1963,8 +1960,8 @@ __global__ __launch_bounds__(256) void k_sl_pp_lhat(DevPlan P, const int32_t* __
 // the rows c1 + kGemmSplitK part .. of B into the front's staging area;
 // k_sl_gemm_sum adds the parts in order and stores as k_sl_gemm does.
 __global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* __restrict__ tasks,
-                                                      int step, const double* __restrict__ W,
-                                                      double* __restrict__ T) {
+                                                      int step, const double* __restrict__ S,
+                                                      const double* __restrict__ W, double* __restrict__ T) {
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
   double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
@@ -1978,29 +1975,20 @@ __global__ __launch_bounds__(512) void k_sl_gemm_part(DevPlan P, const int32_t* 
   const int parts = max(1, (nB + kGemmSplitK - 1) / kGemmSplitK);
   const int ks = part * kGemmSplitK, ke = min(nB, ks + kGemmSplitK);
   const int R0 = c1 + I * kTile;
+  const double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   const double* Lh = Tb + stage_lhat_off(m, step);
   TileAcc acc;
   tile_zero(acc);
-  mma_tile<false, true, 512>(
-      [&](int ii, int kk) {
-        const int i = R0 + ii;
-        return LV{Ws[(int64_t)(c1 + ks + kk) * m + min(i, m - 1)], i < m};
-      },
-      [&](int jj, int kk) {
-        return LV{Lh[(int64_t)min(jj, nb - 1) * m + c1 + ks + kk], jj < nb};
-      },
-      ke - ks, As, Bs, acc);
+  sigma_strip<512>(Sb, Ws, Lh, w, m, nb, R0, c1 + ks, c1 + ke, As, Bs, acc);
   double* Pg = Tb + stage_gemm_off(m) + ((int64_t)I * parts + part) * kTile * kTile;
   if (threadIdx.x < 256) tile_foreach(acc, [&](int ii, int jj, double v) { Pg[jj * kTile + ii] = v; });
 }
 
 __global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* __restrict__ tasks,
                                                      int step, double* __restrict__ S,
-                                                     double* __restrict__ W,
                                                      const double* __restrict__ T) {
-  __shared__ double Ct[kTile][kTile + 1];
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0], I = tk[1];
   const int b = blockIdx.y;
@@ -2011,7 +1999,6 @@ __global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* _
   const int parts = max(1, (nB + kGemmSplitK - 1) / kGemmSplitK);
   const int R0 = c1 + I * kTile;
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
-  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   const double* Pg = T + (int64_t)b * P.t_size + P.t_off[s] + stage_gemm_off(m) + (int64_t)I * parts * kTile * kTile;
   const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
   const int i = R0 + ii;
@@ -2020,18 +2007,7 @@ __global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* _
     const int jj = jb + 4 * u;
     double v = 0.0;
     for (int q = 0; q < parts; ++q) v += Pg[(int64_t)q * kTile * kTile + jj * kTile + ii];
-    Ct[jj][ii] = -v;
-    if (i < m && jj < nb) {
-      Sb[(int64_t)(c0 + jj) * m + i] = -v;
-      Ws[(int64_t)(c0 + jj) * m + i] = -v;
-    }
-  }
-  __syncthreads();
-  // W[c0 + jj][i]: consecutive lanes take consecutive jj
-  for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
-    const int jj = idx & 63, i2 = idx >> 6;
-    const int row = R0 + i2;
-    if (row < m && jj < nb) Ws[(int64_t)row * m + c0 + jj] = Ct[jj][i2];
+    if (i < m && jj < nb) Sb[(int64_t)(c0 + jj) * m + i] = -v;
   }
 }
 
@@ -2044,7 +2020,7 @@ __global__ __launch_bounds__(256) void k_sl_gemm_sum(DevPlan P, const int32_t* _
 __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __restrict__ tasks,
                                                     int step, const double* __restrict__ L,
                                                     const double* __restrict__ Z,
-                                                    const double* __restrict__ W,
+                                                    const double* __restrict__ S,
                                                     double* __restrict__ T) {
   __shared__ double As[64][kPanel + 1];
   __shared__ double Bs[64][kPanel + 1];
@@ -2060,7 +2036,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
   const int ks = part * kSelSplitK, ke = min(m - c1, ks + kSelSplitK);
   const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
-  const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
   double* Tb = T + (int64_t)b * P.t_size + P.t_off[s];
   TileAcc acc;
   tile_zero(acc);
@@ -2087,7 +2063,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
           return LV{Tb[stage_lhat_off(m, step) + (int64_t)min(ii, nb - 1) * m + c1 + ks + kk], ii < nb};
         },
         [&](int jj, int kk) {
-          return LV{Ws[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + ks + kk], jj < nb};
+          return LV{Sb[(int64_t)(c0 + min(jj, nb - 1)) * m + c1 + ks + kk], jj < nb};
         },
         ke - ks, As, Bs, acc);
     tile_negate(acc);
@@ -2098,7 +2074,6 @@ __global__ __launch_bounds__(256) void k_sl_pp_part(DevPlan P, const int32_t* __
 
 __global__ __launch_bounds__(256) void k_sl_pp_sum(DevPlan P, const int32_t* __restrict__ tasks,
                                                    int step, double* __restrict__ S,
-                                                   double* __restrict__ W,
                                                    const double* __restrict__ T) {
   const int32_t* tk = tasks + 3 * blockIdx.x;
   const int s = tk[0];
@@ -2109,7 +2084,6 @@ __global__ __launch_bounds__(256) void k_sl_pp_sum(DevPlan P, const int32_t* __r
   const int nb = c1 - c0;
   const int parts = max(1, (m - c1 + kSelSplitK - 1) / kSelSplitK);
   double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
-  double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
   const double* Pp = T + (int64_t)b * P.t_size + P.t_off[s] + stage_pp_off(m);
   const int ii = threadIdx.x & 63, jb = threadIdx.x >> 6;
 #pragma unroll 4
@@ -2117,11 +2091,7 @@ __global__ __launch_bounds__(256) void k_sl_pp_sum(DevPlan P, const int32_t* __r
     const int jj = jb + 4 * u;
     double v = 0.0;
     for (int q = 0; q < parts; ++q) v += Pp[(int64_t)q * kTile * kTile + jj * kTile + ii];
-    if (ii < nb && jj < nb && ii >= jj) {
-      Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-      Ws[(int64_t)(c0 + jj) * m + c0 + ii] = v;
-      Ws[(int64_t)(c0 + ii) * m + c0 + jj] = v;
-    }
+    if (ii < nb && jj < nb && ii >= jj) Sb[(int64_t)(c0 + jj) * m + c0 + ii] = v;
   }
 }
 
@@ -2388,47 +2358,47 @@ hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int
 }
 
 hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                          double* S, double* W, const double* T, hipStream_t st) {
+                          double* S, const double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(512), 0, st, P, tasks, step, S, W, T);
   return hipGetLastError();
 }
 
 hipError_t launch_sl_gemm_part(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                               const double* W, double* T, hipStream_t st) {
+                               const double* S, const double* W, double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_gemm_part, dim3(count, B), dim3(512), 0, st, P, tasks, step, W, T);
+  hipLaunchKernelGGL(k_sl_gemm_part, dim3(count, B), dim3(512), 0, st, P, tasks, step, S, W, T);
   return hipGetLastError();
 }
 
 hipError_t launch_sl_gemm_sum(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                              double* S, double* W, const double* T, hipStream_t st) {
+                              double* S, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_gemm_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
+  hipLaunchKernelGGL(k_sl_gemm_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, T);
   return hipGetLastError();
 }
 
 hipError_t launch_sl_pp_lhat(const DevPlan& P, const int32_t* pp_tasks, int npp,
                              const int32_t* lhat_tasks, int nlhat, int step, int B, const double* L,
-                             const double* Z, double* S, double* W, double* T, hipStream_t st) {
+                             const double* Z, double* S, double* T, hipStream_t st) {
   if (npp + nlhat <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sl_pp_lhat, dim3(npp + nlhat, B), dim3(256), 0, st, P, pp_tasks, npp, lhat_tasks,
-                     step, L, Z, S, W, T);
+                     step, L, Z, S, T);
   return hipGetLastError();
 }
 
 hipError_t launch_sl_pp_part(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                             const double* L, const double* Z, const double* W, double* T,
+                             const double* L, const double* Z, const double* S, double* T,
                              hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_pp_part, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, W, T);
+  hipLaunchKernelGGL(k_sl_pp_part, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z, S, T);
   return hipGetLastError();
 }
 
 hipError_t launch_sl_pp_sum(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
-                            double* S, double* W, const double* T, hipStream_t st) {
+                            double* S, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_pp_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, W, T);
+  hipLaunchKernelGGL(k_sl_pp_sum, dim3(count, B), dim3(256), 0, st, P, tasks, step, S, T);
   return hipGetLastError();
 }
 

@@ -34,18 +34,18 @@ hipError_t launch_sl_gather_lhat(const DevPlan&, const int32_t*, int, const int3
                                 hipStream_t);
 hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const double*,
                           const double*, double*, hipStream_t);
-hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, double*,
+hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
                           const double*, hipStream_t);
 hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int,
-                             const double*, const double*, double*, double*, double*, hipStream_t);
+                             const double*, const double*, double*, double*, hipStream_t);
 hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
                              const double*, const double*, double*, hipStream_t);
-hipError_t launch_sl_gemm_part(const DevPlan&, const int32_t*, int, int, int, const double*, double*,
-                               hipStream_t);
-hipError_t launch_sl_gemm_sum(const DevPlan&, const int32_t*, int, int, int, double*, double*,
-                              const double*, hipStream_t);
-hipError_t launch_sl_pp_sum(const DevPlan&, const int32_t*, int, int, int, double*, double*,
-                            const double*, hipStream_t);
+hipError_t launch_sl_gemm_part(const DevPlan&, const int32_t*, int, int, int, const double*, const double*,
+                               double*, hipStream_t);
+hipError_t launch_sl_gemm_sum(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
+                              hipStream_t);
+hipError_t launch_sl_pp_sum(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
+                            hipStream_t);
 hipError_t launch_logdet(const DevPlan&, int, const double*, double*, uint32_t*, double*, int32_t*,
                          hipStream_t);
 hipError_t launch_logdet_seg(const DevPlan&, int, const double*, const int64_t*, const int32_t*, int,
@@ -1050,11 +1050,11 @@ struct gmrf_batch {
           if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= kGemmSplitMaxTasks) {
             prof_launch("k_sl_gemm_part", lv.sgpart[k].flops * count, 0.0, stream, [&] {
               return launch_sl_gemm_part(P, tk + 3 * lv.sgpart[k].off, (int)lv.sgpart[k].cnt, (int)k, count,
-                                         Wf, Tf, stream);
+                                         Sf, Wf, Tf, stream);
             });
             prof_launch("k_sl_gemm_sum", 0.0, 0.0, stream, [&] {
               return launch_sl_gemm_sum(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
-                                        Sf, Wf, Tf, stream);
+                                        Sf, Tf, stream);
             });
           } else {
             prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, 0.0, stream, [&] {
@@ -1065,10 +1065,10 @@ struct gmrf_batch {
           if (lv.sppart[k].cnt > 0) {
             prof_launch("k_sl_pp_part", lv.sppart[k].flops * count, 0.0, stream, [&] {
               return launch_sl_pp_part(P, tk + 3 * lv.sppart[k].off, (int)lv.sppart[k].cnt, (int)k, count,
-                                       Lf, Zf, Wf, Tf, stream);
+                                       Lf, Zf, Sf, Tf, stream);
             });
             prof_launch("k_sl_pp_sum", 0.0, 0.0, stream, [&] {
-              return launch_sl_pp_sum(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Sf, Wf,
+              return launch_sl_pp_sum(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, (int)k, count, Sf,
                                       Tf, stream);
             });
             if (next)
@@ -1082,7 +1082,7 @@ struct gmrf_batch {
             prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + (next ? lv.lhat[k + 1].flops : 0.0)) * count, 0.0,
                         stream, [&] {
                           return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, tl, nl, (int)k,
-                                                   count, Lf, Zf, Sf, Wf, Tf, stream);
+                                                   count, Lf, Zf, Sf, Tf, stream);
                         });
           }
         }

@@ -122,8 +122,9 @@ def test_c1_grid_integration_matches_oracle(c1, record_error):
     xs = np.array([r.x for r in nodes])
     wts, mean, var, mlik = ccd.aggregate(wo, lp, xs, np.array(dvar))
     e_lp = record_error("grid log posterior", float(np.max(np.abs(rb.log_post - lp))), MLIK_ATOL, "abs")
-    e_w = record_error("grid weights", float(np.max(np.abs(rb.weights - wts))), 1e-9, "abs")
+    # a log-posterior error eps moves a normalised weight by at most about 2 eps
+    e_w = record_error("grid weights", float(np.max(np.abs(rb.weights - wts))), 2 * MLIK_ATOL, "abs")
     e_m = record_error("mixture mean", _rel(rb.mean, mean), 1e-9)
     e_v = record_error("mixture variance", _rel(rb.var, var), 1e-9)
     e_ml = record_error("mixture mlik", abs(rb.mlik - mlik), MLIK_ATOL, "abs")
-    assert e_lp < MLIK_ATOL and e_w < 1e-9 and e_m < 1e-9 and e_v < 1e-9 and e_ml < MLIK_ATOL
+    assert e_lp < MLIK_ATOL and e_w < 2 * MLIK_ATOL and e_m < 1e-9 and e_v < 1e-9 and e_ml < MLIK_ATOL
