@@ -37,32 +37,135 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, theta, seconds_hint):
-    """Oracle (CPU restatement, 1 core) on one theta-point of the same workload:
-    prior + posterior LDL^T, mode, Takahashi diag (per CCD node, inference.rs:1306-1385)
-    plus a central finite-difference gradient (2m extra Laplace evaluations), the
-    reference's way of differentiating (hyper_opt.rs:197-259)."""
+def _cpu_cores():
+    """Host cores this process may use (the affinity mask), at most 16: the GPU box's
+    CPU share per GPU."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(wl, thetas):
+    """The CPU restatement of the reference (the oracle) on the same CCD theta-points,
+    the reference's way: each node is an independent find_mode + Takahashi diagonal
+    (inference.rs:1306-1385), run across the host cores as its Rayon par_iter over
+    nodes does (the oracle's calls release the interpreter lock, so a thread pool
+    runs them in parallel), and the gradient by central differences, 2m further
+    Laplace evaluations per point (hyper_opt.rs:197-259). Three measurements:
+    node work + FD gradient on all cores for all points (the value, the unit the GPU
+    line reports), node work alone on all cores, and node work on one core for one
+    point."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     from oracle import oracle as orc
 
-    coefs, _, lpar, _ = wl.params(theta)
-    t0 = time.perf_counter()
-    q = wl.gmrf.csc(coefs[0])
-    r = orc.find_mode(q, "gaussian", "identity", wl.y, lpar[0], A=wl.A, keep_post=True)
-    r.post_factor().takahashi_diag()
+    cores = _cpu_cores()
+    m = thetas.shape[1]
     h = 1e-4
-    for l in range(len(theta)):
-        for s in (h, -h):
-            th = np.array(theta, dtype=float)
-            th[l] += s
-            c, _, lp, _ = wl.params(th)
-            orc.find_mode(wl.gmrf.csc(c[0]), "gaussian", "identity", wl.y, lp[0], A=wl.A)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "theta-points/s", "cores": 1, "kind": "port",
-            "sample": f"1 CCD theta-point of the same workload on the oracle (1 thread): node work "
-                      f"(prior+posterior LDLT, mode solve, Takahashi diag) + 2m=6 FD Laplace "
-                      f"evaluations for the gradient; {dt:.1f} s"}
+
+    def node(th):
+        c, _, lp, _ = wl.params(th[None])
+        r = orc.find_mode(wl.gmrf.csc(c[0]), "gaussian", "identity", wl.y, lp[0], A=wl.A, keep_post=True)
+        r.post_factor().takahashi_diag()
+
+    def fd_eval(th):
+        c, _, lp, _ = wl.params(th[None])
+        orc.find_mode(wl.gmrf.csc(c[0]), "gaussian", "identity", wl.y, lp[0], A=wl.A)
+
+    fd_pts = []
+    for th in thetas:
+        for l in range(m):
+            for sgn in (h, -h):
+                t = np.array(th, dtype=float)
+                t[l] += sgn
+                fd_pts.append(t)
+    t0 = time.perf_counter()
+    node(thetas[0])
+    t_one = time.perf_counter() - t0
+    with ThreadPoolExecutor(cores) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(node, thetas))
+        t_nodes = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        list(ex.map(fd_eval, fd_pts))
+        t_fd = time.perf_counter() - t0
+    npts = len(thetas)
+    return {"value": npts / (t_nodes + t_fd), "unit": "theta-points/s", "cores": cores, "kind": "port",
+            "sample": f"the {npts} CCD theta-points of the same workload on the oracle, a thread per node "
+                      f"over {cores} host cores: node work (prior + posterior LDLT, mode, Takahashi diag) "
+                      f"{t_nodes:.1f} s + 2m = {2 * m} finite-difference Laplace evaluations per point for the "
+                      f"gradient {t_fd:.1f} s",
+            "node_only": {"value": npts / t_nodes, "unit": "theta-points/s", "cores": cores},
+            "one_core_node_only": {"value": 1.0 / t_one, "unit": "theta-points/s", "cores": 1,
+                                   "sample": f"1 theta-point, {t_one:.1f} s"},
+            "host_cpus_visible": os.cpu_count()}
+
+
+class OracleObjective:
+    """The interface of inla_gpu.fit.Objective on the oracle: the log posterior
+    (mlik + log prior) of the Gaussian SPDE workload, the points of one call spread
+    over the host cores (one thread per point, as the reference's par_iter over CCD
+    nodes)."""
+
+    def __init__(self, wl, cores):
+        self.wl, self.cores = wl, cores
+        self.calls = 0
+        self.points = 0
+
+    def values(self, thetas, grad=False, want_x=False, want_var=False):
+        from concurrent.futures import ThreadPoolExecutor
+
+        import numpy as np
+
+        from oracle import oracle as orc
+
+        thetas = np.atleast_2d(np.asarray(thetas, dtype=float))
+        wl = self.wl
+
+        def one(th):
+            c, _, lp, _ = wl.params(th[None])
+            r = orc.find_mode(wl.gmrf.csc(c[0]), "gaussian", "identity", wl.y, lp[0], A=wl.A, keep_post=want_var)
+            var = r.post_factor().takahashi_diag() if want_var else None
+            return r.mlik + wl.log_prior(th)[0], r.x, var
+
+        with ThreadPoolExecutor(min(self.cores, len(thetas))) as ex:
+            out = list(ex.map(one, thetas))
+        self.calls += 1
+        self.points += len(thetas)
+        res = {"g": np.array([o[0] for o in out])}
+        if want_x:
+            res["x"] = np.array([o[1] for o in out])
+        if want_var:
+            res["var"] = np.array([o[2] for o in out])
+        return res
+
+
+def cpu_fit_line(wl, theta0):
+    """The reference's fit sequence on the oracle, measured: Nelder-Mead one point at
+    a time (hyper_opt.rs:15-185), the Hessian stencil point by point (hyper_opt.rs:197-259),
+    then every CCD node at once across the host cores (inference.rs:1394) and the
+    mixture."""
+    from inla_gpu import ccd
+    from inla_gpu import fit as F
+
+    cores = _cpu_cores()
+    obj = OracleObjective(wl, cores)
+    t0 = time.perf_counter()
+    mode = F.nelder_mead_mode(obj, theta0)
+    t1 = time.perf_counter()
+    log(f"[cpu fit] Nelder-Mead: {t1 - t0:.1f} s, {obj.calls} evaluations")
+    h = F._hessian_step(mode)
+    import numpy as np
+
+    pts = ccd.hessian_points(mode, h)
+    H = ccd.hessian_from_values(np.concatenate([obj.values(p[None])["g"] for p in pts]), len(mode), h)
+    t2 = time.perf_counter()
+    thetas, w = F._nodes(mode, H)
+    r = obj.values(thetas, want_x=True, want_var=True)
+    _, mean, var, mlik = ccd.aggregate(w, r["g"], r["x"], r["var"])
+    t3 = time.perf_counter()
+    return {"total_s": t3 - t0, "mode_s": t1 - t0, "hessian_s": t2 - t1, "nodes_s": t3 - t2,
+            "evaluations": obj.points, "cores": cores, "mode": [float(v) for v in mode], "mlik": float(mlik)}
 
 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pmc_traffic_spde_gaussian_n20164.md")
@@ -414,6 +517,8 @@ def main():
                     help="add the full-size Besag x AR1 batched evaluation and fit comparison")
     ap.add_argument("--c5", action="store_true",
                     help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
+    ap.add_argument("--cpu-fit", action="store_true",
+                    help="add the reference's fit sequence measured on the oracle (host cores; minutes)")
     ap.add_argument("--nx", type=int, default=142)
     ap.add_argument("--n-obs", type=int, default=50_000)
     args = ap.parse_args()
@@ -483,29 +588,43 @@ def main():
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
-    prof = _lib.profile_summary()
+    prof = _lib.profile_summary(PEAK_FP64, PEAK_HBM)
     _lib.profile_enable(False)
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     name, d = dom
     avg_s = d["ms"] * 1e-3 / d["launches"]
     fl = d["flops"] / d["launches"]
-    achieved = fl / avg_s
-    bound = "mfma" if name in ("k_fl_diag", "k_fl_trsm", "k_fl_update", "k_sl_gemm", "k_sl_pp_lhat", "k_sl_lhat",
-                               "k_sl_gather_lhat", "k_sl_gemm_part", "k_sl_pp_part") else "hbm"
+    by = d["bytes"] / d["launches"]
+    # the bound is the roof the kernel's arithmetic intensity (algorithmic flops per
+    # algorithmic byte) puts it under: FP64 MFMA above the ridge, HBM below it
+    ridge = PEAK_FP64 / PEAK_HBM
+    intensity = fl / by if by > 0 else float("inf")
+    frac_flop = fl / avg_s / PEAK_FP64
+    frac_hbm = by / avg_s / PEAK_HBM
     traffic = traffic_per_launch(name)
-    if bound == "mfma":
-        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64, "traffic": traffic, "kernel": name}
+    if intensity >= ridge:
+        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": frac_flop, "traffic": traffic}
     else:
-        bw = (d["bytes"] / d["launches"]) / avg_s
-        roof = {"bound": "hbm", "achieved": bw / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": bw / PEAK_HBM, "traffic": traffic, "kernel": name}
-    roof["avg_launch_us"] = avg_s * 1e6
-    roof["flops_per_launch"] = fl
+        roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": frac_hbm, "traffic": traffic}
+    roof.update({"kernel": name, "avg_launch_us": avg_s * 1e6, "flops_per_launch": fl,
+                 "algorithmic_bytes_per_launch": by, "intensity_flop_per_byte": intensity,
+                 "ridge_flop_per_byte": ridge, "frac_flop": frac_flop, "frac_hbm": frac_hbm})
     if traffic is not None:
         roof["traffic_from"] = os.path.relpath(TRAFFIC_TABLE, ROOT)
+        roof["traffic_over_algorithmic"] = traffic / by if by > 0 else None
+    # the whole step: the level kernels' time at the roofline, sum over launches of
+    # max(F / P_fp64, B / BW), against the measured time per step
+    roof_ms = sum(v["roof_ms"] for v in prof.values()) / args.steps
+    roof["step_roofline_ms"] = roof_ms
+    roof["step_frac"] = roof_ms / ms_per_step
+    roof["step_roofline_note"] = "level kernels (factorisation, solves, selected inversion) at max(F/P, B/BW) " \
+                                 "per launch, summed, over the measured ms per step"
     kernels = {k: {"launches": v["launches"], "ms_per_step": v["ms"] / args.steps,
-                   "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0}
+                   "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0,
+                   "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0,
+                   "roof_ms_per_step": v["roof_ms"] / args.steps}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
     log(json.dumps(kernels, indent=1))
 
@@ -558,14 +677,16 @@ def main():
         log("== full-size SPDE x RW2 (n = 1.44M): factorise, solve, selected inversion")
         res["c5"] = c5_solver_line()
         log(json.dumps(res["c5"]))
-    if not args.no_cpu_baseline:
-        log("== cpu baseline (oracle, 1 thread, 1 theta-point)")
-        res["cpu_baseline"] = cpu_baseline(wl, wl.theta_true, 25)
+    if args.cpu_fit:
+        log("== the reference's fit sequence on the oracle (host cores)")
+        res["cpu_fit"] = cpu_fit_line(wl, wl.theta_true + np.array([0.5, -0.3, 0.3]))
         if "fit" in res:
-            # the reference sequence on the CPU restatement: per-point cost x its point count
-            pts = res["fit"]["plain_points"]
-            res["fit"]["cpu_plain_s_est"] = pts / res["cpu_baseline"]["value"] * \
-                (1.0 / (1 + 2 * len(wl.theta_true)))
+            res["cpu_fit"]["batched_gpu_speedup"] = res["cpu_fit"]["total_s"] / res["fit"]["batched_s"]
+        log(json.dumps(res["cpu_fit"]))
+    if not args.no_cpu_baseline:
+        log(f"== cpu baseline (oracle, {_cpu_cores()} host cores, the {npts} CCD theta-points)")
+        res["cpu_baseline"] = cpu_baseline(wl, thetas)
+        log(json.dumps(res["cpu_baseline"]))
     print(json.dumps(res), flush=True)
 
 

@@ -306,11 +306,14 @@ gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_g
  * stream (no reference counterpart; used by the benchmark's roofline).
  * gmrf_profile_enable(1) clears and starts recording; gmrf_profile_summary
  * waits for the recorded events and returns per-kernel launch counts, total
- * milliseconds and the algorithmic flops / bytes of the recorded launches;
+ * milliseconds, the algorithmic flops / bytes of the recorded launches and
+ * roof_ms, the sum over launches of max(flops / peak_flops, bytes / peak_bytes)
+ * (peaks in flop/s and byte/s): the time the launches would take at the roofline;
  * `names` receives max_entries strings of name_len bytes. */
 gmrf_status gmrf_profile_enable(int32_t on);
 gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_len,
                                  int64_t* launches, double* total_ms, double* flops, double* bytes,
+                                 double peak_flops, double peak_bytes, double* roof_ms,
                                  int32_t* n_entries);
 
 /* T = A B^T for one 64x64 tile through the FP64 MFMA tile engine (A, B are

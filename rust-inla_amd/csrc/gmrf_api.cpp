@@ -537,13 +537,16 @@ gmrf_status gmrf_profile_enable(int32_t on) {
 
 gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_len,
                                  int64_t* launches, double* total_ms, double* flops, double* bytes,
+                                 double peak_flops, double peak_bytes, double* roof_ms,
                                  int32_t* n_entries) {
   if (!n_entries) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (roof_ms && !(peak_flops > 0.0 && peak_bytes > 0.0))
+    return fail(GMRF_INVALID_ARGUMENT, "roof_ms needs positive peaks");
   try {
     std::lock_guard<std::mutex> lk(g_prof.mu);
     std::vector<std::string> keys;
     std::vector<int64_t> cnt;
-    std::vector<double> ms, fl, by;
+    std::vector<double> ms, fl, by, rf;
     for (auto& r : g_prof.recs) {
       check(hipEventSynchronize(r.b), "hipEventSynchronize");
       float t = 0.0f;
@@ -555,11 +558,13 @@ gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_
         ms.push_back(0);
         fl.push_back(0);
         by.push_back(0);
+        rf.push_back(0);
       }
       cnt[k]++;
       ms[k] += t;
       fl[k] += r.flops;
       by[k] += r.bytes;
+      if (roof_ms) rf[k] += 1e3 * std::max(r.flops / peak_flops, r.bytes / peak_bytes);
     }
     *n_entries = (int32_t)keys.size();
     for (int32_t k = 0; k < (int32_t)keys.size() && k < max_entries; ++k) {
@@ -571,6 +576,7 @@ gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_
       if (total_ms) total_ms[k] = ms[k];
       if (flops) flops[k] = fl[k];
       if (bytes) bytes[k] = by[k];
+      if (roof_ms) roof_ms[k] = rf[k];
     }
     return GMRF_OK;
   } catch (...) {

@@ -1817,6 +1817,26 @@ __global__ __launch_bounds__(256) void k_sl_gather_lhat(DevPlan P, const int32_t
     sl_lhat_task(P, l_tasks + 3 * (blockIdx.x - ng), 0, blockIdx.y, L, Z, T, sh);
 }
 
+// Workgroup -> (task, matrix) of a launch over `count` tasks and B matrices on a 1-D
+// grid. Workgroups are dispatched to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2. With xcd set, every task of matrix b runs on XCD b % 8 (grid
+// 8 ceil(B / 8) count; the workgroups past the last matrix return at once), so that
+// the tiles of one front share their operands through one L2: an off-diagonal
+// entry of Sigma is read by two row tiles, and every tile reads the whole L_hat
+// panel. Without it the grid is count x B in the usual order.
+__device__ __forceinline__ bool wg_task(int count, int B, int xcd, int& task, int& b) {
+  const int L = blockIdx.x;
+  if (xcd) {
+    const int x = L & 7, j = L >> 3, q = j / count;
+    task = j - q * count;
+    b = x + 8 * q;
+    return b < B;
+  }
+  b = L / count;
+  task = L - b * count;
+  return true;
+}
+
 // acc += Sigma[R0 .. R0 + 63][k_lo .. k_hi) Lh[k_lo .. k_hi)[0 .. nb) from the lower
 // triangle of Sigma only (the front's own columns in S, the R x R block in W, the
 // same column stride m). Up to the end of the tile's diagonal block the entries are
@@ -1855,17 +1875,18 @@ __device__ __forceinline__ void sigma_strip(const double* __restrict__ Sb, const
 // Sigma[B,P]: task (s, I) computes rows c1 + 64 I .. of B as
 // -sum_{k in B} Sigma[i][k] Lh[k][p] (K = m - c1, lower triangle only) and stores it
 // into S (column c0 + p, the layout of L).
-__global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks,
-                                                 int step, double* __restrict__ S,
+__global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __restrict__ tasks, int count,
+                                                 int B, int xcd, int step, double* __restrict__ S,
                                                  const double* __restrict__ W,
                                                  const double* __restrict__ T) {
   constexpr int NT = 512;
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   double (*As)[kPanel + 1] = (double (*)[kPanel + 1])sh;
   double (*Bs)[kPanel + 1] = (double (*)[kPanel + 1])(sh + 64 * (kPanel + 1));
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+  int task, b;
+  if (!wg_task(count, B, xcd, task, b)) return;
+  const int32_t* tk = tasks + 3 * task;
   const int s = tk[0], I = tk[1];
-  const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
   int c0, c1, pk;
   sel_panel(w, step, c0, c1, pk);
@@ -2360,7 +2381,11 @@ hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int
 hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
                           double* S, const double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_gemm, dim3(count, B), dim3(512), 0, st, P, tasks, step, S, W, T);
+  // XCD-grouped when every XCD gets a matrix and a matrix's tiles fit one XCD
+  const int xcd = (B >= 8 && count <= kXcdMaxTasks) ? 1 : 0;
+  const int64_t grid = xcd ? (int64_t)8 * ((B + 7) / 8) * count : (int64_t)count * B;
+  hipLaunchKernelGGL(k_sl_gemm, dim3((unsigned)grid), dim3(512), 0, st, P, tasks, count, B, xcd, step, S, W,
+                     T);
   return hipGetLastError();
 }
 

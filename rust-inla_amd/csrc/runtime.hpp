@@ -591,6 +591,8 @@ struct SymData {
         double Dfac = 0, Dsel = 0;
         std::vector<double> dd(maxsteps, 0), dt(maxsteps, 0), du(maxsteps, 0), dl(maxsteps, 0),
             dg(maxsteps, 0), dq(maxsteps, 0);
+        // algorithmic bytes of each launch kind and step: every operand read and every
+        // result written once (triangular blocks as triangles)
         const int64_t np = (w + kTile - 1) / kTile;
         for (int64_t k = 0; k < np; ++k) {
           const double c0 = (double)(k * kTile), nb = (double)std::min<int64_t>(kTile, w - k * kTile);
@@ -599,6 +601,9 @@ struct SymData {
           dt[k] = nb * nb * T;
           du[k] = nb * T * (T + 1.0) / 2.0;
           Dfac += dd[k] + dt[k] + du[k];
+          L.fdiag[k].bytes += 8.0 * (nb * (nb + 1.0) + 64.0 * nb);          // block in and out, Z out
+          L.ftrsm[k].bytes += 8.0 * (2.0 * T * nb + 64.0 * nb);            // A21 in, Z in, L21 out
+          L.fupd[k].bytes += 8.0 * (T * (T + 1.0) + T * nb);               // trailing lower in and out, panel in
           // selinv step k handles panel np - 1 - k
           const int64_t pk = np - 1 - k, sc0 = pk * kTile, sc1 = std::min<int64_t>(w, sc0 + kTile);
           const double nbs = (double)(sc1 - sc0), nB = (double)(m - sc1);
@@ -606,6 +611,9 @@ struct SymData {
           dg[k] = nB * nB * nbs;
           dq[k] = nbs * nbs * nB + nbs * nbs * nbs / 6.0;
           Dsel += dl[k] + dg[k] + dq[k];
+          L.lhat[k].bytes += 8.0 * (2.0 * nB * nbs + 64.0 * nbs);                   // L[B,P], Z in, L_hat out
+          L.sgemm[k].bytes += 8.0 * (nB * (nB + 1.0) / 2.0 + 2.0 * nB * nbs);      // Sigma[B,B] lower, L_hat in, Sigma[B,P] out
+          L.spp[k].bytes += 8.0 * (64.0 * nbs + 2.0 * nB * nbs + nbs * (nbs + 1.0) / 2.0);
         }
         for (int64_t k = 0; k < maxsteps; ++k) {
           if (Dfac > 0) {
@@ -680,6 +688,7 @@ struct SymData {
       for (int64_t s : fronts) {
         int64_t w, r, m;
         mw(s, w, r, m);
+        L.gather.bytes += 8.0 * (double)r * (double)(r + 1);  // parent's Sigma_RR lower in, W lower out
         const int64_t nt = ceil64(r);
         for (int64_t J = 0; J < nt; ++J)
           for (int64_t I = J; I < nt; ++I) add(s, I, J);
@@ -748,6 +757,7 @@ struct SymData {
           }
         L.sgpart[k].cnt = (int64_t)tk.size() / 3 - L.sgpart[k].off;
         L.sgpart[k].flops = L.sgemm[k].flops;
+        L.sgpart[k].bytes = L.sgemm[k].bytes;
         L.spp[k].off = (int64_t)tk.size() / 3;
         bool tall = false;
         for (int64_t s : fronts) {
@@ -766,6 +776,7 @@ struct SymData {
           }
         L.sppart[k].cnt = (int64_t)tk.size() / 3 - L.sppart[k].off;
         L.sppart[k].flops = L.spp[k].flops;
+        L.sppart[k].bytes = L.spp[k].bytes;
       }
     }
     d_tasks = arena.upload(tk.empty() ? std::vector<int32_t>(3, 0) : tk);
@@ -899,24 +910,25 @@ struct gmrf_batch {
         auto fuse = [&](int64_t tiles) { return tiles * count < kFuseDiagBelow; };
         bool diag_done = fuse(lv.asm_t.cnt);
         prof_launch("k_fl_assemble", diag_done && !lv.fdiag.empty() ? lv.fdiag[0].flops * count : 0.0,
-                    lv.asm_t.bytes * count, stream, [&] {
+                    (lv.asm_t.bytes + (diag_done && !lv.fdiag.empty() ? lv.fdiag[0].bytes : 0.0)) * count, stream,
+                    [&] {
                       return launch_fl_assemble(P, tk + 3 * lv.asm_t.off, (int)lv.asm_t.cnt, (int)lv.ain_first,
                                                 count, diag_done, vals, ldv, Lf, Uf, Zf, stream);
                     });
         for (size_t k = 0; k < lv.fdiag.size(); ++k) {
           if (!diag_done)
-            prof_launch("k_fl_diag", lv.fdiag[k].flops * count, 0.0, stream, [&] {
+            prof_launch("k_fl_diag", lv.fdiag[k].flops * count, lv.fdiag[k].bytes * count, stream, [&] {
               return launch_fl_diag(P, tk + 3 * lv.fdiag[k].off, (int)lv.fdiag[k].cnt, (int)k, count,
                                     Lf, Zf, stream);
             });
-          prof_launch("k_fl_trsm", lv.ftrsm[k].flops * count, 0.0, stream, [&] {
+          prof_launch("k_fl_trsm", lv.ftrsm[k].flops * count, lv.ftrsm[k].bytes * count, stream, [&] {
             return launch_fl_trsm(P, tk + 3 * lv.ftrsm[k].off, (int)lv.ftrsm[k].cnt, (int)k, count,
                                   Lf, Zf, stream);
           });
           const bool next = k + 1 < lv.fdiag.size();
           diag_done = next && fuse(lv.fupd[k].cnt);
-          prof_launch("k_fl_update", (lv.fupd[k].flops + (diag_done ? lv.fdiag[k + 1].flops : 0.0)) * count, 0.0,
-                      stream, [&] {
+          prof_launch("k_fl_update", (lv.fupd[k].flops + (diag_done ? lv.fdiag[k + 1].flops : 0.0)) * count,
+                      (lv.fupd[k].bytes + (diag_done ? lv.fdiag[k + 1].bytes : 0.0)) * count, stream, [&] {
                         return launch_fl_update(P, tk + 3 * lv.fupd[k].off, (int)lv.fupd[k].cnt, (int)k, count,
                                                 diag_done, Lf, Uf, Zf, stream);
                       });
@@ -979,19 +991,21 @@ struct gmrf_batch {
     check(launch_gather_perm(P, count * nr, rhs, ldr, Xf, stream, period), "gather");
     for (const LevelInfo& lv : sym->levels) {
       if (lv.se > lv.sb)
-        prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
+        prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count * nr, 2.0 * lv.sol_small * count * nr, stream, [&] {
           return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
                                   Lf, Xf, Vf, stream);
         });
       if (lv.le > lv.lb) {
         const bool sp = lv.sol_split;
-        prof_launch("k_solve_fwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, 0.0, stream, [&] {
+        prof_launch("k_solve_fwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr,
+                    2.0 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, stream, [&] {
           return launch_solve_fwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb),
                                       sp ? lv.maxw_large : lv.maxm_large, count, nr, !sp, Lf, Zf, Xf, Vf,
                                       stream);
         });
         if (sp)
-          prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
+          prof_launch("k_solve_fwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr,
+                      2.0 * (lv.sol_large - lv.sol_tri) * count * nr, stream, [&] {
             return launch_solve_fwd_rows(P, tk + 3 * lv.frow.off, (int)lv.frow.cnt, lv.maxw_large, count,
                                          nr, Lf, Xf, Vf, stream);
           });
@@ -1003,18 +1017,20 @@ struct gmrf_batch {
       if (lv.le > lv.lb) {
         const bool sp = lv.sol_split;
         if (sp)
-          prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr, 0.0, stream, [&] {
+          prof_launch("k_solve_bwd_rows", 0.5 * (lv.sol_large - lv.sol_tri) * count * nr,
+                      2.0 * (lv.sol_large - lv.sol_tri) * count * nr, stream, [&] {
             return launch_solve_bwd_rows(P, tk + 3 * lv.brow.off, (int)lv.brow.cnt, count, nr, Lf, Xf, Tf,
                                          stream);
           });
-        prof_launch("k_solve_bwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, 0.0, stream, [&] {
+        prof_launch("k_solve_bwd_tri", 0.5 * (sp ? lv.sol_tri : lv.sol_large) * count * nr,
+                    2.0 * (sp ? lv.sol_tri : lv.sol_large) * count * nr, stream, [&] {
           return launch_solve_bwd_tri(P, sym->d_large + lv.lb, (int)(lv.le - lv.lb),
                                       sp ? lv.maxw_large : lv.maxm_large, count, nr, !sp, Lf, Zf, Xf, Tf,
                                       stream);
         });
       }
       if (lv.se > lv.sb)
-        prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count * nr, 0.0, stream, [&] {
+        prof_launch("k_solve_bwd", 0.5 * lv.sol_small * count * nr, 2.0 * lv.sol_small * count * nr, stream, [&] {
           return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
                                   Lf, Xf, stream);
         });
@@ -1038,7 +1054,8 @@ struct gmrf_batch {
         // L_hat of panel step k sits in buffer k % 2 of the staging area; the gather
         // launch forms the L_hat of step 0, the launch that closes step k that of k + 1
         const bool l0 = !lv.lhat.empty();
-        prof_launch("k_sl_gather_lhat", l0 ? lv.lhat[0].flops * count : 0.0, 0.0, stream, [&] {
+        prof_launch("k_sl_gather_lhat", l0 ? lv.lhat[0].flops * count : 0.0,
+                    (lv.gather.bytes + (l0 ? lv.lhat[0].bytes : 0.0)) * count, stream, [&] {
           return launch_sl_gather_lhat(P, tk + 3 * lv.gather.off, (int)lv.gather.cnt,
                                        tk + 3 * (l0 ? lv.lhat[0].off : 0), l0 ? (int)lv.lhat[0].cnt : 0, count,
                                        Sf, Uf, Wf, Lf, Zf, Tf, stream);
@@ -1048,7 +1065,7 @@ struct gmrf_batch {
           // few row blocks with a long K (the matrices of tall fronts, a small batch):
           // split K so that the product spreads over the device
           if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= kGemmSplitMaxTasks) {
-            prof_launch("k_sl_gemm_part", lv.sgpart[k].flops * count, 0.0, stream, [&] {
+            prof_launch("k_sl_gemm_part", lv.sgpart[k].flops * count, lv.sgpart[k].bytes * count, stream, [&] {
               return launch_sl_gemm_part(P, tk + 3 * lv.sgpart[k].off, (int)lv.sgpart[k].cnt, (int)k, count,
                                          Sf, Wf, Tf, stream);
             });
@@ -1057,13 +1074,13 @@ struct gmrf_batch {
                                         Sf, Tf, stream);
             });
           } else {
-            prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, 0.0, stream, [&] {
+            prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, lv.sgemm[k].bytes * count, stream, [&] {
               return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
                                     Sf, Wf, Tf, stream);
             });
           }
           if (lv.sppart[k].cnt > 0) {
-            prof_launch("k_sl_pp_part", lv.sppart[k].flops * count, 0.0, stream, [&] {
+            prof_launch("k_sl_pp_part", lv.sppart[k].flops * count, lv.sppart[k].bytes * count, stream, [&] {
               return launch_sl_pp_part(P, tk + 3 * lv.sppart[k].off, (int)lv.sppart[k].cnt, (int)k, count,
                                        Lf, Zf, Sf, Tf, stream);
             });
@@ -1072,15 +1089,15 @@ struct gmrf_batch {
                                       Tf, stream);
             });
             if (next)
-              prof_launch("k_sl_lhat", lv.lhat[k + 1].flops * count, 0.0, stream, [&] {
+              prof_launch("k_sl_lhat", lv.lhat[k + 1].flops * count, lv.lhat[k + 1].bytes * count, stream, [&] {
                 return launch_sl_lhat(P, tk + 3 * lv.lhat[k + 1].off, (int)lv.lhat[k + 1].cnt, (int)k + 1,
                                       count, Lf, Zf, Tf, stream);
               });
           } else {
             const int nl = next ? (int)lv.lhat[k + 1].cnt : 0;
             const int32_t* tl = tk + 3 * (next ? lv.lhat[k + 1].off : 0);
-            prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + (next ? lv.lhat[k + 1].flops : 0.0)) * count, 0.0,
-                        stream, [&] {
+            prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + (next ? lv.lhat[k + 1].flops : 0.0)) * count,
+                        (lv.spp[k].bytes + (next ? lv.lhat[k + 1].bytes : 0.0)) * count, stream, [&] {
                           return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, tl, nl, (int)k,
                                                    count, Lf, Zf, Sf, Tf, stream);
                         });

@@ -147,7 +147,8 @@ SIGNATURES = {
     "gmrf_obs_loglik": (_i32, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gmrf_model_set_stream": (_i32, [_vp, _vp, _i32]),
     "gmrf_profile_enable": (_i32, [_i32]),
-    "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _pi32]),
+    "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _dbl, _dbl, _pdbl,
+                                    _pi32]),
 }
 
 
@@ -177,19 +178,21 @@ def profile_enable(on: bool) -> None:
         raise RuntimeError(last_error())
 
 
-def profile_summary() -> dict:
-    """{kernel: dict(launches, ms, flops, bytes)} of the recorded launches."""
+def profile_summary(peak_flops: float = 78.6e12, peak_bytes: float = 8.0e12) -> dict:
+    """{kernel: dict(launches, ms, flops, bytes, roof_ms)} of the recorded launches;
+    roof_ms = sum over launches of max(flops / peak_flops, bytes / peak_bytes)."""
     cap, nl = 64, 64
     names = C.create_string_buffer(cap * nl)
     la = (C.c_int64 * cap)()
-    ms, fl, by = (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)()
+    ms, fl, by, rf = (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)()
     ne = C.c_int32(0)
-    if lib.gmrf_profile_summary(cap, names, nl, la, ms, fl, by, C.byref(ne)) != GMRF_OK:
+    if lib.gmrf_profile_summary(cap, names, nl, la, ms, fl, by, peak_flops, peak_bytes, rf,
+                                C.byref(ne)) != GMRF_OK:
         raise RuntimeError(last_error())
     out = {}
     for k in range(min(ne.value, cap)):
         nm = names.raw[k * nl:(k + 1) * nl].split(b"\0", 1)[0].decode()
-        out[nm] = {"launches": int(la[k]), "ms": ms[k], "flops": fl[k], "bytes": by[k]}
+        out[nm] = {"launches": int(la[k]), "ms": ms[k], "flops": fl[k], "bytes": by[k], "roof_ms": rf[k]}
     return out
 
 
