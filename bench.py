@@ -561,8 +561,14 @@ def main():
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
     coefs, dcoefs, lpar, dlog = dev(coefs), dev(dcoefs), dev(lpar), dev(dlog)
 
+    # the outputs live in fixed buffers, so that every step replays the captured graph
+    outs = {"mlik": torch.empty(npts, dtype=torch.float64, device="cuda"),
+            "x": torch.empty((npts, g.n), dtype=torch.float64, device="cuda"),
+            "var": torch.empty((npts, g.n), dtype=torch.float64, device="cuda"),
+            "grad": torch.empty((npts, m), dtype=torch.float64, device="cuda")}
+
     def step():
-        return model.eval(coefs, lpar=lpar, dcoefs=dcoefs, dlog_lpar=dlog, want_x=True, want_var=True)
+        return model.eval(coefs, lpar=lpar, dcoefs=dcoefs, dlog_lpar=dlog, want_x=True, want_var=True, out=outs)
 
     log(f"== warmup {args.warmup}")
     for _ in range(args.warmup):

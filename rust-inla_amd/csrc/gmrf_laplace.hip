@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "../../include/gmrf_c.h"
 #include "plan.hpp"
 
 namespace gmrf {
@@ -697,6 +698,57 @@ static inline int grid_for(int64_t tot) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Fill kernels: clearing inside a captured graph goes through a kernel, not a memset node.
+__global__ void k_fill_f64(int64_t len, double v, double* __restrict__ p) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+__global__ void k_fill_i32(int64_t len, int32_t v, int32_t* __restrict__ p) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+
+// End of the Gaussian-identity evaluation on the device, one thread per point b:
+// status (the Newton step's validity, then the prior's and the posterior's factor
+// flags, in the order of the host path), the Laplace marginal likelihood
+// sum log p - x^T Q x / 2 + log|Q_prior| / 2 - log|Q_post| / 2 (inference.rs:972) and
+// the gradient sum_k dcoef_k (tr(Sigma_prior Q_k) - tr(Sigma_post Q_k) - x^T Q_k x) / 2
+// plus, for the Gaussian precision, dlog(prec) (sum dlogp/dlog(prec) - sum v w / 2).
+// fstat[b] is the posterior's factor status, fstat[B + b] the prior's; ld likewise.
+__global__ void k_gauss_finish(int B, int K, int mt, int fam, const int32_t* __restrict__ fstat,
+                               const int32_t* __restrict__ bad, const double* __restrict__ maxstep,
+                               const double* __restrict__ ld, const double* __restrict__ quad,
+                               const double* __restrict__ loglik, const double* __restrict__ tr_post,
+                               const double* __restrict__ tr_prior, const double* __restrict__ qx,
+                               const double* __restrict__ dcoefs, const double* __restrict__ dlog_lpar,
+                               const double* __restrict__ sum_dl, const double* __restrict__ sum_vw,
+                               double* __restrict__ mlik, double* __restrict__ grad, int32_t* __restrict__ status) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  // factor flags: bit 1 a zero pivot (not positive definite), bit 0 a pivot below
+  // 1e-14 or non-finite (singular)
+  auto code = [](int32_t f) { return (f & 2) ? (int)GMRF_NOT_POSITIVE_DEFINITE : (f & 1) ? (int)GMRF_SINGULAR : 0; };
+  int st = (bad[b] || !isfinite(maxstep[b])) ? (int)GMRF_SOLVE_FAILURE : 0;
+  const int fp = code(fstat[B + b]), fq = code(fstat[b]);
+  if (fp != 0)
+    st = fp;
+  else if (st == 0 && fq != 0)
+    st = fq;
+  status[b] = st;
+  mlik[b] = st == 0 ? loglik[b] - 0.5 * quad[b] + 0.5 * ld[B + b] - 0.5 * ld[b] : -INFINITY;
+  if (grad) {
+    for (int l = 0; l < mt; ++l) {
+      double gq = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double c = dcoefs[((int64_t)b * mt + l) * K + k];
+        gq += c * (-0.5 * qx[b * K + k] + 0.5 * tr_prior[b * K + k] - 0.5 * tr_post[b * K + k]);
+      }
+      if (fam == FAM_GAUSSIAN && dlog_lpar) gq += dlog_lpar[(int64_t)b * mt + l] * (sum_dl[b] - 0.5 * sum_vw[b]);
+      grad[(int64_t)b * mt + l] = st == 0 ? gq : NAN;
+    }
+  }
+}
+
 hipError_t lap_lik(int B, int64_t nobs, int fam, int link, const double* eta, const double* y,
                    const double* pa, const double* pb, const double* lpar, double* logp, double* grad,
                    double* w, double* dw, double* dlpar, int32_t* bad, int32_t* invalid, hipStream_t st) {
@@ -841,6 +893,28 @@ hipError_t lap_mul(int64_t tot, const double* a, const double* b, double* out, h
 hipError_t lap_blend(int B, int64_t len, const double* keep, const double* xt, double* x,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_blend, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, keep, xt, x);
+  return hipGetLastError();
+}
+
+hipError_t lap_fill_f64(int64_t len, double v, double* p, hipStream_t st) {
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(len)), dim3(256), 0, st, len, v, p);
+  return hipGetLastError();
+}
+
+hipError_t lap_fill_i32(int64_t len, int32_t v, int32_t* p, hipStream_t st) {
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(len)), dim3(256), 0, st, len, v, p);
+  return hipGetLastError();
+}
+
+hipError_t lap_gauss_finish(int B, int K, int mt, int fam, const int32_t* fstat, const int32_t* bad,
+                            const double* maxstep, const double* ld, const double* quad, const double* loglik,
+                            const double* tr_post, const double* tr_prior, const double* qx, const double* dcoefs,
+                            const double* dlog_lpar, const double* sum_dl, const double* sum_vw, double* mlik,
+                            double* grad, int32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_gauss_finish, dim3((B + 63) / 64), dim3(64), 0, st, B, K, mt, fam, fstat, bad, maxstep, ld,
+                     quad, loglik, tr_post, tr_prior, qx, dcoefs, dlog_lpar, sum_dl, sum_vw, mlik, grad, status);
   return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 // no reference counterpart, the reference differentiates by finite differences,
 // crates/inla_math/src/hyper_opt.rs:197-259).
 #include <cmath>
+#include <cstdlib>
 
 #include "runtime.hpp"
 
@@ -46,10 +47,17 @@ hipError_t lap_lowrank_axpy(int, int, int64_t, const double*, int64_t, int64_t, 
                             double*, int64_t, hipStream_t);
 hipError_t lap_lowrank_diag(int, int, int64_t, const double*, int64_t, int64_t, const double*,
                             double*, int64_t, hipStream_t);
+hipError_t lap_fill_f64(int64_t, double, double*, hipStream_t);
+hipError_t lap_fill_i32(int64_t, int32_t, int32_t*, hipStream_t);
+hipError_t lap_gauss_finish(int, int, int, int, const int32_t*, const int32_t*, const double*, const double*,
+                            const double*, const double*, const double*, const double*, const double*,
+                            const double*, const double*, const double*, const double*, double*, double*,
+                            int32_t*, hipStream_t);
 }  // namespace gmrf
 
 namespace {
 constexpr double kPriorJitter = 1e-12;  // inference.rs:831-834
+constexpr size_t kGraphCache = 4;       // captured Gaussian evaluations kept per model
 enum { RED_MAXABS = 0, RED_SUM = 1, RED_DOT = 2, RED_MAXDIFF = 3 };
 }  // namespace
 
@@ -140,7 +148,36 @@ struct gmrf_model {
   double *red = nullptr, *alpha = nullptr, *cres = nullptr;
   int32_t* bad = nullptr;
   double *tr_post = nullptr, *tr_prior = nullptr, *qx = nullptr, *ql = nullptr, *cpart = nullptr;
+  int32_t* dstat = nullptr;  // per-point status written on the device (Gaussian path)
+  // The Gaussian evaluation captured as HIP graphs, one per (point count, gradient
+  // length, device pointers of the call), the most recently used kept (kGraphCache).
+  struct GaussGraph {
+    int B = 0, mt = 0;
+    const void* ptr[8] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool same(int b, const gmrf_eval_args* a, int m) const {
+      const void* p[8] = {a->coefs, a->lpar, a->dcoefs, a->dlog_lpar, a->mlik, a->x, a->var, a->grad};
+      if (b != B || m != mt) return false;
+      for (int i = 0; i < 8; ++i)
+        if (p[i] != ptr[i]) return false;
+      return true;
+    }
+    void destroy() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  };
+  std::vector<GaussGraph> graphs;  // most recently used last
+  hipEvent_t gev_in = nullptr, gev_out = nullptr;
   hipStream_t stream() const { return bt->stream; }
+  ~gmrf_model() {
+    for (auto& g : graphs) g.destroy();
+    if (gev_in) (void)hipEventDestroy(gev_in);
+    if (gev_out) (void)hipEventDestroy(gev_out);
+  }
 };
 
 namespace {
@@ -353,6 +390,159 @@ std::vector<double> lowrank_traces(gmrf_model* M, int B, bool prior) {
     }
   }
   return out;
+}
+
+// The Gaussian-identity evaluation without constraints (inference.rs:845-870, one
+// Newton system at x = 0, then the mlik, the selected inversions, the variances and
+// the gradient) as one sequence of device work on the model's stream, with no host
+// synchronisation inside: it runs eagerly or under stream capture alike. The same
+// kernels as the general path, with the clears as fill kernels and the per-point
+// status, mlik and gradient finished on the device (k_gauss_finish) into M->dstat,
+// a->mlik and a->grad. Reduction slices of M->red: [0, B) max |step|, [B, 2B) x^T Q x,
+// [2B, 3B) sum log p, [3B, 4B) sum dlogp/dlog(prec), [4B, 5B) sum v w.
+void enqueue_gauss(gmrf_model* M, const gmrf_eval_args* a, int B) {
+  const hipStream_t st = M->stream();
+  const int64_t n = M->n, nnz = M->nnz, m = M->nobs;
+  const int K = M->K;
+  gmrf_batch* bt = M->bt.get();
+  const bool want_grad = a->dcoefs != nullptr && a->n_theta > 0 && a->grad != nullptr;
+  const int sel_count = want_grad ? 2 * B : (a->var ? B : 0);
+  double* red = M->red;
+  check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st), "affine");
+  double* vprior_f = M->vals + (size_t)B * nnz;
+  check(hipMemcpyAsync(vprior_f, M->vprior, (size_t)B * nnz * sizeof(double), hipMemcpyDeviceToDevice, st),
+        "copy prior");
+  check(lap_add_diag(B, n, nnz, M->diag_idx, kPriorJitter, vprior_f, st), "jitter");
+  check(lap_fill_f64((int64_t)B * n, 0.0, M->x, st), "zero x");
+  check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d,
+                 M->ar_long.n, st),
+        "A x");
+  check(lap_fill_i32(B, 0, M->bad, st), "zero bad");
+  check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w, nullptr,
+                nullptr, M->bad, nullptr, st),
+        "lik");
+  check(lap_add_atwa(B, nnz, m, M->atwa_ptr, M->atwa_obs, M->atwa_coef, M->w, M->vprior, M->vals,
+                     M->atwa_long.d, M->atwa_long.n, st),
+        "atwa");
+  check(lap_spmv(B, n, M->a_cptr, M->a_ridx, M->a_cval, 0, M->g, m, 1.0, 0.0, M->rhs, n, M->ac_long.d,
+                 M->ac_long.n, st),
+        "A^T g");
+  check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, -1.0, 1.0, M->rhs, n, M->q_long.d,
+                 M->q_long.n, st),
+        "- Q x");
+  if (sel_count > 0) {
+    // the inversion needs the factors only: it starts before the log determinants
+    bt->factorize_levels(2 * B, M->vals, nnz, 0);
+    bt->selinv_async(sel_count, 0);
+    bt->logdet(2 * B, 0);
+  } else {
+    bt->factorize(2 * B, M->vals, nnz, 0);
+  }
+  bt->solve(B, M->rhs, n, M->step, n, 0);
+  check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, red, st), "max step");
+  check(hipMemcpyAsync(M->x, M->step, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x = step");
+  // mlik terms at x*
+  check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d,
+                 M->ar_long.n, st),
+        "eta*");
+  check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w, M->dw, M->dl,
+                nullptr, nullptr, st),
+        "lik*");
+  check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d,
+                 M->q_long.n, st),
+        "Q x*");
+  check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, red + B, st), "quad");
+  check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, red + 2 * B, st), "loglik");
+  if (a->x)
+    check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
+  if (sel_count > 0) bt->selinv_join();
+  if (a->var) check(launch_diag_out(M->sym->plan, B, bt->S, a->var, n, st), "var");
+  if (want_grad) {
+    const DevPlan& P = M->sym->plan;
+    check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col, M->x, M->x, n,
+                       M->tr_post, M->qx, M->cpart, st),
+          "contract post");
+    check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full, nullptr,
+                       nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st),
+          "contract prior");
+    check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
+    check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, red + 3 * B, st), "sum dl");
+    check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, red + 4 * B, st), "sum v w");
+  }
+  check(lap_gauss_finish(B, K, want_grad ? a->n_theta : 0, M->fam, bt->d_status, M->bad, red, bt->d_logdet,
+                         red + B, red + 2 * B, M->tr_post, M->tr_prior, M->qx, a->dcoefs, a->dlog_lpar,
+                         red + 3 * B, red + 4 * B, a->mlik, want_grad ? a->grad : nullptr, M->dstat, st),
+        "finish");
+}
+
+// Runs enqueue_gauss, as a HIP graph unless per-launch profiling is on or
+// GMRF_NO_GRAPH is set: the graph is captured on the model's own non-blocking stream
+// (the caller's may be the null stream, which cannot be captured) and launched there
+// between events on the caller's stream. Then reads the per-point status back.
+void eval_gauss(gmrf_model* M, gmrf_eval_args* a, int B) {
+  gmrf_batch* bt = M->bt.get();
+  const hipStream_t cs = bt->stream;
+  static const bool no_graph = std::getenv("GMRF_NO_GRAPH") != nullptr;
+  const int mt = (a->dcoefs && a->n_theta > 0 && a->grad) ? a->n_theta : 0;
+  if (g_prof.on || no_graph) {
+    enqueue_gauss(M, a, B);
+  } else {
+    if (!M->gev_in) {
+      check(hipEventCreateWithFlags(&M->gev_in, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&M->gev_out, hipEventDisableTiming), "hipEventCreate");
+    }
+    auto& gs = M->graphs;
+    size_t hit = gs.size();
+    for (size_t i = 0; i < gs.size(); ++i)
+      if (gs[i].same(B, a, mt)) hit = i;
+    if (hit == gs.size()) {
+      if (gs.size() >= kGraphCache) {
+        check(hipStreamSynchronize(bt->own), "sync");  // the oldest graph may still run
+        gs.front().destroy();
+        gs.erase(gs.begin());
+      }
+      gmrf_model::GaussGraph G;
+      bt->stream = bt->own;
+      check(hipStreamBeginCapture(bt->own, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        enqueue_gauss(M, a, B);
+      } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(bt->own, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        bt->stream = cs;
+        throw;
+      }
+      bt->stream = cs;
+      check(hipStreamEndCapture(bt->own, &G.graph), "hipStreamEndCapture");
+      const hipError_t ie = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        G.destroy();
+        check(ie, "hipGraphInstantiate");
+      }
+      const void* p[8] = {a->coefs, a->lpar, a->dcoefs, a->dlog_lpar, a->mlik, a->x, a->var, a->grad};
+      std::copy(p, p + 8, G.ptr);
+      G.B = B;
+      G.mt = mt;
+      gs.push_back(G);
+      hit = gs.size() - 1;
+    } else if (hit + 1 != gs.size()) {
+      std::rotate(gs.begin() + hit, gs.begin() + hit + 1, gs.end());
+      hit = gs.size() - 1;
+    }
+    check(hipEventRecord(M->gev_in, cs), "hipEventRecord");
+    check(hipStreamWaitEvent(bt->own, M->gev_in, 0), "hipStreamWaitEvent");
+    check(hipGraphLaunch(gs[hit].exec, bt->own), "hipGraphLaunch");
+    check(hipEventRecord(M->gev_out, bt->own), "hipEventRecord");
+    check(hipStreamWaitEvent(cs, M->gev_out, 0), "hipStreamWaitEvent");
+  }
+  int32_t* hs = reinterpret_cast<int32_t*>(M->pin.get((size_t)B * sizeof(int32_t)));
+  check(hipMemcpyAsync(hs, M->dstat, (size_t)B * sizeof(int32_t), hipMemcpyDeviceToHost, cs), "D2H status");
+  check(hipStreamSynchronize(cs), "sync");
+  for (int b = 0; b < B; ++b) {
+    if (a->status) a->status[b] = hs[b];
+    if (a->n_iter) a->n_iter[b] = 1;
+  }
 }
 
 }  // namespace
@@ -601,7 +791,8 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->dl = M->arena.alloc<double>(C * m);
     M->v = M->arena.alloc<double>(C * m);
     M->u = M->arena.alloc<double>(C * m);
-    M->red = M->arena.alloc<double>(4 * C);
+    M->red = M->arena.alloc<double>(8 * C);
+    M->dstat = M->arena.alloc<int32_t>(C);
     M->alpha = M->arena.alloc<double>(C);
     M->cres = M->arena.alloc<double>(C * std::max(M->kc, 1));
     M->bad = M->arena.alloc<int32_t>(C);
@@ -633,6 +824,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   const bool want_grad = a->dcoefs != nullptr && a->n_theta > 0 && a->grad != nullptr;
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
+    if (M->fam == 0 && M->link == 0 && M->kc == 0 && a->max_iter > 0) {
+      eval_gauss(M, a, B);
+      return GMRF_OK;
+    }
     const hipStream_t st = M->stream();
     const int64_t n = M->n, nnz = M->nnz, m = M->nobs;
     const int K = M->K;

@@ -131,12 +131,14 @@ class LaplaceModel:
             _free(h)
 
     def eval(self, coefs, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
-             max_iter=200, tol=1e-5):
+             max_iter=200, tol=1e-5, out=None):
         """Evaluate a batch of theta-points.
 
         coefs: (B, K); lpar: (B,) Gaussian precision multipliers; dcoefs: (B, m, K);
         dlog_lpar: (B, m). Returns a dict of device tensors (mlik, x, var, grad) plus
-        numpy status / n_iter.
+        numpy status / n_iter. ``out``: preallocated float64 CUDA tensors for some of
+        mlik (B,), x (B, n), var (B, n), grad (B, m); reusing the same tensors call after
+        call lets the library replay its captured graph of the Gaussian evaluation.
         """
         c = _dev(coefs)
         if c.dim() == 1:
@@ -153,13 +155,23 @@ class LaplaceModel:
             dc = dc.reshape(B, -1, self.K)
             m = dc.shape[1]
         dl = _dev(dlog_lpar, (B, m)) if dlog_lpar is not None else None
-        out = {"mlik": torch.empty(B, dtype=torch.float64, device="cuda")}
+        given = dict(out or {})
+        shapes = {"mlik": (B,), "x": (B, self.n), "var": (B, self.n), "grad": (B, m)}
+        for k, t in given.items():
+            if k not in shapes or not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                                       and t.is_contiguous() and tuple(t.shape) == shapes[k]):
+                raise InvalidArgument(f"out[{k!r}] must be a contiguous float64 CUDA tensor of shape {shapes[k]}")
+
+        def buf(k):
+            return given[k] if k in given else torch.empty(shapes[k], dtype=torch.float64, device="cuda")
+
+        out = {"mlik": buf("mlik")}
         if want_x:
-            out["x"] = torch.empty((B, self.n), dtype=torch.float64, device="cuda")
+            out["x"] = buf("x")
         if want_var:
-            out["var"] = torch.empty((B, self.n), dtype=torch.float64, device="cuda")
+            out["var"] = buf("var")
         if dc is not None:
-            out["grad"] = torch.empty((B, m), dtype=torch.float64, device="cuda")
+            out["grad"] = buf("grad")
         status = np.zeros(B, dtype=np.int32)
         niter = np.zeros(B, dtype=np.int32)
         a = EvalArgs()
