@@ -517,6 +517,7 @@ def main():
                     help="add the full-size Besag x AR1 batched evaluation and fit comparison")
     ap.add_argument("--c5", action="store_true",
                     help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default launch schedule")
     ap.add_argument("--cpu-fit", action="store_true",
                     help="add the reference's fit sequence measured on the oracle (host cores; minutes)")
     ap.add_argument("--nx", type=int, default=142)
@@ -531,6 +532,13 @@ def main():
     import numpy as np
     import torch
 
+    # the tuned schedule goes to a fresh file, so that the line can report it
+    if not args.no_autotune and "GMRF_TUNE_FILE" not in os.environ:
+        import tempfile
+
+        fd, tune_path = tempfile.mkstemp(prefix="gmrf_schedule_", suffix=".txt")
+        os.close(fd)
+        os.environ["GMRF_TUNE_FILE"] = tune_path
     from inla_gpu import _lib, ccd
     from inla_gpu import workloads as W
     from inla_gpu.laplace import LaplaceModel
@@ -576,6 +584,28 @@ def main():
     torch.cuda.synchronize()
     if args.warmup > 0:
         assert (out["status"] == 0).all(), out["status"]
+    tune = None
+    if not args.no_autotune:
+        # the launch schedule tuned on this batch (factorisation + selected inversion of
+        # the 2 x 15 prior and posterior matrices), before timing
+        step()
+        t_def, t_tuned = model.autotune()
+        tune = {"default_ms": t_def, "tuned_ms": t_tuned,
+                "work": "factorisation + selected inversion of the 30 prior and posterior matrices"}
+        try:
+            with open(os.environ["GMRF_TUNE_FILE"]) as fh:
+                lines = [ln.split() for ln in fh if ln.startswith("gmrf-schedule")]
+            if lines:
+                f = lines[-1]
+                tune["schedule"] = {"batch": int(f[5]), "fuse_below": int(f[6]), "split_max": int(f[7]),
+                                    "xcd_max": int(f[8]), "merge_factorisation": f[9], "merge_selinv": f[10]}
+            else:
+                tune["schedule"] = "defaults kept (no candidate was faster)"
+        except OSError:
+            pass
+        log(f"== autotune: {t_def:.3f} ms -> {t_tuned:.3f} ms")
+        out = step()
+        torch.cuda.synchronize()
     log(f"== timed {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -657,6 +687,8 @@ def main():
                    "mlik_mode": float(mlik[0]), "grad_mode_norm": float(np.linalg.norm(grad[0]))},
         "roofline": roof,
     }
+    if tune is not None:
+        res["autotune"] = tune
     if not args.no_fit:
         log("== plain per-point GPU path")
         res["plain_gpu"] = plain_gpu(model, wl, thetas)

@@ -73,6 +73,7 @@ typedef struct gmrf_symbolic_stats {
   double flops_selinv;   /* sum_j 2 r_j (r_j + 1) + n */
   double flops_solve;    /* 4 sum_j r_j + n */
   double sum_r2;
+  uint64_t pattern_hash; /* hash of (n, indptr, indices): the key of tuned schedules */
 } gmrf_symbolic_stats;
 
 /* Thread-local message of the last failure on this thread ("" if none). */
@@ -140,6 +141,19 @@ gmrf_status gmrf_batch_solve_multi(gmrf_batch* bt, int32_t count, int32_t nrhs, 
  * race; calls ordered on one stream do not. */
 gmrf_status gmrf_batch_solve_noalloc(gmrf_batch* bt, int32_t count, int32_t nrhs, const double* rhs,
                                      int64_t ld_rhs, double* x, int64_t ld_x);
+/* Autotuner (no reference counterpart). Times factorisation + selected inversion of
+ * `count` matrices of this pattern (device values as for gmrf_batch_factorize) under
+ * candidate launch schedules and keeps the fastest for the pattern and batch size:
+ * small-front size classes run per level separately or merged into one launch
+ * (chosen level by level, for factorisation and selected inversion separately), the
+ * workgroup threshold below which a trailing-update launch also factors the next
+ * diagonal block, the split of Sigma[B,P] over K, and XCD grouping of its tiles. The
+ * tuned schedule is kept only if it beats the defaults; it is also appended to the
+ * file named by the GMRF_TUNE_FILE environment variable (if set), from which later
+ * processes load it by pattern hash. default_ms / tuned_ms (host, may be NULL) receive
+ * the measured times. The batch's factors are overwritten: factorise again after. */
+gmrf_status gmrf_batch_autotune(gmrf_batch* bt, int32_t count, const double* values, int64_t ld_values,
+                                double* default_ms, double* tuned_ms);
 /* Selected inversion on the supernodal pattern of L (Takahashi recurrence). */
 gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count);
 /* diag(Q_b^-1) in original order after gmrf_batch_selinv (sparse_ldlt.rs:421-431). */
@@ -300,6 +314,11 @@ gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t
                             const double* eta, const double* y, const double* obs_a,
                             const double* obs_b, const double* lpar, double* logp, void* stream);
 gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_given);
+/* gmrf_batch_autotune for the model's batch (the prior and posterior matrices of the
+ * last gmrf_model_eval, 2 count matrices) on that evaluation's values; call after an
+ * evaluation. Later evaluations of the same size use the tuned schedule (the model's
+ * captured graphs are rebuilt). */
+gmrf_status gmrf_model_autotune(gmrf_model* model, double* default_ms, double* tuned_ms);
 
 /* ---- diagnostics -------------------------------------------------------------
  * Per-launch timing of the level kernels with HIP events on the launching

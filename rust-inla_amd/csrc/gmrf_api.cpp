@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -97,6 +101,182 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
 }
 
 
+// Tuned schedules persist as text lines in the file named by GMRF_TUNE_FILE (if set):
+//   gmrf-schedule 1 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
+//   <xcd_max> <factorisation merge masks> <selected-inversion merge masks>
+// one hex digit per level for each mask string; a later line for the same key wins.
+void SymData::load_schedules() {
+  const char* path = std::getenv("GMRF_TUNE_FILE");
+  if (!path) return;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    std::string tag, hash, mf, ms;
+    int ver = 0, count = 0;
+    int64_t n = 0, nl = 0;
+    Schedule sc;
+    if (!(ss >> tag >> ver >> hash >> n >> nl >> count >> sc.fuse_below >> sc.split_max >> sc.xcd_max >> mf >> ms))
+      continue;
+    if (tag != "gmrf-schedule" || ver != 1 || n != S.n || nl != (int64_t)S.nlevels || count < 1) continue;
+    if (std::strtoull(hash.c_str(), nullptr, 16) != S.pattern_hash) continue;
+    if ((int64_t)mf.size() != nl || (int64_t)ms.size() != nl) continue;
+    for (int64_t l = 0; l < nl; ++l) {
+      sc.merge_fac.push_back((uint8_t)(std::strtol(std::string(1, mf[l]).c_str(), nullptr, 16) & 7));
+      sc.merge_sel.push_back((uint8_t)(std::strtol(std::string(1, ms[l]).c_str(), nullptr, 16) & 7));
+    }
+    sched[count] = sc;
+  }
+}
+
+namespace {
+
+void save_schedule(const SymData& d, int count, const Schedule& sc) {
+  const char* path = std::getenv("GMRF_TUNE_FILE");
+  if (!path) return;
+  std::FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::string mf, ms;
+  for (int l = 0; l < d.S.nlevels; ++l) {
+    mf += "01234567"[sc.fac_mask(l) & 7];
+    ms += "01234567"[sc.sel_mask(l) & 7];
+  }
+  std::fprintf(f, "gmrf-schedule 1 %016llx %lld %d %d %lld %lld %d %s %s\n", (unsigned long long)d.S.pattern_hash,
+               (long long)d.S.n, d.S.nlevels, count, (long long)sc.fuse_below, (long long)sc.split_max, sc.xcd_max,
+               mf.c_str(), ms.c_str());
+  std::fclose(f);
+}
+
+// Time (ms, best of `reps`) of factorisation + selected inversion of `count`
+// matrices under schedule sc; with `marks`, the per-level small-front times of the
+// last repetition go to fac_ms / sel_ms.
+double time_schedule(gmrf_batch* bt, int count, const double* vals, int64_t ldv, const Schedule& sc, int reps,
+                     gmrf_batch::TuneMarks* marks, std::vector<double>* fac_ms, std::vector<double>* sel_ms) {
+  SymData& sd = *bt->sym;
+  sd.set_schedule(count, sc);
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "hipEventCreate");
+  check(hipEventCreate(&e1), "hipEventCreate");
+  double best = 1e300;
+  try {
+    for (int r = 0; r < reps; ++r) {
+      bt->marks = (marks && r == reps - 1) ? marks : nullptr;
+      check(hipEventRecord(e0, bt->stream), "hipEventRecord");
+      bt->factorize(count, vals, ldv);
+      bt->selinv(count);
+      check(hipEventRecord(e1, bt->stream), "hipEventRecord");
+      check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      best = std::min(best, (double)ms);
+    }
+    bt->marks = nullptr;
+    if (marks) {
+      const size_t nl = sd.levels.size();
+      fac_ms->assign(nl, 0.0);
+      sel_ms->assign(nl, 0.0);
+      for (size_t l = 0; l < nl; ++l) {
+        float a = 0.0f, b = 0.0f;
+        check(hipEventElapsedTime(&a, marks->fac_a[l], marks->fac_b[l]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&b, marks->sel_a[l], marks->sel_b[l]), "hipEventElapsedTime");
+        (*fac_ms)[l] = a;
+        (*sel_ms)[l] = b;
+      }
+    }
+  } catch (...) {
+    bt->marks = nullptr;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+}  // namespace
+
+// The autotuner behind gmrf_batch_autotune (and gmrf_model_autotune).
+gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_t ldv, double* default_ms,
+                           double* tuned_ms) {
+  try {
+    check(hipSetDevice(bt->sym->device), "hipSetDevice");
+    bt->selinv_join();
+    SymData& sd = *bt->sym;
+    const size_t nl = sd.levels.size();
+    const int reps = 3;
+    const Schedule def;
+    const Schedule prev = sd.schedule(count);
+    const double t_def = time_schedule(bt, count, vals, ldv, def, reps, nullptr, nullptr, nullptr);
+    // 1. small-front class grouping, level by level: every level under each of the 8
+    //    masks, the per-level times from events around its small-front launches
+    gmrf_batch::TuneMarks marks;
+    for (auto* v : {&marks.fac_a, &marks.fac_b, &marks.sel_a, &marks.sel_b}) {
+      v->resize(nl);
+      for (auto& e : *v) check(hipEventCreate(&e), "hipEventCreate");
+    }
+    Schedule cur = def;
+    std::vector<double> bf(nl, 1e300), bs(nl, 1e300), fm, sm;
+    std::vector<uint8_t> mf(nl, 0), ms(nl, 0);
+    try {
+      for (int mask = 0; mask < 8; ++mask) {
+        Schedule s = cur;
+        s.merge_fac.assign(nl, (uint8_t)mask);
+        s.merge_sel.assign(nl, (uint8_t)mask);
+        time_schedule(bt, count, vals, ldv, s, 2, &marks, &fm, &sm);
+        for (size_t l = 0; l < nl; ++l) {
+          if (fm[l] < bf[l] * 0.97) bf[l] = fm[l], mf[l] = (uint8_t)mask;  // a clear gain, else the simpler mask
+          if (sm[l] < bs[l] * 0.97) bs[l] = sm[l], ms[l] = (uint8_t)mask;
+        }
+      }
+    } catch (...) {
+      for (auto* v : {&marks.fac_a, &marks.fac_b, &marks.sel_a, &marks.sel_b})
+        for (auto& e : *v) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto* v : {&marks.fac_a, &marks.fac_b, &marks.sel_a, &marks.sel_b})
+      for (auto& e : *v) (void)hipEventDestroy(e);
+    cur.merge_fac = mf;
+    cur.merge_sel = ms;
+    double t_cur = time_schedule(bt, count, vals, ldv, cur, reps, nullptr, nullptr, nullptr);
+    // 2. the launch thresholds, one at a time
+    for (int64_t v : {(int64_t)0, (int64_t)512, (int64_t)kFuseDiagBelow, (int64_t)8192, (int64_t)1 << 40}) {
+      Schedule s = cur;
+      s.fuse_below = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    for (int64_t v : {(int64_t)0, (int64_t)32, (int64_t)kGemmSplitMaxTasks, (int64_t)256, (int64_t)1024}) {
+      Schedule s = cur;
+      s.split_max = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    for (int32_t v : {0, 64, kXcdMaxTasks, 1 << 30}) {
+      Schedule s = cur;
+      s.xcd_max = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    // keep the tuned schedule only if it beats the defaults on a fresh measurement
+    const double t_def2 = time_schedule(bt, count, vals, ldv, def, reps, nullptr, nullptr, nullptr);
+    const double t_best = time_schedule(bt, count, vals, ldv, cur, reps, nullptr, nullptr, nullptr);
+    const double t_ref = std::min(t_def, t_def2);
+    if (t_best < t_ref) {
+      sd.set_schedule(count, cur);
+      save_schedule(sd, count, cur);
+    } else {
+      sd.set_schedule(count, def);
+    }
+    (void)prev;
+    if (default_ms) *default_ms = t_ref;
+    if (tuned_ms) *tuned_ms = std::min(t_best, t_ref);
+    return GMRF_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 extern "C" {
 
 const char* gmrf_last_error(void) { return g_err.c_str(); }
@@ -144,6 +324,7 @@ gmrf_status gmrf_symbolic_get_stats(const gmrf_symbolic* sym, gmrf_symbolic_stat
   o->flops_selinv = S.flops_selinv;
   o->flops_solve = S.flops_solve;
   o->sum_r2 = S.sum_r2;
+  o->pattern_hash = S.pattern_hash;
   return GMRF_OK;
 }
 
@@ -306,6 +487,14 @@ gmrf_status gmrf_batch_solve_noalloc(gmrf_batch* bt, int32_t count, int32_t nrhs
   } catch (...) {
     return from_exception();
   }
+}
+
+gmrf_status gmrf_batch_autotune(gmrf_batch* bt, int32_t count, const double* values, int64_t ld_values,
+                                double* default_ms, double* tuned_ms) {
+  if (!bt || !values) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (count < 1 || count > bt->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds batch capacity");
+  if (ld_values < bt->sym->S.nnz_full) return fail(GMRF_INVALID_ARGUMENT, "ld_values < nnz");
+  return autotune_batch(bt, count, values, ld_values, default_ms, tuned_ms);
 }
 
 gmrf_status gmrf_batch_selinv(gmrf_batch* bt, int32_t count) {

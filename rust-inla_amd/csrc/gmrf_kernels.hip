@@ -2378,11 +2378,11 @@ hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int
   return hipGetLastError();
 }
 
-hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B, int xcd_max,
                           double* S, const double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   // XCD-grouped when every XCD gets a matrix and a matrix's tiles fit one XCD
-  const int xcd = (B >= 8 && count <= kXcdMaxTasks) ? 1 : 0;
+  const int xcd = (B >= 8 && count <= xcd_max) ? 1 : 0;
   const int64_t grid = xcd ? (int64_t)8 * ((B + 7) / 8) * count : (int64_t)count * B;
   hipLaunchKernelGGL(k_sl_gemm, dim3((unsigned)grid), dim3(512), 0, st, P, tasks, count, B, xcd, step, S, W,
                      T);

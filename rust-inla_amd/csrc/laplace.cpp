@@ -171,6 +171,7 @@ struct gmrf_model {
     }
   };
   std::vector<GaussGraph> graphs;  // most recently used last
+  int last_count = 0;              // points of the last evaluation (0: none yet)
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
   hipStream_t stream() const { return bt->stream; }
   ~gmrf_model() {
@@ -824,6 +825,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   const bool want_grad = a->dcoefs != nullptr && a->n_theta > 0 && a->grad != nullptr;
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
+    M->last_count = B;
     if (M->fam == 0 && M->link == 0 && M->kc == 0 && a->max_iter > 0) {
       eval_gauss(M, a, B);
       return GMRF_OK;
@@ -1178,6 +1180,20 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   } catch (...) {
     return from_exception();
   }
+}
+
+gmrf_status gmrf_model_autotune(gmrf_model* M, double* default_ms, double* tuned_ms) {
+  if (!M) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (M->last_count < 1) return fail(GMRF_INVALID_ARGUMENT, "autotune needs an evaluation first");
+  try {
+    check(hipSetDevice(M->sym->device), "hipSetDevice");
+    check(hipStreamSynchronize(M->bt->own), "sync");
+    for (auto& g : M->graphs) g.destroy();
+    M->graphs.clear();
+  } catch (...) {
+    return from_exception();
+  }
+  return autotune_batch(M->bt.get(), 2 * M->last_count, M->vals, M->nnz, default_ms, tuned_ms);
 }
 
 gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,

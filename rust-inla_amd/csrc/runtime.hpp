@@ -34,7 +34,7 @@ hipError_t launch_sl_gather_lhat(const DevPlan&, const int32_t*, int, const int3
                                 hipStream_t);
 hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const double*,
                           const double*, double*, hipStream_t);
-hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
+hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, int, double*, const double*,
                           const double*, hipStream_t);
 hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int,
                              const double*, const double*, double*, double*, hipStream_t);
@@ -342,6 +342,40 @@ inline Pools plan_pools(const Symbolic& S) {
 
 using namespace gmrf_rt;
 
+// Launch schedule of a pattern for one batch size (matrices per launch): the
+// autotuner's choice (gmrf_batch_autotune), the defaults otherwise.
+//  - merge_fac / merge_sel, per level: bit c set runs small-front class c in one
+//    launch with class c + 1 (the larger class's kernel, which handles any smaller
+//    front): fewer launches for more padding;
+//  - fuse_below: launches of fewer workgroups also factor the next diagonal block;
+//  - split_max: Sigma[B,P] launches of at most this many row tiles split K;
+//  - xcd_max: Sigma[B,P] launches with at most this many tasks per matrix keep a
+//    matrix's tiles on one XCD.
+struct Schedule {
+  std::vector<uint8_t> merge_fac, merge_sel;
+  int64_t fuse_below = kFuseDiagBelow;
+  int64_t split_max = kGemmSplitMaxTasks;
+  int32_t xcd_max = kXcdMaxTasks;
+  uint8_t fac_mask(size_t l) const { return l < merge_fac.size() ? merge_fac[l] : 0; }
+  uint8_t sel_mask(size_t l) const { return l < merge_sel.size() ? merge_sel[l] : 0; }
+};
+
+// Small-front launches of one level under a class-merge mask: f(c, e, hi) for each
+// group of classes [c, e] holding fronts, hi the largest class with fronts.
+template <typename F>
+inline void small_groups(const int64_t* cls_b, uint8_t mask, F f) {
+  int c = 0;
+  while (c < kSmallClasses) {
+    int e = c;
+    while (e + 1 < kSmallClasses && ((mask >> e) & 1)) ++e;
+    int hi = -1;
+    for (int q = c; q <= e; ++q)
+      if (cls_b[q + 1] > cls_b[q]) hi = q;
+    if (hi >= 0) f(c, e, hi);
+    c = e + 1;
+  }
+}
+
 // Symbolic analysis plus its device plan; shared by batches and solvers.
 struct SymData {
   Symbolic S;
@@ -359,6 +393,27 @@ struct SymData {
   std::vector<int64_t> given;  // given permutation (cache key)
   std::mutex dev_mu;
   bool dev_built = false;
+  std::mutex sched_mu;
+  std::map<int, Schedule> sched;  // by batch size
+  bool sched_loaded = false;
+
+  // The schedule for `count` matrices per launch (tuned, loaded from the tuning
+  // file on first use, or the defaults).
+  Schedule schedule(int count) {
+    std::lock_guard<std::mutex> lk(sched_mu);
+    if (!sched_loaded) {
+      sched_loaded = true;
+      load_schedules();
+    }
+    auto it = sched.find(count);
+    return it != sched.end() ? it->second : Schedule{};
+  }
+  void set_schedule(int count, const Schedule& sc) {
+    std::lock_guard<std::mutex> lk(sched_mu);
+    sched_loaded = true;
+    sched[count] = sc;
+  }
+  void load_schedules();  // gmrf_api.cpp
 
   // Upload the plan on first use (symbolic analysis itself needs no device).
   void ensure_device() {
@@ -815,6 +870,11 @@ struct gmrf_batch {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* TS = nullptr;
   bool sel_inflight = false;
+  // autotuner: events around each level's small-front launches (a[l] before, b[l]
+  // after; factorisation in fac, selected inversion in sel), recorded when set
+  struct TuneMarks {
+    std::vector<hipEvent_t> fac_a, fac_b, sel_a, sel_b;
+  }* marks = nullptr;
 
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
@@ -896,18 +956,24 @@ struct gmrf_batch {
     double* Lf = L + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
     double* Zf = Z + (int64_t)first * P.z_size;
-    for (const LevelInfo& lv : sym->levels) {
-      for (int c = 0; c < kSmallClasses; ++c)
-        if (lv.cls_b[c + 1] > lv.cls_b[c])
-          prof_launch("k_factor_small", lv.fl_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
-            return launch_factor_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
-                                       kClassM[c], count, vals, ldv, Lf, Uf, stream);
-          });
+    const Schedule sc = sym->schedule(count);
+    for (size_t li = 0; li < sym->levels.size(); ++li) {
+      const LevelInfo& lv = sym->levels[li];
+      if (marks) check(hipEventRecord(marks->fac_a[li], stream), "hipEventRecord");
+      small_groups(lv.cls_b, sc.fac_mask(li), [&](int c, int e, int hi) {
+        double fl = 0.0, by = 0.0;
+        for (int q = c; q <= e; ++q) fl += lv.fl_cls[q], by += lv.by_cls[q];
+        prof_launch("k_factor_small", fl * count, by * count, stream, [&] {
+          return launch_factor_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[e + 1] - lv.cls_b[c]),
+                                     kClassM[hi], count, vals, ldv, Lf, Uf, stream);
+        });
+      });
+      if (marks) check(hipEventRecord(marks->fac_b[li], stream), "hipEventRecord");
       if (lv.le > lv.lb) {
         const int32_t* tk = sym->d_tasks;
         // launches of few tiles also factor the next diagonal blocks (one launch
         // less per panel step); large launches keep their occupancy
-        auto fuse = [&](int64_t tiles) { return tiles * count < kFuseDiagBelow; };
+        auto fuse = [&](int64_t tiles) { return tiles * count < sc.fuse_below; };
         bool diag_done = fuse(lv.asm_t.cnt);
         prof_launch("k_fl_assemble", diag_done && !lv.fdiag.empty() ? lv.fdiag[0].flops * count : 0.0,
                     (lv.asm_t.bytes + (diag_done && !lv.fdiag.empty() ? lv.fdiag[0].bytes : 0.0)) * count, stream,
@@ -1048,6 +1114,7 @@ struct gmrf_batch {
     double* Wf = W + (int64_t)first * P.w_size;
     const double* Zf = Z + (int64_t)first * P.z_size;
     const int32_t* tk = sym->d_tasks;
+    const Schedule sc = sym->schedule(count);
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
@@ -1064,7 +1131,7 @@ struct gmrf_batch {
           const bool next = k + 1 < lv.lhat.size() && lv.lhat[k + 1].cnt > 0;
           // few row blocks with a long K (the matrices of tall fronts, a small batch):
           // split K so that the product spreads over the device
-          if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= kGemmSplitMaxTasks) {
+          if (lv.sgpart[k].cnt > 0 && lv.sgemm[k].cnt * (int64_t)count <= sc.split_max) {
             prof_launch("k_sl_gemm_part", lv.sgpart[k].flops * count, lv.sgpart[k].bytes * count, stream, [&] {
               return launch_sl_gemm_part(P, tk + 3 * lv.sgpart[k].off, (int)lv.sgpart[k].cnt, (int)k, count,
                                          Sf, Wf, Tf, stream);
@@ -1075,7 +1142,7 @@ struct gmrf_batch {
             });
           } else {
             prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, lv.sgemm[k].bytes * count, stream, [&] {
-              return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count,
+              return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count, sc.xcd_max,
                                     Sf, Wf, Tf, stream);
             });
           }
@@ -1104,12 +1171,17 @@ struct gmrf_batch {
           }
         }
       }
-      for (int c = 0; c < kSmallClasses; ++c)
-        if (lv.cls_b[c + 1] > lv.cls_b[c])
-          prof_launch("k_selinv_small", lv.sel_cls[c] * count, lv.by_cls[c] * count, stream, [&] {
-            return launch_selinv_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[c + 1] - lv.cls_b[c]),
-                                       kClassM[c], lv.maxw_cls[c], count, Lf, Sf, Uf, Wf, stream);
-          });
+      if (marks) check(hipEventRecord(marks->sel_a[l], stream), "hipEventRecord");
+      small_groups(lv.cls_b, sc.sel_mask(l), [&](int c, int e, int hi) {
+        double fl = 0.0, by = 0.0;
+        int maxw = 0;
+        for (int q = c; q <= e; ++q) fl += lv.sel_cls[q], by += lv.by_cls[q], maxw = std::max(maxw, lv.maxw_cls[q]);
+        prof_launch("k_selinv_small", fl * count, by * count, stream, [&] {
+          return launch_selinv_small(P, sym->d_small + lv.cls_b[c], (int)(lv.cls_b[e + 1] - lv.cls_b[c]),
+                                     kClassM[hi], maxw, count, Lf, Sf, Uf, Wf, stream);
+        });
+      });
+      if (marks) check(hipEventRecord(marks->sel_b[l], stream), "hipEventRecord");
     }
   }
 };
@@ -1146,6 +1218,11 @@ struct gmrf_solver {
 
 std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                         int order, const int64_t* given);
+// Autotuner (gmrf_api.cpp): times candidate schedules of factorisation + selected
+// inversion of `count` matrices on the given values and keeps the fastest.
+gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_t ldv, double* default_ms,
+                           double* tuned_ms);
+
 // The same analysis kept out of the process-wide cache (dense inputs: the solver
 // holds the full pattern's analysis itself, keyed by n).
 std::shared_ptr<SymData> analyze_uncached(int64_t n, const int64_t* indptr, const int64_t* indices,

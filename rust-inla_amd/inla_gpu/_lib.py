@@ -50,6 +50,7 @@ class SymbolicStats(C.Structure):
         ("flops_selinv", _dbl),
         ("flops_solve", _dbl),
         ("sum_r2", _dbl),
+        ("pattern_hash", C.c_uint64),
     ]
 
 
@@ -118,6 +119,7 @@ SIGNATURES = {
     "gmrf_batch_solve": (_i32, [_vp, _i32, _vp, _i64, _vp, _i64]),
     "gmrf_batch_solve_multi": (_i32, [_vp, _i32, _i32, _vp, _i64, _vp, _i64]),
     "gmrf_batch_solve_noalloc": (_i32, [_vp, _i32, _i32, _vp, _i64, _vp, _i64]),
+    "gmrf_batch_autotune": (_i32, [_vp, _i32, _vp, _i64, _pdbl, _pdbl]),
     "gmrf_batch_selinv": (_i32, [_vp, _i32]),
     "gmrf_batch_selinv_diag": (_i32, [_vp, _i32, _vp, _i64]),
     "gmrf_batch_selinv_pattern": (_i32, [_vp, _i32, _vp, _i64]),
@@ -146,6 +148,7 @@ SIGNATURES = {
     "gmrf_model_stream": (_vp, [_vp]),
     "gmrf_obs_loglik": (_i32, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gmrf_model_set_stream": (_i32, [_vp, _vp, _i32]),
+    "gmrf_model_autotune": (_i32, [_vp, _pdbl, _pdbl]),
     "gmrf_profile_enable": (_i32, [_i32]),
     "gmrf_profile_summary": (_i32, [_i32, C.c_char_p, _i32, _pi64, _pdbl, _pdbl, _pdbl, _dbl, _dbl, _pdbl,
                                     _pi32]),

@@ -194,6 +194,15 @@ class LaplaceModel:
         return out
 
 
+    def autotune(self):
+        """Tune the launch schedule on the last evaluation's matrices (see
+        gmrf_model_autotune); returns (default_ms, tuned_ms) of factorisation +
+        selected inversion."""
+        d, t = C.c_double(), C.c_double()
+        check(L.lib.gmrf_model_autotune(self._h, C.byref(d), C.byref(t)))
+        return d.value, t.value
+
+
 class LogMarginal(torch.autograd.Function):
     """mlik(theta) for a batch of theta-points with the adjoint gradient.
 

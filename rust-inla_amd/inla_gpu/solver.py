@@ -239,6 +239,16 @@ class Batch:
                                            C.c_void_p(out.data_ptr()), self.n))
         return out
 
+    def autotune(self, values):
+        """Tune the launch schedule for these matrices (gmrf_batch_autotune); the
+        factors are overwritten. Returns (default_ms, tuned_ms)."""
+        v = self._dev(values, self.nnz, "values")
+        self._stream()
+        d, t = C.c_double(), C.c_double()
+        check(L.lib.gmrf_batch_autotune(self._h, v.shape[0], C.c_void_p(v.data_ptr()), self.nnz, C.byref(d),
+                                        C.byref(t)))
+        return d.value, t.value
+
     def selinv(self, count=None):
         count = self.count if count is None else count
         self._stream()
