@@ -331,6 +331,105 @@ def hetero_compare(k=64, reps=3):
             "work": "LDL^T + selected-inverse diagonal + one solve per model"}
 
 
+def hetero_laplace_line(k=64, reps=3, cpu_sample=True):
+    """The heterogeneous Laplace batch: k BYM2 + Poisson models with an intercept each
+    (Delaunay-neighbour graphs, sizes uniform in [500, 20000], phi = 0.5, seeded), 9 CCD
+    theta-points per model (m = 2, theta = 0.5 z around (log tau, logit phi) = (0, 0)):
+    Newton, mlik, marginal variances and the adjoint gradient of every (point, model)
+    pair as one block-diagonal model, against the k models evaluated one after another
+    on the same GPU, and (cpu_sample) the oracle on the models' centre points across the
+    host cores (node work: Newton, mlik, Takahashi diagonal; no gradient)."""
+    import numpy as np
+    import torch
+
+    from inla_gpu import ccd
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    t0 = time.perf_counter()
+    wl = W.bym2_poisson_batch(k=k)
+    t_build = time.perf_counter() - t0
+    log(f"   workload: {k} models, n = {wl.n}, {t_build:.1f} s")
+    z, _ = ccd.ccd_design(2, 1.0)
+    B = len(z)
+    thetas = np.broadcast_to((0.5 * z)[:, None, :], (B, k, 2)).copy()
+    c, dc, _, _ = wl.params(thetas)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    union = LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A,
+                         blocks=wl.block_ptr, capacity=B)
+    t_setup = time.perf_counter() - t0
+    cd, dcd = dev(c), dev(dc)
+
+    def run_union():
+        return union.eval(cd, dcoefs=dcd)
+
+    bp = wl.block_ptr
+    singles = []
+    for q in range(k):
+        e0, e1 = wl.indptr[bp[q]], wl.indptr[bp[q + 1]]
+        singles.append(LaplaceModel(bp[q + 1] - bp[q], wl.indptr[bp[q]:bp[q + 1] + 1] - e0,
+                                    wl.indices[e0:e1] - bp[q], wl.components[:, e0:e1], wl.block_y(q),
+                                    family="poisson", A=wl.block_A(q), capacity=B))
+    cq = [dev(c[:, q]) for q in range(k)]
+    dcq = [dev(dc[:, q]) for q in range(k)]
+
+    def run_single():
+        return [singles[q].eval(cq[q], dcoefs=dcq[q]) for q in range(k)]
+
+    def timed(fn):
+        out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
+    tu, ru = timed(run_union)
+    log(f"   union: {tu * 1e3:.1f} ms per evaluation of {B} x {k} (point, model) pairs")
+    ts, rs = timed(run_single)
+    log(f"   separate models: {ts * 1e3:.1f} ms")
+    ml_u = ru["mlik"].cpu().numpy()
+    ml_s = np.stack([r["mlik"].cpu().numpy() for r in rs], axis=1)
+    npts = B * k
+    res = {"value": npts / tu, "unit": "theta-points/s", "ms_per_step": tu * 1e3,
+           "theta_points": npts, "models": k, "points_per_model": B, "n_total": int(wl.n),
+           "nnz_q_total": int(len(wl.indices)), "sizes_min_max": [int(min(m.n for m in wl.models)),
+                                                                   int(max(m.n for m in wl.models))],
+           "newton_iters_min_max": [int(ru["n_iter"].min()), int(ru["n_iter"].max())],
+           "status_ok": bool((ru["status"] == 0).all()),
+           "separate_models": {"value": npts / ts, "ms_per_step": ts * 1e3},
+           "speedup_vs_separate": ts / tu,
+           "max_abs_mlik_diff_union_vs_separate": float(np.max(np.abs(ml_u - ml_s))),
+           "setup_s": {"workload": t_build, "model": t_setup},
+           "work": "per (theta-point, model): Newton to the mode, mlik, marginal variances, adjoint gradient",
+           "workload": f"{k} BYM2 + Poisson models (intercept each), Delaunay graphs, sizes U[500, 20000], "
+                       f"CCD m=2 ({B} theta-points per model)"}
+    if cpu_sample:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as orc
+
+        cores = _cpu_cores()
+
+        def node(q):
+            r = orc.find_mode(wl.block_prior(q, thetas[0, q]), "poisson", "log", wl.block_y(q), 1.0,
+                              A=wl.block_A(q), keep_post=True)
+            r.post_factor().takahashi_diag()
+            return r.mlik
+
+        with ThreadPoolExecutor(cores) as ex:
+            t0 = time.perf_counter()
+            ml_o = np.array(list(ex.map(node, range(k))))
+            t_cpu = time.perf_counter() - t0
+        res["cpu_sample"] = {"value": k / t_cpu, "unit": "theta-points/s", "cores": cores, "kind": "port",
+                             "sample": f"the {k} models' centre theta-points on the oracle, a thread per model "
+                                       f"over {cores} host cores, node work only (no gradient): {t_cpu:.1f} s",
+                             "max_abs_mlik_diff_vs_gpu": float(np.max(np.abs(ml_o - ml_u[0])))}
+    return res
+
+
 def big_solver_line(Q, label, probes=6):
     """Full-size solver line at batch 1: factorise, solve and selected-inverse diagonal
     of one posterior precision, with size-independent checks: log|2Q| - log|Q| = n log 2,
@@ -703,6 +802,9 @@ def main():
         log("== heterogeneous batch: 64 BYM2 models")
         res["hetero"] = hetero_compare()
         log(json.dumps(res["hetero"]))
+        log("== heterogeneous Laplace: 64 BYM2 + Poisson models x 9 CCD points")
+        res["hetero_laplace"] = hetero_laplace_line(cpu_sample=not args.no_cpu_baseline)
+        log(json.dumps(res["hetero_laplace"]))
     if args.c4:
         log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
         res["c4"] = c4_solver_line()

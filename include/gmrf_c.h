@@ -269,6 +269,22 @@ typedef struct gmrf_model_desc {
                                 precision, so long constraint rows do not densify the pattern
                                 (no reference counterpart; k <= 32) */
   int32_t order;             /* GMRF_ORDER_CHOOSER (the reference's choice) or GMRF_ORDER_AMD */
+  int32_t n_blocks;          /* 0 or 1: one model. nb > 1: nb independent models (a heterogeneous
+                                batch, e.g. many areal models of uneven size) as one
+                                block-diagonal union: block q owns the latent indices
+                                [block_ptr[q], block_ptr[q + 1]), the pattern entries of those
+                                columns (the pattern must be block diagonal) and the
+                                observations that load them (each observation loads one block
+                                and every block has observations). Each block is ordered on its
+                                own pattern; the factorisations, solves and selected inversions
+                                run on the union in one launch sequence, while the Newton steps,
+                                convergence, status, mlik and gradient are per (point, block):
+                                in gmrf_eval_args every per-point quantity (coefs, lpar, dcoefs,
+                                dlog_lpar, mlik, grad, status, n_iter) gains a block index,
+                                [point][block][...]; x and var stay count x n. No constraints.
+                                No reference counterpart: the reference fits such models one
+                                at a time (crates/inla_stats/src/inference.rs:776-975 each) */
+  const int64_t* block_ptr;  /* nb + 1 latent offsets (host), block_ptr[0] = 0, [nb] = n */
 } gmrf_model_desc;
 
 typedef struct gmrf_eval_args {

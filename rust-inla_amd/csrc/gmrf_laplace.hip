@@ -213,33 +213,37 @@ __global__ void k_lik(int B, int64_t nobs, int fam, int link, const double* __re
                       const double* __restrict__ pb, const double* __restrict__ lpar,
                       double* __restrict__ logp, double* __restrict__ grad, double* __restrict__ w,
                       double* __restrict__ dw, double* __restrict__ dlpar, int32_t* __restrict__ bad,
-                      int32_t* __restrict__ invalid) {
+                      int32_t* __restrict__ invalid, int nb, const int32_t* __restrict__ oblk) {
   const int64_t tot = (int64_t)B * nobs;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / nobs);
     const int64_t i = e - (int64_t)b * nobs;
-    const LikOut o = eval_obs(fam, link, eta[e], y[i], pa[i], pb ? pb[i] : 0.0, lpar ? lpar[b] : 1.0);
+    const int vp = b * nb + (oblk ? oblk[i] : 0);  // (point, block)
+    const LikOut o = eval_obs(fam, link, eta[e], y[i], pa[i], pb ? pb[i] : 0.0, lpar ? lpar[vp] : 1.0);
     if (logp) logp[e] = o.logp;
     if (grad) grad[e] = o.grad;
     if (w) w[e] = o.w;
     if (dw) dw[e] = o.dw;
     if (dlpar) dlpar[e] = o.dl_par;
-    if (!o.ok && bad) bad[b] = 1;  // benign race: every writer stores 1
-    if (o.inv && invalid) invalid[b] = 1;
+    if (!o.ok && bad) bad[vp] = 1;  // benign race: every writer stores 1
+    if (o.inv && invalid) invalid[vp] = 1;
   }
 }
 
 // vals[b][e] = sum_k coefs[b][k] comps[k][e]
+// (with blocks: the coefficients of the entry's block, coefs[(b nb + eblk[e]) K + k])
 __global__ void k_affine(int B, int K, int64_t nnz, const double* __restrict__ coefs,
-                         const double* __restrict__ comps, double* __restrict__ out) {
+                         const double* __restrict__ comps, double* __restrict__ out, int nb,
+                         const int32_t* __restrict__ eblk) {
   const int64_t tot = (int64_t)B * nnz;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / nnz);
     const int64_t q = e - (int64_t)b * nnz;
+    const double* cb = coefs + (int64_t)(b * nb + (eblk ? eblk[q] : 0)) * K;
     double s = 0.0;
-    for (int k = 0; k < K; ++k) s += coefs[b * K + k] * comps[(int64_t)k * nnz + q];
+    for (int k = 0; k < K; ++k) s += cb[k] * comps[(int64_t)k * nnz + q];
     out[e] = s;
   }
 }
@@ -330,14 +334,16 @@ __global__ __launch_bounds__(256) void k_gather_spmv_long(const int64_t* __restr
 }
 
 // out[b][i] = a[b][i] + alpha[b] * c[b][i]
+// (with blocks: alpha of the element's block, alpha[b nb + blk[i]])
 __global__ void k_axpy_b(int B, int64_t len, const double* __restrict__ a,
                          const double* __restrict__ alpha, const double* __restrict__ c,
-                         double* __restrict__ out) {
+                         double* __restrict__ out, int nb, const int32_t* __restrict__ blk) {
   const int64_t tot = (int64_t)B * len;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / len);
-    out[e] = a[e] + alpha[b] * c[e];
+    const int64_t i = e - (int64_t)b * len;
+    out[e] = a[e] + alpha[b * nb + (blk ? blk[i] : 0)] * c[e];
   }
 }
 
@@ -347,26 +353,31 @@ __device__ __forceinline__ double red_combine(int op, double a, double v) {
   return (op == 0 || op == 3) ? fmax(a, v) : a + v;
 }
 
+// With blocks (seg non-null, nb segments per row) block b nb + q reduces the row's
+// segment [seg[q], seg[q + 1]) into out[b nb + q].
 __global__ __launch_bounds__(1024) void k_rowreduce(int64_t len, int op, const double* __restrict__ x,
                                                     const double* __restrict__ y,
-                                                    double* __restrict__ out) {
+                                                    double* __restrict__ out, int nb,
+                                                    const int64_t* __restrict__ seg) {
   __shared__ double red[16];
-  const int b = blockIdx.x, tid = threadIdx.x;
+  const int bq = blockIdx.x, tid = threadIdx.x;
+  const int b = bq / nb, q = bq - b * nb;
+  const int64_t s0 = seg ? seg[q] : 0, s1 = seg ? seg[q + 1] : len;
   const double* xb = x + (int64_t)b * len;
   const double* yb = y ? y + (int64_t)b * len : xb;
   double acc = 0.0;
-  for (int64_t i0 = 0; i0 < len; i0 += 8 * 1024) {
+  for (int64_t i0 = s0; i0 < s1; i0 += 8 * 1024) {
     double xv[8], yv[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int64_t i = min(i0 + tid + 1024 * u, len - 1);
+      const int64_t i = min(i0 + tid + 1024 * u, s1 - 1);
       xv[u] = xb[i];
       yv[u] = yb[i];
     }
     LAP_ISSUE_FENCE();
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      if (i0 + tid + 1024 * u >= len) continue;
+      if (i0 + tid + 1024 * u >= s1) continue;
       const double v = xv[u];
       if (op == 0)
         acc = isfinite(v) && isfinite(acc) ? fmax(acc, fabs(v)) : INFINITY;
@@ -384,8 +395,8 @@ __global__ __launch_bounds__(1024) void k_rowreduce(int64_t len, int op, const d
   __syncthreads();
   if (tid == 0) {
     double t = red[0];
-    for (int q = 1; q < 16; ++q) t = red_combine(op, t, red[q]);
-    out[b] = t;
+    for (int r = 1; r < 16; ++r) t = red_combine(op, t, red[r]);
+    out[bq] = t;
   }
 }
 
@@ -398,6 +409,8 @@ __global__ __launch_bounds__(1024) void k_rowreduce(int64_t len, int op, const d
 constexpr int kContractChunks = 64;
 constexpr int kContractGroup = 8;
 
+// With blocks (eseg non-null) the sums run per (b, block q) over the block's entry
+// range [eseg[q], eseg[q + 1]), chunked the same way; grid z = B nb.
 __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
                                                        const double* __restrict__ comps,
                                                        const double* __restrict__ S, int64_t l_size,
@@ -406,13 +419,16 @@ __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
                                                        const int32_t* __restrict__ qcol,
                                                        const double* __restrict__ x,
                                                        const double* __restrict__ z, int64_t n,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part, int nb,
+                                                       const int64_t* __restrict__ eseg) {
   constexpr int G = kContractGroup;
   __shared__ double red[4][2 * G];
-  const int ch = blockIdx.x, k0 = blockIdx.y * G, b = blockIdx.z;
+  const int ch = blockIdx.x, k0 = blockIdx.y * G, bq = blockIdx.z;
+  const int b = bq / nb, qb = bq - b * nb;
   const int kn = min(G, K - k0);
-  const int64_t len = (nnz + kContractChunks - 1) / kContractChunks;
-  const int64_t e0 = ch * len, e1 = min(nnz, e0 + len);
+  const int64_t g0 = eseg ? eseg[qb] : 0, g1 = eseg ? eseg[qb + 1] : nnz;
+  const int64_t len = (g1 - g0 + kContractChunks - 1) / kContractChunks;
+  const int64_t e0 = g0 + ch * len, e1 = min(g1, e0 + len);
   const double* Sb = S ? S + (int64_t)b * l_size : nullptr;
   const double* xb = x ? x + (int64_t)b * n : nullptr;
   const double* zb = z ? z + (int64_t)b * n : nullptr;
@@ -478,7 +494,7 @@ __global__ __launch_bounds__(256) void k_contract_part(int K, int64_t nnz,
   if ((int)threadIdx.x < 2 * kn) {
     const int t = threadIdx.x;
     const double v = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
-    const int64_t q = ((int64_t)b * K + k0 + (t >> 1)) * kContractChunks + ch;
+    const int64_t q = ((int64_t)bq * K + k0 + (t >> 1)) * kContractChunks + ch;
     part[2 * q + (t & 1)] = v;
   }
 }
@@ -682,12 +698,14 @@ __global__ void k_mul(int64_t tot, const double* __restrict__ a, const double* _
 
 // x[b] <- keep[b] ? xt[b] : x[b]
 __global__ void k_blend(int B, int64_t len, const double* __restrict__ keep,
-                        const double* __restrict__ xt, double* __restrict__ x) {
+                        const double* __restrict__ xt, double* __restrict__ x, int nb,
+                        const int32_t* __restrict__ blk) {
   const int64_t tot = (int64_t)B * len;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / len);
-    if (keep[b] != 0.0) x[e] = xt[e];
+    const int64_t i = e - (int64_t)b * len;
+    if (keep[b * nb + (blk ? blk[i] : 0)] != 0.0) x[e] = xt[e];
   }
 }
 
@@ -751,16 +769,18 @@ __global__ void k_gauss_finish(int B, int K, int mt, int fam, const int32_t* __r
 
 hipError_t lap_lik(int B, int64_t nobs, int fam, int link, const double* eta, const double* y,
                    const double* pa, const double* pb, const double* lpar, double* logp, double* grad,
-                   double* w, double* dw, double* dlpar, int32_t* bad, int32_t* invalid, hipStream_t st) {
+                   double* w, double* dw, double* dlpar, int32_t* bad, int32_t* invalid, hipStream_t st,
+                   const LapBlocks* bk) {
   hipLaunchKernelGGL(k_lik, dim3(grid_for((int64_t)B * nobs)), dim3(256), 0, st, B, nobs, fam, link,
-                     eta, y, pa, pb, lpar, logp, grad, w, dw, dlpar, bad, invalid);
+                     eta, y, pa, pb, lpar, logp, grad, w, dw, dlpar, bad, invalid, bk ? bk->nb : 1,
+                     bk ? bk->oblk : nullptr);
   return hipGetLastError();
 }
 
 hipError_t lap_affine(int B, int K, int64_t nnz, const double* coefs, const double* comps,
-                      double* out, hipStream_t st) {
+                      double* out, hipStream_t st, const LapBlocks* bk) {
   hipLaunchKernelGGL(k_affine, dim3(grid_for((int64_t)B * nnz)), dim3(256), 0, st, B, K, nnz, coefs,
-                     comps, out);
+                     comps, out, bk ? bk->nb : 1, bk ? bk->eblk : nullptr);
   return hipGetLastError();
 }
 
@@ -798,28 +818,29 @@ hipError_t lap_spmv(int B, int64_t nout, const int64_t* ptr, const int32_t* idx,
 }
 
 hipError_t lap_axpy(int B, int64_t len, const double* a, const double* alpha, const double* c,
-                    double* out, hipStream_t st) {
+                    double* out, hipStream_t st, const LapBlocks* bk) {
   hipLaunchKernelGGL(k_axpy_b, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, a, alpha,
-                     c, out);
+                     c, out, bk ? bk->nb : 1, bk ? bk->lblk : nullptr);
   return hipGetLastError();
 }
 
 hipError_t lap_rowreduce(int B, int64_t len, int op, const double* x, const double* y, double* out,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_rowreduce, dim3(B), dim3(1024), 0, st, len, op, x, y, out);
+                         hipStream_t st, int nb, const int64_t* seg) {
+  hipLaunchKernelGGL(k_rowreduce, dim3(B * nb), dim3(1024), 0, st, len, op, x, y, out, nb, seg);
   return hipGetLastError();
 }
 
 hipError_t lap_contract(int B, int K, int64_t nnz, const double* comps, const double* S,
                         int64_t l_size, const int64_t* slot_full, const int32_t* qrow,
                         const int32_t* qcol, const double* x, const double* z, int64_t n, double* tr,
-                        double* quad, double* part, hipStream_t st) {
-  hipLaunchKernelGGL(k_contract_part, dim3(kContractChunks, (K + kContractGroup - 1) / kContractGroup, B),
+                        double* quad, double* part, hipStream_t st, const LapBlocks* bk) {
+  const int nb = bk ? bk->nb : 1;
+  hipLaunchKernelGGL(k_contract_part, dim3(kContractChunks, (K + kContractGroup - 1) / kContractGroup, B * nb),
                      dim3(256), 0, st, K, nnz, comps, S,
-                     l_size, slot_full, qrow, qcol, x, z, n, part);
+                     l_size, slot_full, qrow, qcol, x, z, n, part, nb, bk ? bk->eseg : nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_contract_sum, dim3((B * K + 63) / 64), dim3(64), 0, st, B * K, part, tr, quad);
+  hipLaunchKernelGGL(k_contract_sum, dim3((B * nb * K + 63) / 64), dim3(64), 0, st, B * nb * K, part, tr, quad);
   return hipGetLastError();
 }
 
@@ -891,8 +912,9 @@ hipError_t lap_mul(int64_t tot, const double* a, const double* b, double* out, h
 }
 
 hipError_t lap_blend(int B, int64_t len, const double* keep, const double* xt, double* x,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(k_blend, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, keep, xt, x);
+                     hipStream_t st, const LapBlocks* bk) {
+  hipLaunchKernelGGL(k_blend, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, keep, xt, x,
+                     bk ? bk->nb : 1, bk ? bk->lblk : nullptr);
   return hipGetLastError();
 }
 

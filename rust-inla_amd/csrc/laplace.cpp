@@ -18,8 +18,9 @@
 namespace gmrf {
 hipError_t lap_lik(int, int64_t, int, int, const double*, const double*, const double*,
                    const double*, const double*, double*, double*, double*, double*, double*,
-                   int32_t*, int32_t*, hipStream_t);
-hipError_t lap_affine(int, int, int64_t, const double*, const double*, double*, hipStream_t);
+                   int32_t*, int32_t*, hipStream_t, const LapBlocks*);
+hipError_t lap_affine(int, int, int64_t, const double*, const double*, double*, hipStream_t,
+                      const LapBlocks*);
 hipError_t lap_add_diag(int, int64_t, int64_t, const int64_t*, double, double*, hipStream_t);
 hipError_t lap_add_atwa(int, int64_t, int64_t, const int64_t*, const int32_t*, const double*,
                         const double*, const double*, double*, const int64_t*, int64_t, hipStream_t);
@@ -27,16 +28,18 @@ hipError_t lap_spmv(int, int64_t, const int64_t*, const int32_t*, const double*,
                     const double*, int64_t, double, double, double*, int64_t, const int64_t*, int64_t,
                     hipStream_t);
 hipError_t lap_axpy(int, int64_t, const double*, const double*, const double*, double*,
-                    hipStream_t);
-hipError_t lap_rowreduce(int, int64_t, int, const double*, const double*, double*, hipStream_t);
+                    hipStream_t, const LapBlocks*);
+hipError_t lap_rowreduce(int, int64_t, int, const double*, const double*, double*, hipStream_t, int,
+                         const int64_t*);
 hipError_t lap_contract(int, int, int64_t, const double*, const double*, int64_t, const int64_t*,
                         const int32_t*, const int32_t*, const double*, const double*, int64_t,
-                        double*, double*, double*, hipStream_t);
+                        double*, double*, double*, hipStream_t, const LapBlocks*);
 hipError_t lap_vdiag(int, int64_t, const int64_t*, const int64_t*, const double*, const double*,
                      int64_t, double*, hipStream_t);
 hipError_t lap_constr_dot(int, int, int64_t, const double*, const double*, double*, hipStream_t);
 hipError_t lap_mul(int64_t, const double*, const double*, double*, hipStream_t);
-hipError_t lap_blend(int, int64_t, const double*, const double*, double*, hipStream_t);
+hipError_t lap_blend(int, int64_t, const double*, const double*, double*, hipStream_t,
+                     const LapBlocks*);
 hipError_t lap_constr_apply(int, int, int64_t, const double*, const double*, double*,
                             hipStream_t, double);
 hipError_t lap_ground(int, int, int64_t, const int64_t*, const double*, double*, double*,
@@ -171,6 +174,17 @@ struct gmrf_model {
     }
   };
   std::vector<GaussGraph> graphs;  // most recently used last
+  // Independent blocks: a heterogeneous batch of models as one block-diagonal union
+  // (nb = 1: a single model). Per-point quantities become per (point b, block q),
+  // index b nb + q; the factorisations, solves and selected inversions run on the
+  // union, the Newton steps, convergence, mlik and gradient per block.
+  int nb = 1;
+  LapBlocks blk;                    // device maps and segments (nb > 1)
+  int64_t* fseg_ptr = nullptr;      // [nb + 1] factor positions of each block
+  int32_t* fseg_pos = nullptr;      // [n]
+  double* bld = nullptr;            // [2 cap nb] per-block log det, posterior slots then prior
+  int32_t* bst = nullptr;           // [2 cap nb] per-block factor status flags
+  const LapBlocks* blocks() const { return nb > 1 ? &blk : nullptr; }
   int last_count = 0;              // points of the last evaluation (0: none yet)
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
   hipStream_t stream() const { return bt->stream; }
@@ -409,7 +423,7 @@ void enqueue_gauss(gmrf_model* M, const gmrf_eval_args* a, int B) {
   const bool want_grad = a->dcoefs != nullptr && a->n_theta > 0 && a->grad != nullptr;
   const int sel_count = want_grad ? 2 * B : (a->var ? B : 0);
   double* red = M->red;
-  check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st), "affine");
+  check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st, nullptr), "affine");
   double* vprior_f = M->vals + (size_t)B * nnz;
   check(hipMemcpyAsync(vprior_f, M->vprior, (size_t)B * nnz * sizeof(double), hipMemcpyDeviceToDevice, st),
         "copy prior");
@@ -420,7 +434,7 @@ void enqueue_gauss(gmrf_model* M, const gmrf_eval_args* a, int B) {
         "A x");
   check(lap_fill_i32(B, 0, M->bad, st), "zero bad");
   check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w, nullptr,
-                nullptr, M->bad, nullptr, st),
+                nullptr, M->bad, nullptr, st, nullptr),
         "lik");
   check(lap_add_atwa(B, nnz, m, M->atwa_ptr, M->atwa_obs, M->atwa_coef, M->w, M->vprior, M->vals,
                      M->atwa_long.d, M->atwa_long.n, st),
@@ -440,20 +454,20 @@ void enqueue_gauss(gmrf_model* M, const gmrf_eval_args* a, int B) {
     bt->factorize(2 * B, M->vals, nnz, 0);
   }
   bt->solve(B, M->rhs, n, M->step, n, 0);
-  check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, red, st), "max step");
+  check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, red, st, 1, nullptr), "max step");
   check(hipMemcpyAsync(M->x, M->step, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x = step");
   // mlik terms at x*
   check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d,
                  M->ar_long.n, st),
         "eta*");
   check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w, M->dw, M->dl,
-                nullptr, nullptr, st),
+                nullptr, nullptr, st, nullptr),
         "lik*");
   check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d,
                  M->q_long.n, st),
         "Q x*");
-  check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, red + B, st), "quad");
-  check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, red + 2 * B, st), "loglik");
+  check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, red + B, st, 1, nullptr), "quad");
+  check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, red + 2 * B, st, 1, nullptr), "loglik");
   if (a->x)
     check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
   if (sel_count > 0) bt->selinv_join();
@@ -461,14 +475,14 @@ void enqueue_gauss(gmrf_model* M, const gmrf_eval_args* a, int B) {
   if (want_grad) {
     const DevPlan& P = M->sym->plan;
     check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col, M->x, M->x, n,
-                       M->tr_post, M->qx, M->cpart, st),
+                       M->tr_post, M->qx, M->cpart, st, nullptr),
           "contract post");
     check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full, nullptr,
-                       nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st),
+                       nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st, nullptr),
           "contract prior");
     check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
-    check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, red + 3 * B, st), "sum dl");
-    check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, red + 4 * B, st), "sum v w");
+    check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, red + 3 * B, st, 1, nullptr), "sum dl");
+    check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, red + 4 * B, st, 1, nullptr), "sum v w");
   }
   check(lap_gauss_finish(B, K, want_grad ? a->n_theta : 0, M->fam, bt->d_status, M->bad, red, bt->d_logdet,
                          red + B, red + 2 * B, M->tr_post, M->tr_prior, M->qx, a->dcoefs, a->dlog_lpar,
@@ -565,6 +579,20 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     return fail(GMRF_INVALID_ARGUMENT, "low-rank constraint augmentation needs 1..32 constraints");
   if (d->constr_kappa > 0.0 && (int64_t)d->capacity * 2 * d->n_constr > 65535)
     return fail(GMRF_INVALID_ARGUMENT, "capacity x 2 k exceeds 65535 right-hand sides");
+  if (d->n_blocks < 0) return fail(GMRF_INVALID_ARGUMENT, "negative block count");
+  const int nb = d->n_blocks > 1 ? d->n_blocks : 1;
+  if (nb > 1) {
+    if (!d->block_ptr) return fail(GMRF_INVALID_ARGUMENT, "blocks need block_ptr");
+    if (d->block_ptr[0] != 0 || d->block_ptr[nb] != d->n)
+      return fail(GMRF_INVALID_ARGUMENT, "block_ptr must run from 0 to n");
+    for (int q = 0; q < nb; ++q)
+      if (d->block_ptr[q + 1] <= d->block_ptr[q])
+        return fail(GMRF_INVALID_ARGUMENT, "block_ptr must increase strictly");
+    if (d->n_constr > 0 || d->constr_kappa > 0.0)
+      return fail(GMRF_INVALID_ARGUMENT, "constraints are not supported with blocks");
+    if ((int64_t)d->capacity * nb > (int64_t)1 << 30)
+      return fail(GMRF_INVALID_ARGUMENT, "capacity x blocks too large");
+  }
   try {
     auto M = std::make_unique<gmrf_model>();
     const int64_t n = d->n;
@@ -575,7 +603,37 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->cap = d->capacity;
     M->nobs = d->n_obs;
     M->kc = d->n_constr;
-    M->sym = analyze_cached(n, d->indptr, d->indices, d->order, nullptr);
+    M->nb = nb;
+    std::vector<int64_t> bp(nb + 1, 0);  // latent segment of each block
+    bp[nb] = n;
+    if (nb > 1) std::copy(d->block_ptr, d->block_ptr + nb + 1, bp.begin());
+    std::vector<int32_t> lblk(n, 0);
+    for (int q = 0; q < nb; ++q)
+      for (int64_t i = bp[q]; i < bp[q + 1]; ++i) lblk[i] = q;
+    if (nb == 1) {
+      M->sym = analyze_cached(n, d->indptr, d->indices, d->order, nullptr);
+    } else {
+      // each block ordered on its own pattern, as its model alone would be; the union
+      // is analysed in that (block-diagonal) ordering
+      std::vector<int64_t> perm(n);
+      const OrderRequest req = d->order == GMRF_ORDER_AMD ? OrderRequest::AmdOnly : OrderRequest::Chooser;
+      for (int q = 0; q < nb; ++q) {
+        const int64_t n0 = bp[q], nq = bp[q + 1] - bp[q];
+        std::vector<int64_t> ipq(nq + 1), ixq;
+        for (int64_t c = 0; c < nq; ++c) {
+          ipq[c] = (int64_t)ixq.size();
+          for (int64_t p = d->indptr[n0 + c]; p < d->indptr[n0 + c + 1]; ++p) {
+            const int64_t r = d->indices[p];
+            if (r < n0 || r >= bp[q + 1]) throw std::runtime_error("blocks: the pattern must be block diagonal");
+            ixq.push_back(r - n0);
+          }
+        }
+        ipq[nq] = (int64_t)ixq.size();
+        const Symbolic Sq = analyze(nq, ipq.data(), ixq.data(), req, nullptr);
+        for (int64_t p = 0; p < nq; ++p) perm[n0 + p] = n0 + Sq.perm[p];
+      }
+      M->sym = analyze_cached(n, d->indptr, d->indices, GMRF_ORDER_GIVEN, perm.data());
+    }
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     const Symbolic& S = M->sym->S;
     const int64_t nnz = S.nnz_full;
@@ -665,6 +723,45 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->a_rptr = M->arena.upload(arp);
     M->a_cidx = M->arena.upload(aci);
     M->a_rval = M->arena.upload(arv);
+    // ---- blocks: each pattern entry, observation and latent index belongs to one
+    if (nb > 1) {
+      std::vector<int32_t> oblk(m), eblk(nnz);
+      std::vector<int64_t> oseg(nb + 1, 0), eseg(nb + 1, 0);
+      for (int64_t r = 0; r < m; ++r) {
+        if (arp[r + 1] == arp[r]) throw std::runtime_error("blocks: every observation must load a latent index");
+        const int32_t q = lblk[aci[arp[r]]];
+        for (int64_t t = arp[r]; t < arp[r + 1]; ++t)
+          if (lblk[aci[t]] != q) throw std::runtime_error("blocks: an observation loads two blocks");
+        if (r > 0 && q < oblk[r - 1]) throw std::runtime_error("blocks: observation order");
+        oblk[r] = q;
+        oseg[q + 1]++;
+      }
+      for (int q = 0; q < nb; ++q) {
+        if (oseg[q + 1] == 0) throw std::runtime_error("blocks: a block has no observations");
+        oseg[q + 1] += oseg[q];
+      }
+      for (int64_t c = 0; c < n; ++c)
+        for (int64_t p = ip[c]; p < ip[c + 1]; ++p) eblk[p] = lblk[c];
+      for (int q = 0; q <= nb; ++q) eseg[q] = ip[bp[q]];
+      M->blk.nb = nb;
+      M->blk.lblk = M->arena.upload(lblk);
+      M->blk.oblk = M->arena.upload(oblk);
+      M->blk.eblk = M->arena.upload(eblk);
+      M->blk.lseg = M->arena.upload(bp);
+      M->blk.oseg = M->arena.upload(oseg);
+      M->blk.eseg = M->arena.upload(eseg);
+      // factor positions grouped by block, for the per-block log determinants
+      std::vector<int64_t> fptr(nb + 1, 0);
+      for (int64_t p = 0; p < n; ++p) fptr[lblk[S.perm[p]] + 1]++;
+      for (int q = 0; q < nb; ++q) fptr[q + 1] += fptr[q];
+      std::vector<int32_t> fpos(n);
+      std::vector<int64_t> fill(fptr.begin(), fptr.end() - 1);
+      for (int64_t p = 0; p < n; ++p) fpos[fill[lblk[S.perm[p]]]++] = (int32_t)p;
+      M->fseg_ptr = M->arena.upload(fptr);
+      M->fseg_pos = M->arena.upload(fpos);
+      M->bld = M->arena.alloc<double>((size_t)2 * M->cap * nb);
+      M->bst = M->arena.alloc<int32_t>((size_t)2 * M->cap * nb);
+    }
     // ---- A^T W A gather map (per Q entry) and v map (per observation)
     std::vector<int64_t> cnt(nnz + 1, 0);
     std::vector<std::pair<int64_t, int64_t>> pe;  // (entry, obs) with coefficient below
@@ -774,8 +871,8 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
       L.ok_pr.assign(C, 0);
       L.ok_po.assign(C, 0);
     }
-    // ---- workspaces
-    const size_t C = (size_t)M->cap;
+    // ---- workspaces (per-point scalars per (point, block))
+    const size_t C = (size_t)M->cap, CV = C * nb;
     M->vprior = M->arena.alloc<double>(C * nnz);
     M->vals = M->arena.alloc<double>(2 * C * nnz);
     M->x = M->arena.alloc<double>(C * n);
@@ -792,16 +889,16 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->dl = M->arena.alloc<double>(C * m);
     M->v = M->arena.alloc<double>(C * m);
     M->u = M->arena.alloc<double>(C * m);
-    M->red = M->arena.alloc<double>(8 * C);
+    M->red = M->arena.alloc<double>(8 * CV);
     M->dstat = M->arena.alloc<int32_t>(C);
-    M->alpha = M->arena.alloc<double>(C);
+    M->alpha = M->arena.alloc<double>(CV);
     M->cres = M->arena.alloc<double>(C * std::max(M->kc, 1));
-    M->bad = M->arena.alloc<int32_t>(C);
-    M->tr_post = M->arena.alloc<double>(C * M->K);
-    M->tr_prior = M->arena.alloc<double>(C * M->K);
-    M->qx = M->arena.alloc<double>(C * M->K);
-    M->ql = M->arena.alloc<double>(C * M->K);
-    M->cpart = M->arena.alloc<double>(C * M->K * 64 * 2);  // contraction partial sums
+    M->bad = M->arena.alloc<int32_t>(CV);
+    M->tr_post = M->arena.alloc<double>(CV * M->K);
+    M->tr_prior = M->arena.alloc<double>(CV * M->K);
+    M->qx = M->arena.alloc<double>(CV * M->K);
+    M->ql = M->arena.alloc<double>(CV * M->K);
+    M->cpart = M->arena.alloc<double>(CV * M->K * 64 * 2);  // contraction partial sums
     *out = M.release();
     return GMRF_OK;
   } catch (...) {
@@ -826,7 +923,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     M->last_count = B;
-    if (M->fam == 0 && M->link == 0 && M->kc == 0 && a->max_iter > 0) {
+    if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && a->max_iter > 0) {
       eval_gauss(M, a, B);
       return GMRF_OK;
     }
@@ -834,9 +931,14 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     const int64_t n = M->n, nnz = M->nnz, m = M->nobs;
     const int K = M->K;
     gmrf_batch* bt = M->bt.get();
+    // per-point scalars run over the V = B nb (point, block) pairs, index b nb + q
+    const int nb = M->nb, V = B * nb;
+    const LapBlocks* bk = M->blocks();
+    const int64_t* lseg = bk ? bk->lseg : nullptr;
+    const int64_t* oseg = bk ? bk->oseg : nullptr;
     // ---- Q(theta); the prior matrices occupy batch slots [B, 2B) right after the
     // posterior ones, so the first Newton factorisation covers both in one call
-    check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st), "affine");
+    check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st, bk), "affine");
     double* vprior_f = M->vals + (size_t)B * nnz;
     check(hipMemcpyAsync(vprior_f, M->vprior, (size_t)B * nnz * sizeof(double),
                          hipMemcpyDeviceToDevice, st),
@@ -847,10 +949,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     bool prior_pending = true;
     // ---- Newton (inference.rs:842-942)
     check(hipMemsetAsync(M->x, 0, (size_t)B * n * sizeof(double), st), "zero x");
-    std::vector<int> state(B, 0);  // 0 active, 1 confirming, 2 done, 3 failed
-    std::vector<int32_t> status(B, GMRF_OK), niter(B, 0);
-    std::vector<double> hred(4 * B), halpha(B);
-    std::vector<int32_t> hbad(B);
+    std::vector<int> state(V, 0);  // 0 active, 1 confirming, 2 done, 3 failed
+    std::vector<int32_t> status(V, GMRF_OK), niter(V, 0);
+    std::vector<double> hred(4 * V), halpha(V);
+    std::vector<int32_t> hbad(V);
     const bool gauss_id = M->fam == 0 && M->link == 0;
     const int sel_count = want_grad ? 2 * B : (a->var ? B : 0);
     bool sel_started = false;
@@ -861,9 +963,9 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       // eta = A x; likelihood; Q_post = Q + A^T diag(w) A; rhs = A^T g - Q x
       check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d, M->ar_long.n, st),
             "A x");
-      check(hipMemsetAsync(M->bad, 0, B * sizeof(int32_t), st), "zero bad");
+      check(hipMemsetAsync(M->bad, 0, V * sizeof(int32_t), st), "zero bad");
       check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
-                    nullptr, nullptr, M->bad, nullptr, st),
+                    nullptr, nullptr, M->bad, nullptr, st, bk),
             "lik");
       check(lap_add_atwa(B, nnz, m, M->atwa_ptr, M->atwa_obs, M->atwa_coef, M->w, M->vprior,
                          M->vals, M->atwa_long.d, M->atwa_long.n, st),
@@ -892,31 +994,31 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       if (lr) lowrank_factor(M, B, 0, false);
       bt->solve(B, M->rhs, n, M->step, n, 0);
       if (lr) lowrank_correct(M, B, false, M->step);
-      check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, M->red, st), "max step");
+      check(lap_rowreduce(B, n, RED_MAXABS, M->step, nullptr, M->red, st, nb, lseg), "max step");
       {
-        char* pb = M->pin.get((size_t)B * (sizeof(double) + sizeof(int32_t)));
-        d2h_async(pb, M->red, B * sizeof(double), st);
-        d2h_async(pb + B * sizeof(double), M->bad, B * sizeof(int32_t), st);
+        char* pb = M->pin.get((size_t)V * (sizeof(double) + sizeof(int32_t)));
+        d2h_async(pb, M->red, V * sizeof(double), st);
+        d2h_async(pb + V * sizeof(double), M->bad, V * sizeof(int32_t), st);
         check(hipStreamSynchronize(st), "sync");
-        std::memcpy(hred.data(), pb, B * sizeof(double));
-        std::memcpy(hbad.data(), pb + B * sizeof(double), B * sizeof(int32_t));
+        std::memcpy(hred.data(), pb, V * sizeof(double));
+        std::memcpy(hbad.data(), pb + V * sizeof(double), V * sizeof(int32_t));
       }
     };
     if (gauss_id) {
       if (a->max_iter == 0) {
-        for (int b = 0; b < B; ++b) status[b] = GMRF_SOLVE_FAILURE, state[b] = 3;
+        for (int v = 0; v < V; ++v) status[v] = GMRF_SOLVE_FAILURE, state[v] = 3;
         bt->factorize(B, vprior_f, nnz, B);
         if (lr) lowrank_factor(M, B, B, true);
         prior_pending = false;
       } else {
         newton_system(true);
-        for (int b = 0; b < B; ++b) {
-          niter[b] = 1;
-          if (hbad[b] || !std::isfinite(hred[b])) {
-            status[b] = GMRF_SOLVE_FAILURE;
-            state[b] = 3;
+        for (int v = 0; v < V; ++v) {
+          niter[v] = 1;
+          if (hbad[v] || !std::isfinite(hred[v])) {
+            status[v] = GMRF_SOLVE_FAILURE;
+            state[v] = 3;
           } else {
-            state[b] = 2;
+            state[v] = 2;
           }
         }
         check(hipMemcpyAsync(M->x, M->step, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice,
@@ -927,52 +1029,52 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     } else {
       for (int it = 0; it <= a->max_iter; ++it) {
         bool any = false;
-        for (int b = 0; b < B; ++b) any |= state[b] < 2;
+        for (int v = 0; v < V; ++v) any |= state[v] < 2;
         if (!any) break;
         bool fin = true;  // no member still iterating: this system confirms every mode
-        for (int b = 0; b < B; ++b) fin &= state[b] != 0;
+        for (int v = 0; v < V; ++v) fin &= state[v] != 0;
         newton_system(fin);
         bool any_active = false;
-        for (int b = 0; b < B; ++b) {
-          if (state[b] == 1) {
-            state[b] = 2;  // confirmation factorisation at x* done
-          } else if (state[b] == 0) {
-            niter[b] = it + 1;
-            if (hbad[b] || !std::isfinite(hred[b]) || it == a->max_iter) {
-              state[b] = 3;
-              status[b] = GMRF_SOLVE_FAILURE;
-              halpha[b] = 0.0;
+        for (int v = 0; v < V; ++v) {
+          if (state[v] == 1) {
+            state[v] = 2;  // confirmation factorisation at x* done
+          } else if (state[v] == 0) {
+            niter[v] = it + 1;
+            if (hbad[v] || !std::isfinite(hred[v]) || it == a->max_iter) {
+              state[v] = 3;
+              status[v] = GMRF_SOLVE_FAILURE;
+              halpha[v] = 0.0;
             } else {
-              halpha[b] = hred[b] > 10.0 ? 10.0 / hred[b] : 1.0;  // inference.rs:888-894
+              halpha[v] = hred[v] > 10.0 ? 10.0 / hred[v] : 1.0;  // inference.rs:888-894
               any_active = true;
             }
           } else {
-            halpha[b] = 0.0;
+            halpha[v] = 0.0;
           }
         }
         if (!any_active) continue;
         // backtracking on a non-finite objective (inference.rs:896-914)
-        std::vector<double> maxstep(hred.begin(), hred.begin() + B);
+        std::vector<double> maxstep(hred.begin(), hred.begin() + V);
         for (int bt_it = 0; bt_it < 12; ++bt_it) {
-          for (int b = 0; b < B; ++b)
-            if (state[b] != 0) halpha[b] = 0.0;
-          h2d(M->alpha, halpha.data(), B * sizeof(double), st);
-          check(lap_axpy(B, n, M->x, M->alpha, M->step, M->xt, st), "x trial");
+          for (int v = 0; v < V; ++v)
+            if (state[v] != 0) halpha[v] = 0.0;
+          h2d(M->alpha, halpha.data(), V * sizeof(double), st);
+          check(lap_axpy(B, n, M->x, M->alpha, M->step, M->xt, st, bk), "x trial");
           if (!project(M, B, M->xt)) throw std::runtime_error("project_constraints: singular A A^T");
           check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->xt, n, 1.0, 0.0, M->eta, m, M->ar_long.d, M->ar_long.n, st),
                 "A x trial");
-          check(hipMemsetAsync(M->bad, 0, B * sizeof(int32_t), st), "zero bad");
+          check(hipMemsetAsync(M->bad, 0, V * sizeof(int32_t), st), "zero bad");
           check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, nullptr,
-                        nullptr, nullptr, nullptr, M->bad, nullptr, st),
+                        nullptr, nullptr, nullptr, M->bad, nullptr, st, bk),
                 "lik trial");
-          d2h(hbad.data(), M->bad, B * sizeof(int32_t), st);
+          d2h(hbad.data(), M->bad, V * sizeof(int32_t), st);
           bool retry = false;
-          for (int b = 0; b < B; ++b) {
-            if (state[b] != 0 || !hbad[b]) continue;
-            halpha[b] *= 0.5;
-            if (halpha[b] < 1e-8) {
-              state[b] = 3;
-              status[b] = GMRF_SOLVE_FAILURE;
+          for (int v = 0; v < V; ++v) {
+            if (state[v] != 0 || !hbad[v]) continue;
+            halpha[v] *= 0.5;
+            if (halpha[v] < 1e-8) {
+              state[v] = 3;
+              status[v] = GMRF_SOLVE_FAILURE;
             } else {
               retry = true;
             }
@@ -980,19 +1082,19 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
           if (!retry) break;
         }
         // x <- x_trial for active members; convergence test (inference.rs:916-941)
-        check(lap_rowreduce(B, n, RED_MAXDIFF, M->xt, M->x, M->red, st), "max dx");
-        d2h(hred.data(), M->red, B * sizeof(double), st);
-        std::vector<double> keep(B);
-        for (int b = 0; b < B; ++b) keep[b] = state[b] == 0 ? 1.0 : 0.0;
-        h2d(M->alpha, keep.data(), B * sizeof(double), st);
-        check(lap_blend(B, n, M->alpha, M->xt, M->x, st), "x update");
-        for (int b = 0; b < B; ++b) {
-          if (state[b] != 0) continue;
-          if (hred[b] < a->tol || maxstep[b] < a->tol) state[b] = 1;
+        check(lap_rowreduce(B, n, RED_MAXDIFF, M->xt, M->x, M->red, st, nb, lseg), "max dx");
+        d2h(hred.data(), M->red, V * sizeof(double), st);
+        std::vector<double> keep(V);
+        for (int v = 0; v < V; ++v) keep[v] = state[v] == 0 ? 1.0 : 0.0;
+        h2d(M->alpha, keep.data(), V * sizeof(double), st);
+        check(lap_blend(B, n, M->alpha, M->xt, M->x, st, bk), "x update");
+        for (int v = 0; v < V; ++v) {
+          if (state[v] != 0) continue;
+          if (hred[v] < a->tol || maxstep[v] < a->tol) state[v] = 1;
         }
       }
-      for (int b = 0; b < B; ++b)
-        if (state[b] < 2 && status[b] == GMRF_OK) status[b] = GMRF_SOLVE_FAILURE, state[b] = 3;
+      for (int v = 0; v < V; ++v)
+        if (state[v] < 2 && status[v] == GMRF_OK) status[v] = GMRF_SOLVE_FAILURE, state[v] = 3;
       if (!project(M, B, M->x)) throw std::runtime_error("project_constraints: singular A A^T");
     }
     if (prior_pending) {  // no Newton system was formed
@@ -1003,31 +1105,39 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     // ---- mlik at x*: sum log p(y | A x*) - x*^T Q x* / 2 + (log|Q_prior| - log|Q_post|) / 2
     check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval, 0, M->x, n, 1.0, 0.0, M->eta, m, M->ar_long.d, M->ar_long.n, st), "eta*");
     check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
-                  M->dw, M->dl, nullptr, nullptr, st),
+                  M->dw, M->dl, nullptr, nullptr, st, bk),
           "lik*");
     check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d, M->q_long.n, st), "Q x*");
-    check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st), "quad");
-    check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + B, st), "loglik");
+    check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st, nb, lseg), "quad");
+    check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + V, st, nb, oseg), "loglik");
+    // per-block log determinants of the posterior (slots [0, B)) and prior ([B, 2B))
+    // factors, in the layout of the whole-matrix ones: [0, V) posterior, [V, 2V) prior
+    if (bk)
+      check(launch_logdet_seg(M->sym->plan, 2 * B, bt->L, M->fseg_ptr, M->fseg_pos, nb, M->bld, M->bst, st),
+            "logdet_seg");
+    const double* d_ld = bk ? M->bld : bt->d_logdet;
+    const int32_t* d_fs = bk ? M->bst : bt->d_status;
     if (lr) check(lap_constr_dot(B, M->kc, n, M->ca, M->x, M->cres, st), "C x*");
     // ---- log-determinants, factor status and the mlik terms in one host round trip
-    std::vector<double> ld(2 * B), cx(lr ? (size_t)B * M->kc : 0);
-    std::vector<int32_t> fst(2 * B);
+    std::vector<double> ld(2 * V), cx(lr ? (size_t)B * M->kc : 0);
+    std::vector<int32_t> fst(2 * V);
     {
-      const size_t o1 = 2 * B * sizeof(double), o2 = o1 + 2 * B * sizeof(double),
-                   o3 = o2 + 2 * B * sizeof(int32_t);
+      const size_t o1 = 2 * V * sizeof(double), o2 = o1 + 2 * V * sizeof(double),
+                   o3 = o2 + 2 * V * sizeof(int32_t);
       char* pb = M->pin.get(o3 + cx.size() * sizeof(double));
-      d2h_async(pb, bt->d_logdet, 2 * B * sizeof(double), st);
-      d2h_async(pb + o1, M->red, 2 * B * sizeof(double), st);
-      d2h_async(pb + o2, bt->d_status, 2 * B * sizeof(int32_t), st);
+      d2h_async(pb, d_ld, 2 * V * sizeof(double), st);
+      d2h_async(pb + o1, M->red, 2 * V * sizeof(double), st);
+      d2h_async(pb + o2, d_fs, 2 * V * sizeof(int32_t), st);
       if (lr) d2h_async(pb + o3, M->cres, cx.size() * sizeof(double), st);
       check(hipStreamSynchronize(st), "sync");
-      std::memcpy(ld.data(), pb, 2 * B * sizeof(double));
-      std::memcpy(hred.data(), pb + o1, 2 * B * sizeof(double));
-      std::memcpy(fst.data(), pb + o2, 2 * B * sizeof(int32_t));
+      std::memcpy(ld.data(), pb, 2 * V * sizeof(double));
+      std::memcpy(hred.data(), pb + o1, 2 * V * sizeof(double));
+      std::memcpy(fst.data(), pb + o2, 2 * V * sizeof(int32_t));
       if (lr) std::memcpy(cx.data(), pb + o3, cx.size() * sizeof(double));
     }
-    for (int b = 0; b < B; ++b) {
-      const int32_t fp = fst[B + b], fq = fst[b];
+    // (the low-rank terms below exist only for a single model, where V = B)
+    for (int b = 0; b < V; ++b) {
+      const int32_t fp = fst[V + b], fq = fst[b];
       auto code = [](int32_t f) {
         return (f & 2) ? GMRF_NOT_POSITIVE_DEFINITE : (f & 1) ? GMRF_SINGULAR : GMRF_OK;
       };
@@ -1035,7 +1145,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       else if (status[b] == GMRF_OK && code(fq) != GMRF_OK) status[b] = code(fq);
       if (lr) {
         ld[b] += M->lr.ld_po[b];
-        ld[B + b] += M->lr.ld_pr[b];
+        ld[V + b] += M->lr.ld_pr[b];
         if (status[b] == GMRF_OK && (!M->lr.ok_pr[b] || !M->lr.ok_po[b]))
           status[b] = GMRF_NOT_POSITIVE_DEFINITE;
       }
@@ -1045,15 +1155,15 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       for (int b = 0; b < B; ++b)
         for (int t = 0; t < k; ++t) hred[b] += M->kappa * cx[(size_t)b * k + t] * cx[(size_t)b * k + t];
     }
-    std::vector<double> mlik(B);
-    for (int b = 0; b < B; ++b) {
-      mlik[b] = hred[B + b] - 0.5 * hred[b] + 0.5 * ld[B + b] - 0.5 * ld[b];
-      if (status[b] != GMRF_OK) mlik[b] = -INFINITY;
+    std::vector<double> mlik(V);
+    for (int v = 0; v < V; ++v) {
+      mlik[v] = hred[V + v] - 0.5 * hred[v] + 0.5 * ld[V + v] - 0.5 * ld[v];
+      if (status[v] != GMRF_OK) mlik[v] = -INFINITY;
     }
     double* pout = reinterpret_cast<double*>(
-        M->pin_out.get((size_t)B * (1 + std::max(a->n_theta, 0)) * sizeof(double)));
-    std::memcpy(pout, mlik.data(), B * sizeof(double));
-    h2d_async(a->mlik, pout, B * sizeof(double), st);
+        M->pin_out.get((size_t)V * (1 + std::max(a->n_theta, 0)) * sizeof(double)));
+    std::memcpy(pout, mlik.data(), V * sizeof(double));
+    h2d_async(a->mlik, pout, V * sizeof(double), st);
     if (a->x) check(hipMemcpyAsync(a->x, M->x, (size_t)B * n * sizeof(double), hipMemcpyDeviceToDevice, st), "x out");
     // ---- selected inversion, variances
     // posterior (and, for the gradient, prior) selected inverses in one call
@@ -1075,10 +1185,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       const DevPlan& P = M->sym->plan;
       // tr(Sigma_post Q_k), x*^T Q_k x*, tr(Sigma_prior Q_k)
       check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col,
-                         M->x, M->x, n, M->tr_post, M->qx, M->cpart, st),
+                         M->x, M->x, n, M->tr_post, M->qx, M->cpart, st, bk),
             "contract post");
       check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full,
-                         nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st),
+                         nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
             "contract prior");
       // v_i = (A Sigma_post A^T)_ii
       check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
@@ -1091,7 +1201,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_lowrank_diag(B, R, m, M->lr.W, m, (int64_t)R * m, M->lr.kinv_po, M->v, m, st),
               "v correction");
       }
-      std::vector<double> hql((size_t)B * K, 0.0);
+      std::vector<double> hql((size_t)V * K, 0.0);
       if (!gauss_id) {
         // mode sensitivity: u = v .* dw/deta, lambda = Q_post^-1 A^T u
         check(lap_mul(B * m, M->v, M->dw, M->u, st), "u");
@@ -1100,12 +1210,12 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         bt->solve(B, M->rhs, n, M->lam, n, 0);
         if (lr) lowrank_correct(M, B, false, M->lam);
         check(lap_contract(B, K, nnz, M->comps, nullptr, 0, nullptr, M->q_row, M->q_col, M->lam, M->x,
-                           n, nullptr, M->ql, M->cpart, st),
+                           n, nullptr, M->ql, M->cpart, st, bk),
               "lambda^T Q_k x");
       }
       // Gaussian precision hyperparameter at fixed eta: sum dlogp/dlog(prec), sum v w
-      check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, M->red, st), "sum dl");
-      check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, M->red + B, st), "sum v w");
+      check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, M->red, st, nb, oseg), "sum dl");
+      check(lap_rowreduce(B, m, RED_DOT, M->v, M->w, M->red + V, st, nb, oseg), "sum v w");
       // and, where the weights depend on eta (a non-identity link), the mode's
       // sensitivity to the precision: g scales with it, so dx*/dlog(prec) =
       // Q_post^-1 A^T g(x*), contributing -1/2 lambda^T A^T g(x*)
@@ -1114,11 +1224,11 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_spmv(B, n, M->a_cptr, M->a_ridx, M->a_cval, 0, M->g, m, 1.0, 0.0, M->tmpn, n, M->ac_long.d,
                        M->ac_long.n, st),
               "A^T g*");
-        check(lap_rowreduce(B, n, RED_DOT, M->lam, M->tmpn, M->red + 2 * B, st), "lambda^T A^T g");
+        check(lap_rowreduce(B, n, RED_DOT, M->lam, M->tmpn, M->red + 2 * V, st, nb, lseg), "lambda^T A^T g");
       }
-      std::vector<double> hs((gauss_link ? 3 : 2) * B);
-      std::vector<double> htp((size_t)B * K), htq((size_t)B * K), hqx((size_t)B * K);
-      std::vector<double> dc((size_t)B * mt * K), dlp((size_t)B * mt, 0.0);
+      std::vector<double> hs((gauss_link ? 3 : 2) * V);
+      std::vector<double> htp((size_t)V * K), htq((size_t)V * K), hqx((size_t)V * K);
+      std::vector<double> dc((size_t)V * mt * K), dlp((size_t)V * mt, 0.0);
       {
         // one pinned area, one host round trip
         std::vector<std::pair<double*, size_t>> parts = {{hs.data(), hs.size()},   {htp.data(), htp.size()},
@@ -1155,8 +1265,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
           htq[e] -= cpr[e];
         }
       }
-      grad.assign((size_t)B * mt, 0.0);
-      for (int b = 0; b < B; ++b)
+      grad.assign((size_t)V * mt, 0.0);
+      for (int b = 0; b < V; ++b)
         for (int l = 0; l < mt; ++l) {
           double gq = 0.0;
           for (int k = 0; k < K; ++k) {
@@ -1165,16 +1275,16 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
                        0.5 * hql[b * K + k]);
           }
           if (M->fam == 0)
-            gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[B + b] - (gauss_link ? 0.5 * hs[2 * B + b] : 0.0));
+            gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[V + b] - (gauss_link ? 0.5 * hs[2 * V + b] : 0.0));
           grad[(size_t)b * mt + l] = status[b] == GMRF_OK ? gq : NAN;
         }
       std::memcpy(pout + B, grad.data(), grad.size() * sizeof(double));
       h2d_async(a->grad, pout + B, grad.size() * sizeof(double), st);
     }
     check(hipStreamSynchronize(st), "sync");
-    for (int b = 0; b < B; ++b) {
-      if (a->status) a->status[b] = status[b];
-      if (a->n_iter) a->n_iter[b] = niter[b];
+    for (int v = 0; v < V; ++v) {
+      if (a->status) a->status[v] = status[v];
+      if (a->n_iter) a->n_iter[v] = niter[v];
     }
     return GMRF_OK;
   } catch (...) {
@@ -1211,7 +1321,7 @@ gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t
     check(hipMalloc(&flag, count * sizeof(int32_t)), "hipMalloc");
     check(hipMemsetAsync(flag, 0, count * sizeof(int32_t), st), "zero flag");
     check(lap_lik(count, n_obs, family, link, eta, y, obs_a, obs_b, lpar, logp, nullptr, nullptr, nullptr,
-                  nullptr, nullptr, flag, st),
+                  nullptr, nullptr, flag, st, nullptr),
           "lik");
     std::vector<int32_t> inv(count);
     check(hipMemcpyAsync(inv.data(), flag, count * sizeof(int32_t), hipMemcpyDeviceToHost, st), "D2H");

@@ -53,6 +53,22 @@ struct DevPlan {
   const int32_t* cinv;        // parent front position -> child row index, -1 if absent
 };
 
+// Independent blocks of a Laplace model (a heterogeneous batch of models in one
+// block-diagonal union): block q owns the latent indices [lseg[q], lseg[q + 1]), the
+// observations [oseg[q], oseg[q + 1]) (the model's internal order) and the pattern
+// entries [eseg[q], eseg[q + 1]); lblk / oblk / eblk give each element's block. A
+// per-point quantity of the model becomes per (point b, block q), index b nb + q.
+// nb = 1 with null maps and segments is a single model.
+struct LapBlocks {
+  int32_t nb = 1;
+  const int32_t* lblk = nullptr;
+  const int32_t* oblk = nullptr;
+  const int32_t* eblk = nullptr;
+  const int64_t* lseg = nullptr;
+  const int64_t* oseg = nullptr;
+  const int64_t* eseg = nullptr;
+};
+
 constexpr int kSmallFront = 64;  // fronts with m <= 64 run one workgroup each, in LDS
 // small-front classes by front size: one kernel instance per class, 4 m / 16 waves
 constexpr int kSmallClasses = 4;

@@ -76,6 +76,8 @@ class ModelDesc(C.Structure):
         ("capacity", C.c_int32),
         ("constr_kappa", C.c_double),
         ("order", C.c_int32),
+        ("n_blocks", C.c_int32),
+        ("block_ptr", _pi64),
     ]
 
 

@@ -8,6 +8,10 @@ factorisation, Newton loop, Laplace marginal likelihood), the marginal
 variances (inference.rs:1371) and, when coefficient derivatives are given, the
 gradient of the marginal likelihood from the selected inverses.
 
+With ``blocks`` the model is a heterogeneous batch of independent models in one
+block-diagonal union (gmrf_model_desc.n_blocks): every per-point quantity gains a
+block index.
+
 ``LogMarginal`` exposes that gradient to torch.autograd.
 """
 
@@ -54,12 +58,14 @@ class LaplaceModel:
 
     def __init__(self, n, indptr, indices, components, y, family="gaussian", link=None, A=None,
                  obs_a=None, obs_b=None, constraints=None, capacity=16, constraint_kappa=None,
-                 order="chooser"):
+                 order="chooser", blocks=None):
         """constraints = (A_c (k x n), e): hard constraints by projection. With
         constraint_kappa, the augmentation Q + kappa A_c^T A_c of the prior and the
         posterior is applied as an exact low-rank correction (the components then
         must not contain it); otherwise a kappa A_c^T A_c component carries it, as
-        the reference forms it. order: "chooser" (the reference's choice) or "amd"."""
+        the reference forms it. order: "chooser" (the reference's choice) or "amd".
+        blocks: nb + 1 latent offsets of independent models in a block-diagonal union
+        (observations in block order; see gmrf_model_desc.n_blocks)."""
         link = link or {"gaussian": "identity", "poisson": "log", "binomial": "logit",
                         "nbinomial": "log"}[family]
         self.n = int(n)
@@ -117,6 +123,14 @@ class LaplaceModel:
                 raise InvalidArgument("constraint_kappa needs constraints")
             d.constr_kappa = float(constraint_kappa)
         d.order = {"chooser": L.ORDER_CHOOSER, "amd": L.ORDER_AMD}[order]
+        self.nb = 1
+        if blocks is not None:
+            bp = _arr64(blocks)
+            keep.append(bp)
+            self.nb = len(bp) - 1
+            self.block_ptr = bp
+            d.n_blocks = self.nb
+            d.block_ptr = bp.ctypes.data_as(_p64)
         h = C.c_void_p()
         check(L.lib.gmrf_model_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -139,24 +153,29 @@ class LaplaceModel:
         numpy status / n_iter. ``out``: preallocated float64 CUDA tensors for some of
         mlik (B,), x (B, n), var (B, n), grad (B, m); reusing the same tensors call after
         call lets the library replay its captured graph of the Gaussian evaluation.
+        With blocks the per-point arrays carry the block after the point: coefs
+        (B, nb, K), lpar (B, nb), dcoefs (B, nb, m, K), dlog_lpar (B, nb, m), mlik
+        (B, nb), grad (B, nb, m), status and n_iter (B, nb).
         """
+        nb = self.nb
+        pp = (nb,) if nb > 1 else ()
         c = _dev(coefs)
-        if c.dim() == 1:
+        if c.dim() == 1 + len(pp):
             c = c.unsqueeze(0)
         B = c.shape[0]
-        if c.shape[1] != self.K:
-            raise InvalidArgument("coefs must be B x K")
+        if tuple(c.shape[1:]) != pp + (self.K,):
+            raise InvalidArgument("coefs must be B x K" if nb == 1 else "coefs must be B x n_blocks x K")
         if B > self.capacity:
             raise InvalidArgument("batch exceeds model capacity")
-        lp = _dev(lpar, (B,)) if lpar is not None else None
+        lp = _dev(lpar, (B,) + pp) if lpar is not None else None
         dc = _dev(dcoefs) if dcoefs is not None else None
         m = 0
         if dc is not None:
-            dc = dc.reshape(B, -1, self.K)
-            m = dc.shape[1]
-        dl = _dev(dlog_lpar, (B, m)) if dlog_lpar is not None else None
+            dc = dc.reshape((B,) + pp + (-1, self.K))
+            m = dc.shape[-2]
+        dl = _dev(dlog_lpar, (B,) + pp + (m,)) if dlog_lpar is not None else None
         given = dict(out or {})
-        shapes = {"mlik": (B,), "x": (B, self.n), "var": (B, self.n), "grad": (B, m)}
+        shapes = {"mlik": (B,) + pp, "x": (B, self.n), "var": (B, self.n), "grad": (B,) + pp + (m,)}
         for k, t in given.items():
             if k not in shapes or not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
                                        and t.is_contiguous() and tuple(t.shape) == shapes[k]):
@@ -172,8 +191,8 @@ class LaplaceModel:
             out["var"] = buf("var")
         if dc is not None:
             out["grad"] = buf("grad")
-        status = np.zeros(B, dtype=np.int32)
-        niter = np.zeros(B, dtype=np.int32)
+        status = np.zeros((B,) + pp, dtype=np.int32)
+        niter = np.zeros((B,) + pp, dtype=np.int32)
         a = EvalArgs()
         a.count, a.n_theta = B, m
         a.coefs = c.data_ptr()

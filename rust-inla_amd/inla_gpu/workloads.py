@@ -100,16 +100,15 @@ def values_on_pattern(m, indptr, indices):
     m.sum_duplicates()
     m.sort_indices()
     n = len(indptr) - 1
+    indptr = np.asarray(indptr, dtype=np.int64)
+    # entries by column-major key col * n + row: sorted in a sorted CSC pattern
+    pkey = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr)) * n + np.asarray(indices, dtype=np.int64)
+    mkey = np.repeat(np.arange(n, dtype=np.int64), np.diff(m.indptr)) * n + m.indices.astype(np.int64)
+    pos = np.searchsorted(pkey, mkey)
+    if np.any(pos >= len(pkey)) or np.any(pkey[np.minimum(pos, len(pkey) - 1)] != mkey):
+        raise ValueError("matrix has entries outside the pattern")
     out = np.zeros(len(indices))
-    for c in range(n):
-        a, b = indptr[c], indptr[c + 1]
-        ma, mb = m.indptr[c], m.indptr[c + 1]
-        if mb == ma:
-            continue
-        pos = np.searchsorted(indices[a:b], m.indices[ma:mb])
-        if np.any(pos >= b - a) or np.any(indices[a:b][np.minimum(pos, b - a - 1)] != m.indices[ma:mb]):
-            raise ValueError("matrix has entries outside the pattern")
-        out[a + pos] = m.data[ma:mb]
+    out[pos] = m.data
     return out
 
 
@@ -493,6 +492,99 @@ def bym2_hetero_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.
             m.y = rng.poisson(np.exp(intercept + u)).astype(float)
         models.append(m)
     return models
+
+
+PHI_CLAMP = 1e-6  # phi = clamp(sigmoid(theta), 1e-6, 1 - 1e-6) (structured.rs:335-337)
+
+
+@dataclass
+class Bym2PoissonBatch:
+    """k independent BYM2 + Poisson models as one block-diagonal Laplace model
+    (LaplaceModel(blocks=block_ptr)). Block q has the latent (beta_q, u_q), the
+    intercept first with the fixed prior precision int_prec, and one observation per
+    region, eta_i = beta_q + u_qi, y_i ~ Poisson(exp(eta_i)). theta of each model =
+    (log tau, logit phi) and Q_q(theta) = tau (1 - phi) I + tau phi Q*_q on u_q
+    (besag.rs:192-219): components (I on u, Q* on u, e_beta e_beta^T)."""
+
+    models: list
+    n: int
+    indptr: np.ndarray
+    indices: np.ndarray
+    components: np.ndarray  # (3, nnz)
+    A: sp.csc_matrix
+    y: np.ndarray
+    block_ptr: np.ndarray   # k + 1 latent offsets
+    obs_ptr: np.ndarray     # k + 1 observation offsets
+    int_prec: float
+
+    @property
+    def k(self):
+        return len(self.models)
+
+    def params(self, thetas):
+        """thetas (B, k, 2) -> (coefs (B, k, 3), dcoefs (B, k, 2, 3), None, None)."""
+        th = np.asarray(thetas, dtype=float).reshape(-1, self.k, 2)
+        tau = np.exp(th[..., 0])
+        s = 1.0 / (1.0 + np.exp(-th[..., 1]))
+        phi = np.clip(s, PHI_CLAMP, 1.0 - PHI_CLAMP)
+        dphi = np.where((s > PHI_CLAMP) & (s < 1.0 - PHI_CLAMP), s * (1.0 - s), 0.0)
+        coefs = np.stack([tau * (1.0 - phi), tau * phi, np.full_like(tau, self.int_prec)], axis=-1)
+        dco = np.zeros(th.shape[:2] + (2, 3))
+        dco[..., 0, 0], dco[..., 0, 1] = tau * (1.0 - phi), tau * phi
+        dco[..., 1, 0], dco[..., 1, 1] = -tau * dphi, tau * dphi
+        return coefs, dco, None, None
+
+    def block_prior(self, q, theta) -> sp.csc_matrix:
+        """The block's own Q(theta) ((n_q + 1) x (n_q + 1), sorted CSC)."""
+        c, _, _, _ = self.params(np.broadcast_to(np.asarray(theta, dtype=float), (self.k, 2))[None])
+        a, b = self.block_ptr[q], self.block_ptr[q + 1]
+        e0, e1 = self.indptr[a], self.indptr[b]
+        ip = self.indptr[a:b + 1] - e0
+        v = c[0, q] @ self.components[:, e0:e1]
+        Q = sp.csc_matrix((v, self.indices[e0:e1] - a, ip), shape=(b - a, b - a))
+        Q.sort_indices()
+        return Q
+
+    def block_A(self, q) -> sp.csc_matrix:
+        a, b = self.block_ptr[q], self.block_ptr[q + 1]
+        o0, o1 = self.obs_ptr[q], self.obs_ptr[q + 1]
+        return sp.csc_matrix(self.A[o0:o1, a:b])
+
+    def block_y(self, q) -> np.ndarray:
+        return self.y[self.obs_ptr[q]:self.obs_ptr[q + 1]]
+
+
+def bym2_poisson_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.5, intercept=1.0,
+                       int_prec=1e-3, models=None):
+    """The heterogeneous Laplace batch: bym2_hetero_batch's k models (sizes uniform in
+    [nmin, nmax], y ~ Poisson(exp(intercept + u))) with an intercept each."""
+    models = models if models is not None else bym2_hetero_batch(k, nmin, nmax, seed, tau, phi, intercept)
+    ips, ixs, comps, As = [], [], [], []
+    bp, op = [0], [0]
+    for mdl in models:
+        nq = mdl.n
+        N = nq + 1
+        E0 = sp.csc_matrix(([1.0], ([0], [0])), shape=(N, N))
+        Iu = sp.block_diag([sp.csc_matrix((1, 1)), sp.identity(nq)], format="csc")
+        Ru = sp.block_diag([sp.csc_matrix((1, 1)), mdl.R * mdl.scale], format="csc")
+        Aq = sp.csc_matrix(sp.hstack([sp.csc_matrix(np.ones((nq, 1))), sp.identity(nq)]))
+        ip, ix = union_pattern([E0, Iu, Ru, sp.csc_matrix(Aq.T @ Aq)])
+        ips.append(ip)
+        ixs.append(ix)
+        comps.append(np.stack([values_on_pattern(Iu, ip, ix), values_on_pattern(Ru, ip, ix),
+                               values_on_pattern(E0, ip, ix)]))
+        As.append(Aq)
+        bp.append(bp[-1] + N)
+        op.append(op[-1] + nq)
+    e_off = np.cumsum([0] + [len(ix) for ix in ixs])
+    indptr = np.concatenate([np.array([0], dtype=np.int64)] +
+                            [ip[1:] + e_off[q] for q, ip in enumerate(ips)]).astype(np.int64)
+    indices = np.concatenate([ix + bp[q] for q, ix in enumerate(ixs)]).astype(np.int64)
+    A = sp.block_diag(As, format="csc")
+    A.sort_indices()
+    y = np.concatenate([m.y for m in models])
+    return Bym2PoissonBatch(models, bp[-1], indptr, indices, np.concatenate(comps, axis=1), A, y,
+                            np.array(bp, dtype=np.int64), np.array(op, dtype=np.int64), int_prec)
 
 
 @dataclass
