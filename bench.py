@@ -132,6 +132,8 @@ class OracleObjective:
             out = list(ex.map(one, thetas))
         self.calls += 1
         self.points += len(thetas)
+        if self.calls % 10 == 0:
+            log(f"[cpu fit] {self.calls} calls, {self.points} theta-points")
         res = {"g": np.array([o[0] for o in out])}
         if want_x:
             res["x"] = np.array([o[1] for o in out])
