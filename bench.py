@@ -603,6 +603,80 @@ def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
                               "Q_post at tau=0.5, kappa=3", probes)
 
 
+def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
+    """Full-size SPDE x RW2 negative binomial Laplace evaluation (n = 1,440,600, 500,000
+    observations, size 5) at one theta-point with the adjoint gradient and the marginal
+    variances, in the lean form (prior factorised and inverted alone first: one factor
+    and one selected inverse on the device at a time). Synthetic counts from a smooth
+    space-time field (no draw from Q at this size). With fd, the gradient's first
+    component against central differences (step 1e-4) of two further evaluations."""
+    import numpy as np
+    import torch
+
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    t0 = time.perf_counter()
+    wl = W.spde_rw2_workload(nx=nx, T=T, n_obs=n_obs, field="smooth")
+    g = wl.gmrf
+    log(f"[c5 laplace] workload n={g.n} nnz={len(g.indices)} in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="nbinomial", A=wl.A,
+                         obs_a=wl.obs_a, obs_b=wl.obs_b, capacity=1, lean=True, order=order)
+    t_setup = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    log(f"[c5 laplace] model set up in {t_setup:.1f} s")
+    th = wl.theta_true.copy()
+    c, dc, _, _ = wl.params(th[None])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = model.eval(c, dcoefs=dc)
+    torch.cuda.synchronize()
+    t_eval = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    free2, _ = torch.cuda.mem_get_info()
+    mlik = float(r["mlik"][0].item())
+    grad = r["grad"][0].cpu().numpy()
+    var = r["var"][0].cpu().numpy()
+    log(f"[c5 laplace] one theta-point with gradient: {t_eval:.1f} s, {int(r['n_iter'][0])} Newton steps, "
+        f"mlik {mlik:.6f}")
+    res = {"value": 1.0 / t_eval, "unit": "theta-points/s", "s_per_point": t_eval,
+           "newton_iters": int(r["n_iter"][0]), "status_ok": bool(r["status"][0] == 0),
+           "n": int(g.n), "n_obs": int(n_obs), "nnz_q": int(len(g.indices)), "order": order,
+           "mlik": mlik, "grad": [float(v) for v in grad],
+           "var_min_max": [float(var.min()), float(var.max())], "setup_s": t_setup,
+           "device_free_gb_before_eval": free / 1e9, "device_free_gb_after_eval": free2 / 1e9,
+           "device_total_gb": total / 1e9,
+           "work": "prior LDLT + selected inversion, Newton to the mode (one LDLT + solve per step), "
+                   "posterior selected inversion, mlik, variances, adjoint gradient (one extra solve)",
+           "workload": f"spde {nx}x{nx} x rw2(T={T}), n={g.n}, {n_obs} NB(size 5) obs from a smooth field, "
+                       "lean evaluation at theta=(log 0.5, log 3)"}
+    if fd:
+        # central differences at two steps and their Richardson extrapolation
+        # (g(h) = g + c h^2 + ...): at this size the mlik of an evaluation carries a
+        # rounding noise of order 1e-3 (a sum over 1.44M log pivots and 500k terms),
+        # which a step of 1e-4 would amplify to O(10)
+        fdres = []
+        for l in range(len(th)):
+            est = []
+            for h in (1e-2, 5e-3):
+                vals = []
+                for sgn in (h, -h):
+                    t = th.copy()
+                    t[l] += sgn
+                    cc, _, _, _ = wl.params(t[None])
+                    t0 = time.perf_counter()
+                    rr = model.eval(cc, want_x=False, want_var=False)
+                    vals.append(float(rr["mlik"][0].item()))
+                    log(f"[c5 laplace] FD evaluation theta[{l}] {sgn:+.0e}: {time.perf_counter() - t0:.1f} s")
+                est.append((vals[0] - vals[1]) / (2 * h))
+            rich = (4.0 * est[1] - est[0]) / 3.0
+            fdres.append({"fd_h1e-2": est[0], "fd_h5e-3": est[1], "richardson": rich,
+                          "adjoint": float(grad[l]), "rel_err_vs_richardson": abs(grad[l] - rich) / max(abs(rich), 1.0)})
+        res["gradient_vs_fd"] = fdres
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,6 +692,9 @@ def main():
                     help="add the full-size Besag x AR1 batched evaluation and fit comparison")
     ap.add_argument("--c5", action="store_true",
                     help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
+    ap.add_argument("--c5-laplace", action="store_true",
+                    help="add the full-size SPDE x RW2 negative binomial Laplace evaluation (lean; minutes)")
+    ap.add_argument("--c5-order", default="chooser", choices=["chooser", "amd"])
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch schedule")
     ap.add_argument("--cpu-fit", action="store_true",
                     help="add the reference's fit sequence measured on the oracle (host cores; minutes)")
@@ -819,6 +896,10 @@ def main():
         log("== full-size SPDE x RW2 (n = 1.44M): factorise, solve, selected inversion")
         res["c5"] = c5_solver_line()
         log(json.dumps(res["c5"]))
+    if args.c5_laplace:
+        log("== full-size SPDE x RW2 negative binomial Laplace (n = 1.44M, lean)")
+        res["c5_laplace"] = c5_laplace_line(order=args.c5_order)
+        log(json.dumps(res["c5_laplace"]))
     if args.cpu_fit:
         log("== the reference's fit sequence on the oracle (host cores)")
         res["cpu_fit"] = cpu_fit_line(wl, wl.theta_true + np.array([0.5, -0.3, 0.3]))

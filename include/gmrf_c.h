@@ -285,6 +285,14 @@ typedef struct gmrf_model_desc {
                                 No reference counterpart: the reference fits such models one
                                 at a time (crates/inla_stats/src/inference.rs:776-975 each) */
   const int64_t* block_ptr;  /* nb + 1 latent offsets (host), block_ptr[0] = 0, [nb] = n */
+  int32_t lean;              /* 0: the prior and posterior factors of the capacity points share one
+                                batch of 2 capacity matrices. 1: the prior is factorised (and, for the
+                                gradient, inverted) first and alone in the posterior's slots, keeping
+                                only its log determinant, status and traces tr(Sigma_prior Q_k); the
+                                batch then holds capacity matrices (half the device memory), for
+                                models whose factor fills most of the device. Same results; no graph
+                                capture. No reference counterpart (the reference factorises the
+                                prior in its own solver, inference.rs:827-836) */
 } gmrf_model_desc;
 
 typedef struct gmrf_eval_args {

@@ -11,6 +11,7 @@
 // no reference counterpart, the reference differentiates by finite differences,
 // crates/inla_math/src/hyper_opt.rs:197-259).
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 #include "runtime.hpp"
@@ -61,6 +62,12 @@ hipError_t lap_gauss_finish(int, int, int, int, const int32_t*, const int32_t*, 
 namespace {
 constexpr double kPriorJitter = 1e-12;  // inference.rs:831-834
 constexpr size_t kGraphCache = 4;       // captured Gaussian evaluations kept per model
+// GMRF_TRACE_NEWTON set: one line on stderr per Newton system and per phase of the
+// general evaluation (long evaluations of large models report their progress)
+bool trace_newton() {
+  static const bool on = std::getenv("GMRF_TRACE_NEWTON") != nullptr;
+  return on;
+}
 enum { RED_MAXABS = 0, RED_SUM = 1, RED_DOT = 2, RED_MAXDIFF = 3 };
 }  // namespace
 
@@ -179,11 +186,15 @@ struct gmrf_model {
   // index b nb + q; the factorisations, solves and selected inversions run on the
   // union, the Newton steps, convergence, mlik and gradient per block.
   int nb = 1;
+  // Lean: the prior is factorised (and, for the gradient, inverted) alone before the
+  // Newton loop, in the posterior's batch slots; only its log determinant, status and
+  // traces are kept, so the batch holds cap matrices instead of 2 cap.
+  bool lean = false;
   LapBlocks blk;                    // device maps and segments (nb > 1)
   int64_t* fseg_ptr = nullptr;      // [nb + 1] factor positions of each block
   int32_t* fseg_pos = nullptr;      // [n]
-  double* bld = nullptr;            // [2 cap nb] per-block log det, posterior slots then prior
-  int32_t* bst = nullptr;           // [2 cap nb] per-block factor status flags
+  double* bld = nullptr;            // [2 cap nb] log det per (point, block): posterior, then prior
+  int32_t* bst = nullptr;           // [2 cap nb] factor status flags, same layout (blocks or lean)
   const LapBlocks* blocks() const { return nb > 1 ? &blk : nullptr; }
   int last_count = 0;              // points of the last evaluation (0: none yet)
   hipEvent_t gev_in = nullptr, gev_out = nullptr;
@@ -604,6 +615,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->nobs = d->n_obs;
     M->kc = d->n_constr;
     M->nb = nb;
+    M->lean = d->lean != 0;
     std::vector<int64_t> bp(nb + 1, 0);  // latent segment of each block
     bp[nb] = n;
     if (nb > 1) std::copy(d->block_ptr, d->block_ptr + nb + 1, bp.begin());
@@ -638,7 +650,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     const Symbolic& S = M->sym->S;
     const int64_t nnz = S.nnz_full;
     M->nnz = nnz;
-    M->bt.reset(new gmrf_batch(M->sym, 2 * M->cap));
+    M->bt.reset(new gmrf_batch(M->sym, M->lean ? M->cap : 2 * M->cap));
     // ---- pattern arrays
     std::vector<int64_t> ip(d->indptr, d->indptr + n + 1), ix(d->indices, d->indices + nnz);
     std::vector<int32_t> qidx(nnz), qrow(nnz), qcol(nnz);
@@ -759,6 +771,8 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
       for (int64_t p = 0; p < n; ++p) fpos[fill[lblk[S.perm[p]]]++] = (int32_t)p;
       M->fseg_ptr = M->arena.upload(fptr);
       M->fseg_pos = M->arena.upload(fpos);
+    }
+    if (nb > 1 || M->lean) {
       M->bld = M->arena.alloc<double>((size_t)2 * M->cap * nb);
       M->bst = M->arena.alloc<int32_t>((size_t)2 * M->cap * nb);
     }
@@ -923,7 +937,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     M->last_count = B;
-    if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && a->max_iter > 0) {
+    if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && !M->lean && a->max_iter > 0) {
       eval_gauss(M, a, B);
       return GMRF_OK;
     }
@@ -947,6 +961,42 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     const bool lr = M->kappa > 0.0;
     if (lr) check(lap_ground(B, M->kc, nnz, M->lr.gidx, M->vprior, vprior_f, M->lr.alpha, st), "ground");
     bool prior_pending = true;
+    const DevPlan& plan = M->sym->plan;
+    // log det and status of the factors in slots [0, B) into bld / bst at offset off
+    // (per block with blocks)
+    auto keep_logdet = [&](size_t off) {
+      if (bk) {
+        check(launch_logdet_seg(plan, B, bt->L, M->fseg_ptr, M->fseg_pos, nb, M->bld + off, M->bst + off, st),
+              "logdet_seg");
+      } else {
+        check(hipMemcpyAsync(M->bld + off, bt->d_logdet, B * sizeof(double), hipMemcpyDeviceToDevice, st),
+              "copy logdet");
+        check(hipMemcpyAsync(M->bst + off, bt->d_status, B * sizeof(int32_t), hipMemcpyDeviceToDevice, st),
+              "copy status");
+      }
+    };
+    if (M->lean) {
+      // the prior alone, in the posterior's slots: log det, status and (for the
+      // gradient) tr(Sigma_prior Q_k) kept; the slots are then free for the Newton loop
+      bt->factorize(B, vprior_f, nnz, 0);
+      if (lr) lowrank_factor(M, B, 0, true);
+      keep_logdet((size_t)V);
+      if (trace_newton()) {
+        check(hipStreamSynchronize(st), "sync");
+        std::fprintf(stderr, "[gmrf] lean: prior factorised\n");
+      }
+      if (want_grad) {
+        bt->selinv(B, 0);
+        check(lap_contract(B, K, nnz, M->comps, bt->S, plan.l_size, plan.slot_full, nullptr, nullptr, nullptr,
+                           nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
+              "contract prior");
+        if (trace_newton()) {
+          check(hipStreamSynchronize(st), "sync");
+          std::fprintf(stderr, "[gmrf] lean: prior inverted, traces formed\n");
+        }
+      }
+      prior_pending = false;
+    }
     // ---- Newton (inference.rs:842-942)
     check(hipMemsetAsync(M->x, 0, (size_t)B * n * sizeof(double), st), "zero x");
     std::vector<int> state(V, 0);  // 0 active, 1 confirming, 2 done, 3 failed
@@ -954,7 +1004,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     std::vector<double> hred(4 * V), halpha(V);
     std::vector<int32_t> hbad(V);
     const bool gauss_id = M->fam == 0 && M->link == 0;
-    const int sel_count = want_grad ? 2 * B : (a->var ? B : 0);
+    const int sel_count = (want_grad && !M->lean) ? 2 * B : ((want_grad || a->var) ? B : 0);
     bool sel_started = false;
     // final_factor: every member's posterior factor of this system is its last (the
     // prior's, from the first system, is never refactorised), so their selected
@@ -1003,13 +1053,20 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         std::memcpy(hred.data(), pb, V * sizeof(double));
         std::memcpy(hbad.data(), pb + V * sizeof(double), V * sizeof(int32_t));
       }
+      if (trace_newton()) {
+        double mx = 0.0;
+        for (int v = 0; v < V; ++v) mx = std::max(mx, hred[v]);
+        std::fprintf(stderr, "[gmrf] Newton system: %d factors, max |step| %.3e\n", nf, mx);
+      }
     };
     if (gauss_id) {
       if (a->max_iter == 0) {
         for (int v = 0; v < V; ++v) status[v] = GMRF_SOLVE_FAILURE, state[v] = 3;
-        bt->factorize(B, vprior_f, nnz, B);
-        if (lr) lowrank_factor(M, B, B, true);
-        prior_pending = false;
+        if (prior_pending) {
+          bt->factorize(B, vprior_f, nnz, B);
+          if (lr) lowrank_factor(M, B, B, true);
+          prior_pending = false;
+        }
       } else {
         newton_system(true);
         for (int v = 0; v < V; ++v) {
@@ -1110,13 +1167,16 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d, M->q_long.n, st), "Q x*");
     check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st, nb, lseg), "quad");
     check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + V, st, nb, oseg), "loglik");
-    // per-block log determinants of the posterior (slots [0, B)) and prior ([B, 2B))
-    // factors, in the layout of the whole-matrix ones: [0, V) posterior, [V, 2V) prior
-    if (bk)
-      check(launch_logdet_seg(M->sym->plan, 2 * B, bt->L, M->fseg_ptr, M->fseg_pos, nb, M->bld, M->bst, st),
+    // log determinants of the posterior and prior factors, [0, V) posterior, [V, 2V)
+    // prior: per block from slots [0, 2B), or (lean) the posterior's from slots [0, B)
+    // beside the prior's kept before the Newton loop
+    if (M->lean)
+      keep_logdet(0);
+    else if (bk)
+      check(launch_logdet_seg(plan, 2 * B, bt->L, M->fseg_ptr, M->fseg_pos, nb, M->bld, M->bst, st),
             "logdet_seg");
-    const double* d_ld = bk ? M->bld : bt->d_logdet;
-    const int32_t* d_fs = bk ? M->bst : bt->d_status;
+    const double* d_ld = (bk || M->lean) ? M->bld : bt->d_logdet;
+    const int32_t* d_fs = (bk || M->lean) ? M->bst : bt->d_status;
     if (lr) check(lap_constr_dot(B, M->kc, n, M->ca, M->x, M->cres, st), "C x*");
     // ---- log-determinants, factor status and the mlik terms in one host round trip
     std::vector<double> ld(2 * V), cx(lr ? (size_t)B * M->kc : 0);
@@ -1187,9 +1247,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col,
                          M->x, M->x, n, M->tr_post, M->qx, M->cpart, st, bk),
             "contract post");
-      check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full,
-                         nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
-            "contract prior");
+      if (!M->lean)  // (lean: formed before the Newton loop)
+        check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full,
+                           nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
+              "contract prior");
       // v_i = (A Sigma_post A^T)_ii
       check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
       if (lr) {  // v_i -= (A Y Kinv Y^T A^T)_ii
@@ -1303,7 +1364,7 @@ gmrf_status gmrf_model_autotune(gmrf_model* M, double* default_ms, double* tuned
   } catch (...) {
     return from_exception();
   }
-  return autotune_batch(M->bt.get(), 2 * M->last_count, M->vals, M->nnz, default_ms, tuned_ms);
+  return autotune_batch(M->bt.get(), (M->lean ? 1 : 2) * M->last_count, M->vals, M->nnz, default_ms, tuned_ms);
 }
 
 gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,

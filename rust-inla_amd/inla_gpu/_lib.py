@@ -78,6 +78,7 @@ class ModelDesc(C.Structure):
         ("order", C.c_int32),
         ("n_blocks", C.c_int32),
         ("block_ptr", _pi64),
+        ("lean", C.c_int32),
     ]
 
 

@@ -58,14 +58,16 @@ class LaplaceModel:
 
     def __init__(self, n, indptr, indices, components, y, family="gaussian", link=None, A=None,
                  obs_a=None, obs_b=None, constraints=None, capacity=16, constraint_kappa=None,
-                 order="chooser", blocks=None):
+                 order="chooser", blocks=None, lean=False):
         """constraints = (A_c (k x n), e): hard constraints by projection. With
         constraint_kappa, the augmentation Q + kappa A_c^T A_c of the prior and the
         posterior is applied as an exact low-rank correction (the components then
         must not contain it); otherwise a kappa A_c^T A_c component carries it, as
         the reference forms it. order: "chooser" (the reference's choice) or "amd".
         blocks: nb + 1 latent offsets of independent models in a block-diagonal union
-        (observations in block order; see gmrf_model_desc.n_blocks)."""
+        (observations in block order; see gmrf_model_desc.n_blocks). lean: the prior is
+        factorised alone before the Newton loop, so the device holds capacity factors
+        instead of 2 capacity (gmrf_model_desc.lean)."""
         link = link or {"gaussian": "identity", "poisson": "log", "binomial": "logit",
                         "nbinomial": "log"}[family]
         self.n = int(n)
@@ -131,6 +133,7 @@ class LaplaceModel:
             self.block_ptr = bp
             d.n_blocks = self.nb
             d.block_ptr = bp.ctypes.data_as(_p64)
+        d.lean = 1 if lean else 0
         h = C.c_void_p()
         check(L.lib.gmrf_model_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -747,35 +747,48 @@ def besag_ar1_workload(nr=100, T=20, seed=20241206, tau=2.0, rho=0.7, trials=10,
                         np.array([math.log(tau), th_rho]), params, log_prior)
 
 
-def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3):
+def _space_time_projector(nx, T, n_obs, rng):
+    """Barycentric projector of n_obs uniform points of uniform time slices of the
+    nx x nx lattice (group-major latent index t nx^2 + s): (A (n_obs x nx^2 T, CSC),
+    points, time slots)."""
+    ns = nx * nx
+    tslot = rng.integers(0, T, size=n_obs)
+    pts = rng.uniform(size=(n_obs, 2))
+    As = sp.csr_matrix(lattice_projector(nx, nx, pts))
+    cnt = np.diff(As.indptr)
+    cols = As.indices.astype(np.int64) + np.repeat(tslot.astype(np.int64) * ns, cnt)
+    A = sp.csc_matrix(sp.csr_matrix((As.data, cols, As.indptr), shape=(n_obs, ns * T)))
+    A.sort_indices()
+    return A, pts, tslot
+
+
+def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3, field="sample"):
     """SPDE alpha=2 (nx x nx lattice) x RW2 over T time steps, group-major; negative
     binomial (size) observations uniform in space x time through the barycentric
-    projector of their time slice: Q = Q_spde(tau, kappa) x (R_rw2 + eps I)."""
+    projector of their time slice: Q = Q_spde(tau, kappa) x (R_rw2 + eps I).
+    field="sample": eta = 0.5 + A x with x drawn from Q(theta_true) (test sizes);
+    field="smooth": eta = 0.5 + 0.6 sin(2 pi s1) cos(2 pi s2) + 0.3 cos(2 pi t / T), no
+    draw (the full size, where a host factorisation for the draw is out of reach)."""
     g_s, _, _ = spde_components(nx, nx)
     Rt = sp.csc_matrix(rw2_structure(T) + eps * sp.identity(T))
     mats = [sp.kron(Rt, sp.csc_matrix((g_s.components[k], g_s.indices, g_s.indptr), shape=(g_s.n, g_s.n)),
                     format="csc") for k in range(3)]
     indptr, indices = union_pattern(mats)
     comps = np.stack([values_on_pattern(m_, indptr, indices) for m_ in mats])
+    del mats
     n = g_s.n * T
     g = AffineGmrf(n, indptr, indices, comps, ["Rt x C0", "Rt x G1", "Rt x G1C0invG1"])
     rng = np.random.default_rng(seed)
-    tslot = rng.integers(0, T, size=n_obs)
-    As = lattice_projector(nx, nx, rng.uniform(size=(n_obs, 2)))
-    As = sp.csr_matrix(As)
-    rows, cols, vals = [], [], []
-    for i in range(n_obs):
-        a, b = As.indptr[i], As.indptr[i + 1]
-        rows += [i] * (b - a)
-        cols += list(tslot[i] * g_s.n + As.indices[a:b])
-        vals += list(As.data[a:b])
-    A = sp.csc_matrix((vals, (rows, cols)), shape=(n_obs, n))
-    A.sort_indices()
     # the pattern must contain A^T A (neighbours within a triangle of one time slice: already in G1)
+    A, pts, tslot = _space_time_projector(nx, T, n_obs, rng)
     th = np.array([math.log(0.5), math.log(3.0)])
-    c, _ = spde_coefs(th[0], th[1])
-    x = sample_gmrf(g.csc(c), rng)
-    mu = np.exp(0.5 + A @ x)
+    if field == "sample":
+        c, _ = spde_coefs(th[0], th[1])
+        x = sample_gmrf(g.csc(c), rng)
+        mu = np.exp(0.5 + A @ x)
+    else:
+        mu = np.exp(0.5 + 0.6 * np.sin(2 * math.pi * pts[:, 0]) * np.cos(2 * math.pi * pts[:, 1])
+                    + 0.3 * np.cos(2 * math.pi * tslot / T))
     y = rng.negative_binomial(size, size / (size + mu)).astype(float)
 
     def params(thetas):
