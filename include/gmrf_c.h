@@ -312,6 +312,14 @@ typedef struct gmrf_eval_args {
   /* outputs, host pointers */
   int32_t* status;           /* count: GMRF_OK or the failure of that point */
   int32_t* n_iter;           /* count: Newton iterations used */
+  /* outputs, device pointers, count x n_obs in the caller's observation order (NULL to
+     skip): the linear predictor at the mode, eta = A x* (x* without A), and its variance
+     in the reference's diagonal form sum_j A_ij^2 diag(Q_post^-1)_j (x's variances
+     without A): the cond_eta / cond_eta_var of crates/inla_stats/src/inference.rs:1371-1379
+     (predictor_variances_diag, crates/inla_math/src/design.rs:198-210) that DIC, WAIC and
+     CPO/PIT take. Requesting them runs the general (uncaptured) evaluation. */
+  double* eta;
+  double* eta_var;
 } gmrf_eval_args;
 
 gmrf_status gmrf_model_create(const gmrf_model_desc* desc, gmrf_model** out);
@@ -337,6 +345,14 @@ void* gmrf_model_stream(gmrf_model* model);
 gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
                             const double* eta, const double* y, const double* obs_a,
                             const double* obs_b, const double* lpar, double* logp, void* stream);
+/* gmrf_obs_loglik with the derivatives eval_likelihood returns: grad = d log pi / d eta
+ * and neg_hess = -d2 log pi / d eta2 (the Newton weight of the Laplace evaluation; Fisher
+ * scoring where the reference scores, inference.rs:170-311), as
+ * check_cpo_failure_for_obs takes them (crates/inla_stats/src/model_selection.rs:328-396).
+ * Any of logp, grad, neg_hess may be NULL. Same errors as gmrf_obs_loglik. */
+gmrf_status gmrf_obs_eval(int32_t family, int32_t link, int32_t count, int64_t n_obs, const double* eta,
+                          const double* y, const double* obs_a, const double* obs_b, const double* lpar,
+                          double* logp, double* grad, double* neg_hess, void* stream);
 gmrf_status gmrf_model_set_stream(gmrf_model* model, void* stream, int32_t use_given);
 /* gmrf_batch_autotune for the model's batch (the prior and posterior matrices of the
  * last gmrf_model_eval, 2 count matrices) on that evaluation's values; call after an

@@ -23,6 +23,10 @@
 //    likelihoods of inference.rs:170-311; pinned by the Gaussian closed form,
 //    stationarity of every family's mode, and a dense brute force of the
 //    constrained (kappa-augmented, projected) Gaussian case.
+//  * compute_cpo_pit with check_cpo_failure_for_obs and the CDFs of observation_cdf
+//    (crates/inla_stats/src/model_selection.rs:222-511); pinned by the reference's own
+//    tests there (failure cases, CDF known answers), scipy's Poisson / binomial CDFs,
+//    the erfc identity of the normal_cdf formula, and the harmonic mean of CPO.
 //  * ccd_design / grid_design / invert_symmetric_matrix / jacobi_eigen: restated
 //    from crates/inla_math/src/integration.rs; pinned by what the mathematics fixes
 //    (the CCD mixture of a standard normal has identity covariance, the grid of a
@@ -74,6 +78,11 @@ enum Plant {
   P_NNZ_L,           // diagonal not counted
   P_PERM,            // inverse permutation returned
   P_D,               // D returned in original instead of factor order
+  P_CPO_FAIL,        // CPO_DIFF border test at 1 instead of 3
+  P_CPO,             // max shift not restored in 1 / CPO
+  P_NORMAL_CDF,      // exp(-x^2) instead of exp(-x^2 / 2)
+  P_POISSON_CDF,     // sum stops one term early
+  P_BINOMIAL_CDF,    // p and 1 - p swapped
 };
 int g_plant = P_NONE;
 
@@ -642,6 +651,154 @@ int orc_dense_ldlt(i64 n, const double* a, double* logdet, double* diag_inv) {
 }
 
 double orc_log_gamma(double z) { return lanczos_log_gamma(z); }
+
+// ---- CPO / PIT (crates/inla_stats/src/model_selection.rs:222-511)
+// The normal CDF formula of model_selection.rs:450-465 (the Abramowitz-Stegun 7.1.26
+// rational form applied with exp(-x^2 / 2), as the reference writes it).
+double orc_normal_cdf(double x) {
+  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+               A5 = 1.061405429, P = 0.3275911;
+  const double sign = x < 0.0 ? -1.0 : 1.0, ax = std::fabs(x);
+  const double t = 1.0 / (1.0 + P * ax);
+  const double e = g_plant == P_NORMAL_CDF ? std::exp(-ax * ax) : std::exp(-ax * ax / 2.0);
+  const double y = 1.0 - (((((A5 * t + A4) * t) + A3) * t + A2) * t + A1) * t * e;
+  return 0.5 * (1.0 + sign * y);
+}
+
+// Pr(X <= k), X ~ Poisson(lambda), by direct summation (model_selection.rs:470-485)
+double orc_poisson_cdf(double lambda, i64 k) {
+  if (lambda <= 0.0) return 1.0;
+  double cdf = 0.0, lp = -lambda;
+  const i64 last = g_plant == P_POISSON_CDF ? k - 1 : k;
+  for (i64 j = 0; j <= last; ++j) {
+    cdf += std::exp(lp);
+    if (j < k) lp += std::log(lambda) - std::log((double)(j + 1));
+  }
+  return std::min(cdf, 1.0);
+}
+
+// Pr(X <= k), X ~ Binomial(n, p) (model_selection.rs:488-503)
+double orc_binomial_cdf(i64 n, double p, i64 k) {
+  if (p <= 0.0) return 1.0;
+  if (p >= 1.0) return k >= n ? 1.0 : 0.0;
+  const double q = g_plant == P_BINOMIAL_CDF ? p : 1.0 - p, pp = g_plant == P_BINOMIAL_CDF ? 1.0 - p : p;
+  double cdf = 0.0;
+  for (i64 j = 0; j <= k; ++j) {
+    const double lc = lanczos_log_gamma((double)n + 1.0) - lanczos_log_gamma((double)j + 1.0) -
+                      lanczos_log_gamma((double)(n - j) + 1.0);
+    cdf += std::exp(lc + (double)j * std::log(pp) + (double)(n - j) * std::log(q));
+  }
+  return std::min(cdf, 1.0);
+}
+
+// check_cpo_failure_for_obs (model_selection.rs:327-396): 1 when unreliable
+int orc_cpo_failure(int fam, int link, double y, double a, double b, double mu, double var) {
+  if (!(var > 0.0) || !std::isfinite(var)) return 1;
+  const Eval ev = eval_lik(fam, link, mu, y, a, b);
+  if (!ev.ok) return 1;
+  const double g = ev.grad, c = -ev.hess;
+  const double tau_loo = 1.0 / var - c;
+  if (!(tau_loo > 0.0) || !std::isfinite(tau_loo)) return 1;
+  const double var_loo = 1.0 / tau_loo, sd = std::sqrt(var_loo);
+  const double mu_loo = var_loo * (mu / var - c * mu + g);
+  const double steps[5] = {-3.0, -1.5, 0.0, 1.5, 3.0};
+  double d[5];
+  for (int j = 0; j < 5; ++j) {
+    const double x = mu_loo + steps[j] * sd;
+    const Eval e = eval_lik(fam, link, x, y, a, b);
+    if (!e.ok) return 1;
+    d[j] = -0.5 * (x - mu) * (x - mu) / var + e.logp;
+  }
+  bool inc = true, dec = true;
+  for (int j = 1; j < 5; ++j) {
+    if (d[j] < d[j - 1]) inc = false;
+    if (d[j] > d[j - 1]) dec = false;
+  }
+  if (inc || dec) return 1;
+  double mx = -INFINITY;
+  for (int j = 0; j < 5; ++j) mx = std::max(mx, d[j]);
+  const double diff = g_plant == P_CPO_FAIL ? 1.0 : 3.0;  // CPO_DIFF
+  return (mx - d[0] < diff || mx - d[4] < diff) ? 1 : 0;
+}
+
+// observation_cdf (model_selection.rs:398-447) for the four families: NAN for None
+// (negative binomial); the link is taken as identity / log / logit as the reference does
+double obs_cdf(int fam, double eta, double variance, double y, double a) {
+  if (fam == 0) {
+    const double tv = 1.0 / a + variance;
+    if (tv <= 0.0) return NAN;
+    return orc_normal_cdf((y - eta) / std::sqrt(tv));
+  }
+  if (fam == 1) {
+    const double lambda = a * std::exp(eta);
+    if (lambda < 0.0 || !std::isfinite(lambda)) return NAN;
+    const i64 yi = (i64)std::floor(y);
+    if (yi < 0) return 0.0;
+    return orc_poisson_cdf(lambda, yi);
+  }
+  if (fam == 2) {
+    const double p = eta >= 0.0 ? 1.0 / (1.0 + std::exp(-eta)) : std::exp(eta) / (1.0 + std::exp(eta));
+    const i64 n = (i64)a;
+    const i64 yi = (i64)std::floor(y);
+    if (yi < 0) return 0.0;
+    return orc_binomial_cdf(n, p, std::min(yi, n));
+  }
+  return NAN;
+}
+
+// compute_cpo_pit (model_selection.rs:222-325): K nodes x n observations, eta and var
+// row-major [K][n], normalised weights w[K]; observation i has family fam with a[i]
+// (Gaussian precision / exposure / trials) and b[i] (NB size). cpo[i] / pit[i] are NAN
+// where the reference returns None. Returns the failure count.
+i64 orc_cpo_pit(int fam, int link, int K, i64 n, const double* eta, const double* var, const double* w,
+                const double* y, const double* a, const double* b, double* cpo, double* pit) {
+  i64 fails = 0;
+  for (i64 i = 0; i < n; ++i) {
+    cpo[i] = NAN;
+    pit[i] = NAN;
+    const double bi = b ? b[i] : 0.0;
+    bool failed = false;
+    for (int k = 0; k < K && !failed; ++k)
+      failed = orc_cpo_failure(fam, link, y[i], a[i], bi, eta[(i64)k * n + i], var[(i64)k * n + i]) != 0;
+    if (failed) {
+      ++fails;
+      continue;
+    }
+    std::vector<double> ll;
+    bool bad = false;
+    for (int k = 0; k < K; ++k) {
+      const Eval e = eval_lik(fam, link, eta[(i64)k * n + i], y[i], a[i], bi);
+      if (!e.ok) {
+        bad = true;
+        break;
+      }
+      ll.push_back(e.logp);
+    }
+    if (bad || ll.size() < 2) {
+      ++fails;
+      continue;
+    }
+    double max_neg = -INFINITY;
+    for (double v : ll) max_neg = std::max(max_neg, -v);
+    double inv = 0.0;
+    for (int k = 0; k < K; ++k) inv += w[k] * std::exp(-ll[k] - max_neg);
+    const double inv_scaled = g_plant == P_CPO ? inv : inv * std::exp(max_neg);
+    if (!std::isfinite(inv_scaled) || inv_scaled <= 0.0) {
+      ++fails;
+      continue;
+    }
+    cpo[i] = 1.0 / inv_scaled;
+    double ps = 0.0;
+    bool ok = true;
+    for (int k = 0; k < K && ok; ++k) {
+      const double f = obs_cdf(fam, eta[(i64)k * n + i], var[(i64)k * n + i], y[i], a[i]);
+      if (std::isnan(f)) ok = false;
+      else ps += w[k] * f;
+    }
+    if (ok && std::isfinite(ps)) pit[i] = std::min(std::max(ps, 0.0), 1.0);
+  }
+  return fails;
+}
 
 void orc_eval_lik(int fam, int link, i64 m, const double* eta, const double* y, const double* a,
                   const double* b, double* logp, double* grad, double* hess) {

@@ -57,6 +57,11 @@ def _load():
         "orc_grid_design": (i32, [i32, dbl, dbl, EVAL_FN, vp, pd, i32]),
         "orc_invert_symmetric": (i32, [i32, pd, pd]),
         "orc_jacobi_eigen": (i32, [i32, pd, i32, pd, pd]),
+        "orc_normal_cdf": (dbl, [dbl]),
+        "orc_poisson_cdf": (dbl, [dbl, i64]),
+        "orc_binomial_cdf": (dbl, [i64, dbl, i64]),
+        "orc_cpo_failure": (i32, [i32, i32, dbl, dbl, dbl, dbl, dbl]),
+        "orc_cpo_pit": (i64, [i32, i32, i32, i64, pd, pd, pd, pd, pd, pd, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -294,7 +299,42 @@ def ccd_design(m: int, f0: float = 1.0):
 PLANTS = ["none", "lower_bw", "bw_fwd", "time_major", "rcm", "choose", "min_degree", "ldlt", "logdet",
           "solve", "takahashi", "selinv_lookup", "dense_ldlt", "lgamma", "lik_gauss", "lik_poisson",
           "lik_binomial", "lik_nbinomial", "newton_rhs", "project", "mlik", "kappa", "ccd5", "ccd_weight",
-          "grid", "jacobi", "invert", "nnz_l", "perm", "d"]
+          "grid", "jacobi", "invert", "nnz_l", "perm", "d", "cpo_fail", "cpo", "normal_cdf", "poisson_cdf",
+          "binomial_cdf"]
+
+
+def normal_cdf(x):
+    return lib.orc_normal_cdf(float(x))
+
+
+def poisson_cdf(lam, k):
+    return lib.orc_poisson_cdf(float(lam), int(k))
+
+
+def binomial_cdf(n, p, k):
+    return lib.orc_binomial_cdf(int(n), float(p), int(k))
+
+
+def cpo_failure(family, link, y, a, mu, var, b=0.0):
+    """check_cpo_failure_for_obs (model_selection.rs:327-396) for one observation."""
+    return bool(lib.orc_cpo_failure(FAMILIES[family], LINKS[link], float(y), float(a), float(b), float(mu),
+                                    float(var)))
+
+
+def cpo_pit(family, link, y, eta, var, weights, a, b=None):
+    """compute_cpo_pit (model_selection.rs:222-325): eta, var (K, n); (cpo, pit, failures)
+    with NaN where the reference returns None."""
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    K, n = eta.shape
+    var = np.ascontiguousarray(np.broadcast_to(var, (K, n)), dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    a = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+    bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, (n,)), dtype=np.float64)
+    cpo, pit = np.empty(n), np.empty(n)
+    nf = lib.orc_cpo_pit(FAMILIES[family], LINKS[link], K, n, _pd(eta), _pd(var), _pd(w), _pd(y), _pd(a),
+                         None if bb is None else _pd(bb), _pd(cpo), _pd(pit))
+    return cpo, pit, int(nf)
 
 
 class planted:

@@ -689,6 +689,18 @@ __global__ __launch_bounds__(128) void k_lowrank_diag(int R, int64_t n, const do
 }
 
 // out = a .* b (flat)
+// out[b][perm[k]] = in[b][k]: rows of B vectors from the internal observation order
+// to the caller's (perm[k] = the caller's index of internal observation k)
+__global__ void k_scatter_rows(int B, int64_t len, const int64_t* __restrict__ perm,
+                               const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t tot = (int64_t)B * len;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / len, k = e - b * len;
+    out[b * len + (perm ? perm[k] : k)] = in[e];
+  }
+}
+
 __global__ void k_mul(int64_t tot, const double* __restrict__ a, const double* __restrict__ b,
                       double* __restrict__ out) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
@@ -903,6 +915,12 @@ hipError_t lap_lowrank_diag(int B, int R, int64_t n, const double* Y, int64_t ld
   }
   hipLaunchKernelGGL(k_lowrank_diag, dim3((unsigned)((n + 127) / 128), B), dim3(128), sh, st, R, n, Y,
                      ldy, ldyb, Kinv, d, ldd);
+  return hipGetLastError();
+}
+
+hipError_t lap_scatter_rows(int B, int64_t len, const int64_t* perm, const double* in, double* out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for((int64_t)B * len)), dim3(256), 0, st, B, len, perm, in, out);
   return hipGetLastError();
 }
 

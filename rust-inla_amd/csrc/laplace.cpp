@@ -39,6 +39,7 @@ hipError_t lap_vdiag(int, int64_t, const int64_t*, const int64_t*, const double*
                      int64_t, double*, hipStream_t);
 hipError_t lap_constr_dot(int, int, int64_t, const double*, const double*, double*, hipStream_t);
 hipError_t lap_mul(int64_t, const double*, const double*, double*, hipStream_t);
+hipError_t lap_scatter_rows(int, int64_t, const int64_t*, const double*, double*, hipStream_t);
 hipError_t lap_blend(int, int64_t, const double*, const double*, double*, hipStream_t,
                      const LapBlocks*);
 hipError_t lap_constr_apply(int, int, int64_t, const double*, const double*, double*,
@@ -115,6 +116,8 @@ struct gmrf_model {
   int64_t *a_rptr = nullptr, *a_cptr = nullptr;
   int32_t *a_cidx = nullptr, *a_ridx = nullptr;
   double *a_rval = nullptr, *a_cval = nullptr;
+  double* a_rval2 = nullptr;  // squared CSR values (predictor variances)
+  int64_t* d_operm = nullptr;  // internal observation -> caller's index (null: identity)
   // A^T W A gather map per Q entry, (A Sigma A^T)_ii map per observation
   int64_t* atwa_ptr = nullptr;
   int32_t* atwa_obs = nullptr;
@@ -735,6 +738,12 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     M->a_rptr = M->arena.upload(arp);
     M->a_cidx = M->arena.upload(aci);
     M->a_rval = M->arena.upload(arv);
+    {
+      std::vector<double> arv2(arv.size());
+      for (size_t t = 0; t < arv.size(); ++t) arv2[t] = arv[t] * arv[t];
+      M->a_rval2 = M->arena.upload(arv2);
+    }
+    if (!operm.empty()) M->d_operm = M->arena.upload(operm);
     // ---- blocks: each pattern entry, observation and latent index belongs to one
     if (nb > 1) {
       std::vector<int32_t> oblk(m), eblk(nnz);
@@ -937,7 +946,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     M->last_count = B;
-    if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && !M->lean && a->max_iter > 0) {
+    if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && !M->lean && a->max_iter > 0 && !a->eta &&
+        !a->eta_var) {
       eval_gauss(M, a, B);
       return GMRF_OK;
     }
@@ -1004,7 +1014,8 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     std::vector<double> hred(4 * V), halpha(V);
     std::vector<int32_t> hbad(V);
     const bool gauss_id = M->fam == 0 && M->link == 0;
-    const int sel_count = (want_grad && !M->lean) ? 2 * B : ((want_grad || a->var) ? B : 0);
+    const bool want_pv = a->var || a->eta_var;  // the posterior's selected inverse
+    const int sel_count = (want_grad && !M->lean) ? 2 * B : ((want_grad || want_pv) ? B : 0);
     bool sel_started = false;
     // final_factor: every member's posterior factor of this system is its last (the
     // prior's, from the first system, is never refactorised), so their selected
@@ -1164,6 +1175,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     check(lap_lik(B, m, M->fam, M->link, M->eta, M->y, M->pa, M->pb, a->lpar, M->logp, M->g, M->w,
                   M->dw, M->dl, nullptr, nullptr, st, bk),
           "lik*");
+    if (a->eta) check(lap_scatter_rows(B, m, M->d_operm, M->eta, a->eta, st), "eta out");
     check(lap_spmv(B, n, M->q_ptr, M->q_idx, M->vprior, nnz, M->x, n, 1.0, 0.0, M->tmpn, n, M->q_long.d, M->q_long.n, st), "Q x*");
     check(lap_rowreduce(B, n, RED_DOT, M->x, M->tmpn, M->red, st, nb, lseg), "quad");
     check(lap_rowreduce(B, m, RED_SUM, M->logp, nullptr, M->red + V, st, nb, oseg), "loglik");
@@ -1339,8 +1351,23 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
             gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[V + b] - (gauss_link ? 0.5 * hs[2 * V + b] : 0.0));
           grad[(size_t)b * mt + l] = status[b] == GMRF_OK ? gq : NAN;
         }
-      std::memcpy(pout + B, grad.data(), grad.size() * sizeof(double));
-      h2d_async(a->grad, pout + B, grad.size() * sizeof(double), st);
+      std::memcpy(pout + V, grad.data(), grad.size() * sizeof(double));
+      h2d_async(a->grad, pout + V, grad.size() * sizeof(double), st);
+    }
+    // ---- predictor variances sum_j A_ij^2 var_j (design.rs:198-210), caller's order
+    if (a->eta_var) {
+      double* var = a->var;
+      if (!var) {
+        var = M->tmpn;
+        check(launch_diag_out(M->sym->plan, B, bt->S, var, n, st), "var");
+        if (lr)
+          check(lap_lowrank_diag(B, M->lr.R, n, M->lr.Ypo, n, (int64_t)M->lr.R * n, M->lr.kinv_po, var, n, st),
+                "var correction");
+      }
+      check(lap_spmv(B, m, M->a_rptr, M->a_cidx, M->a_rval2, 0, var, n, 1.0, 0.0, M->u, m, M->ar_long.d,
+                     M->ar_long.n, st),
+            "A^2 var");
+      check(lap_scatter_rows(B, m, M->d_operm, M->u, a->eta_var, st), "eta_var out");
     }
     check(hipStreamSynchronize(st), "sync");
     for (int v = 0; v < V; ++v) {
@@ -1367,12 +1394,12 @@ gmrf_status gmrf_model_autotune(gmrf_model* M, double* default_ms, double* tuned
   return autotune_batch(M->bt.get(), (M->lean ? 1 : 2) * M->last_count, M->vals, M->nnz, default_ms, tuned_ms);
 }
 
-gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
-                            const double* eta, const double* y, const double* obs_a,
-                            const double* obs_b, const double* lpar, double* logp, void* stream) {
+gmrf_status gmrf_obs_eval(int32_t family, int32_t link, int32_t count, int64_t n_obs, const double* eta,
+                          const double* y, const double* obs_a, const double* obs_b, const double* lpar,
+                          double* logp, double* grad, double* neg_hess, void* stream) {
   if (count < 0 || n_obs < 0) return fail(GMRF_INVALID_ARGUMENT, "negative size");
   if (count == 0 || n_obs == 0) return GMRF_OK;
-  if (!eta || !y || !obs_a || !logp) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (!eta || !y || !obs_a) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (family < 0 || family > 3 || link < 0 || link > 2) return fail(GMRF_INVALID_ARGUMENT, "unknown family or link");
   if (family == 3 && !obs_b) return fail(GMRF_INVALID_ARGUMENT, "negative binomial needs obs_b (size)");
   int32_t* flag = nullptr;
@@ -1381,7 +1408,7 @@ gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t
     const hipStream_t st = (hipStream_t)stream;
     check(hipMalloc(&flag, count * sizeof(int32_t)), "hipMalloc");
     check(hipMemsetAsync(flag, 0, count * sizeof(int32_t), st), "zero flag");
-    check(lap_lik(count, n_obs, family, link, eta, y, obs_a, obs_b, lpar, logp, nullptr, nullptr, nullptr,
+    check(lap_lik(count, n_obs, family, link, eta, y, obs_a, obs_b, lpar, logp, grad, neg_hess, nullptr,
                   nullptr, nullptr, flag, st, nullptr),
           "lik");
     std::vector<int32_t> inv(count);
@@ -1398,6 +1425,13 @@ gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t
     if (flag) (void)hipFree(flag);
     return from_exception();
   }
+}
+
+gmrf_status gmrf_obs_loglik(int32_t family, int32_t link, int32_t count, int64_t n_obs,
+                            const double* eta, const double* y, const double* obs_a,
+                            const double* obs_b, const double* lpar, double* logp, void* stream) {
+  if (count > 0 && n_obs > 0 && !logp) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  return gmrf_obs_eval(family, link, count, n_obs, eta, y, obs_a, obs_b, lpar, logp, nullptr, nullptr, stream);
 }
 
 }  // extern "C"

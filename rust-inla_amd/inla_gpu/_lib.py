@@ -98,6 +98,8 @@ class EvalArgs(C.Structure):
         ("grad", C.c_void_p),
         ("status", C.POINTER(C.c_int32)),
         ("n_iter", C.POINTER(C.c_int32)),
+        ("eta", C.c_void_p),
+        ("eta_var", C.c_void_p),
     ]
 
 
@@ -150,6 +152,7 @@ SIGNATURES = {
     "gmrf_model_eval": (_i32, [_vp, C.POINTER(EvalArgs)]),
     "gmrf_model_stream": (_vp, [_vp]),
     "gmrf_obs_loglik": (_i32, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gmrf_obs_eval": (_i32, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gmrf_model_set_stream": (_i32, [_vp, _vp, _i32]),
     "gmrf_model_autotune": (_i32, [_vp, _pdbl, _pdbl]),
     "gmrf_profile_enable": (_i32, [_i32]),

@@ -148,7 +148,7 @@ class LaplaceModel:
             _free(h)
 
     def eval(self, coefs, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
-             max_iter=200, tol=1e-5, out=None):
+             max_iter=200, tol=1e-5, out=None, want_eta=False):
         """Evaluate a batch of theta-points.
 
         coefs: (B, K); lpar: (B,) Gaussian precision multipliers; dcoefs: (B, m, K);
@@ -156,6 +156,8 @@ class LaplaceModel:
         numpy status / n_iter. ``out``: preallocated float64 CUDA tensors for some of
         mlik (B,), x (B, n), var (B, n), grad (B, m); reusing the same tensors call after
         call lets the library replay its captured graph of the Gaussian evaluation.
+        want_eta adds eta = A x* and eta_var = sum_j A_ij^2 var_j (B, n_obs), the
+        predictor means and variances DIC, WAIC and CPO/PIT take (gmrf_eval_args.eta).
         With blocks the per-point arrays carry the block after the point: coefs
         (B, nb, K), lpar (B, nb), dcoefs (B, nb, m, K), dlog_lpar (B, nb, m), mlik
         (B, nb), grad (B, nb, m), status and n_iter (B, nb).
@@ -178,7 +180,8 @@ class LaplaceModel:
             m = dc.shape[-2]
         dl = _dev(dlog_lpar, (B,) + pp + (m,)) if dlog_lpar is not None else None
         given = dict(out or {})
-        shapes = {"mlik": (B,) + pp, "x": (B, self.n), "var": (B, self.n), "grad": (B,) + pp + (m,)}
+        shapes = {"mlik": (B,) + pp, "x": (B, self.n), "var": (B, self.n), "grad": (B,) + pp + (m,),
+                  "eta": (B, self.n_obs), "eta_var": (B, self.n_obs)}
         for k, t in given.items():
             if k not in shapes or not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
                                        and t.is_contiguous() and tuple(t.shape) == shapes[k]):
@@ -194,6 +197,9 @@ class LaplaceModel:
             out["var"] = buf("var")
         if dc is not None:
             out["grad"] = buf("grad")
+        if want_eta:
+            out["eta"] = buf("eta")
+            out["eta_var"] = buf("eta_var")
         status = np.zeros((B,) + pp, dtype=np.int32)
         niter = np.zeros((B,) + pp, dtype=np.int32)
         a = EvalArgs()
@@ -209,6 +215,8 @@ class LaplaceModel:
         a.grad = out["grad"].data_ptr() if dc is not None else None
         a.status = status.ctypes.data_as(C.POINTER(C.c_int32))
         a.n_iter = niter.ctypes.data_as(C.POINTER(C.c_int32))
+        a.eta = out["eta"].data_ptr() if want_eta else None
+        a.eta_var = out["eta_var"].data_ptr() if want_eta else None
         check(L.lib.gmrf_model_set_stream(self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream), 1))
         check(L.lib.gmrf_model_eval(self._h, C.byref(a)))
         out["status"] = status

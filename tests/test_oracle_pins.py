@@ -349,6 +349,71 @@ def pin_jacobi():
         assert np.allclose(np.sort(lam), np.linalg.eigvalsh(a), rtol=1e-12)
 
 
+# ------------------------------------------------------------------ CPO / PIT
+def pin_cdfs():
+    """The CDFs of observation_cdf (model_selection.rs:449-503): the reference's known
+    answers (model_selection.rs:768-791), scipy's Poisson and binomial CDFs, and for the
+    normal formula the identity it follows: the rational term approximates
+    erfc(|x|) exp(x^2) to 1.5e-7 (Abramowitz-Stegun 7.1.26), so with exp(-x^2 / 2) the
+    formula is 1/2 (1 + sign(x) (1 - erfc(|x|) exp(x^2 / 2))) within 1.5e-7 exp(x^2 / 2)."""
+    assert abs(orc.normal_cdf(0.0) - 0.5) < 1e-6 and orc.normal_cdf(3.0) > 0.998 and orc.normal_cdf(-3.0) < 0.002
+    for x in np.linspace(-3.0, 3.0, 25):
+        ident = 0.5 * (1.0 + math.copysign(1.0, x) * (1.0 - math.erfc(abs(x)) * math.exp(x * x / 2.0)))
+        assert abs(orc.normal_cdf(x) - ident) <= 1.5e-7 * math.exp(x * x / 2.0) + 1e-15, x
+    assert abs(orc.poisson_cdf(1.0, 0) - math.exp(-1.0)) < 1e-10 and orc.poisson_cdf(1.0, 10) > 0.9999
+    for lam, k in ((0.3, 0), (2.5, 3), (7.0, 12), (40.0, 31)):
+        assert abs(orc.poisson_cdf(lam, k) - ss.poisson.cdf(k, lam)) < 1e-12
+    assert abs(orc.binomial_cdf(10, 0.5, 10) - 1.0) < 1e-10
+    assert abs(orc.binomial_cdf(10, 0.5, 4) - 0.376953125) < 1e-8
+    for n, p, k in ((10, 0.3, 2), (25, 0.8, 17), (7, 0.05, 0), (40, 0.6, 30)):
+        assert abs(orc.binomial_cdf(n, p, k) - ss.binom.cdf(k, n, p)) < 1e-12
+
+
+def pin_cpo_failure():
+    """check_cpo_failure_for_obs: the reference's four cases (model_selection.rs:628-673),
+    a non-positive variance and a leave-one-out precision <= 0."""
+    assert not orc.cpo_failure("gaussian", "identity", 1.0, 2.0, 1.0, 0.2)
+    assert orc.cpo_failure("gaussian", "identity", 1.0, 2.0, 1.0, 0.6)
+    assert orc.cpo_failure("gaussian", "identity", 0.0, 5.0, 10.0, 0.1)
+    assert orc.cpo_failure("poisson", "log", 1.0, 10.0, 0.0, 0.05)
+    assert orc.cpo_failure("poisson", "log", 3.0, 1.0, 1.0, 0.0)
+    assert orc.cpo_failure("poisson", "log", 3.0, 1.0, 1.0, 1.0 / np.e + 1e-3)  # 1/var < -hess = e
+    assert not orc.cpo_failure("poisson", "log", 3.0, 1.0, 1.0, 0.1)
+
+
+def pin_cpo():
+    """compute_cpo_pit: CPO_i is the weighted harmonic mean over the nodes of the
+    likelihood at the node predictors (scipy.stats.hmean); equal nodes give the
+    likelihood itself; a single node fails (fewer than two terms); PIT of equal Gaussian
+    nodes is the normal CDF of the standardised residual over 1/precision + variance."""
+    rng = np.random.default_rng(7)
+    n, K = 6, 4
+    y = rng.poisson(3.0, size=n).astype(float)
+    eta = np.log(3.0) + 0.05 * rng.standard_normal((K, n))
+    var = np.full((K, n), 0.04)
+    w = rng.uniform(0.5, 1.5, size=K)
+    w /= w.sum()
+    cpo, pit, nf = orc.cpo_pit("poisson", "log", y, eta, var, w, 1.0)
+    assert nf == 0 and np.all(np.isfinite(cpo)) and np.all(np.isfinite(pit))
+    lik = ss.poisson.pmf(y[None, :], np.exp(eta))
+    assert np.allclose(cpo, [ss.hmean(lik[:, i], weights=w) for i in range(n)], rtol=1e-12)
+    assert np.allclose(pit, np.clip(w @ ss.poisson.cdf(y[None, :], np.exp(eta)), 0, 1), rtol=1e-12)
+    same = np.repeat(eta[:1], K, axis=0)
+    cpo1, _, _ = orc.cpo_pit("poisson", "log", y, same, var, w, 1.0)
+    assert np.allclose(cpo1, ss.poisson.pmf(y, np.exp(eta[0])), rtol=1e-12)
+    c2, p2, nf2 = orc.cpo_pit("poisson", "log", y, eta[:1], var[:1], [1.0], 1.0)
+    assert nf2 == n and np.all(np.isnan(c2)) and np.all(np.isnan(p2))
+    yg = np.array([1.0, 1.1, 1.2])
+    eg = np.repeat(np.array([[0.9, 1.15, 1.0]]), 3, axis=0)
+    cg, pg, nfg = orc.cpo_pit("gaussian", "identity", yg, eg, np.full((3, 3), 0.2), [0.2, 0.5, 0.3], 2.0)
+    assert nfg == 0
+    assert np.allclose(pg, [orc.normal_cdf((yi - e) / math.sqrt(0.5 + 0.2)) for yi, e in zip(yg, eg[0])],
+                       rtol=1e-14)
+    # the negative binomial has no CDF: CPO only
+    cn, pn, _ = orc.cpo_pit("nbinomial", "log", y, eta, var, w, 1.0, b=5.0)
+    assert np.all(np.isfinite(cn)) and np.all(np.isnan(pn))
+
+
 PINS = {k[4:]: v for k, v in dict(globals()).items() if k.startswith("pin_")}
 
 # every exported oracle function with arithmetic, the mistake planted in it, and
@@ -362,7 +427,8 @@ PLANT_PINS = {
     "lik_nbinomial": ["likelihoods"], "newton_rhs": ["stationarity"], "project": ["constrained"],
     "mlik": ["gaussian_closed_form"], "kappa": ["constrained"], "ccd5": ["ccd"], "ccd_weight": ["ccd"],
     "grid": ["grid"], "jacobi": ["jacobi"], "invert": ["invert"], "nnz_l": ["factor_brute"],
-    "perm": ["factor_brute"], "d": ["factor_brute"],
+    "perm": ["factor_brute"], "d": ["factor_brute"], "cpo_fail": ["cpo_failure"], "cpo": ["cpo"],
+    "normal_cdf": ["cdfs"], "poisson_cdf": ["cdfs", "cpo"], "binomial_cdf": ["cdfs"],
 }
 
 
