@@ -298,7 +298,7 @@ typedef struct gmrf_model_desc {
 typedef struct gmrf_eval_args {
   int32_t count;             /* theta-points in this call (<= capacity) */
   int32_t n_theta;           /* m, length of theta (for the gradient) */
-  const double* coefs;       /* device, count x K */
+  const double* coefs;       /* device, count x K (NULL in values mode, qvals below) */
   const double* lpar;        /* device, count: Gaussian precision multiplier (NULL = 1) */
   const double* dcoefs;      /* device, count x m x K: d coef_k / d theta_l (NULL: no gradient) */
   const double* dlog_lpar;   /* device, count x m: d log(lpar) / d theta_l (NULL = 0) */
@@ -320,6 +320,14 @@ typedef struct gmrf_eval_args {
      CPO/PIT take. Requesting them runs the general (uncaptured) evaluation. */
   double* eta;
   double* eta_var;
+  /* values mode (NULL: the coefficient form above). Device, count x nnz: the entries of
+     Q(theta_b) on the model's pattern, used instead of coefs x components, for callers
+     whose Q(theta) is not affine in fixed matrices (the reference's build_prior callback,
+     inference.rs:1246-1300); dqvals (device, count x m x nnz, NULL: no gradient): the
+     entries of dQ/dtheta_l on the pattern, used instead of dcoefs. Not with blocks or
+     the low-rank constraint form. */
+  const double* qvals;
+  const double* dqvals;
 } gmrf_eval_args;
 
 gmrf_status gmrf_model_create(const gmrf_model_desc* desc, gmrf_model** out);

@@ -93,6 +93,25 @@ struct PinnedBuf {
   }
 };
 
+// Device buffer grown on demand (values-mode gradient traces), freed with the model.
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  double* get(size_t count) {
+    if (count > n) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+      check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)), "hipMalloc");
+      n = count;
+    }
+    return p;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 struct GatherLongs {
   int64_t* d = nullptr;  // device list of output indices
   int64_t n = 0;
@@ -161,6 +180,7 @@ struct gmrf_model {
   double *red = nullptr, *alpha = nullptr, *cres = nullptr;
   int32_t* bad = nullptr;
   double *tr_post = nullptr, *tr_prior = nullptr, *qx = nullptr, *ql = nullptr, *cpart = nullptr;
+  DevBuf qv_tr;  // values mode: the four trace arrays and the partial sums for m derivatives
   int32_t* dstat = nullptr;  // per-point status written on the device (Gaussian path)
   // The Gaussian evaluation captured as HIP graphs, one per (point count, gradient
   // length, device pointers of the call), the most recently used kept (kGraphCache).
@@ -939,15 +959,18 @@ gmrf_status gmrf_model_set_stream(gmrf_model* M, void* stream, int32_t use_given
 }
 
 gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
-  if (!M || !a || !a->coefs || !a->mlik) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  if (!M || !a || (!a->coefs && !a->qvals) || !a->mlik) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   const int B = a->count;
   if (B < 1 || B > M->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds model capacity");
-  const bool want_grad = a->dcoefs != nullptr && a->n_theta > 0 && a->grad != nullptr;
+  const bool qv = a->qvals != nullptr;  // values mode: Q(theta_b) given on the pattern
+  if (qv && (M->nb > 1 || M->kappa > 0.0))
+    return fail(GMRF_INVALID_ARGUMENT, "Q values are not supported with blocks or the low-rank constraint form");
+  const bool want_grad = (qv ? a->dqvals != nullptr : a->dcoefs != nullptr) && a->n_theta > 0 && a->grad != nullptr;
   try {
     check(hipSetDevice(M->sym->device), "hipSetDevice");
     M->last_count = B;
     if (M->fam == 0 && M->link == 0 && M->kc == 0 && M->nb == 1 && !M->lean && a->max_iter > 0 && !a->eta &&
-        !a->eta_var) {
+        !a->eta_var && !qv) {
       eval_gauss(M, a, B);
       return GMRF_OK;
     }
@@ -962,11 +985,42 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
     const int64_t* oseg = bk ? bk->oseg : nullptr;
     // ---- Q(theta); the prior matrices occupy batch slots [B, 2B) right after the
     // posterior ones, so the first Newton factorisation covers both in one call
-    check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st, bk), "affine");
+    if (qv)
+      check(hipMemcpyAsync(M->vprior, a->qvals, (size_t)B * nnz * sizeof(double), hipMemcpyDeviceToDevice, st),
+            "Q values");
+    else
+      check(lap_affine(B, K, nnz, a->coefs, M->comps, M->vprior, st, bk), "affine");
     double* vprior_f = M->vals + (size_t)B * nnz;
     check(hipMemcpyAsync(vprior_f, M->vprior, (size_t)B * nnz * sizeof(double),
                          hipMemcpyDeviceToDevice, st),
           "copy prior");
+    // The trace contractions tr(Sigma Q_k), x^T Q_k z over the components (coefficient
+    // mode) or, in values mode, over each point's own derivative matrices dQ/dtheta_l
+    // (KG = m of them per point, one contraction launch per point).
+    const int KG = qv ? std::max(a->n_theta, 1) : K;
+    double *trp = M->tr_post, *trq = M->tr_prior, *qxp = M->qx, *qlp = M->ql, *cpp = M->cpart;
+    if (qv && want_grad) {
+      const size_t s1 = (size_t)B * KG, s2 = (size_t)KG * 64 * 2;
+      double* base = M->qv_tr.get(4 * s1 + s2);
+      trp = base, trq = base + s1, qxp = base + 2 * s1, qlp = base + 3 * s1, cpp = base + 4 * s1;
+    }
+    const DevPlan& cplan = M->sym->plan;
+    auto contract = [&](const double* Sb, const int32_t* qr, const int32_t* qc, const double* xv, const double* zv,
+                        double* tr, double* quad, const char* what) {
+      if (!qv) {
+        check(lap_contract(B, K, nnz, M->comps, Sb, Sb ? cplan.l_size : 0, Sb ? cplan.slot_full : nullptr, qr, qc,
+                           xv, zv, n, tr, quad, cpp, st, bk),
+              what);
+        return;
+      }
+      for (int b = 0; b < B; ++b)
+        check(lap_contract(1, KG, nnz, a->dqvals + (size_t)b * KG * nnz, Sb ? Sb + (size_t)b * cplan.l_size : nullptr,
+                           Sb ? cplan.l_size : 0, Sb ? cplan.slot_full : nullptr, qr, qc,
+                           xv ? xv + (size_t)b * n : nullptr, zv ? zv + (size_t)b * n : nullptr, n,
+                           tr ? tr + (size_t)b * KG : nullptr, quad ? quad + (size_t)b * KG : nullptr, cpp, st,
+                           nullptr),
+              what);
+    };
     if (M->kc == 0) check(lap_add_diag(B, n, nnz, M->diag_idx, kPriorJitter, vprior_f, st), "jitter");
     const bool lr = M->kappa > 0.0;
     if (lr) check(lap_ground(B, M->kc, nnz, M->lr.gidx, M->vprior, vprior_f, M->lr.alpha, st), "ground");
@@ -997,9 +1051,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       }
       if (want_grad) {
         bt->selinv(B, 0);
-        check(lap_contract(B, K, nnz, M->comps, bt->S, plan.l_size, plan.slot_full, nullptr, nullptr, nullptr,
-                           nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
-              "contract prior");
+        contract(bt->S, nullptr, nullptr, nullptr, nullptr, trq, nullptr, "contract prior");
         if (trace_newton()) {
           check(hipStreamSynchronize(st), "sync");
           std::fprintf(stderr, "[gmrf] lean: prior inverted, traces formed\n");
@@ -1256,13 +1308,9 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       const int mt = a->n_theta;
       const DevPlan& P = M->sym->plan;
       // tr(Sigma_post Q_k), x*^T Q_k x*, tr(Sigma_prior Q_k)
-      check(lap_contract(B, K, nnz, M->comps, bt->S, P.l_size, P.slot_full, M->q_row, M->q_col,
-                         M->x, M->x, n, M->tr_post, M->qx, M->cpart, st, bk),
-            "contract post");
+      contract(bt->S, M->q_row, M->q_col, M->x, M->x, trp, qxp, "contract post");
       if (!M->lean)  // (lean: formed before the Newton loop)
-        check(lap_contract(B, K, nnz, M->comps, bt->S + (size_t)B * P.l_size, P.l_size, P.slot_full,
-                           nullptr, nullptr, nullptr, nullptr, n, M->tr_prior, nullptr, M->cpart, st, bk),
-              "contract prior");
+        contract(bt->S + (size_t)B * P.l_size, nullptr, nullptr, nullptr, nullptr, trq, nullptr, "contract prior");
       // v_i = (A Sigma_post A^T)_ii
       check(lap_vdiag(B, m, M->v_ptr, M->v_slot, M->v_coef, bt->S, P.l_size, M->v, st), "vdiag");
       if (lr) {  // v_i -= (A Y Kinv Y^T A^T)_ii
@@ -1274,7 +1322,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_lowrank_diag(B, R, m, M->lr.W, m, (int64_t)R * m, M->lr.kinv_po, M->v, m, st),
               "v correction");
       }
-      std::vector<double> hql((size_t)V * K, 0.0);
+      std::vector<double> hql((size_t)V * KG, 0.0);
       if (!gauss_id) {
         // mode sensitivity: u = v .* dw/deta, lambda = Q_post^-1 A^T u
         check(lap_mul(B * m, M->v, M->dw, M->u, st), "u");
@@ -1282,9 +1330,7 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
               "A^T u");
         bt->solve(B, M->rhs, n, M->lam, n, 0);
         if (lr) lowrank_correct(M, B, false, M->lam);
-        check(lap_contract(B, K, nnz, M->comps, nullptr, 0, nullptr, M->q_row, M->q_col, M->lam, M->x,
-                           n, nullptr, M->ql, M->cpart, st, bk),
-              "lambda^T Q_k x");
+        contract(nullptr, M->q_row, M->q_col, M->lam, M->x, nullptr, qlp, "lambda^T Q_k x");
       }
       // Gaussian precision hyperparameter at fixed eta: sum dlogp/dlog(prec), sum v w
       check(lap_rowreduce(B, m, RED_SUM, M->dl, nullptr, M->red, st, nb, oseg), "sum dl");
@@ -1300,21 +1346,27 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
         check(lap_rowreduce(B, n, RED_DOT, M->lam, M->tmpn, M->red + 2 * V, st, nb, lseg), "lambda^T A^T g");
       }
       std::vector<double> hs((gauss_link ? 3 : 2) * V);
-      std::vector<double> htp((size_t)V * K), htq((size_t)V * K), hqx((size_t)V * K);
-      std::vector<double> dc((size_t)V * mt * K), dlp((size_t)V * mt, 0.0);
+      std::vector<double> htp((size_t)V * KG), htq((size_t)V * KG), hqx((size_t)V * KG);
+      std::vector<double> dc((size_t)V * mt * KG, 0.0), dlp((size_t)V * mt, 0.0);
+      if (qv)  // the contractions ran over dQ/dtheta_l themselves: unit coefficients
+        for (int b = 0; b < V; ++b)
+          for (int l = 0; l < mt; ++l) dc[((size_t)b * mt + l) * KG + l] = 1.0;
       {
         // one pinned area, one host round trip
         std::vector<std::pair<double*, size_t>> parts = {{hs.data(), hs.size()},   {htp.data(), htp.size()},
-                                                         {htq.data(), htq.size()}, {hqx.data(), hqx.size()},
-                                                         {dc.data(), dc.size()}};
-        std::vector<const double*> srcs = {M->red, M->tr_post, M->tr_prior, M->qx, a->dcoefs};
+                                                         {htq.data(), htq.size()}, {hqx.data(), hqx.size()}};
+        std::vector<const double*> srcs = {M->red, trp, trq, qxp};
+        if (!qv) {
+          parts.push_back({dc.data(), dc.size()});
+          srcs.push_back(a->dcoefs);
+        }
         if (a->dlog_lpar) {
           parts.push_back({dlp.data(), dlp.size()});
           srcs.push_back(a->dlog_lpar);
         }
         if (!gauss_id) {
           parts.push_back({hql.data(), hql.size()});
-          srcs.push_back(M->ql);
+          srcs.push_back(qlp);
         }
         size_t tot = 0;
         for (auto& q : parts) tot += q.second;
@@ -1342,10 +1394,10 @@ gmrf_status gmrf_model_eval(gmrf_model* M, gmrf_eval_args* a) {
       for (int b = 0; b < V; ++b)
         for (int l = 0; l < mt; ++l) {
           double gq = 0.0;
-          for (int k = 0; k < K; ++k) {
-            const double c = dc[((size_t)b * mt + l) * K + k];
-            gq += c * (-0.5 * hqx[b * K + k] + 0.5 * htq[b * K + k] - 0.5 * htp[b * K + k] +
-                       0.5 * hql[b * K + k]);
+          for (int k = 0; k < KG; ++k) {
+            const double c = dc[((size_t)b * mt + l) * KG + k];
+            gq += c * (-0.5 * hqx[b * KG + k] + 0.5 * htq[b * KG + k] - 0.5 * htp[b * KG + k] +
+                       0.5 * hql[b * KG + k]);
           }
           if (M->fam == 0)
             gq += dlp[(size_t)b * mt + l] * (hs[b] - 0.5 * hs[V + b] - (gauss_link ? 0.5 * hs[2 * V + b] : 0.0));

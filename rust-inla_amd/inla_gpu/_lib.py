@@ -100,6 +100,8 @@ class EvalArgs(C.Structure):
         ("n_iter", C.POINTER(C.c_int32)),
         ("eta", C.c_void_p),
         ("eta_var", C.c_void_p),
+        ("qvals", C.c_void_p),
+        ("dqvals", C.c_void_p),
     ]
 
 

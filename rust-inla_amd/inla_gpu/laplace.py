@@ -147,8 +147,8 @@ class LaplaceModel:
             self._h = None
             _free(h)
 
-    def eval(self, coefs, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
-             max_iter=200, tol=1e-5, out=None, want_eta=False):
+    def eval(self, coefs=None, lpar=None, dcoefs=None, dlog_lpar=None, want_x=True, want_var=True,
+             max_iter=200, tol=1e-5, out=None, want_eta=False, qvals=None, dqvals=None):
         """Evaluate a batch of theta-points.
 
         coefs: (B, K); lpar: (B,) Gaussian precision multipliers; dcoefs: (B, m, K);
@@ -156,7 +156,9 @@ class LaplaceModel:
         numpy status / n_iter. ``out``: preallocated float64 CUDA tensors for some of
         mlik (B,), x (B, n), var (B, n), grad (B, m); reusing the same tensors call after
         call lets the library replay its captured graph of the Gaussian evaluation.
-        want_eta adds eta = A x* and eta_var = sum_j A_ij^2 var_j (B, n_obs), the
+        Values mode: qvals (B, nnz) gives Q(theta_b) on the model's pattern instead of
+        coefs, and dqvals (B, m, nnz) its theta-derivatives instead of dcoefs
+        (gmrf_eval_args.qvals). want_eta adds eta = A x* and eta_var = sum_j A_ij^2 var_j (B, n_obs), the
         predictor means and variances DIC, WAIC and CPO/PIT take (gmrf_eval_args.eta).
         With blocks the per-point arrays carry the block after the point: coefs
         (B, nb, K), lpar (B, nb), dcoefs (B, nb, m, K), dlog_lpar (B, nb, m), mlik
@@ -164,20 +166,33 @@ class LaplaceModel:
         """
         nb = self.nb
         pp = (nb,) if nb > 1 else ()
-        c = _dev(coefs)
-        if c.dim() == 1 + len(pp):
-            c = c.unsqueeze(0)
-        B = c.shape[0]
-        if tuple(c.shape[1:]) != pp + (self.K,):
-            raise InvalidArgument("coefs must be B x K" if nb == 1 else "coefs must be B x n_blocks x K")
+        qv = dq = c = dc = None
+        m = 0
+        if qvals is not None:
+            nnz = len(self.indices)
+            qv = _dev(qvals)
+            if qv.dim() == 1:
+                qv = qv.unsqueeze(0)
+            B = qv.shape[0]
+            if tuple(qv.shape[1:]) != (nnz,):
+                raise InvalidArgument("qvals must be B x nnz")
+            if dqvals is not None:
+                dq = _dev(dqvals).reshape(B, -1, nnz)
+                m = dq.shape[1]
+        else:
+            c = _dev(coefs)
+            if c.dim() == 1 + len(pp):
+                c = c.unsqueeze(0)
+            B = c.shape[0]
+            if tuple(c.shape[1:]) != pp + (self.K,):
+                raise InvalidArgument("coefs must be B x K" if nb == 1 else "coefs must be B x n_blocks x K")
+            dc = _dev(dcoefs) if dcoefs is not None else None
+            if dc is not None:
+                dc = dc.reshape((B,) + pp + (-1, self.K))
+                m = dc.shape[-2]
         if B > self.capacity:
             raise InvalidArgument("batch exceeds model capacity")
         lp = _dev(lpar, (B,) + pp) if lpar is not None else None
-        dc = _dev(dcoefs) if dcoefs is not None else None
-        m = 0
-        if dc is not None:
-            dc = dc.reshape((B,) + pp + (-1, self.K))
-            m = dc.shape[-2]
         dl = _dev(dlog_lpar, (B,) + pp + (m,)) if dlog_lpar is not None else None
         given = dict(out or {})
         shapes = {"mlik": (B,) + pp, "x": (B, self.n), "var": (B, self.n), "grad": (B,) + pp + (m,),
@@ -195,7 +210,7 @@ class LaplaceModel:
             out["x"] = buf("x")
         if want_var:
             out["var"] = buf("var")
-        if dc is not None:
+        if dc is not None or dq is not None:
             out["grad"] = buf("grad")
         if want_eta:
             out["eta"] = buf("eta")
@@ -204,7 +219,9 @@ class LaplaceModel:
         niter = np.zeros((B,) + pp, dtype=np.int32)
         a = EvalArgs()
         a.count, a.n_theta = B, m
-        a.coefs = c.data_ptr()
+        a.coefs = c.data_ptr() if c is not None else None
+        a.qvals = qv.data_ptr() if qv is not None else None
+        a.dqvals = dq.data_ptr() if dq is not None else None
         a.lpar = lp.data_ptr() if lp is not None else None
         a.dcoefs = dc.data_ptr() if dc is not None else None
         a.dlog_lpar = dl.data_ptr() if dl is not None else None
@@ -212,7 +229,7 @@ class LaplaceModel:
         a.mlik = out["mlik"].data_ptr()
         a.x = out["x"].data_ptr() if want_x else None
         a.var = out["var"].data_ptr() if want_var else None
-        a.grad = out["grad"].data_ptr() if dc is not None else None
+        a.grad = out["grad"].data_ptr() if "grad" in out else None
         a.status = status.ctypes.data_as(C.POINTER(C.c_int32))
         a.n_iter = niter.ctypes.data_as(C.POINTER(C.c_int32))
         a.eta = out["eta"].data_ptr() if want_eta else None
