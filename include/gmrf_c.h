@@ -200,7 +200,10 @@ gmrf_status gmrf_multi_logdet(gmrf_multi* m, int32_t count, double* logdet, int3
 /* ---- InlaSolver mirror (host pointers) --------------------------------------
  * gmrf_solver_factorize = InlaSolver::factorize (solver.rs:70-74): symbolic
  * analysis on the first call or when the pattern changes, numeric otherwise.
- * On failure the previously stored factor is kept, as in the reference.
+ * On failure the previously stored factor is kept, as in the reference. When the
+ * sparse factorisation runs out of device memory, the matrix is factorised densely
+ * instead (DefaultBackend::factorize, backend.rs:110-122; 8 n^2 bytes, as
+ * gmrf_solver_factorize_dense); GMRF_OUT_OF_MEMORY only if that fails too.
  * solve / diag_inv / log_abs_det before any successful factorize return
  * GMRF_SOLVE_FAILURE "Factorization not computed yet" (solver.rs:77-79). */
 gmrf_status gmrf_solver_create(gmrf_solver** out);

@@ -607,13 +607,38 @@ gmrf_status gmrf_solver_factorize(gmrf_solver* s, int64_t n, const int64_t* indp
                                   const int64_t* indices, const double* data) {
   if (!s || !indptr) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (n <= 0) return fail(GMRF_INVALID_ARGUMENT, "empty or invalid matrix: n must be > 0");
+  gmrf_status rc;
   try {
     check(hipSetDevice(g_device), "hipSetDevice");
-    auto sym = analyze_cached(n, indptr, indices, GMRF_ORDER_CHOOSER, nullptr);
-    return solver_factorize_sym(s, sym, data);
+    // GMRF_SIMULATE_SPARSE_OOM set: report the sparse path out of memory (tests of the
+    // fallback below, without exhausting the device)
+    if (std::getenv("GMRF_SIMULATE_SPARSE_OOM")) {
+      rc = fail(GMRF_OUT_OF_MEMORY, "sparse factorization out of memory (simulated)");
+    } else {
+      auto sym = analyze_cached(n, indptr, indices, GMRF_ORDER_CHOOSER, nullptr);
+      rc = solver_factorize_sym(s, sym, data);
+    }
   } catch (...) {
-    return from_exception();
+    rc = from_exception();
   }
+  if (rc != GMRF_OUT_OF_MEMORY) return rc;
+  // DefaultBackend::factorize (backend.rs:110-122): a sparse factorisation that runs out
+  // of memory falls back to the dense one, on the matrix scattered row-major as
+  // DenseBackend::factorize does (backend.rs:39-56); the dense factor needs 8 n^2 bytes
+  std::vector<double> a;
+  try {
+    a.assign((size_t)n * (size_t)n, 0.0);
+  } catch (const std::bad_alloc&) {
+    return fail(GMRF_OUT_OF_MEMORY, "out of memory in the sparse factorization and for the dense fallback");
+  }
+  if (!indices || !data) return fail(GMRF_INVALID_ARGUMENT, "null argument");
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+      const int64_t r = indices[p];
+      if (r < 0 || r >= n) return fail(GMRF_INVALID_ARGUMENT, "row index out of bounds");
+      a[(size_t)r * n + c] = data[p];
+    }
+  return gmrf_solver_factorize_dense(s, n, a.data());
 }
 
 gmrf_status gmrf_solver_factorize_dense(gmrf_solver* s, int64_t n, const double* a) {

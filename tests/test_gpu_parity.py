@@ -329,3 +329,28 @@ def test_factorize_dense_empty_matrix():
     # a later factorisation replaces it as usual
     s.factorize_dense(np.eye(2) * 4.0)
     assert abs(s.log_abs_det() - 2 * np.log(4.0)) < 1e-12
+
+
+def test_sparse_out_of_memory_falls_back_to_dense(monkeypatch):
+    # DefaultBackend::factorize (backend.rs:110-122): an out-of-memory sparse
+    # factorisation is retried densely. The sparse path's failure is forced from the host
+    # (GMRF_SIMULATE_SPARSE_OOM): the factor, a solve and diag(Q^-1) must equal the sparse
+    # path's and the oracle's; a failure of the dense retry keeps the previous factor.
+    from inla_gpu.solver import Singular
+
+    q = M.random_spd(120, 0.06, 5)
+    sp_s = _solver()
+    sp_s.factorize(q)
+    monkeypatch.setenv("GMRF_SIMULATE_SPARSE_OOM", "1")
+    s = _solver()
+    s.factorize(q)
+    ref = orc.Factor(q)
+    b = np.random.default_rng(1).standard_normal(q.shape[0])
+    assert abs(s.log_abs_det() - ref.log_abs_det()) <= LOGDET_RTOL * abs(ref.log_abs_det())
+    assert abs(s.log_abs_det() - sp_s.log_abs_det()) <= 1e-12 * abs(ref.log_abs_det())
+    assert _rel(s.solve(b), ref.solve(b)) < SOLVE_RTOL and _rel(s.solve(b), sp_s.solve(b)) < 1e-12
+    assert _rel(s.diag_inv(), ref.takahashi_diag()) < SELINV_RTOL
+    zero = sp.csc_matrix(q * 0.0)
+    with pytest.raises(Singular):
+        s.factorize(zero)
+    assert abs(s.log_abs_det() - ref.log_abs_det()) <= LOGDET_RTOL * abs(ref.log_abs_det())
