@@ -333,6 +333,36 @@ def hetero_compare(k=64, reps=3):
             "work": "LDL^T + selected-inverse diagonal + one solve per model"}
 
 
+_HETERO = {}
+
+
+def _hetero_setup(k):
+    """The 64-model BYM2 + Poisson batch, its union model (capacity 9) and the models
+    alone, built once per process."""
+    if k in _HETERO:
+        return _HETERO[k]
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    t0 = time.perf_counter()
+    wl = W.bym2_poisson_batch(k=k)
+    t_build = time.perf_counter() - t0
+    log(f"   workload: {k} models, n = {wl.n}, {t_build:.1f} s")
+    t0 = time.perf_counter()
+    union = LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A,
+                         blocks=wl.block_ptr, capacity=9)
+    t_setup = time.perf_counter() - t0
+    bp = wl.block_ptr
+    singles = []
+    for q in range(k):
+        e0, e1 = wl.indptr[bp[q]], wl.indptr[bp[q + 1]]
+        singles.append(LaplaceModel(bp[q + 1] - bp[q], wl.indptr[bp[q]:bp[q + 1] + 1] - e0,
+                                    wl.indices[e0:e1] - bp[q], wl.components[:, e0:e1], wl.block_y(q),
+                                    family="poisson", A=wl.block_A(q), capacity=9))
+    _HETERO[k] = (wl, union, singles, t_build, t_setup)
+    return _HETERO[k]
+
+
 def hetero_laplace_line(k=64, reps=3, cpu_sample=True):
     """The heterogeneous Laplace batch: k BYM2 + Poisson models with an intercept each
     (Delaunay-neighbour graphs, sizes uniform in [500, 20000], phi = 0.5, seeded), 9 CCD
@@ -345,34 +375,18 @@ def hetero_laplace_line(k=64, reps=3, cpu_sample=True):
     import torch
 
     from inla_gpu import ccd
-    from inla_gpu import workloads as W
-    from inla_gpu.laplace import LaplaceModel
 
-    t0 = time.perf_counter()
-    wl = W.bym2_poisson_batch(k=k)
-    t_build = time.perf_counter() - t0
-    log(f"   workload: {k} models, n = {wl.n}, {t_build:.1f} s")
+    wl, union, singles, t_build, t_setup = _hetero_setup(k)
     z, _ = ccd.ccd_design(2, 1.0)
     B = len(z)
     thetas = np.broadcast_to((0.5 * z)[:, None, :], (B, k, 2)).copy()
     c, dc, _, _ = wl.params(thetas)
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
-    t0 = time.perf_counter()
-    union = LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A,
-                         blocks=wl.block_ptr, capacity=B)
-    t_setup = time.perf_counter() - t0
     cd, dcd = dev(c), dev(dc)
 
     def run_union():
         return union.eval(cd, dcoefs=dcd)
 
-    bp = wl.block_ptr
-    singles = []
-    for q in range(k):
-        e0, e1 = wl.indptr[bp[q]], wl.indptr[bp[q + 1]]
-        singles.append(LaplaceModel(bp[q + 1] - bp[q], wl.indptr[bp[q]:bp[q + 1] + 1] - e0,
-                                    wl.indices[e0:e1] - bp[q], wl.components[:, e0:e1], wl.block_y(q),
-                                    family="poisson", A=wl.block_A(q), capacity=B))
     cq = [dev(c[:, q]) for q in range(k)]
     dcq = [dev(dc[:, q]) for q in range(k)]
 
@@ -430,6 +444,52 @@ def hetero_laplace_line(k=64, reps=3, cpu_sample=True):
                                        f"over {cores} host cores, node work only (no gradient): {t_cpu:.1f} s",
                              "max_abs_mlik_diff_vs_gpu": float(np.max(np.abs(ml_o - ml_u[0])))}
     return res
+
+
+def hetero_fit_line(k=64):
+    """The 64 BYM2 + Poisson models fitted at once (fit_blocks): each model's BFGS mode
+    on the adjoint gradient, its Hessian stencil and its 9 CCD nodes with variances, every
+    step one block evaluation of the union; against each model fitted alone with the same
+    driver (fit_batched on its own model, one model after another)."""
+    import numpy as np
+    import torch
+
+    from inla_gpu import fit as F
+    from inla_gpu import workloads as W
+
+    wl, union, singles, _, _ = _hetero_setup(k)
+    theta0 = np.tile([0.5, 0.5], (k, 1))
+    obj = F.BlockObjective(union, wl.params, W.bym2_log_prior)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fits, secs = F.fit_blocks(obj, theta0, wl.block_ptr)
+    torch.cuda.synchronize()
+    t_union = time.perf_counter() - t0
+    log(f"   all {k} fits as blocks: {t_union:.2f} s ({secs['bfgs_rounds']} BFGS rounds, {obj.calls} calls)")
+    t0 = time.perf_counter()
+    alone = []
+    for q in range(k):
+        def params_q(th, q=q):
+            c, dc, _, _ = wl.params(np.repeat(np.atleast_2d(th)[:, None, :], k, axis=1))
+            return c[:, q], dc[:, q], None, None
+
+        alone.append(F.fit_batched(F.Objective(singles[q], params_q, W.bym2_log_prior), theta0[q]))
+    torch.cuda.synchronize()
+    t_alone = time.perf_counter() - t0
+    log(f"   the {k} models fitted one after another: {t_alone:.2f} s")
+    dmode = max(float(np.max(np.abs(f.mode - a.mode))) for f, a in zip(fits, alone))
+    dmean = max(float(np.max(np.abs(f.mean - a.mean)) / np.max(np.abs(a.mean))) for f, a in zip(fits, alone))
+    modes = np.array([f.mode for f in fits])
+    return {"value": k / t_union, "unit": "models fitted/s", "seconds": t_union,
+            "seconds_by_phase": {kk: secs[kk] for kk in ("mode", "hessian", "nodes")},
+            "bfgs_rounds": secs["bfgs_rounds"], "block_evaluations": obj.calls,
+            "models_alone_seconds": t_alone, "speedup_vs_alone": t_alone / t_union,
+            "max_abs_mode_diff_vs_alone": dmode, "max_rel_mean_diff_vs_alone": dmean,
+            "mode_range": [modes.min(axis=0).tolist(), modes.max(axis=0).tolist()],
+            "work": "per model: BFGS on the adjoint gradient from (0.5, 0.5), Hessian stencil, 9 CCD nodes with "
+                    "variances, mixture",
+            "workload": f"{k} BYM2 + Poisson models (intercept each), Delaunay graphs, sizes U[500, 20000]; "
+                        "priors loggamma(1, 5e-5) on tau, N(0, 1) on logit phi"}
 
 
 def big_solver_line(Q, label, probes=6):
@@ -884,6 +944,9 @@ def main():
         log("== heterogeneous Laplace: 64 BYM2 + Poisson models x 9 CCD points")
         res["hetero_laplace"] = hetero_laplace_line(cpu_sample=not args.no_cpu_baseline)
         log(json.dumps(res["hetero_laplace"]))
+        log("== heterogeneous fit: the 64 models' modes, Hessians and CCD nodes as block evaluations")
+        res["hetero_fit"] = hetero_fit_line()
+        log(json.dumps(res["hetero_fit"]))
     if args.c4:
         log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
         res["c4"] = c4_solver_line()

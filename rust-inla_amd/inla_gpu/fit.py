@@ -92,15 +92,16 @@ class Objective:
         return res
 
 
-def bfgs_mode(obj: Objective, theta0, max_iter=100, gtol=1e-3, xtol=1e-6, n_steps=4, max_step=1.0):
-    """Maximise g with BFGS on the adjoint gradient. Each iteration evaluates the
-    step lengths 1, 1/2, ..., 2^-(n_steps-1) along the search direction in one
-    batch and takes the longest one that satisfies the Armijo condition. Steps
-    are capped at ``max_step`` in every coordinate. Returns (theta, iterations)."""
+def _bfgs_steps(theta0, max_iter=100, gtol=1e-3, xtol=1e-6, n_steps=4, max_step=1.0):
+    """BFGS maximising g as a generator: it yields arrays of theta-points to evaluate and
+    receives (g, grad) at them; its return value is (theta, iterations). Each iteration
+    asks for the step lengths 1, 1/2, ..., 2^-(n_steps-1) along the search direction at
+    once and takes the longest one that satisfies the Armijo condition. Steps are capped
+    at ``max_step`` in every coordinate."""
     x = np.asarray(theta0, dtype=float).copy()
     m = len(x)
-    r = obj.values(x[None], grad=True)
-    f, g = -r["g"][0], -r["grad"][0]  # minimise f = -g
+    gv0, dv0 = yield x[None]
+    f, g = -gv0[0], -dv0[0]  # minimise f = -g
     H = np.eye(m)
     it = 0
     while it < max_iter:
@@ -117,12 +118,11 @@ def bfgs_mode(obj: Objective, theta0, max_iter=100, gtol=1e-3, xtol=1e-6, n_step
         ts = 0.5 ** np.arange(n_steps)
         accepted = None
         while accepted is None:
-            trial = x[None, :] + ts[:, None] * p[None, :]
-            rr = obj.values(trial, grad=True)
-            fv, gv = -rr["g"], -rr["grad"]
+            gv, dv = yield x[None, :] + ts[:, None] * p[None, :]
+            fv, gvv = -np.asarray(gv), -np.asarray(dv)
             for k, t in enumerate(ts):
                 if np.isfinite(fv[k]) and fv[k] <= f + 1e-4 * t * (g @ p):
-                    accepted = (t, fv[k], gv[k])
+                    accepted = (t, fv[k], gvv[k])
                     break
             if accepted is None:
                 ts = ts * 0.5 ** n_steps
@@ -142,6 +142,19 @@ def bfgs_mode(obj: Objective, theta0, max_iter=100, gtol=1e-3, xtol=1e-6, n_step
         if np.max(np.abs(s)) < xtol:
             break
     return x, it
+
+
+def bfgs_mode(obj: Objective, theta0, **kw):
+    """Maximise g with BFGS on the adjoint gradient (_bfgs_steps), each request one
+    batched evaluation. Returns (theta, iterations)."""
+    gen = _bfgs_steps(theta0, **kw)
+    pts = next(gen)
+    while True:
+        r = obj.values(pts, grad=True)
+        try:
+            pts = gen.send((r["g"], r["grad"]))
+        except StopIteration as e:
+            return e.value
 
 
 def nelder_mead_mode(obj: Objective, theta0, step=0.1, max_iter=200, tol=1e-6):
@@ -265,3 +278,106 @@ def fit_plain(obj: Objective, theta0, strategy="ccd", f0=1.0) -> FitResult:
     t3 = time.perf_counter()
     secs.update(mode=t1 - t0, hessian=t2 - t1, nodes=t3 - t2, total=t3 - t0)
     return _finish(mode, H, thetas, w, r, obj.calls, obj.points, secs)
+
+
+# ---------------------------------------------------------------------------
+# Independent models as the blocks of one LaplaceModel
+# ---------------------------------------------------------------------------
+class BlockObjective:
+    """log posteriors g_q(theta_q) = mlik_q + log pi_q(theta_q) of k independent models,
+    the blocks of one LaplaceModel (blocks=...): one evaluation takes theta-points of
+    shape (B, k, m), model q at its own theta[b, q].
+
+    ``params(thetas (B, k, m)) -> (coefs (B,k,K), dcoefs (B,k,m,K), lpar, dlog_lpar)``;
+    ``log_prior(theta (m,)) -> (value, gradient)`` (the same prior for every model)."""
+
+    def __init__(self, model, params, log_prior, max_iter=200, tol=1e-5):
+        self.model, self.params, self.log_prior = model, params, log_prior
+        self.k = model.nb
+        self.max_iter, self.tol = max_iter, tol
+        self.calls = 0
+        self.points = 0
+
+    def values(self, thetas, grad=False, want_x=False, want_var=False):
+        thetas = np.asarray(thetas, dtype=float)
+        if thetas.ndim == 2:
+            thetas = thetas[None]
+        acc = {k: [] for k in ("g", "grad", "x", "var", "status")}
+        cap = self.model.capacity
+        for i in range(0, len(thetas), cap):
+            th = thetas[i:i + cap]
+            c, dc, lp, dl = self.params(th)
+            r = self.model.eval(c, lpar=lp, dcoefs=dc if grad else None, dlog_lpar=dl if grad else None,
+                                want_x=want_x, want_var=want_var, max_iter=self.max_iter, tol=self.tol)
+            self.calls += 1
+            self.points += th.shape[0] * th.shape[1]
+            pri = [[self.log_prior(t) for t in row] for row in th]
+            acc["g"].append(r["mlik"].cpu().numpy() + np.array([[p[0] for p in row] for row in pri]))
+            acc["status"].append(r["status"])
+            if grad:
+                acc["grad"].append(r["grad"].cpu().numpy() + np.array([[p[1] for p in row] for row in pri]))
+            if want_x:
+                acc["x"].append(r["x"].cpu().numpy())
+            if want_var:
+                acc["var"].append(r["var"].cpu().numpy())
+        return {k: np.concatenate(v) for k, v in acc.items() if v}
+
+
+def bfgs_blocks(obj: BlockObjective, theta0, **kw):
+    """One BFGS (_bfgs_steps) per model, all advanced together: each round gathers the
+    theta-points every running search asks for into one (B, k, m) evaluation (padded
+    with the model's last point; finished models repeat their mode). Returns (modes
+    (k, m), iterations (k,), rounds)."""
+    theta0 = np.asarray(theta0, dtype=float)
+    k = theta0.shape[0]
+    gens = [_bfgs_steps(theta0[q], **kw) for q in range(k)]
+    want = [next(g) for g in gens]
+    done = [None] * k
+    rounds = 0
+    while any(d is None for d in done):
+        rounds += 1
+        nb = max(len(w) for q, w in enumerate(want) if done[q] is None)
+        T = np.empty((nb, k, theta0.shape[1]))
+        for q in range(k):
+            src = want[q] if done[q] is None else done[q][0][None]
+            T[:, q] = np.concatenate([src, np.repeat(src[-1:], nb - len(src), axis=0)])
+        r = obj.values(T, grad=True)
+        for q in range(k):
+            if done[q] is not None:
+                continue
+            nq = len(want[q])
+            try:
+                want[q] = gens[q].send((r["g"][:nq, q], r["grad"][:nq, q]))
+            except StopIteration as e:
+                done[q] = e.value
+    return np.array([d[0] for d in done]), np.array([d[1] for d in done]), rounds
+
+
+def fit_blocks(obj: BlockObjective, theta0, block_ptr, f0=1.0, **bfgs_kw):
+    """Every model's fit at once: BFGS modes (bfgs_blocks), each model's Hessian stencil
+    in one evaluation, each model's CCD nodes in one evaluation, and the mixture per
+    model (inference.rs:1246-1468 per model). Returns a list of FitResult, one per
+    model (latent moments over the model's own latent segment), and the timings."""
+    secs = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    modes, _, rounds = bfgs_blocks(obj, theta0, **bfgs_kw)
+    t1 = time.perf_counter()
+    k, m = modes.shape
+    hs = [_hessian_step(md) for md in modes]
+    st = np.stack([np.asarray(ccd.hessian_points(modes[q], hs[q])) for q in range(k)], axis=1)
+    gs = obj.values(st)["g"]
+    Hs = [ccd.hessian_from_values(gs[:, q], m, hs[q]) for q in range(k)]
+    t2 = time.perf_counter()
+    nodes = [_nodes(modes[q], Hs[q], "ccd", f0) for q in range(k)]
+    T = np.stack([nd[0] for nd in nodes], axis=1)
+    r = obj.values(T, want_x=True, want_var=True)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out = []
+    for q in range(k):
+        a, b = block_ptr[q], block_ptr[q + 1]
+        rq = {"g": r["g"][:, q], "x": r["x"][:, a:b], "var": r["var"][:, a:b]}
+        out.append(_finish(modes[q], Hs[q], T[:, q], nodes[q][1], rq, obj.calls, obj.points, {}))
+    secs.update(mode=t1 - t0, hessian=t2 - t1, nodes=t3 - t2, total=t3 - t0, bfgs_rounds=rounds)
+    return out, secs

@@ -554,6 +554,15 @@ class Bym2PoissonBatch:
         return self.y[self.obs_ptr[q]:self.obs_ptr[q + 1]]
 
 
+def bym2_log_prior(theta):
+    """log pi(theta) and its gradient for theta = (log tau, logit phi) of the BYM2 batch:
+    loggamma(1, 5e-5) on tau (the C1 prior) and N(0, 1) on logit phi (this workload's
+    choice; the reference's BYM2 defaults to a PC prior on phi, which needs the spectrum
+    of each graph)."""
+    v, g = loggamma_prior(theta[0])
+    return v - 0.5 * theta[1] ** 2 - 0.5 * math.log(2 * math.pi), np.array([float(np.ravel(g)[0]), -theta[1]])
+
+
 def bym2_poisson_batch(k=64, nmin=500, nmax=20000, seed=20241204, tau=1.0, phi=0.5, intercept=1.0,
                        int_prec=1e-3, models=None):
     """The heterogeneous Laplace batch: bym2_hetero_batch's k models (sizes uniform in
