@@ -92,3 +92,54 @@ def test_spde_gaussian_full_size_ccd_batch_matches_oracle(record_error):
     assert e_x < MODE_RTOL
     assert e_v < VAR_RTOL
     assert e_g < GRAD_RTOL, (grad[0], fd)
+
+
+def test_spde_poisson_full_size_ccd_batch_matches_oracle(record_error):
+    """The batched Poisson configuration at full size (the 142 x 142 SPDE mesh plus an
+    intercept, n = 20,165, 50,000 counts y ~ Poisson(exp(1 + u)), CCD m = 2 around the
+    generating theta with unit scaling: 9 points, each its own Newton loop) against the
+    oracle point by point, and the adjoint gradient at the centre against central
+    differences of the oracle's mlik."""
+    from inla_gpu import ccd
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.spde_poisson_workload()
+    g = wl.gmrf
+    assert g.n == 20165 and len(wl.y) == 50_000
+    z, _ = ccd.ccd_design(2, 1.0)
+    thetas = wl.theta_true[None] + 0.3 * z
+    coefs, dcoefs, _, _ = wl.params(thetas)
+    model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="poisson", A=wl.A,
+                         capacity=len(thetas))
+    out = model.eval(coefs, dcoefs=dcoefs)
+    assert (out["status"] == 0).all(), out["status"]
+    mlik, x, var, grad = (out[k].cpu().numpy() for k in ("mlik", "x", "var", "grad"))
+
+    def oracle_point(b):
+        ref = orc.find_mode(g.csc(coefs[b]), "poisson", "log", wl.y, 1.0, A=wl.A, keep_post=True)
+        return ref.mlik, ref.x, ref.post_factor().takahashi_diag()
+
+    def oracle_mlik(th):
+        cc, _, _, _ = wl.params(np.asarray(th)[None])
+        return orc.find_mode(g.csc(cc[0]), "poisson", "log", wl.y, 1.0, A=wl.A).mlik
+
+    h = 1e-4
+    fd_points = []
+    for l in range(2):
+        for s in (h, -h):
+            t = thetas[0].copy()
+            t[l] += s
+            fd_points.append(t)
+    with ThreadPoolExecutor(_workers()) as ex:
+        refs = list(ex.map(oracle_point, range(len(thetas))))
+        fdv = list(ex.map(oracle_mlik, fd_points))
+    for b, (ml, xo, vo) in enumerate(refs):
+        e_ml = record_error("mlik (Poisson, 9 points)", abs(mlik[b] - ml), MLIK_ATOL, "abs")
+        e_x = record_error("mode (Poisson, 9 points)", _rel(x[b], xo), MODE_RTOL)
+        e_v = record_error("variances (Poisson, 9 points)", _rel(var[b], vo), VAR_RTOL)
+        assert e_ml < MLIK_ATOL and e_x < MODE_RTOL and e_v < VAR_RTOL, (b, e_ml, e_x, e_v)
+    fd = np.array([(fdv[2 * l] - fdv[2 * l + 1]) / (2 * h) for l in range(2)])
+    e_g = record_error("gradient vs FD (Poisson, centre)", float(np.max(np.abs(grad[0] - fd) / np.maximum(np.abs(fd), 1.0))),
+                       GRAD_RTOL)
+    assert e_g < GRAD_RTOL, (grad[0], fd)
