@@ -533,7 +533,7 @@ void eval_gauss(gmrf_model* M, gmrf_eval_args* a, int B) {
   const hipStream_t cs = bt->stream;
   static const bool no_graph = std::getenv("GMRF_NO_GRAPH") != nullptr;
   const int mt = (a->dcoefs && a->n_theta > 0 && a->grad) ? a->n_theta : 0;
-  if (g_prof.on || no_graph) {
+  if (g_prof.on || no_graph || gmrf_batch::LevelClock::wanted()) {
     enqueue_gauss(M, a, B);
   } else {
     if (!M->gev_in) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iterator>
 #include <map>
@@ -876,6 +878,47 @@ struct gmrf_batch {
     std::vector<hipEvent_t> fac_a, fac_b, sel_a, sel_b;
   }* marks = nullptr;
 
+  // GMRF_LEVEL_TIMES set: the time of every level of a factorisation / selected
+  // inversion, from events at the level boundaries, printed on stderr when the phase
+  // ends (synchronises, so the model evaluates eagerly, without graph capture; a
+  // diagnostic of where the time goes in the tree).
+  struct LevelClock {
+    std::vector<hipEvent_t> ev;
+    bool on = false;
+    static bool wanted() {
+      static const bool want = std::getenv("GMRF_LEVEL_TIMES") != nullptr;
+      return want;
+    }
+    void start(size_t nl, hipStream_t st) {
+      on = wanted();
+      if (!on) return;
+      if (ev.size() < nl + 1) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        ev.assign(nl + 1, nullptr);
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+      }
+      check(hipEventRecord(ev[0], st), "hipEventRecord");
+    }
+    void mark(size_t i, hipStream_t st) {
+      if (on) check(hipEventRecord(ev[i], st), "hipEventRecord");
+    }
+    // interval i covers level i (top_down: level nl - 1 - i)
+    void report(const char* phase, size_t nl, int count, const std::vector<LevelInfo>& levels, bool top_down) {
+      if (!on) return;
+      check(hipEventSynchronize(ev[nl]), "hipEventSynchronize");
+      for (size_t i = 0; i < nl; ++i) {
+        const size_t l = top_down ? nl - 1 - i : i;
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "hipEventElapsedTime");
+        std::fprintf(stderr, "[gmrf] %s level %zu (%d matrices, %lld large fronts): %.4f ms\n", phase, l, count,
+                     (long long)(levels[l].le - levels[l].lb), ms);
+      }
+    }
+    ~LevelClock() {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+  } lclock;
+
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
     sym->ensure_device();
@@ -957,6 +1000,7 @@ struct gmrf_batch {
     double* Uf = U + (int64_t)first * P.u_size;
     double* Zf = Z + (int64_t)first * P.z_size;
     const Schedule sc = sym->schedule(count);
+    lclock.start(sym->levels.size(), stream);
     for (size_t li = 0; li < sym->levels.size(); ++li) {
       const LevelInfo& lv = sym->levels[li];
       if (marks) check(hipEventRecord(marks->fac_a[li], stream), "hipEventRecord");
@@ -1000,7 +1044,9 @@ struct gmrf_batch {
                       });
         }
       }
+      lclock.mark(li + 1, stream);
     }
+    lclock.report("factorisation", sym->levels.size(), count, sym->levels, false);
   }
   void logdet(int count, int first = 0) {
     const DevPlan& P = sym->plan;
@@ -1115,6 +1161,7 @@ struct gmrf_batch {
     const double* Zf = Z + (int64_t)first * P.z_size;
     const int32_t* tk = sym->d_tasks;
     const Schedule sc = sym->schedule(count);
+    lclock.start(sym->levels.size(), stream);
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
@@ -1182,7 +1229,9 @@ struct gmrf_batch {
         });
       });
       if (marks) check(hipEventRecord(marks->sel_b[l], stream), "hipEventRecord");
+      lclock.mark(sym->levels.size() - l, stream);
     }
+    lclock.report("selected inversion", sym->levels.size(), count, sym->levels, true);
   }
 };
 
