@@ -159,3 +159,54 @@ def test_besag_ar1_lowrank_gradient_matches_central_fd(record_error):
         fd.append((vals[0] - vals[1]) / (2 * h))
     err = record_error("gradient vs FD (low-rank)", _rel(gr, np.array(fd)), 1e-5)
     assert err < 1e-5, (gr, fd)
+
+
+def test_besag_ar1_full_size_properties(record_error):
+    """The full-size space-time configuration (100 x 100 rook lattice x T = 20,
+    n = 200,000, binomial(10), 20 sum-to-zero rows as the low-rank augmentation), where
+    the oracle does not reach, through size-independent properties: every point of a
+    CCD-like batch converges; the mlik, mode and variances do not depend on the fill-
+    reducing order (AMD against the chooser's RCM: the same matrices factorised along
+    different elimination trees); the constraints hold at every mode; and the adjoint
+    gradient matches central differences of the evaluation's own mlik."""
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    nr, T = 100, 20
+    wl = W.besag_ar1_workload(nr=nr, T=T)
+    ca, ce = wl.constraints
+    ca = ca / math.sqrt(nr * nr)
+    g = wl.gmrf
+    assert g.n == 200_000
+
+    def make(order, cap):
+        return LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="binomial", obs_a=wl.obs_a,
+                            constraints=(ca, ce), constraint_kappa=KAPPA, capacity=cap, order=order)
+
+    thetas = wl.theta_true[None] + np.array([[0.0, 0.0], [0.25, 0.0], [0.0, -0.25], [-0.2, 0.2]])
+    c, dc, _, _ = wl.params(thetas)
+    amd = make("amd", 4)
+    ra = amd.eval(c, dcoefs=dc)
+    assert (ra["status"] == 0).all(), ra["status"]
+    rc = make("chooser", 4).eval(c)
+    assert (rc["status"] == 0).all()
+    ml_a, ml_c = ra["mlik"].cpu().numpy(), rc["mlik"].cpu().numpy()
+    e_ml = record_error("mlik AMD vs RCM (n = 200k)", float(np.max(np.abs(ml_a - ml_c))), 1e-8, "abs")
+    e_x = record_error("mode AMD vs RCM (n = 200k)", _rel(ra["x"].cpu().numpy(), rc["x"].cpu().numpy()), 1e-10)
+    e_v = record_error("variances AMD vs RCM (n = 200k)", _rel(ra["var"].cpu().numpy(), rc["var"].cpu().numpy()), 1e-9)
+    assert e_ml < 1e-8 and e_x < 1e-10 and e_v < 1e-9
+    assert np.max(np.abs(ca @ ra["x"].cpu().numpy().T)) < 1e-9
+    h = 1e-4
+    one = make("amd", 1)
+    gr = ra["grad"][0].cpu().numpy()
+    fd = []
+    for l in range(2):
+        v = []
+        for s in (h, -h):
+            t = thetas[0].copy()
+            t[l] += s
+            cc, _, _, _ = wl.params(t[None])
+            v.append(one.eval(cc, want_x=False, want_var=False)["mlik"][0].item())
+        fd.append((v[0] - v[1]) / (2 * h))
+    e_g = record_error("gradient vs FD (n = 200k)", float(np.max(np.abs(gr - fd) / np.maximum(np.abs(fd), 1.0))), 1e-5)
+    assert e_g < 1e-5, (gr, fd)
