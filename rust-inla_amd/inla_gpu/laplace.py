@@ -20,7 +20,6 @@ from __future__ import annotations
 import ctypes as C
 
 import numpy as np
-import scipy.sparse as sp
 import torch
 
 from . import _lib as L

@@ -10,7 +10,6 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-import matrices as M
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
