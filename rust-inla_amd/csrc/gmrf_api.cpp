@@ -758,6 +758,10 @@ gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_
     return fail(GMRF_INVALID_ARGUMENT, "roof_ms needs positive peaks");
   try {
     std::lock_guard<std::mutex> lk(g_prof.mu);
+    // GMRF_PROFILE_DUMP=<path>: every recorded launch (name, flops, bytes, ms) appended
+    // there as one text line (a diagnostic of launch shapes against durations)
+    const char* dump_path = std::getenv("GMRF_PROFILE_DUMP");
+    std::FILE* dump = dump_path ? std::fopen(dump_path, "a") : nullptr;
     std::vector<std::string> keys;
     std::vector<int64_t> cnt;
     std::vector<double> ms, fl, by, rf;
@@ -774,6 +778,7 @@ gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_
         by.push_back(0);
         rf.push_back(0);
       }
+      if (dump) std::fprintf(dump, "%s %.6e %.6e %.6f\n", r.name, r.flops, r.bytes, (double)t);
       cnt[k]++;
       ms[k] += t;
       fl[k] += r.flops;
@@ -792,6 +797,7 @@ gmrf_status gmrf_profile_summary(int32_t max_entries, char* names, int32_t name_
       if (bytes) bytes[k] = by[k];
       if (roof_ms) roof_ms[k] = rf[k];
     }
+    if (dump) std::fclose(dump);
     return GMRF_OK;
   } catch (...) {
     return from_exception();
