@@ -102,9 +102,10 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
 
 
 // Tuned schedules persist as text lines in the file named by GMRF_TUNE_FILE (if set):
-//   gmrf-schedule 1 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
-//   <xcd_max> <factorisation merge masks> <selected-inversion merge masks>
+//   gmrf-schedule 2 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
+//   <xcd_max> <factorisation merge masks> <selected-inversion merge masks> <stagger>
 // one hex digit per level for each mask string; a later line for the same key wins.
+// Version 1 lines (no stagger field) are read with stagger -1.
 void SymData::load_schedules() {
   const char* path = std::getenv("GMRF_TUNE_FILE");
   if (!path) return;
@@ -118,7 +119,10 @@ void SymData::load_schedules() {
     Schedule sc;
     if (!(ss >> tag >> ver >> hash >> n >> nl >> count >> sc.fuse_below >> sc.split_max >> sc.xcd_max >> mf >> ms))
       continue;
-    if (tag != "gmrf-schedule" || ver != 1 || n != S.n || nl != (int64_t)S.nlevels || count < 1) continue;
+    if (tag != "gmrf-schedule" || (ver != 1 && ver != 2) || n != S.n || nl != (int64_t)S.nlevels || count < 1)
+      continue;
+    if (ver == 2 && !(ss >> sc.stagger)) continue;
+    if (sc.stagger < -1 || sc.stagger >= nl) continue;
     if (std::strtoull(hash.c_str(), nullptr, 16) != S.pattern_hash) continue;
     if ((int64_t)mf.size() != nl || (int64_t)ms.size() != nl) continue;
     for (int64_t l = 0; l < nl; ++l) {
@@ -141,9 +145,9 @@ void save_schedule(const SymData& d, int count, const Schedule& sc) {
     mf += "01234567"[sc.fac_mask(l) & 7];
     ms += "01234567"[sc.sel_mask(l) & 7];
   }
-  std::fprintf(f, "gmrf-schedule 1 %016llx %lld %d %d %lld %lld %d %s %s\n", (unsigned long long)d.S.pattern_hash,
+  std::fprintf(f, "gmrf-schedule 2 %016llx %lld %d %d %lld %lld %d %s %s %d\n", (unsigned long long)d.S.pattern_hash,
                (long long)d.S.n, d.S.nlevels, count, (long long)sc.fuse_below, (long long)sc.split_max, sc.xcd_max,
-               mf.c_str(), ms.c_str());
+               mf.c_str(), ms.c_str(), sc.stagger);
   std::fclose(f);
 }
 
@@ -162,8 +166,13 @@ double time_schedule(gmrf_batch* bt, int count, const double* vals, int64_t ldv,
     for (int r = 0; r < reps; ++r) {
       bt->marks = (marks && r == reps - 1) ? marks : nullptr;
       check(hipEventRecord(e0, bt->stream), "hipEventRecord");
-      bt->factorize(count, vals, ldv);
-      bt->selinv(count);
+      const int split = bt->stagger_level(count);
+      if (split >= 0) {
+        bt->factor_selinv_staggered(count, vals, ldv, split);
+      } else {
+        bt->factorize(count, vals, ldv);
+        bt->selinv(count);
+      }
       check(hipEventRecord(e1, bt->stream), "hipEventRecord");
       check(hipEventSynchronize(e1), "hipEventSynchronize");
       float ms = 0.0f;
@@ -258,6 +267,19 @@ gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }
+    // 3. with GMRF_TUNE_STAGGER set: two staggered pipelines of half the batch, from
+    //    each level of the lower two thirds of the tree (the second half's lower
+    //    levels beside the first's upper levels). Opt-in: each launch then covers half
+    //    the matrices, and model evaluations with a staggered schedule run eagerly
+    //    instead of from a captured graph.
+    const bool try_stagger = std::getenv("GMRF_TUNE_STAGGER") != nullptr;
+    if (try_stagger && count >= 2)
+      for (int32_t v = 0; v < (int32_t)(2 * nl / 3); ++v) {
+        Schedule s = cur;
+        s.stagger = v;
+        const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+        if (t < t_cur) t_cur = t, cur = s;
+      }
     // keep the tuned schedule only if it beats the defaults on a fresh measurement
     const double t_def2 = time_schedule(bt, count, vals, ldv, def, reps, nullptr, nullptr, nullptr);
     const double t_best = time_schedule(bt, count, vals, ldv, cur, reps, nullptr, nullptr, nullptr);

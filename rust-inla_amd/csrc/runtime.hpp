@@ -352,12 +352,19 @@ using namespace gmrf_rt;
 //  - fuse_below: launches of fewer workgroups also factor the next diagonal block;
 //  - split_max: Sigma[B,P] launches of at most this many row tiles split K;
 //  - xcd_max: Sigma[B,P] launches with at most this many tasks per matrix keep a
-//    matrix's tiles on one XCD.
+//    matrix's tiles on one XCD;
+//  - stagger (>= 0): the batch runs as two pipelines of half the matrices each
+//    (factorisation, log determinants, selected inversion), the second on a stream of
+//    its own that starts once the first has enqueued the factorisation of level
+//    `stagger`, so that its lower levels (thousands of small fronts) fill the device
+//    while the first walks its upper levels (a few large fronts, launches of few
+//    workgroups). -1: one pipeline of all matrices.
 struct Schedule {
   std::vector<uint8_t> merge_fac, merge_sel;
   int64_t fuse_below = kFuseDiagBelow;
   int64_t split_max = kGemmSplitMaxTasks;
   int32_t xcd_max = kXcdMaxTasks;
+  int32_t stagger = -1;
   uint8_t fac_mask(size_t l) const { return l < merge_fac.size() ? merge_fac[l] : 0; }
   uint8_t sel_mask(size_t l) const { return l < merge_sel.size() ? merge_sel[l] : 0; }
 };
@@ -872,6 +879,12 @@ struct gmrf_batch {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* TS = nullptr;
   bool sel_inflight = false;
+  // the second pipeline of a staggered batch (Schedule::stagger): its own stream,
+  // the event it starts from (recorded mid-factorisation on the main stream) and the
+  // event that ends it
+  hipStream_t aux2 = nullptr;
+  hipEvent_t ev_mid = nullptr, ev_p2 = nullptr;
+  bool p2_inflight = false;
   // autotuner: events around each level's small-front launches (a[l] before, b[l]
   // after; factorisation in fac, selected inversion in sel), recorded when set
   struct TuneMarks {
@@ -947,6 +960,12 @@ struct gmrf_batch {
       (void)hipEventDestroy(ev_fork);
       (void)hipEventDestroy(ev_join);
     }
+    if (aux2) {
+      (void)hipStreamSynchronize(aux2);
+      (void)hipStreamDestroy(aux2);
+      (void)hipEventDestroy(ev_mid);
+      (void)hipEventDestroy(ev_p2);
+    }
     if (own) {
       (void)hipStreamSynchronize(own);
       (void)hipStreamDestroy(own);
@@ -979,10 +998,63 @@ struct gmrf_batch {
     check(hipEventRecord(ev_join, aux), "hipEventRecord");
     sel_inflight = true;
   }
+  // The main stream waits for the work on the other streams: the selected inversion
+  // of selinv_async and the second pipeline of a staggered batch.
   void selinv_join() {
-    if (!sel_inflight) return;
-    check(hipStreamWaitEvent(stream, ev_join, 0), "hipStreamWaitEvent");
-    sel_inflight = false;
+    if (sel_inflight) {
+      check(hipStreamWaitEvent(stream, ev_join, 0), "hipStreamWaitEvent");
+      sel_inflight = false;
+    }
+    if (p2_inflight) {
+      check(hipStreamWaitEvent(stream, ev_p2, 0), "hipStreamWaitEvent");
+      p2_inflight = false;
+    }
+  }
+
+  // Whether a batch of `count` matrices runs as two staggered pipelines, and from
+  // which level (Schedule::stagger); never while level clocks or tuning marks
+  // record, whose events serve one pipeline.
+  // GMRF_STAGGER=<level> (read at each call; -1 for one pipeline) overrides the
+  // schedule: a diagnostic for timing the levels against each other.
+  int stagger_level(int count) const {
+    if (count < 2 || marks || LevelClock::wanted()) return -1;
+    const char* env = std::getenv("GMRF_STAGGER");
+    const int lv = env ? std::atoi(env) : sym->schedule(count).stagger;
+    return lv < (int)sym->levels.size() ? lv : -1;
+  }
+
+  // The second pipeline of a staggered batch: matrices [first, first + count) on
+  // their own stream, from the point where the main stream's factorisation has
+  // enqueued the marked level (factorize_levels with mark_level): factorisation,
+  // log determinants and, with `inv`, selected inversion. selinv_join waits for it.
+  // Must be enqueued before anything else joins (selinv_async joins first).
+  void pipeline2_async(int count, const double* vals, int64_t ldv, int first, bool inv) {
+    hipStream_t main = stream;
+    stream = aux2;
+    try {
+      check(hipStreamWaitEvent(aux2, ev_mid, 0), "hipStreamWaitEvent");
+      factorize_levels(count, vals, ldv, first, -1, false);
+      logdet(count, first);
+      if (inv) selinv(count, first, false);
+      check(hipEventRecord(ev_p2, aux2), "hipEventRecord");
+    } catch (...) {
+      stream = main;
+      throw;
+    }
+    stream = main;
+    p2_inflight = true;
+  }
+
+  // Factorisation, log determinants and selected inversion of matrices [0, count)
+  // as two staggered pipelines from level `split`: [0, c0) on the main stream,
+  // [c0, count) beside it (c0 = ceil(count / 2)); the main stream waits for both.
+  void factor_selinv_staggered(int count, const double* vals, int64_t ldv, int split) {
+    const int c0 = (count + 1) / 2, c1 = count - c0;
+    factorize_levels(c0, vals, ldv, 0, split);
+    pipeline2_async(c1, vals + (int64_t)c0 * ldv, ldv, c0, true);
+    logdet(c0, 0);
+    selinv(c0, 0, false);
+    selinv_join();
   }
 
   // Matrices [first, first + count) of the batch: the level loop, then the log
@@ -993,8 +1065,16 @@ struct gmrf_batch {
   }
   // The factors only (L, Z); logdet() completes the factorisation. A selected
   // inversion may be started in between (it reads L and Z only).
-  void factorize_levels(int count, const double* vals, int64_t ldv, int first = 0) {
-    selinv_join();
+  // With mark_level >= 0 the event that starts a second pipeline (pipeline2_async)
+  // is recorded once that level has been enqueued.
+  void factorize_levels(int count, const double* vals, int64_t ldv, int first = 0, int mark_level = -1,
+                        bool join = true) {
+    if (join) selinv_join();
+    if (mark_level >= 0 && !aux2) {
+      check(hipStreamCreateWithFlags(&aux2, hipStreamNonBlocking), "hipStreamCreate");
+      check(hipEventCreateWithFlags(&ev_mid, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&ev_p2, hipEventDisableTiming), "hipEventCreate");
+    }
     const DevPlan& P = sym->plan;
     double* Lf = L + (int64_t)first * P.l_size;
     double* Uf = U + (int64_t)first * P.u_size;
@@ -1045,6 +1125,7 @@ struct gmrf_batch {
         }
       }
       lclock.mark(li + 1, stream);
+      if ((int)li == mark_level) check(hipEventRecord(ev_mid, stream), "hipEventRecord");
     }
     lclock.report("factorisation", sym->levels.size(), count, sym->levels, false);
   }
@@ -1150,8 +1231,8 @@ struct gmrf_batch {
     check(launch_scatter_perm(P, count * nr, Xf, x, ldx, stream), "scatter");
   }
 
-  void selinv(int count, int first = 0) {
-    if (stream != aux) selinv_join();
+  void selinv(int count, int first = 0, bool join = true) {
+    if (join && stream != aux) selinv_join();
     const DevPlan& P = sym->plan;
     const double* Lf = L + (int64_t)first * P.l_size;
     double* Sf = S + (int64_t)first * P.l_size;
