@@ -813,6 +813,64 @@ def spde_rw2_workload(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=
     return KronWorkload(g, A, y, np.full(n_obs, math.exp(0.5)), np.full(n_obs, size), "nbinomial", None, th, params)
 
 
+def spde_rw2_five_workload(nx=245, T=24, n_obs=500_000, seed=20241208, size=5.0, eps=1e-3, field="sample"):
+    """The SPDE x RW2 space-time model with five hyperparameters, on the pattern of
+    spde_rw2_workload (group-major, n = nx^2 T):
+
+        Q(theta) = Q_spde(tau1, kappa1) x (R_rw2 + eps I) + Q_spde(tau2, kappa2) x I_T + tau3 I,
+
+    a space-time smoothness term, a per-slice spatial term and a nugget;
+    theta = (log tau1, log kappa1, log tau2, log kappa2, log tau3), seven affine
+    components (the three SPDE matrices under each temporal factor, then I). Negative
+    binomial (size) counts uniform in space x time through the barycentric projector of
+    their slice; field="sample" draws x from Q(theta_true) (test sizes), "smooth" uses
+    the deterministic field of spde_rw2_workload (full size). Priors: independent
+    N(theta_true_k, 1)."""
+    g_s, _, _ = spde_components(nx, nx)
+    Rt = sp.csc_matrix(rw2_structure(T) + eps * sp.identity(T))
+    It = sp.identity(T, format="csc")
+    Cs = [sp.csc_matrix((g_s.components[k], g_s.indices, g_s.indptr), shape=(g_s.n, g_s.n)) for k in range(3)]
+    mats = [sp.kron(Rt, Cs[k], format="csc") for k in range(3)] + \
+           [sp.kron(It, Cs[k], format="csc") for k in range(3)] + [sp.identity(g_s.n * T, format="csc")]
+    indptr, indices = union_pattern(mats)
+    comps = np.stack([values_on_pattern(m_, indptr, indices) for m_ in mats])
+    del mats
+    n = g_s.n * T
+    g = AffineGmrf(n, indptr, indices, comps, ["Rt x C0", "Rt x G1", "Rt x G1C0invG1",
+                                               "It x C0", "It x G1", "It x G1C0invG1", "I"])
+    rng = np.random.default_rng(seed)
+    A, pts, tslot = _space_time_projector(nx, T, n_obs, rng)
+    th = np.array([math.log(0.5), math.log(3.0), math.log(0.5), math.log(6.0), math.log(1.0)])
+
+    def params(thetas):
+        B = len(thetas)
+        coefs = np.zeros((B, 7))
+        dco = np.zeros((B, 5, 7))
+        for b, t_ in enumerate(thetas):
+            c1, d1 = spde_coefs(t_[0], t_[1])
+            c2, d2 = spde_coefs(t_[2], t_[3])
+            coefs[b, :3], coefs[b, 3:6], coefs[b, 6] = c1, c2, math.exp(t_[4])
+            dco[b, 0:2, 0:3] = d1
+            dco[b, 2:4, 3:6] = d2
+            dco[b, 4, 6] = math.exp(t_[4])
+        return coefs, dco, None, None
+
+    def log_prior(t_):
+        d = np.asarray(t_, dtype=float) - th
+        return -0.5 * float(d @ d) - 2.5 * math.log(2 * math.pi), -d
+
+    if field == "sample":
+        c, _, _, _ = params(th[None])
+        x = sample_gmrf(g.csc(c[0]), rng)
+        mu = np.exp(0.5 + A @ x)
+    else:
+        mu = np.exp(0.5 + 0.6 * np.sin(2 * math.pi * pts[:, 0]) * np.cos(2 * math.pi * pts[:, 1])
+                    + 0.3 * np.cos(2 * math.pi * tslot / T))
+    y = rng.negative_binomial(size, size / (size + mu)).astype(float)
+    return KronWorkload(g, A, y, np.full(n_obs, math.exp(0.5)), np.full(n_obs, size), "nbinomial", None, th, params,
+                        log_prior)
+
+
 def spde_rw2_posterior_precision(nx=245, T=24, n_obs=500_000, seed=20241207, size=5.0, eps=1e-3,
                                  log_tau=math.log(0.5), log_kappa=math.log(3.0), intercept=0.5):
     """Q_post = Q_spde(tau, kappa) x (R_rw2 + eps I) + A^T W A of the SPDE x RW2 negative
