@@ -663,13 +663,16 @@ def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
                               "Q_post at tau=0.5, kappa=3", probes)
 
 
-def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
-    """Full-size SPDE x RW2 negative binomial Laplace evaluation (n = 1,440,600, 500,000
-    observations, size 5) at one theta-point with the adjoint gradient and the marginal
-    variances, in the lean form (prior factorised and inverted alone first: one factor
-    and one selected inverse on the device at a time). Synthetic counts from a smooth
-    space-time field (no draw from Q at this size). With fd, the gradient's first
-    component against central differences (step 1e-4) of two further evaluations."""
+def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd_components=(1, 4)):
+    """Full-size SPDE x RW2 negative binomial Laplace evaluation with five hyperparameters
+    (n = 1,440,600, 500,000 observations, size 5; Q = Q_spde(tau1, kappa1) x (R_rw2 +
+    eps I) + Q_spde(tau2, kappa2) x I_T + tau3 I) at one theta-point with the adjoint
+    gradient and the marginal variances, in the lean form (prior factorised and inverted
+    alone first: one factor and one selected inverse on the device at a time).
+    Synthetic counts from a smooth space-time field (no draw from Q at this size). The
+    gradient components in fd_components against central differences of further
+    evaluations (two steps and their Richardson extrapolation). The m = 5 CCD design has
+    27 nodes; one node per call is what the device holds at this size."""
     import numpy as np
     import torch
 
@@ -677,9 +680,10 @@ def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
     from inla_gpu.laplace import LaplaceModel
 
     t0 = time.perf_counter()
-    wl = W.spde_rw2_workload(nx=nx, T=T, n_obs=n_obs, field="smooth")
+    wl = W.spde_rw2_five_workload(nx=nx, T=T, n_obs=n_obs, field="smooth")
     g = wl.gmrf
-    log(f"[c5 laplace] workload n={g.n} nnz={len(g.indices)} in {time.perf_counter() - t0:.1f} s")
+    log(f"[c5 laplace] workload n={g.n} nnz={len(g.indices)} K={g.components.shape[0]} in "
+        f"{time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="nbinomial", A=wl.A,
                          obs_a=wl.obs_a, obs_b=wl.obs_b, capacity=1, lean=True, order=order)
@@ -693,7 +697,6 @@ def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
     r = model.eval(c, dcoefs=dc)
     torch.cuda.synchronize()
     t_eval = time.perf_counter() - t0
-    peak = torch.cuda.max_memory_allocated()
     free2, _ = torch.cuda.mem_get_info()
     mlik = float(r["mlik"][0].item())
     grad = r["grad"][0].cpu().numpy()
@@ -701,23 +704,25 @@ def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
     log(f"[c5 laplace] one theta-point with gradient: {t_eval:.1f} s, {int(r['n_iter'][0])} Newton steps, "
         f"mlik {mlik:.6f}")
     res = {"value": 1.0 / t_eval, "unit": "theta-points/s", "s_per_point": t_eval,
+           "ccd_27_nodes_s_computed": 27 * t_eval,
            "newton_iters": int(r["n_iter"][0]), "status_ok": bool(r["status"][0] == 0),
-           "n": int(g.n), "n_obs": int(n_obs), "nnz_q": int(len(g.indices)), "order": order,
+           "n": int(g.n), "n_obs": int(n_obs), "nnz_q": int(len(g.indices)), "components": int(g.components.shape[0]),
+           "n_theta": int(len(th)), "order": order,
            "mlik": mlik, "grad": [float(v) for v in grad],
            "var_min_max": [float(var.min()), float(var.max())], "setup_s": t_setup,
            "device_free_gb_before_eval": free / 1e9, "device_free_gb_after_eval": free2 / 1e9,
            "device_total_gb": total / 1e9,
            "work": "prior LDLT + selected inversion, Newton to the mode (one LDLT + solve per step), "
                    "posterior selected inversion, mlik, variances, adjoint gradient (one extra solve)",
-           "workload": f"spde {nx}x{nx} x rw2(T={T}), n={g.n}, {n_obs} NB(size 5) obs from a smooth field, "
-                       "lean evaluation at theta=(log 0.5, log 3)"}
-    if fd:
+           "workload": f"spde {nx}x{nx} x rw2(T={T}) + per-slice spde + nugget, n={g.n}, {n_obs} NB(size 5) obs "
+                       "from a smooth field, 5 hyperparameters, lean evaluation at theta_true"}
+    if fd_components:
         # central differences at two steps and their Richardson extrapolation
         # (g(h) = g + c h^2 + ...): at this size the mlik of an evaluation carries a
         # rounding noise of order 1e-3 (a sum over 1.44M log pivots and 500k terms),
         # which a step of 1e-4 would amplify to O(10)
         fdres = []
-        for l in range(len(th)):
+        for l in fd_components:
             est = []
             for h in (1e-2, 5e-3):
                 vals = []
@@ -731,7 +736,7 @@ def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd=True):
                     log(f"[c5 laplace] FD evaluation theta[{l}] {sgn:+.0e}: {time.perf_counter() - t0:.1f} s")
                 est.append((vals[0] - vals[1]) / (2 * h))
             rich = (4.0 * est[1] - est[0]) / 3.0
-            fdres.append({"fd_h1e-2": est[0], "fd_h5e-3": est[1], "richardson": rich,
+            fdres.append({"component": int(l), "fd_h1e-2": est[0], "fd_h5e-3": est[1], "richardson": rich,
                           "adjoint": float(grad[l]), "rel_err_vs_richardson": abs(grad[l] - rich) / max(abs(rich), 1.0)})
         res["gradient_vs_fd"] = fdres
     return res
@@ -970,7 +975,7 @@ def main():
         res["c5"] = c5_solver_line()
         log(json.dumps(res["c5"]))
     if args.c5_laplace:
-        log("== full-size SPDE x RW2 negative binomial Laplace (n = 1.44M, lean)")
+        log("== full-size SPDE x RW2 negative binomial Laplace, five hyperparameters (n = 1.44M, lean)")
         res["c5_laplace"] = c5_laplace_line(order=args.c5_order)
         log(json.dumps(res["c5_laplace"]))
     if args.cpu_fit:
