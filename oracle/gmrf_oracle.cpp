@@ -15,7 +15,10 @@
 //    permutation-invariant up to rounding, which is what the tests compare.
 //  * LDL^T, solve, log|Q|: up-looking simplicial LDL^T (no pivoting, no
 //    regularisation), singular threshold 1e-14 as sparse_ldlt.rs:118-119;
-//    pinned by dense brute force, residuals and closed forms (AR1 log-det).
+//    pinned by dense brute force, residuals and closed forms (AR1 log-det). The
+//    ordering of the last pattern factorised on a thread is reused for the same
+//    pattern, as the reference's thread-local pattern cache does
+//    (sparse_ldlt.rs:200-207,234; scratch.rs:90-134).
 //  * Takahashi diag: restated loop-for-loop from sparse_ldlt.rs:357-432,
 //    including the binary-search lookup; pinned against dense inverses and the
 //    reference's own known-answer tests (sparse_ldlt.rs:613-665).
@@ -514,17 +517,34 @@ void orc_min_degree(i64 n, const i64* ip, const i64* ix, i64* order_out) {
 
 // Factorise; perm (perm[k] = original index) may be NULL: then the chooser's
 // branch is followed (time-major / RCM permutation, else exact minimum degree).
-// *status: 0 ok, 1 singular.
+// The ordering of the last pattern factorised on this thread is kept and reused when
+// the same pattern comes again, as factorize_sparse reuses its symbolic analysis
+// through the thread-local scratch's single-entry pattern cache
+// (crates/inla_math/src/sparse_ldlt.rs:200-207,234; scratch.rs:90-134). The cache
+// is bypassed while a deliberate mistake is switched on (orc_plant), so that the
+// ordering pins see the mistake. *status: 0 ok, 1 singular.
 void* orc_factor(i64 n, const i64* ip, const i64* ix, const double* x, const i64* perm, int* status) {
+  struct PatternCache {
+    std::vector<i64> indptr, indices, perm;
+  };
+  thread_local PatternCache cache;
   std::vector<i64> p(n);
   if (perm) {
     std::copy(perm, perm + n, p.begin());
+  } else if (g_plant == 0 && (i64)cache.indptr.size() == n + 1 && std::equal(ip, ip + n + 1, cache.indptr.begin()) &&
+             std::equal(ix, ix + ip[n], cache.indices.begin())) {
+    p = cache.perm;
   } else {
     std::vector<i64> fwd;
     if (choose(n, ip, ix, fwd)) {
       for (i64 o = 0; o < n; ++o) p[fwd[o]] = o;
     } else {
       min_degree(n, ip, ix, p);
+    }
+    if (g_plant == 0) {
+      cache.indptr.assign(ip, ip + n + 1);
+      cache.indices.assign(ix, ix + ip[n]);
+      cache.perm = p;
     }
   }
   Handle* h = new Handle;

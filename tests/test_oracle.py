@@ -206,3 +206,25 @@ def test_ccd_design_reference_properties():
     n_exp = len(z)
     wexp = 1.0 / ((n_exp - 1) * (1.0 + math.exp(-0.5 * f0 * f0) * (f0 * f0 / m - 1.0)))
     assert abs(w[1] - wexp) < 1e-15 and abs(w[0] - (1 - (n_exp - 1) * wexp)) < 1e-15
+
+
+def test_ordering_reused_for_a_repeated_pattern():
+    # the oracle keeps the last pattern's ordering per thread (the reference's
+    # factorize_sparse with its thread-local pattern cache, sparse_ldlt.rs:200-207,234):
+    # the same pattern with new values gets the same ordering and the results of an
+    # explicitly ordered factorisation; another pattern gets its own ordering
+    q1 = M.random_spd(300, 0.02, 5)
+    q2 = sp.csc_matrix((q1.data * 1.7 + 0.3 * (q1.indices == np.repeat(np.arange(300), np.diff(q1.indptr))),
+                        q1.indices, q1.indptr), shape=q1.shape)
+    f1 = orc.Factor(q1)
+    f2 = orc.Factor(q2)
+    assert np.array_equal(f1.perm(), f2.perm())
+    g2 = orc.Factor(q2, perm=orc.min_degree(q2))
+    assert np.array_equal(f2.perm(), g2.perm())
+    assert f2.log_abs_det() == g2.log_abs_det()
+    rhs = np.random.default_rng(0).standard_normal(300)
+    assert np.array_equal(f2.solve(rhs), g2.solve(rhs))
+    q3 = M.random_spd(300, 0.02, 6)
+    f3 = orc.Factor(q3)
+    assert np.array_equal(f3.perm(), orc.Factor(q3, perm=orc.min_degree(q3)).perm())
+    assert abs(f3.log_abs_det() - np.linalg.slogdet(q3.toarray())[1]) < 1e-9 * abs(f3.log_abs_det())
