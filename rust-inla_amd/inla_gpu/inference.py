@@ -79,6 +79,16 @@ class InferenceResult:
     evaluations: int = 0
     extra: dict = field(default_factory=dict)
 
+    def lincomb(self, combs):
+        """Gaussian summaries of v = a^T x (the binding's lincomb, py-inla/src/lib.rs:339-366):
+        combs is ``[(name, [(latent_index, weight), ...]), ...]``; the variance comes from
+        the posterior precision stored at the mode (posterior.rs:53-91)."""
+        if self.posterior_precision is None:
+            raise InvalidArgument("lincomb: posterior precision was not stored")
+        from .posterior import lincomb_summaries
+
+        return lincomb_summaries(self.latent_means, self.posterior_precision, combs)
+
 
 # --------------------------------------------------------------------------- observations
 def parse_obs(obs, gaussian_free_prec=False):

@@ -45,7 +45,8 @@ def _dev(t, shape=None):
     if t is None:
         return None
     if not isinstance(t, torch.Tensor):
-        t = torch.as_tensor(np.asarray(t, dtype=np.float64))
+        a = np.asarray(t, dtype=np.float64)
+        t = torch.as_tensor(a if a.flags.writeable else a.copy())
     t = t.to(device="cuda", dtype=torch.float64).contiguous()
     if shape is not None:
         t = t.reshape(shape)
