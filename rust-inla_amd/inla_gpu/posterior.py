@@ -5,8 +5,9 @@ combination v = a^T x, mean a^T mu from the mixture mean and Var(v) = a^T Q^-1 a
 the posterior precision stored at the mode (the Gaussian-at-mode approximation), with
 the 1.96 sd bounds. The reference factorises Q once (FaerCpuSolver, posterior.rs:68-69)
 and solves one combination at a time; here Q is factorised once on the device and all
-combination vectors are solved in one multi-right-hand-side launch sequence
-(gmrf_batch_solve_multi), in chunks of ``chunk`` vectors.
+combination vectors are solved in multi-right-hand-side launch sequences
+(gmrf_batch_solve_multi) of up to ``chunk`` vectors (fewer for large n: at most 256 MB
+of dense vectors per sequence).
 """
 
 from __future__ import annotations
@@ -54,6 +55,8 @@ def lincomb_summaries(mean, q_post, combs, chunk=256):
     bt.factorize(torch.as_tensor(q.data[None, :], dtype=torch.float64, device="cuda"))
     _, status = bt.logdet()
     check(int(status[0]))
+    # at most 2^25 doubles (256 MB) of dense combination vectors per solve
+    chunk = max(1, min(int(chunk), (1 << 25) // max(n, 1)))
     out = []
     for c0 in range(0, len(combs), chunk):
         part = combs[c0:c0 + chunk]
