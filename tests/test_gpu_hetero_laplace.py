@@ -159,3 +159,50 @@ def test_block_arguments_are_checked(batch):
     with pytest.raises(InvalidArgument):
         LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A, blocks=bp,
                      constraints=(np.ones((1, wl.n)), np.zeros(1)))
+
+
+def test_full_size_64_models_match_oracle_per_model(record_error):
+    """The full heterogeneous configuration (64 BYM2 + Poisson models with an intercept
+    each, Delaunay-neighbour graphs of sizes uniform in [500, 20000], n = 592,548) as
+    one block-diagonal model at three theta-points per model; every (point, model)
+    pair's mlik, mode and marginal variances against the oracle run on that model
+    alone, the oracle's calls spread over the host cores."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    from inla_gpu import workloads as W
+    from inla_gpu.laplace import LaplaceModel
+
+    wl = W.bym2_poisson_batch(k=64)
+    k = len(wl.models)
+    assert k == 64
+    model = LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A,
+                         blocks=wl.block_ptr, capacity=3)
+    offs = np.array([[0.0, 0.0], [0.4, -0.3], [-0.3, 0.5]])
+    thetas = np.broadcast_to(offs[:, None, :], (3, k, 2)).copy()
+    c, _, _, _ = wl.params(thetas)
+    r = model.eval(torch.as_tensor(c, device="cuda"))
+    assert (r["status"] == 0).all()
+    ml, x, var = r["mlik"].cpu().numpy(), r["x"].cpu().numpy(), r["var"].cpu().numpy()
+    bp = wl.block_ptr
+
+    def ref(job):
+        b, q = job
+        o = orc.find_mode(wl.block_prior(q, thetas[b, q]), "poisson", "log", wl.block_y(q), 1.0,
+                          A=wl.block_A(q), keep_post=True)
+        return b, q, o.mlik, o.x, o.post_factor().takahashi_diag()
+
+    jobs = [(b, q) for b in range(3) for q in range(k)]
+    with ThreadPoolExecutor(max(1, min(16, len(os.sched_getaffinity(0))))) as ex:
+        refs = list(ex.map(ref, jobs))
+    e_ml = e_x = e_v = 0.0
+    for b, q, o_ml, o_x, o_v in refs:
+        e_ml = max(e_ml, abs(ml[b, q] - o_ml))
+        e_x = max(e_x, _rel(x[b, bp[q]:bp[q + 1]], o_x))
+        e_v = max(e_v, _rel(var[b, bp[q]:bp[q + 1]], o_v))
+    record_error("64 models x 3 points: mlik", e_ml, MLIK_ATOL, "abs")
+    record_error("64 models x 3 points: mode", e_x, MODE_RTOL)
+    record_error("64 models x 3 points: variances", e_v, VAR_RTOL)
+    assert e_ml < MLIK_ATOL and e_x < MODE_RTOL and e_v < VAR_RTOL
