@@ -855,7 +855,8 @@ def main():
                 f = lines[-1]
                 tune["schedule"] = {"batch": int(f[5]), "fuse_below": int(f[6]), "split_max": int(f[7]),
                                     "xcd_max": int(f[8]), "merge_factorisation": f[9], "merge_selinv": f[10],
-                                    "stagger": int(f[11]) if len(f) > 11 else -1}
+                                    "stagger": int(f[11]) if len(f) > 11 else -1,
+                                    "pp_split_max": int(f[12]) if len(f) > 12 else None}
             else:
                 tune["schedule"] = "defaults kept (no candidate was faster)"
         except OSError:

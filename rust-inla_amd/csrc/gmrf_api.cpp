@@ -102,10 +102,12 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
 
 
 // Tuned schedules persist as text lines in the file named by GMRF_TUNE_FILE (if set):
-//   gmrf-schedule 2 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
+//   gmrf-schedule 3 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
 //   <xcd_max> <factorisation merge masks> <selected-inversion merge masks> <stagger>
+//   <pp_split_max>
 // one hex digit per level for each mask string; a later line for the same key wins.
-// Version 1 lines (no stagger field) are read with stagger -1.
+// Version 1 lines (no stagger field) are read with stagger -1, and version 1 and 2
+// lines (no pp_split_max) with the default pp_split_max.
 void SymData::load_schedules() {
   const char* path = std::getenv("GMRF_TUNE_FILE");
   if (!path) return;
@@ -119,9 +121,10 @@ void SymData::load_schedules() {
     Schedule sc;
     if (!(ss >> tag >> ver >> hash >> n >> nl >> count >> sc.fuse_below >> sc.split_max >> sc.xcd_max >> mf >> ms))
       continue;
-    if (tag != "gmrf-schedule" || (ver != 1 && ver != 2) || n != S.n || nl != (int64_t)S.nlevels || count < 1)
+    if (tag != "gmrf-schedule" || ver < 1 || ver > 3 || n != S.n || nl != (int64_t)S.nlevels || count < 1)
       continue;
-    if (ver == 2 && !(ss >> sc.stagger)) continue;
+    if (ver >= 2 && !(ss >> sc.stagger)) continue;
+    if (ver >= 3 && !(ss >> sc.pp_split_max)) continue;
     if (sc.stagger < -1 || sc.stagger >= nl) continue;
     if (std::strtoull(hash.c_str(), nullptr, 16) != S.pattern_hash) continue;
     if ((int64_t)mf.size() != nl || (int64_t)ms.size() != nl) continue;
@@ -145,9 +148,9 @@ void save_schedule(const SymData& d, int count, const Schedule& sc) {
     mf += "01234567"[sc.fac_mask(l) & 7];
     ms += "01234567"[sc.sel_mask(l) & 7];
   }
-  std::fprintf(f, "gmrf-schedule 2 %016llx %lld %d %d %lld %lld %d %s %s %d\n", (unsigned long long)d.S.pattern_hash,
-               (long long)d.S.n, d.S.nlevels, count, (long long)sc.fuse_below, (long long)sc.split_max, sc.xcd_max,
-               mf.c_str(), ms.c_str(), sc.stagger);
+  std::fprintf(f, "gmrf-schedule 3 %016llx %lld %d %d %lld %lld %d %s %s %d %lld\n",
+               (unsigned long long)d.S.pattern_hash, (long long)d.S.n, d.S.nlevels, count, (long long)sc.fuse_below,
+               (long long)sc.split_max, sc.xcd_max, mf.c_str(), ms.c_str(), sc.stagger, (long long)sc.pp_split_max);
   std::fclose(f);
 }
 
@@ -264,6 +267,12 @@ gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_
     for (int32_t v : {0, 64, kXcdMaxTasks, 1 << 30}) {
       Schedule s = cur;
       s.xcd_max = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    for (int64_t v : {(int64_t)0, (int64_t)256, kPpSplitMaxTasks, (int64_t)4096, (int64_t)1 << 40}) {
+      Schedule s = cur;
+      s.pp_split_max = v;
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }

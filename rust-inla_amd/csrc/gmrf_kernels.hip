@@ -196,6 +196,64 @@ __device__ __forceinline__ void tile_foreach(const TileAcc& acc, F f) {
       }
 }
 
+// 32 x 32 tile on 256 threads: T[i][j] += sum_k A(i, k) B(j, k) for 32 rows i and
+// 32 columns j. Wave w owns the 16 x 16 MFMA block of row block w & 1 and column block
+// w >> 1 over the whole K, so a workgroup does a quarter of a 64 x 64 tile's MFMA
+// work per chunk: for small squares with a long K in launches of few workgroups. Both
+// operands must be contiguous along k (lanes along k), each chunk staged in LDS
+// (sh: 2 x 32 x (kPanel + 1) doubles) with masks and factors applied at the store, as
+// in mma_tile. acc[v] of lane l holds T[16 (w & 1) + l % 16][16 (w >> 1) + 4 v + l / 16].
+template <typename FA, typename FB>
+__device__ __forceinline__ void mma_tile32(FA fa, FB fb, int K, double* sh, d4& acc) {
+  constexpr int LD = kPanel + 1, NL = 32 * kPanel / 256;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int rb = wv & 1, cb = wv >> 1, lr = lane & 15, lk = lane >> 4;
+  double* As = sh;
+  double* Bs = sh + 32 * LD;
+  decltype(fa(0, 0)) ra[NL];
+  decltype(fb(0, 0)) rb_[NL];
+  auto fetch = [&](int k0) {
+    const int kc = min(kPanel, K - k0);
+#pragma unroll
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + 256 * it;
+      const int ii = idx / kPanel, kk = idx % kPanel;
+      ra[it] = fa(ii, k0 + min(kk, kc - 1));
+      rb_[it] = fb(ii, k0 + min(kk, kc - 1));
+    }
+    ISSUE_FENCE();
+  };
+  if (K > 0) fetch(0);
+  for (int k0 = 0; k0 < K; k0 += kPanel) {
+    const int kc = min(kPanel, K - k0);
+#pragma unroll
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + 256 * it;
+      const int ii = idx / kPanel, kk = idx % kPanel;
+      As[ii * LD + kk] = kk < kc ? lv_get(ra[it]) : 0.0;
+      Bs[ii * LD + kk] = kk < kc ? lv_get(rb_[it]) : 0.0;
+    }
+    __syncthreads();
+    if (k0 + kPanel < K) fetch(k0 + kPanel);
+#pragma unroll
+    for (int st = 0; st < kPanel / 4; ++st) {
+      const int kk = 4 * st;
+      if (kk >= kc) break;  // uniform: short last chunk (zero padded)
+      acc = mfma16(Bs[(cb * 16 + lr) * LD + kk + lk], As[(rb * 16 + lr) * LD + kk + lk], acc);
+    }
+    __syncthreads();
+  }
+}
+
+// Visit the accumulator elements of mma_tile32 with their tile coordinates (i, j).
+template <typename F>
+__device__ __forceinline__ void tile_foreach32(const d4& acc, F f) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = (wv & 1) * 16 + (lane & 15);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) f(i, (wv >> 1) * 16 + 4 * v + (lane >> 4), acc[v]);
+}
+
 // The engine on one LDS block sh of 2 x 64 x (kPanel + 1) doubles (A, then B).
 template <bool AK, bool BK, typename FA, typename FB>
 __device__ __forceinline__ void tile_product(FA fa, FB fb, int K, double* sh, TileAcc& acc) {
@@ -1960,20 +2018,83 @@ __device__ __forceinline__ void sl_pp_task(const DevPlan& P, const int32_t* tk, 
     });
 }
 
+// sl_pp_task as three 32 x 32 blocks of the lower triangle of the 64 x 64 Sigma[P,P]
+// (part 0: rows and columns 0-31; part 1: rows 32-63, columns 0-31; part 2: rows and
+// columns 32-63), one workgroup each: a quarter of the tile's MFMA work per workgroup
+// and the strictly upper block skipped. Same products and stores as sl_pp_task.
+__device__ __forceinline__ void sl_pp_task32(const DevPlan& P, const int32_t* tk, int part, int step, int b,
+                                             const double* __restrict__ L, const double* __restrict__ Z,
+                                             double* __restrict__ S, const double* __restrict__ T, double* sh,
+                                             double* dinv) {
+  const int s = tk[0];
+  const int tid = threadIdx.x;
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int p0 = part == 0 ? 0 : 32, q0 = part == 2 ? 32 : 0;
+  if (p0 >= nb) return;  // uniform: the block lies beyond the panel
+  const double* Lb = L + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)pk * kTile * kTile;
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
+  if (tid < kTile) {
+    const int q = min(tid, nb - 1);
+    dinv[tid] = 1.0 / Lb[(int64_t)(c0 + q) * m + c0 + q];
+  }
+  __syncthreads();
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  // (Z^T D^-1 Z)[p][q] = sum_k Z[k][p] Z[k][q] / d_k, Zb[q * 64 + k] = Z[k][q]
+  mma_tile32(
+      [&](int ii, int k) {
+        const int pp = min(p0 + ii, nb - 1);
+        return LVs{Zb[pp * kTile + k], dinv[k], p0 + ii < nb};
+      },
+      [&](int jj, int k) {
+        const int qq = min(q0 + jj, nb - 1);
+        return LV{Zb[qq * kTile + k], q0 + jj < nb};
+      },
+      nb, sh, acc);
+  if (m > c1) {
+    // acc - Lh^T Sigma[B,P] as -(-acc + Lh^T Sigma[B,P]) (negation is exact)
+    acc = -acc;
+    mma_tile32(
+        [&](int ii, int kk) {
+          return LV{Lh[(int64_t)min(p0 + ii, nb - 1) * m + c1 + kk], p0 + ii < nb};
+        },
+        [&](int jj, int kk) {
+          return LV{Sb[(int64_t)(c0 + min(q0 + jj, nb - 1)) * m + c1 + kk], q0 + jj < nb};
+        },
+        m - c1, sh, acc);
+    acc = -acc;
+  }
+  tile_foreach32(acc, [&](int ii, int jj, double v) {
+    const int p = p0 + ii, q = q0 + jj;
+    if (p < nb && q < nb && p >= q) Sb[(int64_t)(c0 + q) * m + c0 + p] = v;
+  });
+}
+
 // Sigma[P,P] of panel step `step` and L_hat of step + 1 in one launch: the first
 // npp workgroups take the Sigma[P,P] tasks, the rest the L_hat tasks (they read
 // only L and Z and write the other L_hat buffer, so the two sets are independent).
+// With split, each Sigma[P,P] task runs as three workgroups (sl_pp_task32).
 __global__ __launch_bounds__(256) void k_sl_pp_lhat(DevPlan P, const int32_t* __restrict__ pp_tasks,
                                                     int npp, const int32_t* __restrict__ lhat_tasks,
-                                                    int step, const double* __restrict__ L,
+                                                    int step, int split, const double* __restrict__ L,
                                                     const double* __restrict__ Z,
                                                     double* __restrict__ S, double* __restrict__ T) {
   __shared__ double sh[2 * 64 * (kPanel + 1)];
   __shared__ double dinv[kTile];
-  if ((int)blockIdx.x < npp)
-    sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, T, sh, dinv);
-  else
-    sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp), step + 1, blockIdx.y, L, Z, T, sh);
+  const int npp_wg = split ? 3 * npp : npp;
+  if ((int)blockIdx.x < npp_wg) {
+    if (split)
+      sl_pp_task32(P, pp_tasks + 3 * (blockIdx.x / 3), (int)blockIdx.x % 3, step, blockIdx.y, L, Z, S, T, sh,
+                   dinv);
+    else
+      sl_pp_task(P, pp_tasks + 3 * blockIdx.x, step, blockIdx.y, L, Z, S, T, sh, dinv);
+  } else {
+    sl_lhat_task(P, lhat_tasks + 3 * (blockIdx.x - npp_wg), step + 1, blockIdx.y, L, Z, T, sh);
+  }
 }
 
 // Sigma[B,P] split over K for launches with few row blocks (tall fronts of a small
@@ -2404,11 +2525,11 @@ hipError_t launch_sl_gemm_sum(const DevPlan& P, const int32_t* tasks, int count,
 }
 
 hipError_t launch_sl_pp_lhat(const DevPlan& P, const int32_t* pp_tasks, int npp,
-                             const int32_t* lhat_tasks, int nlhat, int step, int B, const double* L,
-                             const double* Z, double* S, double* T, hipStream_t st) {
+                             const int32_t* lhat_tasks, int nlhat, int step, int B, bool split,
+                             const double* L, const double* Z, double* S, double* T, hipStream_t st) {
   if (npp + nlhat <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sl_pp_lhat, dim3(npp + nlhat, B), dim3(256), 0, st, P, pp_tasks, npp, lhat_tasks,
-                     step, L, Z, S, T);
+  hipLaunchKernelGGL(k_sl_pp_lhat, dim3((split ? 3 : 1) * npp + nlhat, B), dim3(256), 0, st, P, pp_tasks, npp,
+                     lhat_tasks, step, split ? 1 : 0, L, Z, S, T);
   return hipGetLastError();
 }
 

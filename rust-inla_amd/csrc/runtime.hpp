@@ -38,7 +38,7 @@ hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const d
                           const double*, double*, hipStream_t);
 hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, int, double*, const double*,
                           const double*, hipStream_t);
-hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int,
+hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int, bool,
                              const double*, const double*, double*, double*, hipStream_t);
 hipError_t launch_sl_pp_part(const DevPlan&, const int32_t*, int, int, int, const double*,
                              const double*, const double*, double*, hipStream_t);
@@ -353,6 +353,8 @@ using namespace gmrf_rt;
 //  - split_max: Sigma[B,P] launches of at most this many row tiles split K;
 //  - xcd_max: Sigma[B,P] launches with at most this many tasks per matrix keep a
 //    matrix's tiles on one XCD;
+//  - pp_split_max: Sigma[P,P] launches of at most this many tasks (fronts x
+//    matrices) run each task as three 32 x 32 workgroups;
 //  - stagger (>= 0): the batch runs as two pipelines of half the matrices each
 //    (factorisation, log determinants, selected inversion), the second on a stream of
 //    its own that starts once the first has enqueued the factorisation of level
@@ -365,6 +367,7 @@ struct Schedule {
   int64_t split_max = kGemmSplitMaxTasks;
   int32_t xcd_max = kXcdMaxTasks;
   int32_t stagger = -1;
+  int64_t pp_split_max = kPpSplitMaxTasks;
   uint8_t fac_mask(size_t l) const { return l < merge_fac.size() ? merge_fac[l] : 0; }
   uint8_t sel_mask(size_t l) const { return l < merge_sel.size() ? merge_sel[l] : 0; }
 };
@@ -1294,7 +1297,8 @@ struct gmrf_batch {
             prof_launch("k_sl_pp_lhat", (lv.spp[k].flops + (next ? lv.lhat[k + 1].flops : 0.0)) * count,
                         (lv.spp[k].bytes + (next ? lv.lhat[k + 1].bytes : 0.0)) * count, stream, [&] {
                           return launch_sl_pp_lhat(P, tk + 3 * lv.spp[k].off, (int)lv.spp[k].cnt, tl, nl, (int)k,
-                                                   count, Lf, Zf, Sf, Tf, stream);
+                                                   count, lv.spp[k].cnt * (int64_t)count <= sc.pp_split_max,
+                                                   Lf, Zf, Sf, Tf, stream);
                         });
           }
         }

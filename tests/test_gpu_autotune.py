@@ -83,3 +83,32 @@ def test_autotune_keeps_results(monkeypatch, tmp_path):
     assert (st1 == 0).all()
     assert np.max(np.abs(ld1 - ld0)) <= 1e-12 * np.max(np.abs(ld0))
     assert _rel(x1, x0) < 1e-12 and _rel(d1, d0) < 1e-12
+
+
+@pytest.mark.parametrize("pp_split", [0, 1 << 40])
+def test_sigma_pp_split_schedule_matches_default(tmp_path, monkeypatch, pp_split):
+    # a version-3 schedule line: Sigma[P,P] never split (0) or always split into three
+    # 32 x 32 workgroups; factors, solves and selected inverses equal the other form's
+    # and the oracle's
+    from inla_gpu import workloads as W
+    from inla_gpu.solver import Symbolic
+
+    g, _, _ = W.spde_components(33, 26)
+    c, _ = W.spde_coefs(-0.2, 1.1)
+    q = sp.csc_matrix(g.csc(c) + 1.5 * sp.identity(g.n, format="csc"))
+    q.sort_indices()
+    path = tmp_path / "sched.txt"
+    monkeypatch.setenv("GMRF_TUNE_FILE", str(path))
+    sym = Symbolic.from_csc(q)
+    st = sym.stats()
+    nl = st["nlevels"]
+    path.write_text(f"gmrf-schedule 3 {st['pattern_hash']:016x} {st['n']} {nl} 3 2048 96 256 "
+                    f"{'0' * nl} {'0' * nl} -1 {pp_split}\n")
+    ld3, st3, x3, d3, rhs3 = _run(sym, q.data, 3)    # the line's schedule
+    ld2, st2, x2, d2, rhs2 = _run(sym, q.data, 2)    # the defaults (split up to 1024 tasks)
+    assert (st3 == 0).all() and (st2 == 0).all()
+    assert abs(ld3[0] - ld2[0]) <= 1e-12 * abs(ld2[0])
+    assert _rel(x3[:2], x2) < 1e-12
+    assert _rel(d3[:2], d2) < 1e-12
+    ref = orc.Factor(q)
+    assert _rel(d3[0], ref.takahashi_diag()) < 1e-9
