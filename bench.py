@@ -856,7 +856,9 @@ def main():
                 tune["schedule"] = {"batch": int(f[5]), "fuse_below": int(f[6]), "split_max": int(f[7]),
                                     "xcd_max": int(f[8]), "merge_factorisation": f[9], "merge_selinv": f[10],
                                     "stagger": int(f[11]) if len(f) > 11 else -1,
-                                    "pp_split_max": int(f[12]) if len(f) > 12 else None}
+                                    "pp_split_max": int(f[12]) if len(f) > 12 else None,
+                                    "trsm32_max": int(f[13]) if len(f) > 13 else None,
+                                    "gemm32_max": int(f[14]) if len(f) > 14 else None}
             else:
                 tune["schedule"] = "defaults kept (no candidate was faster)"
         except OSError:

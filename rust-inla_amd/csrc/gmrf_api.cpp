@@ -102,12 +102,12 @@ std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const 
 
 
 // Tuned schedules persist as text lines in the file named by GMRF_TUNE_FILE (if set):
-//   gmrf-schedule 3 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
+//   gmrf-schedule 4 <pattern hash, hex> <n> <levels> <count> <fuse_below> <split_max>
 //   <xcd_max> <factorisation merge masks> <selected-inversion merge masks> <stagger>
-//   <pp_split_max>
+//   <pp_split_max> <trsm32_max> <gemm32_max>
 // one hex digit per level for each mask string; a later line for the same key wins.
-// Version 1 lines (no stagger field) are read with stagger -1, and version 1 and 2
-// lines (no pp_split_max) with the default pp_split_max.
+// Lines of earlier versions are read with the defaults for the fields they lack
+// (version 1: no stagger; 2: no pp_split_max; 3: no trsm32_max / gemm32_max).
 void SymData::load_schedules() {
   const char* path = std::getenv("GMRF_TUNE_FILE");
   if (!path) return;
@@ -121,10 +121,11 @@ void SymData::load_schedules() {
     Schedule sc;
     if (!(ss >> tag >> ver >> hash >> n >> nl >> count >> sc.fuse_below >> sc.split_max >> sc.xcd_max >> mf >> ms))
       continue;
-    if (tag != "gmrf-schedule" || ver < 1 || ver > 3 || n != S.n || nl != (int64_t)S.nlevels || count < 1)
+    if (tag != "gmrf-schedule" || ver < 1 || ver > 4 || n != S.n || nl != (int64_t)S.nlevels || count < 1)
       continue;
     if (ver >= 2 && !(ss >> sc.stagger)) continue;
     if (ver >= 3 && !(ss >> sc.pp_split_max)) continue;
+    if (ver >= 4 && !(ss >> sc.trsm32_max >> sc.gemm32_max)) continue;
     if (sc.stagger < -1 || sc.stagger >= nl) continue;
     if (std::strtoull(hash.c_str(), nullptr, 16) != S.pattern_hash) continue;
     if ((int64_t)mf.size() != nl || (int64_t)ms.size() != nl) continue;
@@ -148,9 +149,10 @@ void save_schedule(const SymData& d, int count, const Schedule& sc) {
     mf += "01234567"[sc.fac_mask(l) & 7];
     ms += "01234567"[sc.sel_mask(l) & 7];
   }
-  std::fprintf(f, "gmrf-schedule 3 %016llx %lld %d %d %lld %lld %d %s %s %d %lld\n",
+  std::fprintf(f, "gmrf-schedule 4 %016llx %lld %d %d %lld %lld %d %s %s %d %lld %lld %lld\n",
                (unsigned long long)d.S.pattern_hash, (long long)d.S.n, d.S.nlevels, count, (long long)sc.fuse_below,
-               (long long)sc.split_max, sc.xcd_max, mf.c_str(), ms.c_str(), sc.stagger, (long long)sc.pp_split_max);
+               (long long)sc.split_max, sc.xcd_max, mf.c_str(), ms.c_str(), sc.stagger, (long long)sc.pp_split_max,
+               (long long)sc.trsm32_max, (long long)sc.gemm32_max);
   std::fclose(f);
 }
 
@@ -273,6 +275,18 @@ gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_
     for (int64_t v : {(int64_t)0, (int64_t)256, kPpSplitMaxTasks, (int64_t)4096, (int64_t)1 << 40}) {
       Schedule s = cur;
       s.pp_split_max = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    for (int64_t v : {(int64_t)0, (int64_t)64, kTile32MaxTasks, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
+      Schedule s = cur;
+      s.trsm32_max = v;
+      const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
+      if (t < t_cur) t_cur = t, cur = s;
+    }
+    for (int64_t v : {(int64_t)0, (int64_t)64, kTile32MaxTasks, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
+      Schedule s = cur;
+      s.gemm32_max = v;
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }

@@ -199,12 +199,14 @@ __device__ __forceinline__ void tile_foreach(const TileAcc& acc, F f) {
 // 32 x 32 tile on 256 threads: T[i][j] += sum_k A(i, k) B(j, k) for 32 rows i and
 // 32 columns j. Wave w owns the 16 x 16 MFMA block of row block w & 1 and column block
 // w >> 1 over the whole K, so a workgroup does a quarter of a 64 x 64 tile's MFMA
-// work per chunk: for small squares with a long K in launches of few workgroups. Both
-// operands must be contiguous along k (lanes along k), each chunk staged in LDS
-// (sh: 2 x 32 x (kPanel + 1) doubles) with masks and factors applied at the store, as
-// in mma_tile. acc[v] of lane l holds T[16 (w & 1) + l % 16][16 (w >> 1) + 4 v + l / 16].
-template <typename FA, typename FB>
+// work per chunk: for small tiles with a long K in launches of few workgroups. AK / BK
+// as in mma_tile (the operand is contiguous along k: lanes along k, else along rows);
+// each chunk is staged in LDS (sh: 2 x 32 x (kPanel + 1) doubles) with masks and
+// factors applied at the store, as in mma_tile. acc[v] of lane l holds
+// T[16 (w & 1) + l % 16][16 (w >> 1) + 4 v + l / 16].
+template <bool AK = true, bool BK = true, typename FA, typename FB>
 __device__ __forceinline__ void mma_tile32(FA fa, FB fb, int K, double* sh, d4& acc) {
+  static_assert(kPanel == 32, "the 32 x 32 engine stages 32 x 32 operand chunks");
   constexpr int LD = kPanel + 1, NL = 32 * kPanel / 256;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int rb = wv & 1, cb = wv >> 1, lr = lane & 15, lk = lane >> 4;
@@ -217,9 +219,14 @@ __device__ __forceinline__ void mma_tile32(FA fa, FB fb, int K, double* sh, d4& 
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
       const int idx = tid + 256 * it;
-      const int ii = idx / kPanel, kk = idx % kPanel;
-      ra[it] = fa(ii, k0 + min(kk, kc - 1));
-      rb_[it] = fb(ii, k0 + min(kk, kc - 1));
+      {
+        const int ii = AK ? (idx >> 5) : (idx & 31), kk = AK ? (idx & 31) : (idx >> 5);
+        ra[it] = fa(ii, k0 + min(kk, kc - 1));
+      }
+      {
+        const int ii = BK ? (idx >> 5) : (idx & 31), kk = BK ? (idx & 31) : (idx >> 5);
+        rb_[it] = fb(ii, k0 + min(kk, kc - 1));
+      }
     }
     ISSUE_FENCE();
   };
@@ -229,9 +236,14 @@ __device__ __forceinline__ void mma_tile32(FA fa, FB fb, int K, double* sh, d4& 
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
       const int idx = tid + 256 * it;
-      const int ii = idx / kPanel, kk = idx % kPanel;
-      As[ii * LD + kk] = kk < kc ? lv_get(ra[it]) : 0.0;
-      Bs[ii * LD + kk] = kk < kc ? lv_get(rb_[it]) : 0.0;
+      {
+        const int ii = AK ? (idx >> 5) : (idx & 31), kk = AK ? (idx & 31) : (idx >> 5);
+        As[ii * LD + kk] = kk < kc ? lv_get(ra[it]) : 0.0;
+      }
+      {
+        const int ii = BK ? (idx >> 5) : (idx & 31), kk = BK ? (idx & 31) : (idx >> 5);
+        Bs[ii * LD + kk] = kk < kc ? lv_get(rb_[it]) : 0.0;
+      }
     }
     __syncthreads();
     if (k0 + kPanel < K) fetch(k0 + kPanel);
@@ -866,13 +878,15 @@ __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __res
 
 // Panel solve as a product: task (s, I) computes 64 rows below the diagonal block,
 // L21[i][p] = sum_q A21[i][q] Z[p][q] / d_p  (L21 = A21 L11^-T D^-1), in place.
+// With split, each tile runs as four 32 x 32 workgroups (rows 32 (part & 1) .., panel
+// columns 32 (part >> 1) ..; mma_tile32), for launches of few tiles.
 __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __restrict__ tasks,
-                                                 int step, double* __restrict__ L,
+                                                 int step, int split, double* __restrict__ L,
                                                  const double* __restrict__ Z) {
   __shared__ double As[64][kPanel + 1];
   __shared__ double Bs[64][kPanel + 1];
   __shared__ double dinv[kTile];
-  const int32_t* tk = tasks + 3 * blockIdx.x;
+  const int32_t* tk = tasks + 3 * (split ? (blockIdx.x >> 2) : blockIdx.x);
   const int s = tk[0], I = tk[1];
   const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
@@ -882,6 +896,26 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
   const double* Zb = Z + (int64_t)b * P.z_size + P.z_off[s] + (int64_t)step * kTile * kTile;
   if (threadIdx.x < nb) dinv[threadIdx.x] = 1.0 / Lb[(int64_t)(c0 + threadIdx.x) * m + c0 + threadIdx.x];
   __syncthreads();
+  if (split) {
+    const int part = blockIdx.x & 3, r0 = R0 + (part & 1) * 32, q0 = (part >> 1) * 32;
+    if (r0 >= m || q0 >= nb) return;  // uniform: the block lies beyond the front or the panel
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    mma_tile32<false, false>(
+        [&](int ii, int q) {
+          const int i = r0 + ii;
+          return LV{Lb[(int64_t)(c0 + q) * m + min(i, m - 1)], i < m};
+        },
+        [&](int pp, int q) {
+          const int p = q0 + pp;
+          return LVs{Zb[q * kTile + p], dinv[p], p < nb};
+        },
+        nb, &As[0][0], acc);
+    tile_foreach32(acc, [&](int ii, int pp, double v) {
+      const int i = r0 + ii, p = q0 + pp;
+      if (i < m && p < nb) Lb[(int64_t)(c0 + p) * m + i] = v;
+    });
+    return;
+  }
   TileAcc acc;
   tile_zero(acc);
   mma_tile(
@@ -1964,6 +1998,65 @@ __global__ __launch_bounds__(512) void k_sl_gemm(DevPlan P, const int32_t* __res
     });
 }
 
+// sigma_strip on a 32 x 32 block: rows R0 .. R0 + 31 and the panel columns q0 .. q0 + 31
+// of Lh (mma_tile32), the diagonal block boundary at R0 + 32.
+__device__ __forceinline__ void sigma_strip32(const double* __restrict__ Sb, const double* __restrict__ Ws,
+                                              const double* __restrict__ Lh, int w, int m, int nb, int R0,
+                                              int q0, int k_lo, int k_hi, double* sh, d4& acc) {
+  const int kd = min(max(R0 + 32, k_lo), k_hi), nq = nb - q0;
+  if (kd > k_lo)
+    mma_tile32<false, true>(
+        [&](int ii, int kk) {
+          const int i = min(R0 + ii, m - 1), k = k_lo + kk;
+          const int hi = max(i, k), lo = min(i, k);
+          return LV{(lo < w ? Sb : Ws)[(int64_t)lo * m + hi], R0 + ii < m};
+        },
+        [&](int jj, int kk) {
+          return LV{Lh[(int64_t)(q0 + min(jj, nq - 1)) * m + k_lo + kk], jj < nq};
+        },
+        kd - k_lo, sh, acc);
+  if (k_hi > kd)
+    mma_tile32<true, true>(
+        [&](int ii, int kk) {
+          const int i = min(R0 + ii, m - 1);
+          return LV{(i < w ? Sb : Ws)[(int64_t)i * m + kd + kk], R0 + ii < m};
+        },
+        [&](int jj, int kk) {
+          return LV{Lh[(int64_t)(q0 + min(jj, nq - 1)) * m + kd + kk], jj < nq};
+        },
+        k_hi - kd, sh, acc);
+}
+
+// k_sl_gemm with each task's 64 x 64 tile as four 32 x 32 workgroups (task 4 t + part:
+// rows 32 (part & 1) .., columns 32 (part >> 1) ..), for launches of few tiles; the
+// same XCD grouping over the 4 count block tasks.
+__global__ __launch_bounds__(256) void k_sl_gemm32(DevPlan P, const int32_t* __restrict__ tasks, int count,
+                                                   int B, int xcd, int step, double* __restrict__ S,
+                                                   const double* __restrict__ W,
+                                                   const double* __restrict__ T) {
+  __shared__ double sh[2 * 32 * (kPanel + 1)];
+  int t4, b;
+  if (!wg_task(4 * count, B, xcd, t4, b)) return;
+  const int32_t* tk = tasks + 3 * (t4 >> 2);
+  const int part = t4 & 3;
+  const int s = tk[0], I = tk[1];
+  const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
+  int c0, c1, pk;
+  sel_panel(w, step, c0, c1, pk);
+  const int nb = c1 - c0;
+  const int R0 = c1 + I * kTile + (part & 1) * 32, q0 = (part >> 1) * 32;
+  if (R0 >= m || q0 >= nb) return;  // uniform: the block lies beyond the front or the panel
+  double* Sb = S + (int64_t)b * P.l_size + P.l_off[s];
+  const double* Ws = W + (int64_t)b * P.w_size + P.w_off[s];
+  const double* Lh = T + (int64_t)b * P.t_size + P.t_off[s] + stage_lhat_off(m, step);
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  sigma_strip32(Sb, Ws, Lh, w, m, nb, R0, q0, c1, m, sh, acc);
+  tile_foreach32(acc, [&](int ii, int jj, double v) {
+    const int i = R0 + ii, j = q0 + jj;
+    if (i < m && j < nb) Sb[(int64_t)(c0 + j) * m + i] = -v;
+  });
+}
+
 // Sigma[P,P]: task (s) = Z^T D^-1 Z - Lh^T Sigma[B,P], both products on MFMA
 // (K = nb, then K = m - c1; Sigma[B,P] read back from S); lower part into S. The
 // strict upper triangle of S's diagonal blocks is not part of the layout's
@@ -2307,10 +2400,11 @@ hipError_t launch_fl_diag(const DevPlan& P, const int32_t* tasks, int count, int
   return hipGetLastError();
 }
 
-hipError_t launch_fl_trsm(const DevPlan& P, const int32_t* tasks, int count, int step, int B,
+hipError_t launch_fl_trsm(const DevPlan& P, const int32_t* tasks, int count, int step, int B, bool split,
                           double* L, const double* Z, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_trsm, dim3(count, B), dim3(256), 0, st, P, tasks, step, L, Z);
+  hipLaunchKernelGGL(k_fl_trsm, dim3((split ? 4 : 1) * count, B), dim3(256), 0, st, P, tasks, step,
+                     split ? 1 : 0, L, Z);
   return hipGetLastError();
 }
 
@@ -2500,13 +2594,18 @@ hipError_t launch_sl_lhat(const DevPlan& P, const int32_t* tasks, int count, int
 }
 
 hipError_t launch_sl_gemm(const DevPlan& P, const int32_t* tasks, int count, int step, int B, int xcd_max,
-                          double* S, const double* W, const double* T, hipStream_t st) {
+                          bool split, double* S, const double* W, const double* T, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   // XCD-grouped when every XCD gets a matrix and a matrix's tiles fit one XCD
   const int xcd = (B >= 8 && count <= xcd_max) ? 1 : 0;
-  const int64_t grid = xcd ? (int64_t)8 * ((B + 7) / 8) * count : (int64_t)count * B;
-  hipLaunchKernelGGL(k_sl_gemm, dim3((unsigned)grid), dim3(512), 0, st, P, tasks, count, B, xcd, step, S, W,
-                     T);
+  const int64_t n = (split ? 4 : 1) * (int64_t)count;
+  const int64_t grid = xcd ? (int64_t)8 * ((B + 7) / 8) * n : n * B;
+  if (split)
+    hipLaunchKernelGGL(k_sl_gemm32, dim3((unsigned)grid), dim3(256), 0, st, P, tasks, count, B, xcd, step, S, W,
+                       T);
+  else
+    hipLaunchKernelGGL(k_sl_gemm, dim3((unsigned)grid), dim3(512), 0, st, P, tasks, count, B, xcd, step, S, W,
+                       T);
   return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ hipError_t launch_fl_assemble(const DevPlan&, const int32_t*, int, int, int, boo
                               hipStream_t);
 hipError_t launch_fl_diag(const DevPlan&, const int32_t*, int, int, int, double*, double*,
                           hipStream_t);
-hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, double*, const double*,
+hipError_t launch_fl_trsm(const DevPlan&, const int32_t*, int, int, int, bool, double*, const double*,
                           hipStream_t);
 hipError_t launch_fl_update(const DevPlan&, const int32_t*, int, int, int, bool, double*, double*, double*,
                             hipStream_t);
@@ -36,7 +36,7 @@ hipError_t launch_sl_gather_lhat(const DevPlan&, const int32_t*, int, const int3
                                 hipStream_t);
 hipError_t launch_sl_lhat(const DevPlan&, const int32_t*, int, int, int, const double*,
                           const double*, double*, hipStream_t);
-hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, int, double*, const double*,
+hipError_t launch_sl_gemm(const DevPlan&, const int32_t*, int, int, int, int, bool, double*, const double*,
                           const double*, hipStream_t);
 hipError_t launch_sl_pp_lhat(const DevPlan&, const int32_t*, int, const int32_t*, int, int, int, bool,
                              const double*, const double*, double*, double*, hipStream_t);
@@ -355,6 +355,8 @@ using namespace gmrf_rt;
 //    matrix's tiles on one XCD;
 //  - pp_split_max: Sigma[P,P] launches of at most this many tasks (fronts x
 //    matrices) run each task as three 32 x 32 workgroups;
+//  - trsm32_max / gemm32_max: panel-solve / Sigma[B,P] launches of at most this many
+//    64 x 64 tiles (tiles x matrices) run each tile as four 32 x 32 workgroups;
 //  - stagger (>= 0): the batch runs as two pipelines of half the matrices each
 //    (factorisation, log determinants, selected inversion), the second on a stream of
 //    its own that starts once the first has enqueued the factorisation of level
@@ -368,6 +370,8 @@ struct Schedule {
   int32_t xcd_max = kXcdMaxTasks;
   int32_t stagger = -1;
   int64_t pp_split_max = kPpSplitMaxTasks;
+  int64_t trsm32_max = kTile32MaxTasks;
+  int64_t gemm32_max = kTile32MaxTasks;
   uint8_t fac_mask(size_t l) const { return l < merge_fac.size() ? merge_fac[l] : 0; }
   uint8_t sel_mask(size_t l) const { return l < merge_sel.size() ? merge_sel[l] : 0; }
 };
@@ -1116,7 +1120,7 @@ struct gmrf_batch {
             });
           prof_launch("k_fl_trsm", lv.ftrsm[k].flops * count, lv.ftrsm[k].bytes * count, stream, [&] {
             return launch_fl_trsm(P, tk + 3 * lv.ftrsm[k].off, (int)lv.ftrsm[k].cnt, (int)k, count,
-                                  Lf, Zf, stream);
+                                  lv.ftrsm[k].cnt * (int64_t)count <= sc.trsm32_max, Lf, Zf, stream);
           });
           const bool next = k + 1 < lv.fdiag.size();
           diag_done = next && fuse(lv.fupd[k].cnt);
@@ -1274,7 +1278,7 @@ struct gmrf_batch {
           } else {
             prof_launch("k_sl_gemm", lv.sgemm[k].flops * count, lv.sgemm[k].bytes * count, stream, [&] {
               return launch_sl_gemm(P, tk + 3 * lv.sgemm[k].off, (int)lv.sgemm[k].cnt, (int)k, count, sc.xcd_max,
-                                    Sf, Wf, Tf, stream);
+                                    lv.sgemm[k].cnt * (int64_t)count <= sc.gemm32_max, Sf, Wf, Tf, stream);
             });
           }
           if (lv.sppart[k].cnt > 0) {

@@ -112,3 +112,33 @@ def test_sigma_pp_split_schedule_matches_default(tmp_path, monkeypatch, pp_split
     assert _rel(d3[:2], d2) < 1e-12
     ref = orc.Factor(q)
     assert _rel(d3[0], ref.takahashi_diag()) < 1e-9
+
+
+@pytest.mark.parametrize("t32", [0, 1 << 40])
+def test_tile32_schedules_match_default(tmp_path, monkeypatch, t32):
+    # a version-4 schedule line: panel solves and Sigma[B,P] never (0) or always as four
+    # 32 x 32 workgroups per tile, Sigma[P,P] likewise never or always split; factors,
+    # solves and selected inverses equal the defaults' and the oracle's
+    from inla_gpu import workloads as W
+    from inla_gpu.solver import Symbolic
+
+    g, _, _ = W.spde_components(29, 34)
+    c, _ = W.spde_coefs(0.3, 1.3)
+    q = sp.csc_matrix(g.csc(c) + 2.5 * sp.identity(g.n, format="csc"))
+    q.sort_indices()
+    path = tmp_path / "sched.txt"
+    monkeypatch.setenv("GMRF_TUNE_FILE", str(path))
+    sym = Symbolic.from_csc(q)
+    st = sym.stats()
+    nl = st["nlevels"]
+    path.write_text(f"gmrf-schedule 4 {st['pattern_hash']:016x} {st['n']} {nl} 3 2048 96 256 "
+                    f"{'0' * nl} {'0' * nl} -1 {t32} {t32} {t32}\n")
+    ld3, st3, x3, d3, _ = _run(sym, q.data, 3)    # the line's schedule
+    ld2, st2, x2, d2, _ = _run(sym, q.data, 2)    # the defaults
+    assert (st3 == 0).all() and (st2 == 0).all()
+    assert abs(ld3[0] - ld2[0]) <= 1e-12 * abs(ld2[0])
+    assert _rel(x3[:2], x2) < 1e-12
+    assert _rel(d3[:2], d2) < 1e-12
+    ref = orc.Factor(q)
+    assert abs(ld3[0] - ref.log_abs_det()) <= 1e-10 * abs(ref.log_abs_det())
+    assert _rel(d3[0], ref.takahashi_diag()) < 1e-9
