@@ -272,19 +272,19 @@ gmrf_status autotune_batch(gmrf_batch* bt, int count, const double* vals, int64_
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }
-    for (int64_t v : {(int64_t)0, (int64_t)256, kPpSplitMaxTasks, (int64_t)4096, (int64_t)1 << 40}) {
+    for (int64_t v : {(int64_t)0, (int64_t)256, (int64_t)1024, (int64_t)4096, kPpSplitMaxTasks}) {
       Schedule s = cur;
       s.pp_split_max = v;
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }
-    for (int64_t v : {(int64_t)0, (int64_t)64, kTile32MaxTasks, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
+    for (int64_t v : {(int64_t)0, (int64_t)64, (int64_t)128, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
       Schedule s = cur;
       s.trsm32_max = v;
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);
       if (t < t_cur) t_cur = t, cur = s;
     }
-    for (int64_t v : {(int64_t)0, (int64_t)64, kTile32MaxTasks, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
+    for (int64_t v : {(int64_t)0, (int64_t)64, (int64_t)128, (int64_t)256, (int64_t)512, (int64_t)1 << 40}) {
       Schedule s = cur;
       s.gemm32_max = v;
       const double t = time_schedule(bt, count, vals, ldv, s, reps, nullptr, nullptr, nullptr);

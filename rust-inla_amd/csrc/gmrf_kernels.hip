@@ -878,15 +878,18 @@ __global__ __launch_bounds__(256) void k_fl_diag(DevPlan P, const int32_t* __res
 
 // Panel solve as a product: task (s, I) computes 64 rows below the diagonal block,
 // L21[i][p] = sum_q A21[i][q] Z[p][q] / d_p  (L21 = A21 L11^-T D^-1), in place.
-// With split, each tile runs as four 32 x 32 workgroups (rows 32 (part & 1) .., panel
-// columns 32 (part >> 1) ..; mma_tile32), for launches of few tiles.
+// With split, each tile runs as two workgroups of 32 rows (part = blockIdx.x & 1), for
+// launches of few tiles; each forms both 32-column halves of its rows (mma_tile32 twice)
+// before it stores them. The solve is in place: a workgroup's K operand is every panel
+// column of its rows, so the column halves of one row block cannot be split between
+// workgroups (one would read columns the other has already overwritten).
 __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __restrict__ tasks,
                                                  int step, int split, double* __restrict__ L,
                                                  const double* __restrict__ Z) {
   __shared__ double As[64][kPanel + 1];
   __shared__ double Bs[64][kPanel + 1];
   __shared__ double dinv[kTile];
-  const int32_t* tk = tasks + 3 * (split ? (blockIdx.x >> 2) : blockIdx.x);
+  const int32_t* tk = tasks + 3 * (split ? (blockIdx.x >> 1) : blockIdx.x);
   const int s = tk[0], I = tk[1];
   const int b = blockIdx.y;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
@@ -897,23 +900,31 @@ __global__ __launch_bounds__(256) void k_fl_trsm(DevPlan P, const int32_t* __res
   if (threadIdx.x < nb) dinv[threadIdx.x] = 1.0 / Lb[(int64_t)(c0 + threadIdx.x) * m + c0 + threadIdx.x];
   __syncthreads();
   if (split) {
-    const int part = blockIdx.x & 3, r0 = R0 + (part & 1) * 32, q0 = (part >> 1) * 32;
-    if (r0 >= m || q0 >= nb) return;  // uniform: the block lies beyond the front or the panel
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
-    mma_tile32<false, false>(
-        [&](int ii, int q) {
-          const int i = r0 + ii;
-          return LV{Lb[(int64_t)(c0 + q) * m + min(i, m - 1)], i < m};
-        },
-        [&](int pp, int q) {
-          const int p = q0 + pp;
-          return LVs{Zb[q * kTile + p], dinv[p], p < nb};
-        },
-        nb, &As[0][0], acc);
-    tile_foreach32(acc, [&](int ii, int pp, double v) {
-      const int i = r0 + ii, p = q0 + pp;
-      if (i < m && p < nb) Lb[(int64_t)(c0 + p) * m + i] = v;
-    });
+    const int r0 = R0 + (blockIdx.x & 1) * 32;
+    if (r0 >= m) return;  // uniform: the row block lies beyond the front
+    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q0 = 32 * h;
+      if (q0 >= nb) break;  // uniform
+      mma_tile32<false, false>(
+          [&](int ii, int q) {
+            const int i = r0 + ii;
+            return LV{Lb[(int64_t)(c0 + q) * m + min(i, m - 1)], i < m};
+          },
+          [&](int pp, int q) {
+            const int p = q0 + pp;
+            return LVs{Zb[q * kTile + p], dinv[p], p < nb};
+          },
+          nb, &As[0][0], acc[h]);
+    }
+    // every read of the rows' panel columns is done (the engine ends on a barrier)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      tile_foreach32(acc[h], [&](int ii, int pp, double v) {
+        const int i = r0 + ii, p = 32 * h + pp;
+        if (i < m && p < nb) Lb[(int64_t)(c0 + p) * m + i] = v;
+      });
     return;
   }
   TileAcc acc;
@@ -2403,7 +2414,7 @@ hipError_t launch_fl_diag(const DevPlan& P, const int32_t* tasks, int count, int
 hipError_t launch_fl_trsm(const DevPlan& P, const int32_t* tasks, int count, int step, int B, bool split,
                           double* L, const double* Z, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fl_trsm, dim3((split ? 4 : 1) * count, B), dim3(256), 0, st, P, tasks, step,
+  hipLaunchKernelGGL(k_fl_trsm, dim3((split ? 2 : 1) * count, B), dim3(256), 0, st, P, tasks, step,
                      split ? 1 : 0, L, Z);
   return hipGetLastError();
 }

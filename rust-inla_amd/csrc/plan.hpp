@@ -83,8 +83,9 @@ constexpr int kSelSplitK = 512;  // rows below a panel per partial product of Si
 constexpr int kGemmSplitK = 512;  // K per partial product of Sigma[B,P] when a level has few row blocks
 constexpr int kGemmSplitMaxTasks = 96;  // split Sigma[B,P] when a launch has at most this many row blocks
 constexpr int kXcdMaxTasks = 256;  // XCD-grouped launches: at most this many tasks per matrix
-constexpr int64_t kPpSplitMaxTasks = 1024;  // Sigma[P,P] launches of at most this many tasks split each in three
-constexpr int64_t kTile32MaxTasks = 128;  // panel-solve / Sigma[B,P] launches of at most this many tiles split each in four
+constexpr int64_t kPpSplitMaxTasks = (int64_t)1 << 30;  // Sigma[P,P] launches of at most this many tasks split each in three
+constexpr int64_t kTrsm32MaxTasks = 64;   // panel-solve launches of at most this many tiles split each in four
+constexpr int64_t kGemm32MaxTasks = 512;  // Sigma[B,P] launches of at most this many tiles split each in four
 
 // Staging area of a large front (T, per level): L_hat of two panels (m x 64 each,
 // panel step k in buffer k % 2, so that the next panel's L_hat can be formed while

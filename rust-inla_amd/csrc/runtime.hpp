@@ -370,8 +370,8 @@ struct Schedule {
   int32_t xcd_max = kXcdMaxTasks;
   int32_t stagger = -1;
   int64_t pp_split_max = kPpSplitMaxTasks;
-  int64_t trsm32_max = kTile32MaxTasks;
-  int64_t gemm32_max = kTile32MaxTasks;
+  int64_t trsm32_max = kTrsm32MaxTasks;
+  int64_t gemm32_max = kGemm32MaxTasks;
   uint8_t fac_mask(size_t l) const { return l < merge_fac.size() ? merge_fac[l] : 0; }
   uint8_t sel_mask(size_t l) const { return l < merge_sel.size() ? merge_sel[l] : 0; }
 };
