@@ -899,7 +899,7 @@ struct gmrf_batch {
   }* marks = nullptr;
 
   // GMRF_LEVEL_TIMES set: the time of every level of a factorisation / selected
-  // inversion, from events at the level boundaries, printed on stderr when the phase
+  // inversion / solve, from events at the level boundaries, printed on stderr when the phase
   // ends (synchronises, so the model evaluates eagerly, without graph capture; a
   // diagnostic of where the time goes in the tree).
   struct LevelClock {
@@ -937,7 +937,7 @@ struct gmrf_batch {
     ~LevelClock() {
       for (auto& e : ev) (void)hipEventDestroy(e);
     }
-  } lclock;
+  } lclock, sclock;  // sclock: the level-scheduled solves (forward, then backward)
 
   gmrf_batch(std::shared_ptr<SymData> s, int capacity) : sym(std::move(s)), cap(capacity) {
     check(hipSetDevice(sym->device), "hipSetDevice");
@@ -1189,7 +1189,9 @@ struct gmrf_batch {
     const double* Zf = Z + (int64_t)first * P.z_size;
     const int32_t* tk = sym->d_tasks;
     check(launch_gather_perm(P, count * nr, rhs, ldr, Xf, stream, period), "gather");
-    for (const LevelInfo& lv : sym->levels) {
+    sclock.start(sym->levels.size(), stream);
+    for (size_t li = 0; li < sym->levels.size(); ++li) {
+      const LevelInfo& lv = sym->levels[li];
       if (lv.se > lv.sb)
         prof_launch("k_solve_fwd", 0.5 * lv.sol_small * count * nr, 2.0 * lv.sol_small * count * nr, stream, [&] {
           return launch_solve_fwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
@@ -1210,8 +1212,11 @@ struct gmrf_batch {
                                          nr, Lf, Xf, Vf, stream);
           });
       }
+      sclock.mark(li + 1, stream);
     }
+    sclock.report("forward solve", sym->levels.size(), count * nr, sym->levels, false);
     check(launch_diag_scale(P, count, nr, Lf, Xf, stream), "diag_scale");
+    sclock.start(sym->levels.size(), stream);
     for (size_t l = sym->levels.size(); l-- > 0;) {
       const LevelInfo& lv = sym->levels[l];
       if (lv.le > lv.lb) {
@@ -1234,7 +1239,9 @@ struct gmrf_batch {
           return launch_solve_bwd(P, sym->d_small + lv.sb, (int)(lv.se - lv.sb), lv.maxm_small, lv.maxw_small, count, nr,
                                   Lf, Xf, stream);
         });
+      sclock.mark(sym->levels.size() - l, stream);
     }
+    sclock.report("backward solve", sym->levels.size(), count * nr, sym->levels, true);
     check(launch_scatter_perm(P, count * nr, Xf, x, ldx, stream), "scatter");
   }
 
