@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 (MFMA), FLOP/s
 PEAK_HBM = 8.0e12     # MI355X HBM3E, B/s
 METRIC = "CCD θ-points/sec (LDLᵀ+selinv+grad) FP64, SPDE n≈20k; % FP64-MFMA/HBM peak"
+# GMRF_ORDER_* of the ordering option; the analysis' order_kind names
+ORDERS = {"chooser": 0, "amd": 1, "nd": 4, "auto": 5}
+ORDER_KINDS = {0: "minimum degree", 1: "time-major", 2: "RCM", 3: "natural", 4: "given", 5: "nested dissection"}
 
 
 def log(msg):
@@ -244,7 +247,7 @@ def fit_compare(model, wl):
     return out, rb
 
 
-def poisson_line(nx, n_obs, steps=5):
+def poisson_line(nx, n_obs, steps=5, order="auto"):
     """The batched Poisson configuration on the same mesh (intercept + SPDE field,
     y ~ Poisson(exp(1 + u)), theta = (log tau, log kappa), CCD m = 2: 9 points):
     every point's prior LDL^T, Newton iterates, selected inversions and gradient
@@ -263,7 +266,7 @@ def poisson_line(nx, n_obs, steps=5):
     hh = max(1e-4, 0.05 * float(np.max(np.abs(wl.theta_true))))
     stencil = ccd.hessian_points(wl.theta_true, hh)
     model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="poisson", A=wl.A,
-                         capacity=max(len(z), len(stencil)))
+                         capacity=max(len(z), len(stencil)), order=order)
     c, _, _, _ = wl.params(stencil)
     r = model.eval(c, want_x=False, want_var=False)
     lpost = r["mlik"].cpu().numpy() + np.array([wl.log_prior(t)[0] for t in stencil])
@@ -774,6 +777,9 @@ def main():
                          "half the matrices per launch)")
     ap.add_argument("--cpu-fit", action="store_true",
                     help="add the reference's fit sequence measured on the oracle (host cores; minutes)")
+    ap.add_argument("--order", default="auto", choices=["auto", "chooser", "nd", "amd"],
+                    help="ordering of the headline and Poisson models (auto: the chooser's ordering or "
+                         "nested dissection, whichever has the shorter estimated level schedule)")
     ap.add_argument("--nx", type=int, default=142)
     ap.add_argument("--n-obs", type=int, default=50_000)
     args = ap.parse_args()
@@ -811,8 +817,9 @@ def main():
     cap = max(npts, len(stencil))
     log(f"== setup: model n={g.n} nnz={len(g.indices)} obs={len(wl.y)} capacity={cap}")
     model = LaplaceModel(g.n, g.indptr, g.indices, g.components, wl.y, family="gaussian", A=wl.A,
-                         capacity=cap)
-    st = Symbolic(g.n, g.indptr, g.indices).stats()
+                         capacity=cap, order=args.order)
+    # the prior's pattern equals the posterior's here (A^T A lies inside the SPDE stencil)
+    st = Symbolic(g.n, g.indptr, g.indices, order=ORDERS[args.order]).stats()
     # Hessian of the log posterior from one batched evaluation of the stencil
     hh = max(1e-4, 0.05 * float(np.max(np.abs(wl.theta_true))))
     c, dc, lp, dl = wl.params(stencil)
@@ -946,6 +953,7 @@ def main():
         "config": {"workload": f"spde-gaussian lattice {args.nx}x{args.nx} n={g.n}, "
                                f"{len(wl.y)} obs, CCD m=3 ({npts} theta-points)",
                    "n": g.n, "n_obs": len(wl.y), "theta_points": npts, "nnz_q": int(len(g.indices)),
+                   "ordering": f"{args.order} ({ORDER_KINDS.get(int(st['order_kind']), st['order_kind'])})",
                    "nnz_l": int(st["nnz_l"]), "nsuper": int(st["nsuper"]), "levels": int(st["nlevels"]),
                    "flops_ldlt": st["flops_ldlt"], "flops_selinv": st["flops_selinv"],
                    "mlik_mode": float(mlik[0]), "grad_mode_norm": float(np.linalg.norm(grad[0]))},
@@ -962,7 +970,7 @@ def main():
         log(json.dumps(res["fit"]))
     if not args.no_extra:
         log("== batched Poisson SPDE (CCD m=2)")
-        res["poisson_spde"] = poisson_line(args.nx, args.n_obs)
+        res["poisson_spde"] = poisson_line(args.nx, args.n_obs, order=args.order)
         log(json.dumps(res["poisson_spde"]))
         log("== heterogeneous batch: 64 BYM2 models")
         res["hetero"] = hetero_compare()

@@ -51,7 +51,14 @@ typedef enum gmrf_order {
   GMRF_ORDER_CHOOSER = 0,
   GMRF_ORDER_AMD = 1,
   GMRF_ORDER_NATURAL = 2,
-  GMRF_ORDER_GIVEN = 3 /* perm[k] = original index at factor position k */
+  GMRF_ORDER_GIVEN = 3, /* perm[k] = original index at factor position k */
+  /* No reference counterpart: orderings for the level-synchronous device schedule.
+   * ND: nested dissection by level-structure separators (short, wide elimination
+   * trees: fewer dependent level launches). AUTO: the CHOOSER's ordering or ND,
+   * whichever has the lower estimated schedule cost (dependent panel steps of the
+   * levels plus flops). Results agree with the CHOOSER's to rounding. */
+  GMRF_ORDER_ND = 4,
+  GMRF_ORDER_AUTO = 5
 } gmrf_order;
 
 typedef struct gmrf_symbolic gmrf_symbolic;
@@ -62,7 +69,7 @@ typedef struct gmrf_symbolic_stats {
   int64_t n;
   int64_t nnz;           /* stored entries of the input CSC */
   int64_t nnz_q_lower;   /* entries with row >= col */
-  int32_t order_kind;    /* 0 AMD, 1 time-major, 2 RCM, 3 natural, 4 given */
+  int32_t order_kind;    /* 0 AMD, 1 time-major, 2 RCM, 3 natural, 4 given, 5 nested dissection */
   int32_t nlevels;       /* height of the supernodal tree */
   int64_t nsuper;
   int64_t nnz_l;         /* structural nnz(L) incl. the diagonal */
@@ -271,7 +278,9 @@ typedef struct gmrf_model_desc {
                                 the posterior as an exact low-rank correction of the grounded
                                 precision, so long constraint rows do not densify the pattern
                                 (no reference counterpart; k <= 32) */
-  int32_t order;             /* GMRF_ORDER_CHOOSER (the reference's choice) or GMRF_ORDER_AMD */
+  int32_t order;             /* GMRF_ORDER_CHOOSER (the reference's choice), GMRF_ORDER_AMD,
+                                GMRF_ORDER_NATURAL (the caller's order), GMRF_ORDER_ND or
+                                GMRF_ORDER_AUTO */
   int32_t n_blocks;          /* 0 or 1: one model. nb > 1: nb independent models (a heterogeneous
                                 batch, e.g. many areal models of uneven size) as one
                                 block-diagonal union: block q owns the latent indices

@@ -56,15 +56,23 @@ constexpr size_t kCacheMax = 16;
 
 }  // namespace
 
+OrderRequest order_request(int order) {
+  switch (order) {
+    case GMRF_ORDER_AMD: return OrderRequest::AmdOnly;
+    case GMRF_ORDER_NATURAL: return OrderRequest::Natural;
+    case GMRF_ORDER_GIVEN: return OrderRequest::Given;
+    case GMRF_ORDER_ND: return OrderRequest::NestedDissection;
+    case GMRF_ORDER_AUTO: return OrderRequest::Auto;
+    default: return OrderRequest::Chooser;
+  }
+}
+
 std::shared_ptr<SymData> analyze_uncached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                           int order, const int64_t* given) {
   std::string err = validate_pattern(n, indptr, indices);
   if (!err.empty()) throw std::runtime_error(err);
   auto d = std::make_shared<SymData>();
-  OrderRequest req = order == GMRF_ORDER_AMD       ? OrderRequest::AmdOnly
-                     : order == GMRF_ORDER_NATURAL ? OrderRequest::Natural
-                     : order == GMRF_ORDER_GIVEN   ? OrderRequest::Given
-                                                   : OrderRequest::Chooser;
+  OrderRequest req = order_request(order);
   d->S = analyze(n, indptr, indices, req, given);
   d->pools = plan_pools(d->S);
   d->device = g_device;
@@ -338,7 +346,7 @@ gmrf_status gmrf_analyze(int64_t n, const int64_t* indptr, const int64_t* indice
                          const int64_t* perm, gmrf_symbolic** out) {
   if (!out || !indptr || (n > 0 && !indices && indptr[n] > 0))
     return fail(GMRF_INVALID_ARGUMENT, "null argument");
-  if (order < 0 || order > 3) return fail(GMRF_INVALID_ARGUMENT, "unknown ordering request");
+  if (order < 0 || order > 5) return fail(GMRF_INVALID_ARGUMENT, "unknown ordering request");
   if (order == GMRF_ORDER_GIVEN && !perm)
     return fail(GMRF_INVALID_ARGUMENT, "GMRF_ORDER_GIVEN needs a permutation");
   try {

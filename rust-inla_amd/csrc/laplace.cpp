@@ -620,8 +620,8 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
     return fail(GMRF_INVALID_ARGUMENT, "unknown likelihood family or link");
   if (!d->a_indptr && d->n_obs != d->n)
     return fail(GMRF_INVALID_ARGUMENT, "Observations length must match prior precision dimension");
-  if (d->order != GMRF_ORDER_CHOOSER && d->order != GMRF_ORDER_AMD)
-    return fail(GMRF_INVALID_ARGUMENT, "model ordering must be the chooser or AMD");
+  if (d->order < GMRF_ORDER_CHOOSER || d->order > GMRF_ORDER_AUTO || d->order == GMRF_ORDER_GIVEN)
+    return fail(GMRF_INVALID_ARGUMENT, "model ordering must be the chooser, AMD, natural, ND or auto");
   if (d->constr_kappa < 0.0 || (d->constr_kappa > 0.0 && (d->n_constr < 1 || d->n_constr > 32)))
     return fail(GMRF_INVALID_ARGUMENT, "low-rank constraint augmentation needs 1..32 constraints");
   if (d->constr_kappa > 0.0 && (int64_t)d->capacity * 2 * d->n_constr > 65535)
@@ -664,7 +664,7 @@ gmrf_status gmrf_model_create(const gmrf_model_desc* d, gmrf_model** out) {
       // each block ordered on its own pattern, as its model alone would be; the union
       // is analysed in that (block-diagonal) ordering
       std::vector<int64_t> perm(n);
-      const OrderRequest req = d->order == GMRF_ORDER_AMD ? OrderRequest::AmdOnly : OrderRequest::Chooser;
+      const OrderRequest req = order_request(d->order);
       for (int q = 0; q < nb; ++q) {
         const int64_t n0 = bp[q], nq = bp[q + 1] - bp[q];
         std::vector<int64_t> ipq(nq + 1), ixq;

@@ -1360,7 +1360,8 @@ struct gmrf_solver {
   }
 };
 
-
+// The analysis request of a GMRF_ORDER_* value.
+OrderRequest order_request(int order);
 std::shared_ptr<SymData> analyze_cached(int64_t n, const int64_t* indptr, const int64_t* indices,
                                         int order, const int64_t* given);
 // Autotuner (gmrf_api.cpp): times candidate schedules of factorisation + selected

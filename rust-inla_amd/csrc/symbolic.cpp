@@ -442,6 +442,182 @@ void amd_order(int64_t n, const int64_t* indptr, const int64_t* indices,
 }
 
 // ---------------------------------------------------------------------------
+// Nested dissection with level-structure separators (A. George, "Nested dissection
+// of a regular finite element mesh", SIAM J. Numer. Anal. 10(2), 1973; the automatic
+// form of George & Liu, "Computer Solution of Large Sparse Positive Definite
+// Systems", 1981, ch. 8). A connected subgraph larger than `leaf` is split by a level
+// set of a breadth-first search from a pseudo-peripheral vertex: the middle level,
+// thinned to the vertices adjacent to the next level, is numbered last and the rest
+// is dissected recursively (each connected component on its own); a subgraph of at
+// most `leaf` vertices keeps the order of its vertices in the work list, so that it
+// becomes a few dense fronts. The elimination tree is short and wide, which the
+// level-synchronous factorisation turns into fewer dependent launches than the
+// deep trees of minimum degree.
+// ---------------------------------------------------------------------------
+void nested_dissection_order(int64_t n, const int64_t* indptr, const int64_t* indices, int64_t leaf,
+                             std::vector<int64_t>& order) {
+  order.assign(n, -1);
+  if (n == 0) return;
+  leaf = std::max<int64_t>(leaf, 1);
+  // symmetric adjacency without the diagonal, from the lower triangle (row > col)
+  std::vector<int64_t> xadj(n + 1, 0);
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p)
+      if (indices[p] > c) {
+        xadj[c + 1]++;
+        xadj[indices[p] + 1]++;
+      }
+  for (int64_t i = 0; i < n; ++i) xadj[i + 1] += xadj[i];
+  std::vector<int64_t> adj(xadj[n]);
+  {
+    std::vector<int64_t> fill(xadj.begin(), xadj.end() - 1);
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+        const int64_t r = indices[p];
+        if (r > c) {
+          adj[fill[c]++] = r;
+          adj[fill[r]++] = c;
+        }
+      }
+  }
+  // stamp[v] == cur: v belongs to the subgraph being split; lev[v]: its BFS level
+  // in the current search (valid where seen[v] == search id)
+  std::vector<int64_t> stamp(n, -1), seen(n, -1), lev(n, 0), placed(n, -1);
+  int64_t cur = 0, search = 0;
+  // dense rows (degree > max(16, 10 sqrt n), as the minimum-degree ordering treats
+  // them) are numbered last and left out of the graph being dissected: one such row
+  // (an intercept) would otherwise put every vertex within two levels of every other
+  const int64_t dense_deg = std::max<int64_t>(16, (int64_t)(10.0 * std::sqrt((double)n)));
+  std::vector<int64_t> buf, queue(n), tmp, dense;
+  for (int64_t v = 0; v < n; ++v) {
+    if (xadj[v + 1] - xadj[v] > dense_deg) {
+      dense.push_back(v);
+      stamp[v] = -3;
+    } else {
+      buf.push_back(v);
+    }
+  }
+  const int64_t nsparse = (int64_t)buf.size();
+  for (int64_t i = 0; i < (int64_t)dense.size(); ++i) order[nsparse + i] = dense[i];
+  buf.resize(n);
+  // BFS from src over the stamped vertices into queue; returns the number visited and
+  // fills lstart with each level's first queue index (plus the end)
+  std::vector<int64_t> lstart;
+  auto bfs = [&](int64_t src) {
+    const int64_t id = search++;
+    int64_t head = 0, tail = 0;
+    queue[tail++] = src;
+    seen[src] = id;
+    lev[src] = 0;
+    lstart.assign(1, 0);
+    while (head < tail) {
+      const int64_t u = queue[head++];
+      for (int64_t p = xadj[u]; p < xadj[u + 1]; ++p) {
+        const int64_t v = adj[p];
+        if (stamp[v] != cur || seen[v] == id) continue;
+        seen[v] = id;
+        lev[v] = lev[u] + 1;
+        queue[tail++] = v;
+      }
+    }
+    // level boundaries (the queue is in nondecreasing level order)
+    for (int64_t q = 1; q < tail; ++q)
+      if (lev[queue[q]] != lev[queue[q - 1]]) lstart.push_back(q);
+    lstart.push_back(tail);
+    return tail;
+  };
+  auto degree_in = [&](int64_t v) {
+    int64_t d = 0;
+    for (int64_t p = xadj[v]; p < xadj[v + 1]; ++p) d += stamp[adj[p]] == cur;
+    return d;
+  };
+  // work list: segment [b, e) of buf, numbered into order[o, o + e - b)
+  struct Task {
+    int64_t b, e, o;
+  };
+  std::vector<Task> work{{0, nsparse, 0}};
+  while (!work.empty()) {
+    const Task t = work.back();
+    work.pop_back();
+    const int64_t sz = t.e - t.b;
+    if (sz <= leaf) {
+      for (int64_t i = 0; i < sz; ++i) order[t.o + i] = buf[t.b + i];
+      continue;
+    }
+    cur++;
+    for (int64_t i = t.b; i < t.e; ++i) stamp[buf[i]] = cur;
+    // connected components, each in BFS order
+    const int64_t reached = bfs(buf[t.b]);
+    if (reached < sz) {
+      tmp.assign(queue.begin(), queue.begin() + reached);
+      std::vector<int64_t> comp_end{reached};
+      for (int64_t q = 0; q < reached; ++q) placed[queue[q]] = cur;
+      for (int64_t i = t.b; i < t.e; ++i) {
+        const int64_t v = buf[i];
+        if (placed[v] == cur) continue;
+        const int64_t k = bfs(v);
+        for (int64_t q = 0; q < k; ++q) placed[queue[q]] = cur;
+        tmp.insert(tmp.end(), queue.begin(), queue.begin() + k);
+        comp_end.push_back((int64_t)tmp.size());
+      }
+      std::copy(tmp.begin(), tmp.end(), buf.begin() + t.b);
+      int64_t c0 = 0;
+      for (int64_t ce : comp_end) {
+        work.push_back({t.b + c0, t.b + ce, t.o + c0});
+        c0 = ce;
+      }
+      continue;
+    }
+    // pseudo-peripheral vertex: restart from a minimum-degree vertex of the last level
+    // while the level structure gets deeper
+    int64_t root = buf[t.b], nlev = (int64_t)lstart.size() - 1;
+    for (int it = 0; it < 5; ++it) {
+      int64_t cand = -1, cd = 0;
+      for (int64_t q = lstart[nlev - 1]; q < lstart[nlev]; ++q) {
+        const int64_t v = queue[q], d = degree_in(v);
+        if (cand < 0 || d < cd) cand = v, cd = d;
+      }
+      bfs(cand);
+      const int64_t nl2 = (int64_t)lstart.size() - 1;
+      if (nl2 <= nlev) break;
+      root = cand;
+      nlev = nl2;
+    }
+    bfs(root);
+    nlev = (int64_t)lstart.size() - 1;
+    if (nlev < 3) {
+      // no separating level: keep the list order
+      for (int64_t i = 0; i < sz; ++i) order[t.o + i] = buf[t.b + i];
+      continue;
+    }
+    const int64_t mid = nlev / 2;
+    const int64_t id = search - 1;
+    tmp.clear();
+    std::vector<int64_t> sep;
+    for (int64_t q = lstart[mid]; q < lstart[mid + 1]; ++q) {
+      const int64_t v = queue[q];
+      bool next = false;
+      for (int64_t p = xadj[v]; p < xadj[v + 1] && !next; ++p) {
+        const int64_t u = adj[p];
+        next = stamp[u] == cur && seen[u] == id && lev[u] == mid + 1;
+      }
+      if (next) sep.push_back(v);
+    }
+    // the rest in list order, then the separator (numbered last)
+    for (int64_t v : sep) stamp[v] = -2;  // out of every later subgraph
+    for (int64_t i = t.b; i < t.e; ++i)
+      if (stamp[buf[i]] != -2) tmp.push_back(buf[i]);
+    const int64_t nr = (int64_t)tmp.size();
+    std::copy(tmp.begin(), tmp.end(), buf.begin() + t.b);
+    for (size_t i = 0; i < sep.size(); ++i) {
+      buf[t.b + nr + i] = sep[i];
+      order[t.o + nr + i] = sep[i];
+    }
+    work.push_back({t.b, t.b + nr, t.o});
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pattern helpers
 // ---------------------------------------------------------------------------
 std::string validate_pattern(int64_t n, const int64_t* indptr, const int64_t* indices) {
@@ -474,6 +650,26 @@ uint64_t hash_pattern(int64_t n, const int64_t* indptr, const int64_t* indices) 
   return h;
 }
 
+// The level schedule runs each level as one launch sequence over all matrices of a
+// batch; a level whose fronts have more than 64 rows takes one dependent 64-column
+// panel step per 64 columns of its widest such front. Cost: 70 us per dependent step
+// (a factorisation and a selected-inversion step of a 30-matrix batch at n = 20k on
+// one MI355X) plus the flops at 0.33 TFLOP/s per matrix (10 TFLOP/s over the batch).
+double level_schedule_cost(const Symbolic& S) {
+  constexpr double kStepSeconds = 70e-6, kFlopRate = 3.3e11;
+  double steps = 0.0;
+  for (int l = 0; l < S.nlevels; ++l) {
+    int64_t p = 1;
+    for (int64_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int64_t s = S.level_sn[q];
+      const int64_t w = S.sn_start[s + 1] - S.sn_start[s], m = w + (S.sn_rows_ptr[s + 1] - S.sn_rows_ptr[s]);
+      if (m > 64) p = std::max<int64_t>(p, (w + 63) / 64);
+    }
+    steps += (double)p;
+  }
+  return steps * kStepSeconds + (S.flops_ldlt + S.flops_selinv) / kFlopRate;
+}
+
 // ---------------------------------------------------------------------------
 // Full analysis
 // ---------------------------------------------------------------------------
@@ -482,6 +678,11 @@ Symbolic analyze(int64_t n, const int64_t* indptr, const int64_t* indices, Order
   std::string err = validate_pattern(n, indptr, indices);
   if (!err.empty()) throw std::runtime_error(err);
   if (n == 0) throw std::runtime_error("empty or invalid matrix: n must be > 0");
+  if (req == OrderRequest::Auto) {
+    Symbolic a = analyze(n, indptr, indices, OrderRequest::Chooser, nullptr);
+    Symbolic b = analyze(n, indptr, indices, OrderRequest::NestedDissection, nullptr);
+    return level_schedule_cost(b) < level_schedule_cost(a) ? b : a;
+  }
   Symbolic S;
   S.n = n;
   S.nnz_full = indptr[n];
@@ -504,6 +705,9 @@ Symbolic analyze(int64_t n, const int64_t* indptr, const int64_t* indices, Order
   } else if (req == OrderRequest::Natural) {
     std::iota(perm0.begin(), perm0.end(), 0);
     S.order_kind = OrderKind::Natural;
+  } else if (req == OrderRequest::NestedDissection) {
+    nested_dissection_order(n, indptr, indices, kNdLeaf, perm0);
+    S.order_kind = OrderKind::NestedDissection;
   } else {
     std::vector<int64_t> fwd, inv;
     OrderKind k = OrderKind::Amd;

@@ -25,10 +25,13 @@ enum class OrderKind : int {
   Rcm = 2,        // reverse Cuthill-McKee
   Natural = 3,    // identity (explicit request only)
   Given = 4,      // caller-supplied permutation (explicit request only)
+  NestedDissection = 5,  // level-structure nested dissection (explicit request or Auto)
 };
 
 // Ordering request for analyze(); Chooser mirrors the reference decision.
-enum class OrderRequest : int { Chooser = 0, AmdOnly = 1, Natural = 2, Given = 3 };
+// Auto: the chooser's ordering or nested dissection, whichever has the lower
+// level-schedule cost (level_schedule_cost).
+enum class OrderRequest : int { Chooser = 0, AmdOnly = 1, Natural = 2, Given = 3, NestedDissection = 4, Auto = 5 };
 
 struct Symbolic {
   int64_t n = 0;
@@ -101,6 +104,16 @@ OrderKind choose_symmetric_order(int64_t n, const int64_t* indptr, const int64_t
 // Approximate minimum degree on the symmetric pattern; order[k] = k-th pivot.
 void amd_order(int64_t n, const int64_t* indptr, const int64_t* indices,
                std::vector<int64_t>& order);
+
+// Nested dissection; order[k] = k-th pivot. Subgraphs of at most `leaf` vertices are
+// not split further.
+void nested_dissection_order(int64_t n, const int64_t* indptr, const int64_t* indices, int64_t leaf,
+                             std::vector<int64_t>& order);
+constexpr int64_t kNdLeaf = 48;
+
+// Estimated seconds of a batched factorisation and selected inversion under this
+// analysis' level schedule (the chain of dependent level launches plus the flops).
+double level_schedule_cost(const Symbolic& S);
 
 // Validate a CSC pattern (square, sorted unique rows in range). Empty on success.
 std::string validate_pattern(int64_t n, const int64_t* indptr, const int64_t* indices);

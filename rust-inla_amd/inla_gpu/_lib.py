@@ -24,6 +24,8 @@ ORDER_CHOOSER = 0
 ORDER_AMD = 1
 ORDER_NATURAL = 2
 ORDER_GIVEN = 3
+ORDER_ND = 4
+ORDER_AUTO = 5
 
 _i32 = C.c_int32
 _i64 = C.c_int64

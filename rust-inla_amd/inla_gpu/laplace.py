@@ -63,7 +63,11 @@ class LaplaceModel:
         constraint_kappa, the augmentation Q + kappa A_c^T A_c of the prior and the
         posterior is applied as an exact low-rank correction (the components then
         must not contain it); otherwise a kappa A_c^T A_c component carries it, as
-        the reference forms it. order: "chooser" (the reference's choice) or "amd".
+        the reference forms it. order: "chooser" (the reference's choice), "amd",
+        "natural" (the caller's order: a permutation the caller applied beforehand),
+        "nd" (nested dissection) or "auto" (the chooser's ordering or nested
+        dissection, whichever has the shorter estimated level schedule; same results
+        to rounding).
         blocks: nb + 1 latent offsets of independent models in a block-diagonal union
         (observations in block order; see gmrf_model_desc.n_blocks). lean: the prior is
         factorised alone before the Newton loop, so the device holds capacity factors
@@ -124,7 +128,8 @@ class LaplaceModel:
             if constraints is None:
                 raise InvalidArgument("constraint_kappa needs constraints")
             d.constr_kappa = float(constraint_kappa)
-        d.order = {"chooser": L.ORDER_CHOOSER, "amd": L.ORDER_AMD}[order]
+        d.order = {"chooser": L.ORDER_CHOOSER, "amd": L.ORDER_AMD, "natural": L.ORDER_NATURAL,
+                   "nd": L.ORDER_ND, "auto": L.ORDER_AUTO}[order]
         self.nb = 1
         if blocks is not None:
             bp = _arr64(blocks)
