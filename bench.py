@@ -503,7 +503,7 @@ def hetero_fit_line(k=64):
                         "priors loggamma(1, 5e-5) on tau, N(0, 1) on logit phi"}
 
 
-def big_solver_line(Q, label, probes=6):
+def big_solver_line(Q, label, probes=6, order="auto"):
     """Full-size solver line at batch 1: factorise, solve and selected-inverse diagonal
     of one posterior precision, with size-independent checks: log|2Q| - log|Q| = n log 2,
     the solve residual, and diagonal entries against solves of e_i."""
@@ -514,7 +514,7 @@ def big_solver_line(Q, label, probes=6):
 
     n = Q.shape[0]
     t0 = time.perf_counter()
-    sym = Symbolic.from_csc(Q)
+    sym = Symbolic.from_csc(Q, order=ORDERS[order])
     st = sym.stats()
     t_sym = time.perf_counter() - t0
     bt = Batch(sym, 1)
@@ -550,7 +550,7 @@ def big_solver_line(Q, label, probes=6):
         dmax = max(dmax, abs(bt.solve(e)[0, i].item() - dg[i]) / dg[i])
     bt.factorize(2.0 * v)
     ld2, _ = bt.logdet()
-    return {"workload": label, "n": int(n), "nnz_q": int(Q.nnz),
+    return {"workload": label, "n": int(n), "nnz_q": int(Q.nnz), "order": order,
             "order_kind": int(st["order_kind"]), "nsuper": int(st["nsuper"]), "levels": int(st["nlevels"]),
             "nnz_l": int(st["nnz_l"]), "l_size": int(st["l_size"]), "max_front": int(st["max_front"]),
             "flops_ldlt": st["flops_ldlt"], "flops_selinv": st["flops_selinv"],
@@ -561,7 +561,7 @@ def big_solver_line(Q, label, probes=6):
             "solve_rel_residual": resid, "selinv_diag_vs_solves_rel": dmax}
 
 
-def c4_solver_line(nr=100, T=20, probes=6):
+def c4_solver_line(nr=100, T=20, probes=6, order="auto"):
     """Full-size Besag x AR1 (100 x 100 rook lattice x T = 20, n = 200,000; the
     reference's chooser picks RCM: a chain of dense fronts): Q_post = Q(tau=2, rho=0.7)
     + diag(2.5) (binomial(10) weights at eta = 0)."""
@@ -573,10 +573,10 @@ def c4_solver_line(nr=100, T=20, probes=6):
     c, _, _, _ = wl.params(wl.theta_true[None])
     Q = sp.csc_matrix(wl.gmrf.csc(c[0]) + 2.5 * sp.identity(wl.n))
     Q.sort_indices()
-    return big_solver_line(Q, f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7", probes)
+    return big_solver_line(Q, f"besag {nr}x{nr} x ar1(T={T}), n={wl.n}, Q_post at tau=2, rho=0.7", probes, order)
 
 
-def c4_fit_line(order="amd", nr=100, T=20):
+def c4_fit_line(order="auto", nr=100, T=20):
     """Full-size Besag x AR1 binomial(10) (100 x 100 rook lattice x T = 20, n = 200,000) with
     its 20 sum-to-zero rows applied as the low-rank augmentation (the explicit
     kappa A_c^T A_c has 2e9 entries): one batched evaluation of the 9 CCD points with
@@ -636,7 +636,7 @@ def c4_fit_line(order="amd", nr=100, T=20):
         mc = make("chooser")
         oc, dtc = timed_eval(mc)
         res["chooser_eval_s"] = dtc
-        res["chooser_vs_amd_mlik_maxdiff"] = float((oc["mlik"] - out["mlik"]).abs().max().item())
+        res["chooser_vs_order_mlik_maxdiff"] = float((oc["mlik"] - out["mlik"]).abs().max().item())
         log(f"[c4 fit] the same under the chooser's order: {dtc:.2f} s")
         del mc
         model = make(order)
@@ -661,7 +661,7 @@ def c4_fit_line(order="amd", nr=100, T=20):
     return res
 
 
-def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
+def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4, order="auto"):
     """Full-size SPDE x RW2 (245 x 245 lattice x 24 time steps, n = 1,440,600; RCM chain of
     fronts up to 13k wide, about 100 GB of L): Q_post with the negative binomial (size 5)
     Fisher weights of 500,000 observations at eta = 0.5."""
@@ -671,10 +671,10 @@ def c5_solver_line(nx=245, T=24, n_obs=500_000, probes=4):
     Q = W.spde_rw2_posterior_precision(nx=nx, T=T, n_obs=n_obs)
     print(f"[c5] Q_post built in {time.perf_counter() - t0:.1f} s (nnz {Q.nnz})", file=sys.stderr, flush=True)
     return big_solver_line(Q, f"spde {nx}x{nx} x rw2(T={T}), n={Q.shape[0]}, {n_obs} NB obs, "
-                              "Q_post at tau=0.5, kappa=3", probes)
+                              "Q_post at tau=0.5, kappa=3", probes, order)
 
 
-def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="chooser", fd_components=(1, 4)):
+def c5_laplace_line(nx=245, T=24, n_obs=500_000, order="auto", fd_components=(1, 4)):
     """Full-size SPDE x RW2 negative binomial Laplace evaluation with five hyperparameters
     (n = 1,440,600, 500,000 observations, size 5; Q = Q_spde(tau1, kappa1) x (R_rw2 +
     eps I) + Q_spde(tau2, kappa2) x I_T + tau3 I) at one theta-point with the adjoint
@@ -770,7 +770,8 @@ def main():
                     help="add the full-size SPDE x RW2 solver line (about 220 GB of HBM)")
     ap.add_argument("--c5-laplace", action="store_true",
                     help="add the full-size SPDE x RW2 negative binomial Laplace evaluation (lean; minutes)")
-    ap.add_argument("--c5-order", default="chooser", choices=["chooser", "amd"])
+    ap.add_argument("--big-order", default="auto", choices=["auto", "chooser", "nd", "amd"],
+                    help="ordering of the full-size C4 / C5 lines")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch schedule")
     ap.add_argument("--stagger", action="store_true",
                     help="let the tuner also try two staggered pipelines of half the batch each (eager, "
@@ -983,19 +984,19 @@ def main():
         log(json.dumps(res["hetero_fit"]))
     if args.c4:
         log("== full-size Besag x AR1 (n = 200k): factorise, solve, selected inversion")
-        res["c4"] = c4_solver_line()
+        res["c4"] = c4_solver_line(order=args.big_order)
         log(json.dumps(res["c4"]))
     if args.c4_fit:
         log("== full-size Besag x AR1 binomial: batched CCD evaluation, BFGS vs Nelder-Mead fit")
-        res["c4_fit"] = c4_fit_line()
+        res["c4_fit"] = c4_fit_line(order=args.big_order)
         log(json.dumps(res["c4_fit"]))
     if args.c5:
         log("== full-size SPDE x RW2 (n = 1.44M): factorise, solve, selected inversion")
-        res["c5"] = c5_solver_line()
+        res["c5"] = c5_solver_line(order=args.big_order)
         log(json.dumps(res["c5"]))
     if args.c5_laplace:
         log("== full-size SPDE x RW2 negative binomial Laplace, five hyperparameters (n = 1.44M, lean)")
-        res["c5_laplace"] = c5_laplace_line(order=args.c5_order)
+        res["c5_laplace"] = c5_laplace_line(order=args.big_order)
         log(json.dumps(res["c5_laplace"]))
     if args.cpu_fit:
         log("== the reference's fit sequence on the oracle (host cores)")
