@@ -488,17 +488,15 @@ __device__ __forceinline__ void base16(double* F, int k0, int kn, double* Zk) {
 }
 
 // Blocked LDL^T of the first ncols columns of an m x m front in F (m <= MB;
-// the calling workgroup has MB / 16 waves). With WITHZ (ncols == m), Zs (same
-// layout as F) receives Z = L^-1. Zk (16 x 17) and Wb (MB x 17) are scratch.
+// the calling workgroup has MB / 16 waves; only F's lower triangle is read). With
+// WITHZ (ncols == m, F's strict upper triangle zero on entry) F's strict upper
+// triangle receives Z = L^-1, transposed: Z[i][j] (i > j) at F[j * LD + i]; Z's
+// diagonal is 1 and its upper part 0. Zk (16 x 17) and Wb (MB x 17) are scratch.
 template <int MB, bool WITHZ>
-__device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zs, double* Zk,
-                                           double* Wb) {
+__device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zk, double* Wb) {
   constexpr int LD = MB + 1, NW = MB / 16;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int vr = lane >> 4, vc = lane & 15;  // accumulator element (4 v + vr, vc)
-  if (WITHZ) {
-    for (int idx = threadIdx.x; idx < MB * LD; idx += 64 * NW) Zs[idx] = 0.0;
-  }
   if (wv == 0 && ncols > 0) base16<MB, LD>(F, 0, min(16, ncols), Zk);
   __syncthreads();
   for (int k0 = 0; k0 < ncols; k0 += 16) {
@@ -532,20 +530,21 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
             [&](int ii, int q) {
               return LV{Zk[ii * 17 + q], q < kn};
             },
-            [&](int q, int jj) {
-              return LV{Zs[(k0 + min(q, kn - 1)) * LD + 16 * jb + jj], q < kn};
+            [&](int q, int jj) {  // Z[k0 + q][16 jb + jj], strictly lower (jb < kb)
+              return LV{F[(16 * jb + jj) * LD + k0 + min(q, kn - 1)], q < kn};
             },
             acc);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int ii = 4 * v + vr;
-          if (ii < kn) Zs[(k0 + ii) * LD + 16 * jb + vc] = acc[v];
+          if (ii < kn) F[(16 * jb + vc) * LD + k0 + ii] = acc[v];
         }
       } else {
+        // Z[kb][kb] = Zk: its strictly lower entries
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int ii = 4 * v + vr;
-          if (ii < kn && vc < kn) Zs[(k0 + ii) * LD + k0 + vc] = Zk[ii * 17 + vc];
+          if (ii < kn && vc < ii) F[(k0 + vc) * LD + k0 + ii] = Zk[ii * 17 + vc];
         }
       }
     }
@@ -590,14 +589,16 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
             [&](int ii, int pp) {
               return LV{F[min(R + ii, m - 1) * LD + k0 + min(pp, kn - 1)], pp < kn};
             },
-            [&](int pp, int jj) {
-              return LV{Zs[(k0 + min(pp, kn - 1)) * LD + 16 * jb + jj], pp < kn};
+            [&](int pp, int jj) {  // Z[k0 + pp][16 jb + jj], jb <= kb (the diagonal block: unit lower)
+              const int zi = k0 + min(pp, kn - 1), zj = 16 * jb + jj;
+              const double zv = zi == zj ? 1.0 : F[min(zi, zj) * LD + max(zi, zj)];
+              return LV{zi >= zj ? zv : 0.0, pp < kn};
             },
             acc);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = R + 4 * v + vr;
-          if (i < m) Zs[i * LD + 16 * jb + vc] -= acc[v];
+          if (i < m) F[(16 * jb + vc) * LD + i] -= acc[v];
         }
       }
     }
@@ -713,7 +714,7 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
     }
     __syncthreads();
   }
-  front_ldlt<MB, false>(F, m, w, nullptr, Zk, Wb);
+  front_ldlt<MB, false>(F, m, w, Zk, Wb);
   for (int idx = tid; idx < m * m; idx += NT) {
     const int j = idx / m, i = idx - j * m;
     if (j < w)
@@ -729,17 +730,16 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
 // (supernode, a, b) whose meaning depends on the kernel.
 // ---------------------------------------------------------------------------
 
-// LDS of one diagonal-block factorisation: F and Zs (64 x 65 each), Zk (16 x 17),
-// Wb (64 x 17).
-constexpr int kDiagLds = 2 * kTile * (kTile + 1) + 16 * 17 + kTile * 17;
+// LDS of one diagonal-block factorisation: F (64 x 65: L and D in the lower
+// triangle, Z = L^-1 transposed in the strict upper one), Zk (16 x 17), Wb (64 x 17).
+constexpr int kDiagLds = kTile * (kTile + 1) + 16 * 17 + kTile * 17;
 
 __device__ __forceinline__ void fl_diag_block(const DevPlan& P, int s, int step, int b,
                                               double* __restrict__ L, double* __restrict__ Z,
                                               double* lds) {
   constexpr int LD = kTile + 1;
   double* F = lds;
-  double* Zs = F + kTile * LD;
-  double* Zk = Zs + kTile * LD;
+  double* Zk = F + kTile * LD;
   double* Wb = Zk + 16 * 17;
   const int tid = threadIdx.x;
   const int a0 = P.sn_start[s], w = P.sn_start[s + 1] - a0, r = P.sn_r[s], m = w + r;
@@ -759,13 +759,13 @@ __device__ __forceinline__ void fl_diag_block(const DevPlan& P, int s, int step,
     }
   }
   __syncthreads();
-  front_ldlt<kTile, true>(F, nb, nb, Zs, Zk, Wb);
+  front_ldlt<kTile, true>(F, nb, nb, Zk, Wb);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int j = c + 4 * t;
     if (i < nb && j < nb) {
       Lb[(int64_t)(c0 + j) * m + c0 + i] = (j <= i) ? F[i * LD + j] : 0.0;
-      Zb[j * kTile + i] = (j <= i) ? Zs[i * LD + j] : 0.0;
+      Zb[j * kTile + i] = (j < i) ? F[j * LD + i] : (j == i ? 1.0 : 0.0);
     }
   }
 }
