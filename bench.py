@@ -345,6 +345,7 @@ def hetero_compare(k=64, reps=3):
 
 
 _HETERO = {}
+HETERO_ORDER = ["chooser"]  # ordering of the heterogeneous models (set from --hetero-order)
 
 
 def _hetero_setup(k):
@@ -361,7 +362,7 @@ def _hetero_setup(k):
     log(f"   workload: {k} models, n = {wl.n}, {t_build:.1f} s")
     t0 = time.perf_counter()
     union = LaplaceModel(wl.n, wl.indptr, wl.indices, wl.components, wl.y, family="poisson", A=wl.A,
-                         blocks=wl.block_ptr, capacity=9)
+                         blocks=wl.block_ptr, capacity=9, order=HETERO_ORDER[0])
     t_setup = time.perf_counter() - t0
     bp = wl.block_ptr
     singles = []
@@ -369,7 +370,7 @@ def _hetero_setup(k):
         e0, e1 = wl.indptr[bp[q]], wl.indptr[bp[q + 1]]
         singles.append(LaplaceModel(bp[q + 1] - bp[q], wl.indptr[bp[q]:bp[q + 1] + 1] - e0,
                                     wl.indices[e0:e1] - bp[q], wl.components[:, e0:e1], wl.block_y(q),
-                                    family="poisson", A=wl.block_A(q), capacity=9))
+                                    family="poisson", A=wl.block_A(q), capacity=9, order=HETERO_ORDER[0]))
     _HETERO[k] = (wl, union, singles, t_build, t_setup)
     return _HETERO[k]
 
@@ -781,6 +782,8 @@ def main():
     ap.add_argument("--order", default="auto", choices=["auto", "chooser", "nd", "amd"],
                     help="ordering of the headline and Poisson models (auto: the chooser's ordering or "
                          "nested dissection, whichever has the shorter estimated level schedule)")
+    ap.add_argument("--hetero-order", default="chooser", choices=["auto", "chooser", "nd", "amd"],
+                    help="ordering of each model of the heterogeneous Laplace lines")
     ap.add_argument("--nx", type=int, default=142)
     ap.add_argument("--n-obs", type=int, default=50_000)
     args = ap.parse_args()
@@ -969,6 +972,7 @@ def main():
         log("== end-to-end fit: batched BFGS + adjoint vs one-point Nelder-Mead")
         res["fit"], _ = fit_compare(model, wl)
         log(json.dumps(res["fit"]))
+    HETERO_ORDER[0] = args.hetero_order
     if not args.no_extra:
         log("== batched Poisson SPDE (CCD m=2)")
         res["poisson_spde"] = poisson_line(args.nx, args.n_obs, order=args.order)
