@@ -491,7 +491,9 @@ __device__ __forceinline__ void base16(double* F, int k0, int kn, double* Zk) {
 // the calling workgroup has MB / 16 waves; only F's lower triangle is read). With
 // WITHZ (ncols == m, F's strict upper triangle zero on entry) F's strict upper
 // triangle receives Z = L^-1, transposed: Z[i][j] (i > j) at F[j * LD + i]; Z's
-// diagonal is 1 and its upper part 0. Zk (16 x 17) and Wb (MB x 17) are scratch.
+// diagonal is 1 and its upper part 0. Zk (16 x 17) is scratch, and so is Wb
+// (MB x 17, the panel rows times D) with WITHZ; without, W[i][p] of block k0 is kept
+// in F's strict upper triangle at F[(k0 + p) * LD + i] and Wb is not used.
 template <int MB, bool WITHZ>
 __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* Zk, double* Wb) {
   constexpr int LD = MB + 1, NW = MB / 16;
@@ -520,7 +522,10 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         for (int v = 0; v < 4; ++v) {
           const int ii = 4 * v + vr, pp = vc;
           if (r0 + ii < m && pp < kn) {
-            Wb[(r0 - t0 + ii) * 17 + pp] = acc[v];
+            if (WITHZ)
+              Wb[(r0 - t0 + ii) * 17 + pp] = acc[v];
+            else
+              F[(k0 + pp) * LD + r0 + ii] = acc[v];  // W[r0 + ii][pp] in F's free upper triangle
             F[(r0 + ii) * LD + k0 + pp] = acc[v] / F[(k0 + pp) * LD + k0 + pp];
           }
         }
@@ -571,7 +576,8 @@ __device__ __forceinline__ void front_ldlt(double* F, int m, int ncols, double* 
         const int R = t0 + 16 * ib, C = t0 + 16 * jb;
         acc = wgemm16(
             [&](int ii, int pp) {
-              return LV{Wb[(16 * ib + ii) * 17 + min(pp, kn - 1)], pp < kn};
+              const int wr = min(t0 + 16 * ib + ii, m - 1), wc = min(pp, kn - 1);
+              return LV{WITHZ ? Wb[(wr - t0) * 17 + wc] : F[(k0 + wc) * LD + wr], pp < kn};
             },
             [&](int pp, int jj) {
               return LV{F[min(C + jj, m - 1) * LD + k0 + min(pp, kn - 1)], (pp < kn && C + jj < m)};
@@ -624,7 +630,6 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
   constexpr int NT = 4 * MB, LD = MB + 1, NCM = 32;
   __shared__ double F[MB * LD];
   __shared__ double Zk[16 * 17];
-  __shared__ double Wb[MB * 17];
   __shared__ int64_t ch_uoff[NCM], ch_rp[NCM];  // the first NCM children's U offset and row list
   __shared__ int ch_rc[NCM];
   const int s = list[blockIdx.x];
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(4 * MB) void k_factor_small(DevPlan P, const int32_
     }
     __syncthreads();
   }
-  front_ldlt<MB, false>(F, m, w, Zk, Wb);
+  front_ldlt<MB, false>(F, m, w, Zk, nullptr);
   for (int idx = tid; idx < m * m; idx += NT) {
     const int j = idx / m, i = idx - j * m;
     if (j < w)
