@@ -464,13 +464,13 @@ gmrf_status gmrf_batch_logdet(gmrf_batch* bt, int32_t count, double* logdet, int
   if (!bt) return fail(GMRF_INVALID_ARGUMENT, "null argument");
   if (count < 1 || count > bt->cap) return fail(GMRF_INVALID_ARGUMENT, "count exceeds batch capacity");
   try {
-    std::vector<double> ld(count);
-    std::vector<int32_t> st(count);
-    check(hipMemcpyAsync(ld.data(), bt->d_logdet, count * sizeof(double), hipMemcpyDeviceToHost,
-                         bt->stream),
+    char* pb = bt->pin.get((size_t)count * (sizeof(double) + sizeof(int32_t)));
+    const double* ld = reinterpret_cast<const double*>(pb);
+    const int32_t* st = reinterpret_cast<const int32_t*>(pb + (size_t)count * sizeof(double));
+    check(hipMemcpyAsync(pb, bt->d_logdet, count * sizeof(double), hipMemcpyDeviceToHost, bt->stream),
           "copy logdet");
-    check(hipMemcpyAsync(st.data(), bt->d_status, count * sizeof(int32_t), hipMemcpyDeviceToHost,
-                         bt->stream),
+    check(hipMemcpyAsync(pb + (size_t)count * sizeof(double), bt->d_status, count * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, bt->stream),
           "copy status");
     check(hipStreamSynchronize(bt->stream), "sync");
     for (int b = 0; b < count; ++b) {
@@ -635,7 +635,10 @@ gmrf_status solver_factorize_sym(gmrf_solver* s, const std::shared_ptr<SymData>&
   }
   const int64_t nnz = sym->S.nnz_full;
   s->ensure(s->d_vals, s->d_vals_cap, (size_t)std::max<int64_t>(nnz, 1));
-  check(hipMemcpyAsync(s->d_vals, data, nnz * sizeof(double), hipMemcpyHostToDevice, sl.bt->stream),
+  // every earlier copy through the staging buffer has completed (each call synchronises)
+  char* pb = s->pin.get((size_t)nnz * sizeof(double));
+  std::memcpy(pb, data, (size_t)nnz * sizeof(double));
+  check(hipMemcpyAsync(s->d_vals, pb, nnz * sizeof(double), hipMemcpyHostToDevice, sl.bt->stream),
         "upload values");
   sl.bt->factorize(1, s->d_vals, nnz);
   double ld;
@@ -741,12 +744,13 @@ gmrf_status gmrf_solver_solve(gmrf_solver* s, const double* rhs, double* x) {
     const int64_t n = bt->sym->S.n;
     check(hipSetDevice(bt->sym->device), "hipSetDevice");
     s->ensure(s->d_vec, s->d_vec_cap, (size_t)n);
-    check(hipMemcpyAsync(s->d_vec, rhs, n * sizeof(double), hipMemcpyHostToDevice, bt->stream),
-          "upload rhs");
+    double* pb = reinterpret_cast<double*>(s->pin.get((size_t)n * sizeof(double)));
+    std::memcpy(pb, rhs, n * sizeof(double));
+    check(hipMemcpyAsync(s->d_vec, pb, n * sizeof(double), hipMemcpyHostToDevice, bt->stream), "upload rhs");
     bt->solve(1, s->d_vec, n, s->d_vec, n);
-    check(hipMemcpyAsync(x, s->d_vec, n * sizeof(double), hipMemcpyDeviceToHost, bt->stream),
-          "download x");
+    check(hipMemcpyAsync(pb, s->d_vec, n * sizeof(double), hipMemcpyDeviceToHost, bt->stream), "download x");
     check(hipStreamSynchronize(bt->stream), "sync");
+    std::memcpy(x, pb, n * sizeof(double));
     return GMRF_OK;
   } catch (...) {
     return from_exception();
@@ -769,9 +773,11 @@ gmrf_status gmrf_solver_diag_inv(gmrf_solver* s, double* out) {
     }
     s->ensure(s->d_vec, s->d_vec_cap, (size_t)n);
     check(launch_diag_out(bt->sym->plan, 1, bt->S, s->d_vec, n, bt->stream), "diag_out");
-    check(hipMemcpyAsync(out, s->d_vec, n * sizeof(double), hipMemcpyDeviceToHost, bt->stream),
+    double* pb = reinterpret_cast<double*>(s->pin.get((size_t)n * sizeof(double)));
+    check(hipMemcpyAsync(pb, s->d_vec, n * sizeof(double), hipMemcpyDeviceToHost, bt->stream),
           "download diag");
     check(hipStreamSynchronize(bt->stream), "sync");
+    std::memcpy(out, pb, n * sizeof(double));
     return GMRF_OK;
   } catch (...) {
     return from_exception();

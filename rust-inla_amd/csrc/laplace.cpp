@@ -72,27 +72,6 @@ bool trace_newton() {
 enum { RED_MAXABS = 0, RED_SUM = 1, RED_DOT = 2, RED_MAXDIFF = 3 };
 }  // namespace
 
-// Pinned host staging for the small per-evaluation read-backs: copies into
-// pageable memory are staged one at a time with a host round trip each.
-struct PinnedBuf {
-  char* p = nullptr;
-  size_t n = 0;
-  // callers ensure no copy into the buffer is in flight
-  char* get(size_t bytes) {
-    if (bytes > n) {
-      if (p) (void)hipHostFree(p);
-      p = nullptr;
-      n = 0;
-      check(hipHostMalloc((void**)&p, bytes, hipHostMallocDefault), "hipHostMalloc");
-      n = bytes;
-    }
-    return p;
-  }
-  ~PinnedBuf() {
-    if (p) (void)hipHostFree(p);
-  }
-};
-
 // Device buffer grown on demand (values-mode gradient traces), freed with the model.
 struct DevBuf {
   double* p = nullptr;

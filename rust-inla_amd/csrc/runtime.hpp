@@ -182,6 +182,31 @@ struct DevArena {
   }
 };
 
+// Pinned host staging, grown on demand: every copy between the host and the device
+// goes through memory of the handle's own (no copies from or into pageable memory,
+// which the runtime pins or stages per call, with many threads copying at once).
+struct PinnedBuf {
+  char* p = nullptr;
+  size_t n = 0;
+  // callers ensure no copy into or out of the buffer is in flight
+  char* get(size_t bytes) {
+    if (bytes > n) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+      check(hipHostMalloc((void**)&p, std::max<size_t>(bytes, 64), hipHostMallocDefault), "hipHostMalloc");
+      n = std::max<size_t>(bytes, 64);
+    }
+    return p;
+  }
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct LevelInfo {
   int64_t sb, se;     // small fronts [sb, se) in small list, ordered by size class
   int64_t cls_b[kSmallClasses + 1];  // class q (m <= kClassM[q]) covers [cls_b[q], cls_b[q + 1])
@@ -873,6 +898,7 @@ struct gmrf_batch {
          *W = nullptr, *Z = nullptr;
   double* d_logdet = nullptr;
   int32_t* d_status = nullptr;
+  PinnedBuf pin;                  // host staging of the log-determinant and status read-back
   double* d_ldpart = nullptr;     // per-matrix chunk partials of log|D|
   uint32_t* d_ldcount = nullptr;  // per-matrix chunk counters (zero between launches)
   // forward/backward workspaces for solves with several right-hand sides per
@@ -958,7 +984,13 @@ struct gmrf_batch {
     d_status = arena.alloc<int32_t>(cap);
     d_ldpart = arena.alloc<double>((size_t)cap * 2 * 16);
     d_ldcount = arena.alloc<uint32_t>(cap);
-    check(hipMemset(d_ldcount, 0, (size_t)cap * sizeof(uint32_t)), "hipMemset");
+    // the counters must be zero before the first k_logdet: cleared on the batch's own
+    // (non-blocking) stream and waited for here. A clear on the null stream is not
+    // ordered before work on a non-blocking stream, and a fresh allocation may reuse
+    // memory another handle has just freed, so the first log determinant of a batch
+    // could otherwise count from a stale value (and never be written).
+    check(hipMemsetAsync(d_ldcount, 0, (size_t)cap * sizeof(uint32_t), own), "hipMemsetAsync");
+    check(hipStreamSynchronize(own), "hipStreamSynchronize");
   }
   ~gmrf_batch() {
     if (aux) {
@@ -1346,6 +1378,7 @@ struct gmrf_solver {
   size_t d_vals_cap = 0;
   double* d_vec = nullptr;
   size_t d_vec_cap = 0;
+  PinnedBuf pin;  // host staging of the values, right-hand side and results
   ~gmrf_solver() {
     if (d_vals) (void)hipFree(d_vals);
     if (d_vec) (void)hipFree(d_vec);
